@@ -1,0 +1,599 @@
+/*
+ * ckks_oracle.c -- CPU restatement of the reference's CKKS hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the GPU product is compared
+ * against; the product never links it.  Only tests/, the smoke check and
+ * bench.py's cpu_baseline leg may load it.
+ *
+ * It restates, in plain C, the scalar semantics and the exact operation order
+ * of the reference (paths relative to the reference root):
+ *   Montgomery unit / compounds   src/montgomery/unit.h:13-31,
+ *                                 src/montgomery/unit.cpp, compound.cpp
+ *   NTT / INTT                    src/operation/ntt.cpp:46-134, :177-272
+ *   per-prime constants           src/ckks/derived_parameters.cpp:19-50,
+ *                                 src/arithmetic/arithmetic.cpp:10-167
+ *   chain-level constants         src/ckks/precomputed.cpp:33-137, :325-382
+ *   partitions                    src/ckks/parameters.cpp:325-360
+ *   tensor                        src/ckks/core.cpp:284-314
+ *   to_ntt / to_intt              src/ckks/core_ntt.cpp:91-150
+ *   MRS / ModUp / ModDown / KS    src/ckks/core_key_switch.cpp:21-160, :293-358
+ *   relinearize                   src/ckks/core.cpp:713-741
+ *   rescale                       src/ckks/core_rescale.cpp:5-25
+ *
+ * Values are int64 with the reference's two's-complement behaviour: the
+ * Montgomery product sign-extends its operands to 128 bits exactly like
+ * static_cast<uint128_t>(int64_t) does, so negative intermediates in ModDown
+ * and rescale reproduce bit for bit.
+ */
+#include "ckks_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+typedef __int128 i128;
+
+#define NBITS 62
+#define FB_MASK ((UINT64_C(1) << NBITS) - 1)
+#define R_VALUE (INT64_C(1) << NBITS)
+
+/* ---------------------------------------------------------------- scalars */
+
+/* src/montgomery/unit.h:13 */
+static inline int64_t guard(int64_t x, int64_t q) { return (x < q) ? x : x - q; }
+
+/* src/montgomery/unit.h:15-23 */
+int64_t oracle_mont_mult(int64_t a, int64_t b, uint64_t q, uint64_t k) {
+  const u128 x = (u128)(i128)a * (u128)(i128)b;
+  const uint64_t s = (uint64_t)(x * k) & FB_MASK;
+  const u128 sq = (u128)s * (u128)q;
+  return (int64_t)(uint64_t)((x + sq) >> NBITS);
+}
+
+/* src/montgomery/unit.h:25-31 */
+int64_t oracle_mont_exit(int64_t x, uint64_t q, uint64_t k) {
+  const uint64_t s = ((uint64_t)x * k) & FB_MASK;
+  const u128 sq = (u128)s * (u128)q;
+  return (int64_t)(uint64_t)(((u128)(i128)x + sq) >> NBITS);
+}
+
+#define MONT oracle_mont_mult
+
+/* src/arithmetic/arithmetic.cpp:10-31 (extended Euclid, 0 if not invertible) */
+static int64_t modulus_invert(int64_t x, int64_t modulus) {
+  int64_t a = x, b = modulus, ra = 1, rb = 0;
+  while (b > 0) {
+    int64_t quotient = a / b;
+    int64_t t = a;
+    a = b;
+    b = t - quotient * b;
+    t = ra;
+    ra = rb;
+    rb = t - quotient * rb;
+  }
+  if (a != 1)
+    return 0;
+  return ra >= 0 ? ra : ra + modulus;
+}
+
+/* src/arithmetic/arithmetic.cpp:33-38 */
+static int64_t modulus_multiply(int64_t a, int64_t b, int64_t m) {
+  i128 p = (i128)a * (i128)b;
+  p %= (i128)m;
+  return (int64_t)p;
+}
+
+/* src/arithmetic/arithmetic.cpp:40-52 */
+static int64_t modulus_power(int64_t base, int64_t e, int64_t m) {
+  int64_t cur = 1;
+  for (; e > 0; e >>= 1) {
+    if (e & 1)
+      cur = modulus_multiply(cur, base, m);
+    base = modulus_multiply(base, base, m);
+  }
+  return cur;
+}
+
+/* src/arithmetic/arithmetic.cpp:98-113: first base whose (q-1)/2N power has
+ * order exactly 2N */
+int64_t oracle_find_root(int64_t q, int coeff_count) {
+  const int64_t unit_power = (q - 1) / (2 * (int64_t)coeff_count);
+  int64_t e = 0;
+  for (int64_t base = 2; base < q; base++) {
+    e = modulus_power(base, unit_power, q);
+    if (modulus_power(e, coeff_count, q) != 1)
+      break;
+  }
+  return e;
+}
+
+/* src/arithmetic/arithmetic.cpp:159-167: k = (R * R^-1 - 1) / q */
+uint64_t oracle_calculate_k(uint64_t q) {
+  const int64_t rinv = modulus_invert(R_VALUE, (int64_t)q);
+  i128 k = (i128)R_VALUE * (i128)rinv - 1;
+  k /= (i128)q;
+  return (uint64_t)k;
+}
+
+static int reverse_bits(int x, int bits) {
+  int y = 0;
+  for (int i = 0; i < bits; i++)
+    y |= ((x >> i) & 1) << (bits - 1 - i);
+  return y;
+}
+
+/* ------------------------------------------------------------------- NTT */
+
+/* src/operation/ntt.cpp:46-134.  Stage s has 2^s groups of span N/2^(s+1);
+ * the reference walks the twiddle array sequentially, i.e. group g of stage
+ * s uses root[2^s - 1 + g]. */
+void oracle_ntt_raw(int64_t *y, const int64_t *root, uint64_t q, uint64_t k,
+                    int log_n) {
+  const int n = 1 << log_n;
+  const int64_t q2 = (int64_t)(q << 1);
+  for (int s = 0; s < log_n; s++) {
+    const int groups = 1 << s;
+    const int span = n >> (s + 1);
+    for (int g = 0; g < groups; g++) {
+      const int64_t w = root[groups - 1 + g];
+      int64_t *lo = y + 2 * g * span, *hi = lo + span;
+      for (int i = 0; i < span; i++) {
+        const int64_t a = lo[i];
+        const int64_t p = MONT(w, hi[i], q, k);
+        lo[i] = guard(a + p, q2);
+        hi[i] = guard(a + q2 - p, q2);
+      }
+    }
+  }
+}
+
+/* src/operation/ntt.cpp:177-272.  Stage t (span 2^t, N/2^(t+1) groups) uses
+ * root_inverse[N - N/2^t + g] (the backward paint order of
+ * src/ckks/precomputed.cpp:15-31). */
+void oracle_intt_raw(int64_t *y, const int64_t *root_inverse, uint64_t q,
+                     uint64_t k, int log_n) {
+  const int n = 1 << log_n;
+  const int64_t q2 = (int64_t)(q << 1);
+  int offset = 0;
+  for (int t = 0; t < log_n; t++) {
+    const int span = 1 << t;
+    const int groups = n >> (t + 1);
+    for (int g = 0; g < groups; g++) {
+      const int64_t w = root_inverse[offset + g];
+      int64_t *lo = y + 2 * g * span, *hi = lo + span;
+      for (int i = 0; i < span; i++) {
+        const int64_t a = lo[i], b = hi[i];
+        lo[i] = guard(a + b, q2);
+        hi[i] = MONT(w, guard(a + q2 - b, q2), q, k);
+      }
+    }
+    offset += groups;
+  }
+}
+
+/* --------------------------------------------------------------- context */
+
+struct oracle_ctx {
+  int log_n, n;
+  int prime_count, chain_count, alpha;
+  uint64_t *q, *k, *q2, *qh;
+  int64_t *r2, *ninv;
+  int64_t *root, *root_inv;   /* [prime][n-1] */
+  int64_t *p_product_mult_r;  /* [chain_count] */
+  int64_t *p_inverse_mult_r;  /* [alpha][prime_count]: sp-th special from the
+                                 end, entry per absolute prime index */
+  int64_t *rescale_scale;     /* [prime_count][prime_count]: [from][to] */
+};
+
+int oracle_ctx_chain_count(const oracle_ctx *ctx) { return ctx->chain_count; }
+
+oracle_ctx *oracle_ctx_create(int log_n, const uint64_t *primes,
+                              int prime_count, int special_count) {
+  oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
+  const int n = 1 << log_n;
+  c->log_n = log_n;
+  c->n = n;
+  c->prime_count = prime_count;
+  c->alpha = special_count;
+  c->chain_count = prime_count - special_count;
+  c->q = malloc(sizeof(uint64_t) * prime_count);
+  c->k = malloc(sizeof(uint64_t) * prime_count);
+  c->q2 = malloc(sizeof(uint64_t) * prime_count);
+  c->qh = malloc(sizeof(uint64_t) * prime_count);
+  c->r2 = malloc(sizeof(int64_t) * prime_count);
+  c->ninv = malloc(sizeof(int64_t) * prime_count);
+  c->root = malloc(sizeof(int64_t) * (size_t)prime_count * (n - 1));
+  c->root_inv = malloc(sizeof(int64_t) * (size_t)prime_count * (n - 1));
+  int64_t *power = malloc(sizeof(int64_t) * n);
+  int64_t *shuffled = malloc(sizeof(int64_t) * n);
+
+  for (int i = 0; i < prime_count; i++) {
+    /* src/ckks/derived_parameters.cpp:19-50 */
+    const uint64_t q = primes[i];
+    c->q[i] = q;
+    c->q2[i] = q << 1;
+    c->qh[i] = q >> 1;
+    c->r2[i] = modulus_multiply(R_VALUE, R_VALUE, (int64_t)q);
+    c->k[i] = oracle_calculate_k(q);
+    c->ninv[i] = modulus_multiply(modulus_invert(n, (int64_t)q), R_VALUE,
+                                  (int64_t)q);
+    const int64_t psi = oracle_find_root((int64_t)q, n);
+    for (int pass = 0; pass < 2; pass++) {
+      const int64_t base = pass == 0 ? psi : modulus_invert(psi, (int64_t)q);
+      power[0] = 1;
+      for (int j = 1; j < n; j++)
+        power[j] = modulus_multiply(power[j - 1], base, (int64_t)q);
+      for (int j = 0; j < n; j++)
+        shuffled[reverse_bits(j, log_n)] = power[j];
+      int64_t *dst = (pass == 0 ? c->root : c->root_inv) + (size_t)i * (n - 1);
+      if (pass == 0) {
+        for (int j = 1; j < n; j++)
+          dst[j - 1] = shuffled[j];
+      } else {
+        /* backward paint: groups N/2.., N/4.., ..., 1 */
+        int o = 0;
+        for (int lg = log_n - 1; lg >= 0; lg--) {
+          const int count = 1 << lg;
+          for (int g = 0; g < count; g++)
+            dst[o++] = shuffled[count + g];
+        }
+      }
+      for (int j = 0; j < n - 1; j++)
+        dst[j] = MONT(dst[j], c->r2[i], q, c->k[i]);
+    }
+  }
+  free(power);
+  free(shuffled);
+
+  /* src/ckks/precomputed.cpp:97-107 */
+  c->p_product_mult_r = malloc(sizeof(int64_t) * c->chain_count);
+  for (int i = 0; i < c->chain_count; i++) {
+    int64_t product = R_VALUE;
+    for (int p = c->chain_count; p < prime_count; p++)
+      product = modulus_multiply(product, (int64_t)c->q[p], (int64_t)c->q[i]);
+    c->p_product_mult_r[i] = product;
+  }
+  /* src/ckks/precomputed.cpp:109-121 */
+  c->p_inverse_mult_r = calloc((size_t)special_count * prime_count,
+                               sizeof(int64_t));
+  for (int sp = 0; sp < special_count; sp++) {
+    const int p_index = prime_count - 1 - sp;
+    for (int i = 0; i < p_index; i++) {
+      const int64_t inv = modulus_invert((int64_t)c->q[p_index], (int64_t)c->q[i]);
+      c->p_inverse_mult_r[(size_t)sp * prime_count + i] =
+          modulus_multiply(inv, R_VALUE, (int64_t)c->q[i]);
+    }
+  }
+  /* src/ckks/precomputed.cpp:123-137 */
+  c->rescale_scale = calloc((size_t)prime_count * prime_count, sizeof(int64_t));
+  for (int from = 0; from < prime_count; from++)
+    for (int to = from + 1; to < prime_count; to++) {
+      const int64_t inv = modulus_invert((int64_t)c->q[from], (int64_t)c->q[to]);
+      c->rescale_scale[(size_t)from * prime_count + to] =
+          modulus_multiply(inv, R_VALUE, (int64_t)c->q[to]);
+    }
+  return c;
+}
+
+void oracle_ctx_destroy(oracle_ctx *c) {
+  if (!c)
+    return;
+  free(c->q); free(c->k); free(c->q2); free(c->qh);
+  free(c->r2); free(c->ninv); free(c->root); free(c->root_inv);
+  free(c->p_product_mult_r); free(c->p_inverse_mult_r);
+  free(c->rescale_scale);
+  free(c);
+}
+
+void oracle_prime_constants(const oracle_ctx *c, int i, int64_t *out,
+                            int64_t *root, int64_t *root_inverse) {
+  out[0] = (int64_t)c->k[i];
+  out[1] = c->r2[i];
+  out[2] = c->ninv[i];
+  if (root)
+    memcpy(root, c->root + (size_t)i * (c->n - 1), sizeof(int64_t) * (c->n - 1));
+  if (root_inverse)
+    memcpy(root_inverse, c->root_inv + (size_t)i * (c->n - 1),
+           sizeof(int64_t) * (c->n - 1));
+}
+
+void oracle_ntt_limb(const oracle_ctx *c, int i, int64_t *x) {
+  oracle_ntt_raw(x, c->root + (size_t)i * (c->n - 1), c->q[i], c->k[i], c->log_n);
+}
+
+void oracle_intt_limb(const oracle_ctx *c, int i, int64_t *x) {
+  oracle_intt_raw(x, c->root_inv + (size_t)i * (c->n - 1), c->q[i], c->k[i],
+                  c->log_n);
+}
+
+#define LIMB(base, idx) ((base) + (size_t)(idx) * (size_t)n)
+
+/* src/ckks/core_ntt.cpp:91-100: mont_enter(R^2) then ntt */
+void oracle_to_ntt(const oracle_ctx *c, int mid, int limbs, int64_t *x) {
+  const int n = c->n;
+  for (int l = 0; l < limbs; l++) {
+    const int pi = mid + l;
+    int64_t *y = LIMB(x, l);
+    for (int j = 0; j < n; j++)
+      y[j] = MONT(y[j], c->r2[pi], c->q[pi], c->k[pi]);
+    oracle_ntt_limb(c, pi, y);
+  }
+}
+
+/* src/ckks/core_ntt.cpp:131-150: intt then mont_enter_exit(N^-1 R)
+ * (src/montgomery/compound.cpp:54-60) */
+void oracle_to_intt(const oracle_ctx *c, int mid, int limbs, int64_t *x) {
+  const int n = c->n;
+  for (int l = 0; l < limbs; l++) {
+    const int pi = mid + l;
+    int64_t *y = LIMB(x, l);
+    oracle_intt_limb(c, pi, y);
+    for (int j = 0; j < n; j++)
+      y[j] = oracle_mont_exit(MONT(y[j], c->ninv[pi], c->q[pi], c->k[pi]),
+                              c->q[pi], c->k[pi]);
+  }
+}
+
+/* src/ckks/core.cpp:284-314 */
+void oracle_tensor(const oracle_ctx *c, int mid, int limbs, const int64_t *a,
+                   const int64_t *b, int64_t *trio) {
+  const int n = c->n;
+  const size_t poly = (size_t)limbs * n;
+  for (int l = 0; l < limbs; l++) {
+    const int pi = mid + l;
+    const uint64_t q = c->q[pi], k = c->k[pi];
+    const int64_t q2 = (int64_t)c->q2[pi];
+    for (int j = 0; j < n; j++) {
+      const size_t o = (size_t)l * n + j;
+      const int64_t a0 = a[o], a1 = a[poly + o], b0 = b[o], b1 = b[poly + o];
+      trio[o] = MONT(a0, b0, q, k);
+      const int64_t z1 = MONT(a0, b1, q, k);
+      const int64_t z3 = MONT(a1, b0, q, k);
+      trio[poly + o] = guard(guard(z1 + z3, q2), (int64_t)q);
+      trio[2 * poly + o] = MONT(a1, b1, q, k);
+    }
+  }
+}
+
+/* src/ckks/parameters.cpp:325-360 */
+static int partition_count(const oracle_ctx *c, int mid) {
+  const int chain = c->chain_count - mid;
+  return (chain + c->alpha - 1) / c->alpha;
+}
+
+int oracle_partitions(const oracle_ctx *c, int mid, int key_mid, int *starts,
+                      int *sizes, int *key_index) {
+  const int chain = c->chain_count - mid;
+  const int count = partition_count(c, mid);
+  int pidx = partition_count(c, key_mid) - count;
+  int o = 0;
+  for (int start = chain - count * c->alpha; start < chain;
+       start += c->alpha, pidx++, o++) {
+    const int lo = start > 0 ? start : 0;
+    const int hi = start + c->alpha < chain ? start + c->alpha : chain;
+    starts[o] = lo;
+    sizes[o] = hi - lo;
+    key_index[o] = pidx;
+  }
+  return count;
+}
+
+/* (prod_{t<j} q[part[t]]) mod modulus, as BigInt % modulus
+ * (src/ckks/precomputed.cpp:325-382) */
+static int64_t partial_product_mod(const oracle_ctx *c, int mid, int start,
+                                   int j, int64_t modulus) {
+  int64_t r = 1 % modulus;
+  for (int t = 0; t < j; t++)
+    r = modulus_multiply(r, (int64_t)(c->q[mid + start + t] % (uint64_t)modulus),
+                         modulus);
+  return r;
+}
+
+/* Exact hybrid key switching of x (INTT form, `chain` limbs at mid)
+ * following src/ckks/core_key_switch.cpp:293-358 (alpha > 1) and the
+ * single-special-prime path :162-291 (alpha == 1), which computes the same
+ * limbs through mont_enter_tiled + NTT per partition. */
+void oracle_key_switch(const oracle_ctx *c, int mid, const int64_t *x,
+                       const int64_t *key, int key_mid, int64_t *out) {
+  const int n = c->n;
+  const int alpha = c->alpha;
+  const int chain = c->chain_count - mid;
+  const int ext = chain + alpha;
+  const int key_chain = c->chain_count - key_mid + alpha;
+  const int key_offset = (c->chain_count - key_mid) - chain;
+  int starts[128], sizes[128], kidx[128];
+  const int parts = oracle_partitions(c, mid, key_mid, starts, sizes, kidx);
+
+  int64_t *acc = calloc((size_t)2 * ext * n, sizeof(int64_t));
+  int64_t *mrs = malloc(sizeof(int64_t) * (size_t)alpha * n);
+  int64_t *extended = malloc(sizeof(int64_t) * (size_t)ext * n);
+
+  for (int p = 0; p < parts; p++) {
+    const int s = starts[p], size = sizes[p];
+    /* rns_to_mrs, src/ckks/core_key_switch.cpp:21-56 */
+    for (int j = 0; j < size; j++)
+      memcpy(LIMB(mrs, j), LIMB(x, s), sizeof(int64_t) * n);
+    for (int j = 0; j + 1 < size; j++) {
+      const int pi = mid + s + j + 1;
+      const int64_t m = (int64_t)c->q[pi];
+      const int64_t qp = partial_product_mod(c, mid, s, j + 1, m);
+      const int64_t inv_r = modulus_multiply(modulus_invert(qp, m), R_VALUE, m);
+      const int64_t q2 = (int64_t)c->q2[pi];
+      int64_t *d = LIMB(mrs, j + 1);
+      const int64_t *xs = LIMB(x, s + j + 1);
+      for (int t = 0; t < n; t++) {
+        const int64_t diff = guard(xs[t] + q2 - d[t], q2);
+        d[t] = guard(MONT(diff, inv_r, c->q[pi], c->k[pi]), m);
+      }
+      for (int tgt = j + 2; tgt < size; tgt++) {
+        const int ti = mid + s + tgt;
+        const int64_t tm = (int64_t)c->q[ti];
+        const int64_t cst = modulus_multiply(
+            partial_product_mod(c, mid, s, j + 1, tm), R_VALUE, tm);
+        const int64_t tq2 = (int64_t)c->q2[ti];
+        int64_t *y = LIMB(mrs, tgt);
+        for (int t = 0; t < n; t++)
+          y[t] = guard(guard(MONT(d[t], cst, c->q[ti], c->k[ti]) + y[t], tq2),
+                       tm);
+      }
+    }
+    /* mod_up, src/ckks/core_key_switch.cpp:92-108 */
+    for (int i = 0; i < ext; i++) {
+      const int pi = mid + i;
+      const uint64_t q = c->q[pi], k = c->k[pi];
+      const int64_t q2 = (int64_t)c->q2[pi];
+      int64_t *y = LIMB(extended, i);
+      for (int t = 0; t < n; t++)
+        y[t] = MONT(mrs[t], c->r2[pi], q, k);
+      for (int j = 1; j < size; j++) {
+        const int64_t cst = modulus_multiply(
+            partial_product_mod(c, mid, s, j, (int64_t)q), c->r2[pi], (int64_t)q);
+        const int64_t *d = LIMB(mrs, j);
+        for (int t = 0; t < n; t++)
+          y[t] = guard(MONT(d[t], cst, q, k) + y[t], q2);
+      }
+      oracle_ntt_limb(c, pi, y);
+    }
+    /* key inner product, src/ckks/core_key_switch.cpp:318-336 */
+    for (int half = 0; half < 2; half++) {
+      const int64_t *kp =
+          key + ((size_t)(2 * kidx[p] + half) * key_chain + key_offset) * n;
+      int64_t *a = acc + (size_t)half * ext * n;
+      for (int i = 0; i < ext; i++) {
+        const int pi = mid + i;
+        const uint64_t q = c->q[pi], k = c->k[pi];
+        const int64_t q2 = (int64_t)c->q2[pi];
+        const int64_t *kl = LIMB(kp, i), *e = LIMB(extended, i);
+        int64_t *al = LIMB(a, i);
+        if (p == 0) {
+          for (int t = 0; t < n; t++)
+            al[t] = MONT(e[t], kl[t], q, k);
+        } else {
+          for (int t = 0; t < n; t++)
+            al[t] = guard(guard(MONT(e[t], kl[t], q, k) + al[t], q2),
+                          (int64_t)q);
+        }
+      }
+    }
+  }
+
+  /* mod_down, src/ckks/core_key_switch.cpp:110-160 */
+  for (int half = 0; half < 2; half++) {
+    int64_t *a = acc + (size_t)half * ext * n;
+    oracle_to_intt(c, mid, ext, a);
+    for (int i = 0; i < ext; i++) {
+      const int64_t q = (int64_t)c->q[mid + i];
+      int64_t *al = LIMB(a, i);
+      for (int t = 0; t < n; t++)
+        al[t] = guard(al[t], q);
+    }
+    int reduced = ext;
+    for (int sp = 0; sp < alpha; sp++) {
+      reduced--;
+      const int64_t *xp = LIMB(a, reduced);
+      for (int i = 0; i < reduced; i++) {
+        const int pi = mid + i;
+        const int64_t cst = c->p_inverse_mult_r[(size_t)sp * c->prime_count + pi];
+        const int64_t q2 = (int64_t)c->q2[pi];
+        int64_t *al = LIMB(a, i);
+        for (int t = 0; t < n; t++)
+          al[t] = MONT(guard(al[t] + q2 - xp[t], q2), cst, c->q[pi], c->k[pi]);
+      }
+    }
+    int64_t *o = out + (size_t)half * chain * n;
+    for (int i = 0; i < chain; i++) {
+      const int64_t q = (int64_t)c->q[mid + i];
+      const int64_t *al = LIMB(a, i);
+      int64_t *ol = LIMB(o, i);
+      for (int t = 0; t < n; t++)
+        ol[t] = al[t] < 0 ? al[t] + q : guard(al[t], q);
+    }
+  }
+  free(acc);
+  free(mrs);
+  free(extended);
+}
+
+/* src/ckks/core.cpp:713-741 */
+void oracle_relinearize(const oracle_ctx *c, int mid, const int64_t *trio,
+                        const int64_t *key, int key_mid, int64_t *duo) {
+  const int n = c->n;
+  const int chain = c->chain_count - mid;
+  const size_t poly = (size_t)chain * n;
+  int64_t *x2 = malloc(sizeof(int64_t) * poly);
+  int64_t *ks = malloc(sizeof(int64_t) * 2 * poly);
+  memcpy(duo, trio, sizeof(int64_t) * 2 * poly);
+  oracle_to_intt(c, mid, chain, duo);
+  oracle_to_intt(c, mid, chain, duo + poly);
+  memcpy(x2, trio + 2 * poly, sizeof(int64_t) * poly);
+  oracle_to_intt(c, mid, chain, x2);
+  oracle_key_switch(c, mid, x2, key, key_mid, ks);
+  for (int h = 0; h < 2; h++)
+    for (int l = 0; l < chain; l++) {
+      const int64_t q = (int64_t)c->q[mid + l], q2 = (int64_t)c->q2[mid + l];
+      int64_t *y = duo + h * poly + (size_t)l * n;
+      const int64_t *z = ks + h * poly + (size_t)l * n;
+      for (int t = 0; t < n; t++)
+        y[t] = guard(guard(y[t] + z[t], q2), q);
+    }
+  free(x2);
+  free(ks);
+}
+
+/* src/ckks/core_rescale.cpp:5-25 with the processor primitives of
+ * src/processor/single.cpp:76-124 */
+void oracle_rescale(const oracle_ctx *c, int mid, int polys, const int64_t *x,
+                    int64_t *y) {
+  const int n = c->n;
+  const int chain = c->chain_count - mid;
+  const int next = chain - 1;
+  const int64_t qh0 = (int64_t)c->qh[mid];
+  for (int p = 0; p < polys; p++) {
+    const int64_t *xp = x + (size_t)p * chain * n;
+    int64_t *yp = y + (size_t)p * next * n;
+    const int64_t *x0 = xp;
+    for (int i = 0; i < next; i++) {
+      const int pi = mid + 1 + i;
+      const int64_t q = (int64_t)c->q[pi], q2 = (int64_t)c->q2[pi];
+      const int64_t cst = c->rescale_scale[(size_t)mid * c->prime_count + pi];
+      const int64_t *xi = LIMB(xp, i + 1);
+      int64_t *yi = LIMB(yp, i);
+      for (int t = 0; t < n; t++) {
+        const int64_t rounder = x0[t] > qh0 ? 1 : 0;
+        int64_t v = xi[t] - x0[t];
+        v = v >= 0 ? v : v + q2;
+        v = guard(v, q);
+        v = MONT(v, cst, c->q[pi], c->k[pi]);
+        v = guard(v + rounder, q2);
+        yi[t] = guard(v, q);
+      }
+    }
+  }
+}
+
+/* src/ckks/engine_multiply.cpp:149-162 on rescaled INTT inputs, followed by
+ * src/ckks/engine_rescale.cpp:38-49 */
+void oracle_hmult_relin_rescale(const oracle_ctx *c, int mid, const int64_t *a,
+                                const int64_t *b, const int64_t *key,
+                                int key_mid, int64_t *out) {
+  const int n = c->n;
+  const int chain = c->chain_count - mid;
+  const size_t duo = (size_t)2 * chain * n;
+  int64_t *an = malloc(sizeof(int64_t) * duo);
+  int64_t *bn = malloc(sizeof(int64_t) * duo);
+  int64_t *trio = malloc(sizeof(int64_t) * (duo + duo / 2));
+  int64_t *rel = malloc(sizeof(int64_t) * duo);
+  memcpy(an, a, sizeof(int64_t) * duo);
+  memcpy(bn, b, sizeof(int64_t) * duo);
+  for (int p = 0; p < 2; p++) {
+    oracle_to_ntt(c, mid, chain, an + (size_t)p * chain * n);
+    oracle_to_ntt(c, mid, chain, bn + (size_t)p * chain * n);
+  }
+  oracle_tensor(c, mid, chain, an, bn, trio);
+  oracle_relinearize(c, mid, trio, key, key_mid, rel);
+  oracle_rescale(c, mid, 2, rel, out);
+  free(an);
+  free(bn);
+  free(trio);
+  free(rel);
+}

@@ -1,0 +1,315 @@
+// Test-only C wrapper around the reference CKKS engine.
+//
+// This file is oracle infrastructure: it is compiled together with the
+// reference's own sources (by oracle/Makefile, outputs under oracle/_ref/) and
+// is loaded only by tests/, the smoke check and bench.py's cpu_baseline leg.
+// It never ships in the product library.
+//
+// Every entry point forwards to the reference's public API
+// (src/engine/abstract_engine.h, src/ckks/engine.h) or to its free functions
+// (src/operation/ntt.h, src/ckks/precomputed.h).  Ciphertexts and keys are
+// exchanged as plain int64 arrays in the reference's limb order
+// [polynomial][limb][coefficient].
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ckks/config.h"
+#include "ckks/engine.h"
+#include "ckks/precomputed.h"
+#include "engine/engine.h"
+#include "operation/ntt.h"
+
+using namespace desilo::fhe;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct RefEngine {
+  CKKSConfig config;
+  CKKSEngine *engine;
+  CKKSPrecomputed *precomputed;
+};
+
+CKKSConfig make_config(int log_n, int special_primes, int bootstrap_preset,
+                       int thread_count) {
+  CKKSConfig config;
+  config.log_coeff_count_ = log_n;
+  config.special_prime_count_ = special_primes;
+  config.scheme_preset_ = bootstrap_preset ? SchemePreset::CKKS_54_60_BOOTSTRAP
+                                           : SchemePreset::CKKS_40_60_DEFAULT;
+  if (thread_count > 1) {
+    config.processor_ = EngineProcessor::CPU_PARALLEL;
+    config.thread_count_ = thread_count;
+  }
+  return config;
+}
+
+void copy_out(Data *data, int64_t *out) {
+  const int polys = data->polynomial_count();
+  const int chains = data->chain_count_;
+  const int n = data->coeff_count_;
+  for (int p = 0; p < polys; p++) {
+    auto poly = data->get_polynomial(p);
+    for (int c = 0; c < chains; c++) {
+      std::memcpy(out + ((size_t)p * chains + c) * n, poly->get(c),
+                  sizeof(int64_t) * n);
+    }
+  }
+}
+
+void copy_in(Data *data, const int64_t *in) {
+  const int polys = data->polynomial_count();
+  const int chains = data->chain_count_;
+  const int n = data->coeff_count_;
+  for (int p = 0; p < polys; p++) {
+    auto poly = data->get_polynomial(p);
+    for (int c = 0; c < chains; c++) {
+      std::memcpy(poly->get(c), in + ((size_t)p * chains + c) * n,
+                  sizeof(int64_t) * n);
+    }
+  }
+}
+
+template <class F> auto guarded(F &&f) -> decltype(f()) {
+  try {
+    return f();
+  } catch (const std::exception &e) {
+    g_error = e.what();
+  } catch (...) {
+    g_error = "unknown exception";
+  }
+  return decltype(f())();
+}
+
+} // namespace
+
+extern "C" {
+
+const char *ref_last_error() { return g_error.c_str(); }
+
+void *ref_engine_create(int log_n, int special_primes, int bootstrap_preset,
+                        int thread_count) {
+  return guarded([&]() -> void * {
+    auto config =
+        make_config(log_n, special_primes, bootstrap_preset, thread_count);
+    config.validate();
+    auto ref = new RefEngine{config, new CKKSEngine(config),
+                             new CKKSPrecomputed(config)};
+    return ref;
+  });
+}
+
+void ref_engine_destroy(void *handle) {
+  auto ref = static_cast<RefEngine *>(handle);
+  if (!ref)
+    return;
+  delete ref->engine;
+  delete ref->precomputed;
+  delete ref;
+}
+
+// out[0..8] = N, prime_count, chain_count, special_prime_count, full_level,
+//             max_level, slot_count, scale_bits, one_more_level
+int ref_engine_info(void *handle, int64_t *out) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    const auto &p = ref->precomputed->parameters_;
+    out[0] = p.get_coeff_count();
+    out[1] = p.get_prime_count();
+    out[2] = p.get_chain_count();
+    out[3] = p.get_special_prime_count();
+    out[4] = p.get_full_level();
+    out[5] = p.get_max_level();
+    out[6] = p.get_slot_count();
+    out[7] = p.get_scale_bit_length();
+    out[8] = p.get_use_one_more_level();
+    return 0;
+  });
+}
+
+int ref_engine_primes(void *handle, uint64_t *out) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    const auto prime_count = ref->precomputed->prime_count_;
+    const auto q = ref->precomputed->get_q(0);
+    for (int i = 0; i < prime_count; i++)
+      out[i] = q[i];
+    return prime_count;
+  });
+}
+
+int ref_moduli_id(void *handle, int level) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    return ref->precomputed->parameters_.get_moduli_id(level);
+  });
+}
+
+// Per-prime derived constants, for checking a restatement of
+// src/ckks/derived_parameters.cpp:19-50 and src/ckks/precomputed.cpp:33-217.
+// out = {k, r_squared, coeff_count_inverse}, root/root_inverse get N-1 each.
+int ref_prime_constants(void *handle, int prime_index, int64_t *out,
+                        int64_t *root, int64_t *root_inverse) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    auto pre = ref->precomputed;
+    const int n = pre->coeff_count_;
+    out[0] = (int64_t)pre->get_k(0)[prime_index];
+    out[1] = pre->get_r_squared(0)[prime_index];
+    out[2] = pre->get_coeff_count_inverse(0)[prime_index];
+    if (root)
+      std::memcpy(root, pre->get_root(0, prime_index),
+                  sizeof(int64_t) * (n - 1));
+    if (root_inverse)
+      std::memcpy(root_inverse, pre->get_root_inverse(0, prime_index),
+                  sizeof(int64_t) * (n - 1));
+    return 0;
+  });
+}
+
+// In-place forward / inverse NTT of one limb with the reference's free
+// functions (src/operation/ntt.cpp:46-134, :177-272).
+int ref_ntt_limb(void *handle, int prime_index, int64_t *x, int inverse) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    auto pre = ref->precomputed;
+    const int n = pre->coeff_count_;
+    const uint64_t q = pre->get_q(0)[prime_index];
+    const uint64_t k = pre->get_k(0)[prime_index];
+    const uint64_t q2 = pre->get_q_double(0)[prime_index];
+    if (inverse)
+      intt(x, x, pre->get_root_inverse(0, prime_index), q2, q, k, n,
+           pre->log_coeff_count_);
+    else
+      ntt(x, x, pre->get_root(0, prime_index), q2, q, k, n,
+          pre->log_coeff_count_);
+    return 0;
+  });
+}
+
+void *ref_secret_key(void *handle) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->create_secret_key();
+  });
+}
+
+void *ref_public_key(void *handle, void *sk) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->create_public_key(
+        static_cast<SecretKey *>(sk));
+  });
+}
+
+void *ref_relin_key(void *handle, void *sk) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->create_relinearization_key(
+        static_cast<SecretKey *>(sk));
+  });
+}
+
+void ref_free_secret_key(void *p) { delete static_cast<SecretKey *>(p); }
+void ref_free_public_key(void *p) { delete static_cast<PublicKey *>(p); }
+void ref_free_relin_key(void *p) { delete static_cast<RelinearizationKey *>(p); }
+void ref_free_ct(void *p) { delete static_cast<UnitCiphertext *>(p); }
+
+// shape[0..3] = moduli_id, polynomial_count, chain_count, coeff_count
+static void data_shape(Data *d, int64_t *shape) {
+  shape[0] = d->moduli_id_;
+  shape[1] = d->polynomial_count();
+  shape[2] = d->chain_count_;
+  shape[3] = d->coeff_count_;
+}
+
+int ref_relin_key_shape(void *rlk, int64_t *shape) {
+  data_shape(static_cast<RelinearizationKey *>(rlk)->key_switching_key_, shape);
+  return 0;
+}
+int ref_relin_key_get(void *rlk, int64_t *out) {
+  copy_out(static_cast<RelinearizationKey *>(rlk)->key_switching_key_, out);
+  return 0;
+}
+int ref_relin_key_set(void *rlk, const int64_t *in) {
+  copy_in(static_cast<RelinearizationKey *>(rlk)->key_switching_key_, in);
+  return 0;
+}
+
+// shape[0..5] = moduli_id, polynomial_count, chain_count, coeff_count, flag,
+//               level
+int ref_ct_shape(void *ct, int64_t *shape) {
+  auto c = static_cast<UnitCiphertext *>(ct);
+  data_shape(c, shape);
+  shape[4] = static_cast<int64_t>(c->flag_);
+  shape[5] = c->level_;
+  return 0;
+}
+double ref_ct_scale(void *ct) {
+  return (double)static_cast<UnitCiphertext *>(ct)->ckks_scale_;
+}
+int ref_ct_get(void *ct, int64_t *out) {
+  copy_out(static_cast<UnitCiphertext *>(ct), out);
+  return 0;
+}
+int ref_ct_set(void *ct, const int64_t *in) {
+  copy_in(static_cast<UnitCiphertext *>(ct), in);
+  return 0;
+}
+
+// Public-key encryption of real slot values at a level (level < 0: max).
+void *ref_encrypt(void *handle, const double *values, int count, void *pk,
+                  int level) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    std::vector<double> input(values, values + count);
+    auto key = static_cast<PublicKey *>(pk);
+    if (level < 0)
+      return ref->engine->encrypt_unit(input, key);
+    return ref->engine->encrypt_unit(input, key, level);
+  });
+}
+
+int ref_decrypt(void *handle, void *ct, void *sk, double *out) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    auto values = ref->engine->decrypt(static_cast<UnitCiphertext *>(ct),
+                                       static_cast<SecretKey *>(sk));
+    std::memcpy(out, values.data(), sizeof(double) * values.size());
+    return (int)values.size();
+  });
+}
+
+void *ref_multiply_relinearize(void *handle, void *a, void *b, void *rlk) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    return ref->engine->multiply(static_cast<UnitCiphertext *>(a),
+                                 static_cast<UnitCiphertext *>(b),
+                                 static_cast<RelinearizationKey *>(rlk));
+  });
+}
+
+void *ref_rescale(void *handle, void *ct) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    return ref->engine->rescale(static_cast<UnitCiphertext *>(ct));
+  });
+}
+
+// One HMult+relin+rescale unit of work through the reference engine:
+// multiply (src/ckks/engine_multiply.cpp:149-162) then rescale
+// (src/ckks/engine_rescale.cpp:38-49).
+void *ref_hmult_relin_rescale(void *handle, void *a, void *b, void *rlk) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    auto product = ref->engine->multiply(static_cast<UnitCiphertext *>(a),
+                                         static_cast<UnitCiphertext *>(b),
+                                         static_cast<RelinearizationKey *>(rlk));
+    auto rescaled = ref->engine->rescale(product);
+    delete product;
+    return rescaled;
+  });
+}
+
+} // extern "C"
