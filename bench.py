@@ -1,0 +1,233 @@
+"""Benchmark: batched HMult+relin+rescale ciphertexts/s on one MI355X.
+
+Workload (one "step"): one batched fhe_hmult_relin_rescale launch over B
+ciphertext pairs of the Gold parameter set (N = 2^16, 38-prime chain with 4
+special primes), inputs rescaled at level 24 (25 limbs, moduli_id 13) with a
+relinearization key at the key-switching level (moduli_id 1, 10 partitions of
+which 7 are used).  Inputs and key are synthetic seeded residues, resident in
+HBM before timing starts.
+
+Printed JSON line: metric/value (whole-job ciphertexts/s), plain (B = 1)
+throughput beside it, the roofline of the dominant kernel measured live with
+HIP events on the library's stream, and the CPU baseline (the reference
+engine built from its sources, CPU_PARALLEL on the host cores) on a bounded
+sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+LOG_N = 16
+ALPHA = 4
+MODULI_ID = 13          # rescaled level 24: 25 limbs
+HBM_PEAK_GBS = 8000.0
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (the library's own)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [
+            ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return ms.value
+
+
+def uniform_limbs(rng, primes, first, limbs, n, factor=1):
+    out = np.empty((limbs, n), np.int64)
+    for l in range(limbs):
+        out[l] = rng.integers(0, int(primes[first + l]) * factor, n,
+                              dtype=np.uint64).astype(np.int64)
+    return out
+
+
+def make_inputs(ctx, batch, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    limbs = ctx.limbs_at(MODULI_ID)
+    key_mid = ctx.key_moduli_id()
+    kp, kc, n = ctx.key_shape(key_mid)
+    a = np.stack([np.stack([uniform_limbs(rng, ctx.primes, MODULI_ID, limbs, n)
+                            for _ in range(2)]) for _ in range(batch)])
+    b = np.stack([np.stack([uniform_limbs(rng, ctx.primes, MODULI_ID, limbs, n)
+                            for _ in range(2)]) for _ in range(batch)])
+    key = np.stack([uniform_limbs(rng, ctx.primes, key_mid, kc, n, 2)
+                    for _ in range(kp)])
+    return a, b, key, key_mid
+
+
+def time_steps(ctx, ev, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    ctx.synchronize()
+    start, stop = ev.event(), ev.event()
+    ev.record(start, ctx.stream)
+    for _ in range(steps):
+        fn()
+    ev.record(stop, ctx.stream)
+    ms = ev.elapsed_ms(start, stop)
+    ctx.synchronize()
+    return ms / steps
+
+
+def cpu_baseline(budget_s=20.0):
+    """The reference engine (oracle/_ref, compiled from its sources) on the
+    host cores; falls back to the oracle port (1 thread) when absent."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from support import Oracle, Reference
+    from seeded_inputs import uniform_limbs
+    threads = min(os.cpu_count() or 1, 16)
+    if Reference.available():
+        ref = Reference(LOG_N, ALPHA, threads=threads)
+        lib, e = ref.lib, ref.e
+        sk = lib.ref_secret_key(e)
+        pk = lib.ref_public_key(e, sk)
+        rlk = lib.ref_relin_key(e, sk)
+        slots = ref.slot_count
+        zeros = np.zeros(slots)
+        level = ref.chain_count - MODULI_ID - 1
+        a0 = lib.ref_encrypt(e, zeros.ctypes.data, slots, pk, level)
+        b0 = lib.ref_encrypt(e, zeros.ctypes.data, slots, pk, level)
+        a, b = lib.ref_rescale(e, a0), lib.ref_rescale(e, b0)
+        done, t0 = 0, time.perf_counter()
+        while True:
+            c = lib.ref_hmult_relin_rescale(e, a, b, rlk)
+            lib.ref_free_ct(c)
+            done += 1
+            if time.perf_counter() - t0 > budget_s or done >= 200:
+                break
+        dt = time.perf_counter() - t0
+        for h in (a0, b0, a, b):
+            lib.ref_free_ct(h)
+        ref.close()
+        return {"value": done / dt, "unit": "ciphertexts/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"{done} HMult+relin+rescale at Gold L=24 through "
+                          f"CKKSEngine::multiply+rescale, CPU_PARALLEL "
+                          f"thread_count={threads}, {dt:.1f} s"}
+    import fhe_amd
+    primes = fhe_amd.resolve_primes(LOG_N, ALPHA)
+    orc = Oracle(LOG_N, primes, ALPHA)
+    rng = np.random.Generator(np.random.PCG64(1))
+    limbs = orc.limbs_at(MODULI_ID)
+    n = 1 << LOG_N
+    a = np.stack([uniform_limbs(rng, primes, MODULI_ID, limbs, n)
+                  for _ in range(2)])
+    key_mid = 1
+    kc = orc.limbs_at(key_mid) + ALPHA
+    kp = 2 * (-(-orc.limbs_at(key_mid) // ALPHA))
+    key = np.stack([uniform_limbs(rng, primes, key_mid, kc, n, 2)
+                    for _ in range(kp)])
+    done, t0 = 0, time.perf_counter()
+    while True:
+        orc.hmult(a, a, key, MODULI_ID, key_mid)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "ciphertexts/s", "cores": 1,
+            "kind": "port", "sample": f"{done} oracle HMult at Gold L=24"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only (--gpus 1): batches shard "
+              "across GPUs with replicated keys and no communication, and "
+              "that multi-GPU form is not measured.", file=sys.stderr)
+        sys.exit(2)
+
+    import fhe_amd
+    ev = HipEvents()
+    with fhe_amd.Context(LOG_N, ALPHA) as ctx:
+        limbs = ctx.limbs_at(MODULI_ID)
+        n = ctx.n
+        a, b, key, key_mid = make_inputs(ctx, args.batch, 7)
+        da, db, dk = ctx.to_device(a), ctx.to_device(b), ctx.to_device(key)
+        out = ctx.array((args.batch, 2, limbs - 1, n))
+        step_ms = time_steps(
+            ctx, ev, lambda: ctx.hmult_relin_rescale(da, db, dk, key_mid, out,
+                                                     MODULI_ID),
+            args.steps, args.warmup)
+        # plain form: one ciphertext pair per launch
+        pa, pb = ctx.to_device(a[:1]), ctx.to_device(b[:1])
+        pout = ctx.array((1, 2, limbs - 1, n))
+        plain_ms = time_steps(
+            ctx, ev, lambda: ctx.hmult_relin_rescale(pa, pb, dk, key_mid, pout,
+                                                     MODULI_ID),
+            max(5, args.steps), 2)
+        # dominant kernel: forward NTT of the extended basis (two passes),
+        # timed alone on the same stream over a batch-sized buffer
+        ext = limbs + ALPHA
+        ntt_buf = ctx.array((args.batch, ext, n))
+        ntt_ms = time_steps(ctx, ev, lambda: ctx.ntt(ntt_buf, MODULI_ID,
+                                                     enter=False),
+                            args.steps, 2)
+        for arr in (da, db, dk, out, pa, pb, pout, ntt_buf):
+            arr.free()
+
+    value = args.batch / (step_ms / 1e3)
+    ntt_bytes = 2 * 8 * n * ext * args.batch  # read + write once (algorithmic)
+    ntt_gbs = ntt_bytes / (ntt_ms / 1e3) / 1e9
+    line = {
+        "metric": "HMult+relin+rescale ciphertexts/s at N=2^16,L=24 (batched)",
+        "value": round(value, 2),
+        "unit": "ciphertexts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64 (62-bit Montgomery residues)",
+        "data": "synthetic",
+        "config": {"workload": "Gold HMult+relin+rescale, N=2^16, L=24 "
+                               "(25->24 limbs), alpha=4, dnum=7, batch "
+                               f"{args.batch} ciphertext pairs per launch",
+                   "batch": args.batch, "log_n": LOG_N, "moduli_id": MODULI_ID},
+        "plain_ciphertexts_per_s": round(1 / (plain_ms / 1e3), 2),
+        "batched_vs_plain": round(value * plain_ms / 1e3, 3),
+        "roofline": {"bound": "hbm", "kernel": "ntt_pass_kernel (forward, 2 "
+                     "passes, extended basis)",
+                     "achieved": round(ntt_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ntt_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

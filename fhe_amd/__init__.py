@@ -1,0 +1,256 @@
+"""Python binding of the MI355X CKKS backend's C ABI (include/fhe_amd.h).
+
+The product is the native library libfhe_amd.so; this module only loads it
+with ctypes and wraps device buffers for tests and the benchmark.  There is
+no fallback: if the library is missing or cannot be loaded, import of the
+binding raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_PKG, "libfhe_amd.so")
+
+FHE_OK = 0
+INFO_FIELDS = ("coeff_count", "prime_count", "chain_count",
+               "special_prime_count", "full_level", "max_level",
+               "max_key_switching_level", "slot_count", "scale_bits")
+
+# every entry point declared in include/fhe_amd.h
+EXPORTED_SYMBOLS = (
+    "fhe_context_create", "fhe_context_destroy", "fhe_last_error",
+    "fhe_context_info", "fhe_context_primes", "fhe_context_prime_constants",
+    "fhe_context_stream", "fhe_synchronize", "fhe_device_alloc",
+    "fhe_device_free", "fhe_copy_to_device", "fhe_copy_to_host", "fhe_ntt",
+    "fhe_intt", "fhe_mont_mult", "fhe_rescale", "fhe_relinearize",
+    "fhe_hmult_relin_rescale", "fhe_params_resolve_primes",
+)
+
+
+class FheError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"fhe error {code}: {message}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("log_coeff_count", ctypes.c_int32),
+                ("special_prime_count", ctypes.c_int32),
+                ("scale_bits", ctypes.c_int32),
+                ("small_prime_bits", ctypes.c_int32),
+                ("one_more_level", ctypes.c_int32),
+                ("max_level", ctypes.c_int32),
+                ("prime_count", ctypes.c_int32),
+                ("primes", ctypes.POINTER(ctypes.c_uint64))]
+
+
+_lib = None
+
+
+def library():
+    """Load libfhe_amd.so once; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise ImportError(
+                f"{LIBRARY_PATH} is not built; run python -m fhe_amd.build")
+        lib = ctypes.CDLL(LIBRARY_PATH)
+        vp, i32, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+        lib.fhe_last_error.restype = ctypes.c_char_p
+        lib.fhe_context_stream.restype = vp
+        lib.fhe_context_stream.argtypes = [vp]
+        lib.fhe_context_destroy.argtypes = [vp]
+        lib.fhe_context_create.argtypes = [ctypes.POINTER(Params),
+                                           ctypes.POINTER(vp)]
+        lib.fhe_params_resolve_primes.argtypes = [
+            ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_uint64), i32,
+            ctypes.POINTER(i32)]
+        lib.fhe_context_info.argtypes = [vp, vp, i32]
+        lib.fhe_context_primes.argtypes = [vp, vp, i32]
+        lib.fhe_context_prime_constants.argtypes = [vp, i32, vp, vp, vp]
+        lib.fhe_synchronize.argtypes = [vp]
+        lib.fhe_device_alloc.argtypes = [vp, sz, ctypes.POINTER(vp)]
+        lib.fhe_device_free.argtypes = [vp, vp]
+        lib.fhe_copy_to_device.argtypes = [vp, vp, vp, sz]
+        lib.fhe_copy_to_host.argtypes = [vp, vp, vp, sz]
+        lib.fhe_ntt.argtypes = [vp, vp, i32, i32, i32, i32]
+        lib.fhe_intt.argtypes = [vp, vp, i32, i32, i32, i32]
+        lib.fhe_mont_mult.argtypes = [vp, vp, vp, vp, i32, i32, i32]
+        lib.fhe_rescale.argtypes = [vp, vp, vp, i32, i32]
+        lib.fhe_relinearize.argtypes = [vp, i32, i32, vp, vp, i32, vp]
+        lib.fhe_hmult_relin_rescale.argtypes = [vp, i32, i32, vp, vp, vp, i32,
+                                                vp]
+        _lib = lib
+    return _lib
+
+
+def _check(code):
+    if code != FHE_OK:
+        raise FheError(code, library().fhe_last_error().decode())
+
+
+def make_params(log_n, special_prime_count, scale_bits=40, small_prime_bits=40,
+                one_more_level=True, max_level=-1, primes=None):
+    p = Params(log_n, special_prime_count, scale_bits, small_prime_bits,
+               1 if one_more_level else 0, max_level, 0, None)
+    if primes is not None:
+        arr = (ctypes.c_uint64 * len(primes))(*[int(q) for q in primes])
+        p.prime_count = len(primes)
+        p.primes = ctypes.cast(arr, ctypes.POINTER(ctypes.c_uint64))
+        p._keep = arr
+    return p
+
+
+def resolve_primes(log_n, special_prime_count, scale_bits=40,
+                   small_prime_bits=40, one_more_level=True):
+    """Host-only: the prime chain the context would use (no GPU needed)."""
+    p = make_params(log_n, special_prime_count, scale_bits, small_prime_bits,
+                    one_more_level)
+    out = (ctypes.c_uint64 * 512)()
+    count = ctypes.c_int(0)
+    _check(library().fhe_params_resolve_primes(ctypes.byref(p), out, 512,
+                                               ctypes.byref(count)))
+    return np.array(out[:count.value], dtype=np.uint64)
+
+
+class DeviceArray:
+    """A device int64 buffer owned by a Context."""
+
+    def __init__(self, ctx, shape):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        ptr = ctypes.c_void_p()
+        _check(library().fhe_device_alloc(ctx.handle, self.nbytes,
+                                          ctypes.byref(ptr)))
+        self.ptr = ptr
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=np.int64)
+        assert host.nbytes == self.nbytes, (host.shape, self.shape)
+        _check(library().fhe_copy_to_device(self.ctx.handle, self.ptr,
+                                            host.ctypes.data, self.nbytes))
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, dtype=np.int64)
+        _check(library().fhe_copy_to_host(self.ctx.handle, out.ctypes.data,
+                                          self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            _check(library().fhe_device_free(self.ctx.handle, self.ptr))
+        self.ptr = None
+
+
+class Context:
+    """fhe_context wrapper: parameters + device tables + stream."""
+
+    def __init__(self, log_n, special_prime_count, scale_bits=40,
+                 small_prime_bits=40, one_more_level=True, max_level=-1,
+                 primes=None):
+        self._params = make_params(log_n, special_prime_count, scale_bits,
+                                   small_prime_bits, one_more_level, max_level,
+                                   primes)
+        h = ctypes.c_void_p()
+        _check(library().fhe_context_create(ctypes.byref(self._params),
+                                            ctypes.byref(h)))
+        self.handle = h
+        info = np.zeros(len(INFO_FIELDS), np.int64)
+        _check(library().fhe_context_info(h, info.ctypes.data, len(info)))
+        self.info = dict(zip(INFO_FIELDS, (int(v) for v in info)))
+        self.n = self.info["coeff_count"]
+        self.log_n = self.n.bit_length() - 1
+        self.alpha = self.info["special_prime_count"]
+        self.chain_count = self.info["chain_count"]
+        q = np.zeros(self.info["prime_count"], np.uint64)
+        _check(library().fhe_context_primes(h, q.ctypes.data, len(q)))
+        self.primes = q
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            library().fhe_context_destroy(self.handle)
+        self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def stream(self):
+        return library().fhe_context_stream(self.handle)
+
+    def limbs_at(self, moduli_id):
+        return self.chain_count - moduli_id
+
+    def key_moduli_id(self):
+        return self.info["full_level"] - self.info["max_key_switching_level"]
+
+    def key_shape(self, key_moduli_id):
+        limbs = self.limbs_at(key_moduli_id)
+        parts = -(-limbs // self.alpha)
+        return (2 * parts, limbs + self.alpha, self.n)
+
+    def array(self, shape):
+        return DeviceArray(self, shape)
+
+    def to_device(self, host):
+        host = np.ascontiguousarray(host, dtype=np.int64)
+        return DeviceArray(self, host.shape).upload(host)
+
+    def synchronize(self):
+        _check(library().fhe_synchronize(self.handle))
+
+    def prime_constants(self, index):
+        c = np.zeros(3, np.int64)
+        root = np.zeros(self.n - 1, np.int64)
+        rinv = np.zeros(self.n - 1, np.int64)
+        _check(library().fhe_context_prime_constants(
+            self.handle, index, c.ctypes.data, root.ctypes.data,
+            rinv.ctypes.data))
+        return c, root, rinv
+
+    # -- batched primitives on device arrays --------------------------------
+    def ntt(self, arr, moduli_id, enter=True):
+        polys, limbs = _polys_limbs(arr, self.n)
+        _check(library().fhe_ntt(self.handle, arr.ptr, polys, limbs,
+                                 moduli_id, 1 if enter else 0))
+
+    def intt(self, arr, moduli_id, post=1):
+        polys, limbs = _polys_limbs(arr, self.n)
+        _check(library().fhe_intt(self.handle, arr.ptr, polys, limbs,
+                                  moduli_id, post))
+
+    def mont_mult(self, a, b, out, moduli_id):
+        polys, limbs = _polys_limbs(a, self.n)
+        _check(library().fhe_mont_mult(self.handle, a.ptr, b.ptr, out.ptr,
+                                       polys, limbs, moduli_id))
+
+    def rescale(self, src, dst, moduli_id):
+        polys = int(np.prod(src.shape[:-2]))
+        _check(library().fhe_rescale(self.handle, src.ptr, dst.ptr, polys,
+                                     moduli_id))
+
+    def relinearize(self, trios, key, key_moduli_id, out, moduli_id):
+        count = int(np.prod(trios.shape[:-3]))
+        _check(library().fhe_relinearize(self.handle, count, moduli_id,
+                                         trios.ptr, key.ptr, key_moduli_id,
+                                         out.ptr))
+
+    def hmult_relin_rescale(self, a, b, key, key_moduli_id, out, moduli_id):
+        count = int(np.prod(a.shape[:-3]))
+        _check(library().fhe_hmult_relin_rescale(self.handle, count, moduli_id,
+                                                 a.ptr, b.ptr, key.ptr,
+                                                 key_moduli_id, out.ptr))
+
+
+def _polys_limbs(arr, n):
+    assert arr.shape[-1] == n
+    limbs = arr.shape[-2]
+    polys = int(np.prod(arr.shape[:-2])) if len(arr.shape) > 2 else 1
+    return polys, limbs

@@ -1,0 +1,361 @@
+// Elementwise and key-switching kernels for the CKKS hot path (gfx950).
+//
+// Each kernel performs the reference's per-coefficient operation sequence in
+// the same order (so lazy and signed intermediates reproduce bit for bit), but
+// fuses the steps that the reference runs as separate full-polynomial passes
+// into one pass over HBM.  Exact-arithmetic modular work: VALU only, no MFMA.
+#include "kernels.h"
+
+#include "params.h"
+
+namespace fhe {
+
+namespace {
+
+constexpr int kMaxAlpha = CkksParams::kMaxAlpha;
+
+__global__ __launch_bounds__(kBlock) void mont_mult_kernel(
+    const int64_t *a, const int64_t *b, int64_t *out, int64_t total, int limbs,
+    int moduli_id, DeviceTables t) {
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= total)
+    return;
+  const int p = moduli_id + int((idx >> t.log_n) % limbs);
+  out[idx] = mont_mult(a[idx], b[idx], t.q[p], t.k[p]);
+}
+
+__global__ __launch_bounds__(kBlock) void tensor_kernel(
+    const int64_t *a, const int64_t *b, int64_t *trio, int64_t total,
+    int limbs, int moduli_id, DeviceTables t) {
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= total)
+    return;
+  const int64_t poly = int64_t(limbs) << t.log_n;
+  const int64_t c = idx / poly;
+  const int64_t r = idx - c * poly;
+  const int p = moduli_id + int(r >> t.log_n);
+  const uint64_t q = t.q[p], k = t.k[p];
+  const int64_t q2 = int64_t(t.q2[p]);
+  const int64_t *ac = a + c * 2 * poly + r;
+  const int64_t *bc = b + c * 2 * poly + r;
+  int64_t *z = trio + c * 3 * poly + r;
+  const int64_t a0 = ac[0], a1 = ac[poly], b0 = bc[0], b1 = bc[poly];
+  z[0] = mont_mult_u(a0, b0, q, k);
+  const int64_t z1 = mont_mult_u(a0, b1, q, k);
+  const int64_t z3 = mont_mult_u(a1, b0, q, k);
+  z[poly] = guard(guard(z1 + z3, q2), int64_t(q));
+  z[2 * poly] = mont_mult_u(a1, b1, q, k);
+}
+
+// One thread per (ciphertext, coefficient): Garner digits of the partition
+// in registers, then every limb of the extended basis.
+template <int MAXZ>
+__global__ __launch_bounds__(kBlock) void modup_kernel(
+    const int64_t *x, int64_t x_stride, int64_t *ext, int64_t ext_stride,
+    int count, int limbs, int moduli_id, int start, int size,
+    const int64_t *cst, DeviceTables t) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= int64_t(count) * n)
+    return;
+  const int64_t c = idx >> t.log_n;
+  const int64_t coef = idx & (n - 1);
+  const int64_t *xs = x + c * x_stride + coef;
+  const int ext_limbs = limbs + t.alpha;
+
+  int64_t d[MAXZ];
+  const int64_t first = xs[int64_t(start) * n];
+#pragma unroll
+  for (int j = 0; j < MAXZ; j++)
+    d[j] = first;
+  // rns_to_mrs (src/ckks/core_key_switch.cpp:21-56)
+#pragma unroll
+  for (int j = 1; j < MAXZ; j++) {
+    if (j < size) {
+      const int pj = moduli_id + start + j;
+      const int64_t m = int64_t(t.q[pj]), m2 = int64_t(t.q2[pj]);
+      const int64_t diff = guard(xs[int64_t(start + j) * n] + m2 - d[j], m2);
+      d[j] = guard(mont_mult(diff, cst[CkksParams::kKsInvOffset + j], t.q[pj],
+                             t.k[pj]),
+                   m);
+#pragma unroll
+      for (int tt = j + 1; tt < MAXZ; tt++) {
+        if (tt < size) {
+          const int pt = moduli_id + start + tt;
+          const int64_t prod = mont_mult(
+              d[j], cst[CkksParams::kKsTileOffset + j * kMaxAlpha + tt],
+              t.q[pt], t.k[pt]);
+          d[tt] = guard(guard(prod + d[tt], int64_t(t.q2[pt])),
+                        int64_t(t.q[pt]));
+        }
+      }
+    }
+  }
+  // mod_up (src/ckks/core_key_switch.cpp:92-108) before its NTT
+  const int64_t *ec = cst + CkksParams::kKsExtOffset;
+  int64_t *out = ext + c * ext_stride + coef;
+  for (int i = 0; i < ext_limbs; i++) {
+    const int p = moduli_id + i;
+    const uint64_t q = t.q[p], k = t.k[p];
+    const int64_t q2 = int64_t(t.q2[p]);
+    int64_t y = mont_mult(d[0], ec[i], q, k);
+#pragma unroll
+    for (int j = 1; j < MAXZ; j++)
+      if (j < size)
+        y = guard(mont_mult(d[j], ec[j * ext_limbs + i], q, k) + y, q2);
+    out[int64_t(i) * n] = y;
+  }
+}
+
+// One thread per (limb, coefficient); the key values are loaded once and
+// applied to every ciphertext of the batch.
+__global__ __launch_bounds__(kBlock) void key_inner_product_kernel(
+    const int64_t *ext, int64_t ext_stride, const int64_t *key0,
+    const int64_t *key1, int64_t *acc, int64_t acc_stride, int count,
+    int ext_limbs, int moduli_id, int first, DeviceTables t) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= int64_t(ext_limbs) * n)
+    return;
+  const int p = moduli_id + int(idx >> t.log_n);
+  const uint64_t q = t.q[p], k = t.k[p];
+  const int64_t q2 = int64_t(t.q2[p]);
+  const int64_t k0 = key0[idx], k1 = key1[idx];
+  const int64_t half = int64_t(ext_limbs) * n;
+  for (int c = 0; c < count; c++) {
+    const int64_t e = ext[c * ext_stride + idx];
+    int64_t *a0 = acc + c * acc_stride + idx;
+    int64_t *a1 = a0 + half;
+    const int64_t p0 = mont_mult_u(e, k0, q, k);
+    const int64_t p1 = mont_mult_u(e, k1, q, k);
+    if (first) {
+      *a0 = p0;
+      *a1 = p1;
+    } else {
+      *a0 = guard(guard(p0 + *a0, q2), int64_t(q));
+      *a1 = guard(guard(p1 + *a1, q2), int64_t(q));
+    }
+  }
+}
+
+// ModDown (src/ckks/core_key_switch.cpp:110-160) of one key-switched
+// polynomial at one coefficient: the special-prime limbs are peeled off from
+// the last one down, each subtraction applied to every remaining limb.
+template <int MAXA> struct ModDown {
+  int64_t peeled[MAXA];
+  int alpha, limbs, moduli_id;
+
+  __device__ void prepare(const int64_t *a, int64_t n, const DeviceTables &t) {
+    int64_t sv[MAXA];
+#pragma unroll
+    for (int j = 0; j < MAXA; j++)
+      sv[j] = j < alpha ? a[int64_t(limbs + j) * n] : 0;
+#pragma unroll
+    for (int sp = 0; sp < MAXA; sp++) {
+      if (sp < alpha) {
+        const int r = alpha - 1 - sp;
+        int64_t xr = 0;
+#pragma unroll
+        for (int j = 0; j < MAXA; j++)
+          if (j == r)
+            xr = sv[j];
+        peeled[sp] = xr;
+#pragma unroll
+        for (int j = 0; j < MAXA; j++) {
+          if (j < r) {
+            const int p = moduli_id + limbs + j;
+            const int64_t q2 = int64_t(t.q2[p]);
+            sv[j] = mont_mult(guard(sv[j] + q2 - xr, q2),
+                              t.p_inverse[int64_t(sp) * t.prime_count + p],
+                              t.q[p], t.k[p]);
+          }
+        }
+      }
+    }
+  }
+
+  // the canonical key-switched value of ordinary limb i
+  __device__ int64_t limb(int64_t v, int i, const DeviceTables &t) const {
+    const int p = moduli_id + i;
+    const uint64_t q = t.q[p], k = t.k[p];
+    const int64_t q2 = int64_t(t.q2[p]);
+#pragma unroll
+    for (int sp = 0; sp < MAXA; sp++)
+      if (sp < alpha)
+        v = mont_mult(guard(v + q2 - peeled[sp], q2),
+                      t.p_inverse[int64_t(sp) * t.prime_count + p], q, k);
+    return v < 0 ? v + int64_t(q) : guard(v, int64_t(q));
+  }
+};
+
+template <int MAXA, bool RESCALE>
+__global__ __launch_bounds__(kBlock) void moddown_relin_kernel(
+    const int64_t *acc, const int64_t *trio, int64_t *out, int count,
+    int limbs, int moduli_id, DeviceTables t) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= int64_t(count) * 2 * n)
+    return;
+  const int64_t ch = idx >> t.log_n; // ciphertext * 2 + half
+  const int64_t coef = idx & (n - 1);
+  const int64_t c = ch >> 1, h = ch & 1;
+  const int ext_limbs = limbs + t.alpha;
+  const int64_t *a = acc + ch * ext_limbs * n + coef;
+  const int64_t *d = trio + (c * 3 + h) * limbs * n + coef;
+
+  ModDown<MAXA> md;
+  md.alpha = t.alpha;
+  md.limbs = limbs;
+  md.moduli_id = moduli_id;
+  md.prepare(a, n, t);
+
+  // relinearize: y_i = reduce(d_i + ks_i) (src/ckks/core.cpp:736-737)
+  auto relin = [&](int i) {
+    const int p = moduli_id + i;
+    const int64_t ks = md.limb(a[int64_t(i) * n], i, t);
+    return guard(guard(d[int64_t(i) * n] + ks, int64_t(t.q2[p])),
+                 int64_t(t.q[p]));
+  };
+
+  if (!RESCALE) {
+    int64_t *o = out + ch * limbs * n + coef;
+    for (int i = 0; i < limbs; i++)
+      o[int64_t(i) * n] = relin(i);
+    return;
+  }
+  // rescale (src/ckks/core_rescale.cpp:5-25): drop limb 0 with rounding
+  const int64_t y0 = relin(0);
+  const int64_t rounder = y0 > int64_t(t.qh[moduli_id]) ? 1 : 0;
+  int64_t *o = out + ch * (limbs - 1) * n + coef;
+  for (int i = 1; i < limbs; i++) {
+    const int p = moduli_id + i;
+    const uint64_t q = t.q[p], k = t.k[p];
+    const int64_t q2 = int64_t(t.q2[p]);
+    int64_t v = relin(i) - y0;
+    v = v >= 0 ? v : v + q2;
+    v = guard(v, int64_t(q));
+    v = mont_mult(v, t.rescale[int64_t(moduli_id) * t.prime_count + p], q, k);
+    v = guard(v + rounder, q2);
+    o[int64_t(i - 1) * n] = guard(v, int64_t(q));
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void rescale_kernel(
+    const int64_t *in, int64_t *out, int polys, int limbs, int moduli_id,
+    DeviceTables t) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= int64_t(polys) * n)
+    return;
+  const int64_t poly = idx >> t.log_n;
+  const int64_t coef = idx & (n - 1);
+  const int64_t *x = in + poly * limbs * n + coef;
+  int64_t *o = out + poly * (limbs - 1) * n + coef;
+  const int64_t x0 = x[0];
+  const int64_t rounder = x0 > int64_t(t.qh[moduli_id]) ? 1 : 0;
+  for (int i = 1; i < limbs; i++) {
+    const int p = moduli_id + i;
+    const uint64_t q = t.q[p], k = t.k[p];
+    const int64_t q2 = int64_t(t.q2[p]);
+    int64_t v = x[int64_t(i) * n] - x0;
+    v = v >= 0 ? v : v + q2;
+    v = guard(v, int64_t(q));
+    v = mont_mult(v, t.rescale[int64_t(moduli_id) * t.prime_count + p], q, k);
+    v = guard(v + rounder, q2);
+    o[int64_t(i - 1) * n] = guard(v, int64_t(q));
+  }
+}
+
+} // namespace
+
+void launch_mont_mult(const DeviceTables &t, const int64_t *a,
+                      const int64_t *b, int64_t *out, int64_t total_limbs,
+                      int limbs, int moduli_id, hipStream_t stream) {
+  const int64_t total = total_limbs << t.log_n;
+  hipLaunchKernelGGL(mont_mult_kernel, dim3(grid_for(total)), dim3(kBlock), 0,
+                     stream, a, b, out, total, limbs, moduli_id, t);
+}
+
+void launch_tensor(const DeviceTables &t, const int64_t *a, const int64_t *b,
+                   int64_t *trio, int count, int limbs, int moduli_id,
+                   hipStream_t stream) {
+  const int64_t total = (int64_t(count) * limbs) << t.log_n;
+  hipLaunchKernelGGL(tensor_kernel, dim3(grid_for(total)), dim3(kBlock), 0,
+                     stream, a, b, trio, total, limbs, moduli_id, t);
+}
+
+void launch_modup(const DeviceTables &t, const int64_t *x, int64_t x_stride,
+                  int64_t *ext, int64_t ext_stride, int count, int limbs,
+                  int moduli_id, int start, int size, const int64_t *constants,
+                  hipStream_t stream) {
+  const dim3 grid(grid_for(int64_t(count) << t.log_n));
+#define FHE_MODUP(W)                                                           \
+  hipLaunchKernelGGL(modup_kernel<W>, grid, dim3(kBlock), 0, stream, x,        \
+                     x_stride, ext, ext_stride, count, limbs, moduli_id,       \
+                     start, size, constants, t)
+  if (size <= 1)
+    FHE_MODUP(1);
+  else if (size <= 2)
+    FHE_MODUP(2);
+  else if (size <= 4)
+    FHE_MODUP(4);
+  else if (size <= 8)
+    FHE_MODUP(8);
+  else
+    FHE_MODUP(16);
+#undef FHE_MODUP
+}
+
+void launch_key_inner_product(const DeviceTables &t, const int64_t *ext,
+                              int64_t ext_stride, const int64_t *key0,
+                              const int64_t *key1, int64_t *acc,
+                              int64_t acc_stride, int count, int ext_limbs,
+                              int moduli_id, bool first, hipStream_t stream) {
+  const dim3 grid(grid_for(int64_t(ext_limbs) << t.log_n));
+  hipLaunchKernelGGL(key_inner_product_kernel, grid, dim3(kBlock), 0, stream,
+                     ext, ext_stride, key0, key1, acc, acc_stride, count,
+                     ext_limbs, moduli_id, first ? 1 : 0, t);
+}
+
+template <bool RESCALE>
+static void launch_moddown(const DeviceTables &t, const int64_t *acc,
+                           const int64_t *trio, int64_t *out, int count,
+                           int limbs, int moduli_id, hipStream_t stream) {
+  const dim3 grid(grid_for((int64_t(count) * 2) << t.log_n));
+#define FHE_MODDOWN(W)                                                         \
+  hipLaunchKernelGGL((moddown_relin_kernel<W, RESCALE>), grid, dim3(kBlock),  \
+                     0, stream, acc, trio, out, count, limbs, moduli_id, t)
+  if (t.alpha <= 1)
+    FHE_MODDOWN(1);
+  else if (t.alpha <= 2)
+    FHE_MODDOWN(2);
+  else if (t.alpha <= 4)
+    FHE_MODDOWN(4);
+  else if (t.alpha <= 8)
+    FHE_MODDOWN(8);
+  else
+    FHE_MODDOWN(16);
+#undef FHE_MODDOWN
+}
+
+void launch_moddown_relin_rescale(const DeviceTables &t, const int64_t *acc,
+                                  const int64_t *trio, int64_t *out, int count,
+                                  int limbs, int moduli_id,
+                                  hipStream_t stream) {
+  launch_moddown<true>(t, acc, trio, out, count, limbs, moduli_id, stream);
+}
+
+void launch_moddown_relin(const DeviceTables &t, const int64_t *acc,
+                          const int64_t *trio, int64_t *out, int count,
+                          int limbs, int moduli_id, hipStream_t stream) {
+  launch_moddown<false>(t, acc, trio, out, count, limbs, moduli_id, stream);
+}
+
+void launch_rescale(const DeviceTables &t, const int64_t *in, int64_t *out,
+                    int polys, int limbs, int moduli_id, hipStream_t stream) {
+  const dim3 grid(grid_for(int64_t(polys) << t.log_n));
+  hipLaunchKernelGGL(rescale_kernel, grid, dim3(kBlock), 0, stream, in, out,
+                     polys, limbs, moduli_id, t);
+}
+
+} // namespace fhe

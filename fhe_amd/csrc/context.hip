@@ -1,0 +1,470 @@
+// fhe_context: device-resident CKKS parameters, workspace and the C ABI
+// declared in include/fhe_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "fhe_amd.h"
+#include "kernels.h"
+#include "params.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct FheError : std::runtime_error {
+  int code;
+  FheError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what) {
+  if (e == hipErrorOutOfMemory)
+    throw FheError(FHE_ERR_OUT_OF_MEMORY, std::string(what) + ": out of memory");
+  if (e != hipSuccess)
+    throw FheError(FHE_ERR_DEVICE,
+                   std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void require(bool ok, const std::string &message) {
+  if (!ok)
+    throw FheError(FHE_ERR_INVALID, message);
+}
+
+template <class F> int guarded(F &&f) {
+  try {
+    f();
+    return FHE_OK;
+  } catch (const FheError &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::invalid_argument &e) {
+    g_last_error = e.what();
+    return FHE_ERR_INVALID;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return FHE_ERR_DEVICE;
+  }
+}
+
+template <class T> T *upload(const std::vector<T> &v, std::vector<void *> &own) {
+  void *p = nullptr;
+  hip_check(hipMalloc(&p, std::max<size_t>(1, v.size() * sizeof(T))),
+            "table allocation");
+  own.push_back(p);
+  if (!v.empty())
+    hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T),
+                        hipMemcpyHostToDevice),
+              "table upload");
+  return static_cast<T *>(p);
+}
+
+} // namespace
+
+struct fhe_context {
+  fhe::CkksParams params;
+  fhe::DeviceTables tables{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::vector<void *> owned;
+  // per-(moduli_id, key_moduli_id) ModUp constants on the device
+  std::map<std::pair<int, int>, int64_t *> modup_cache;
+  // growable scratch for the fused pipelines
+  void *workspace = nullptr;
+  size_t workspace_bytes = 0;
+
+  explicit fhe_context(fhe::CkksParams p) : params(std::move(p)) {
+    hip_check(hipGetDevice(&device), "hipGetDevice");
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking),
+              "stream creation");
+    auto &t = tables;
+    t.q = upload(params.q, owned);
+    t.k = upload(params.k, owned);
+    t.q2 = upload(params.q2, owned);
+    t.qh = upload(params.qh, owned);
+    t.r2 = upload(params.r2, owned);
+    t.ninv = upload(params.ninv, owned);
+    t.root = upload(params.root, owned);
+    t.root_inverse = upload(params.root_inverse, owned);
+    t.p_inverse = upload(params.p_inverse_mult_r, owned);
+    t.rescale = upload(params.rescale_scale, owned);
+    t.prime_count = params.prime_count;
+    t.log_n = params.log_n;
+    t.alpha = params.alpha;
+  }
+
+  ~fhe_context() {
+    if (stream)
+      hipStreamSynchronize(stream);
+    for (auto &kv : modup_cache)
+      hipFree(kv.second);
+    for (void *p : owned)
+      hipFree(p);
+    if (workspace)
+      hipFree(workspace);
+    if (stream)
+      hipStreamDestroy(stream);
+  }
+
+  int64_t n() const { return params.n; }
+
+  int64_t *scratch(size_t bytes) {
+    if (bytes > workspace_bytes) {
+      hip_check(hipStreamSynchronize(stream), "workspace sync");
+      if (workspace)
+        hip_check(hipFree(workspace), "workspace free");
+      workspace = nullptr;
+      workspace_bytes = 0;
+      hip_check(hipMalloc(&workspace, bytes), "workspace allocation");
+      workspace_bytes = bytes;
+    }
+    return static_cast<int64_t *>(workspace);
+  }
+
+  const int64_t *modup_constants(int moduli_id, int key_moduli_id) {
+    auto key = std::make_pair(moduli_id, key_moduli_id);
+    auto it = modup_cache.find(key);
+    if (it != modup_cache.end())
+      return it->second;
+    const auto host = params.modup_constants(moduli_id, key_moduli_id);
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, host.size() * sizeof(int64_t)), "constants");
+    hip_check(hipMemcpy(p, host.data(), host.size() * sizeof(int64_t),
+                        hipMemcpyHostToDevice),
+              "constants upload");
+    modup_cache[key] = static_cast<int64_t *>(p);
+    return static_cast<int64_t *>(p);
+  }
+
+  void check_launch(const char *what) {
+    hip_check(hipGetLastError(), what);
+  }
+
+  void check_moduli(int moduli_id, int min_limbs, const char *op) {
+    require(moduli_id >= 0 && moduli_id < params.chain_count,
+            std::string(op) + ": moduli_id out of range.");
+    require(params.limbs_at(moduli_id) >= min_limbs,
+            std::string(op) + ": the given level is too low.");
+  }
+
+  void check_key(int moduli_id, int key_moduli_id, const char *op) {
+    require(key_moduli_id >= 0 && key_moduli_id <= moduli_id,
+            std::string(op) +
+                ": the key level is lower than the ciphertext level.");
+    require(params.max_key_switching_level() >=
+                params.level_of(key_moduli_id, false),
+            std::string(op) + ": the key moduli id is above the key-switching "
+                              "level.");
+  }
+
+  // Key switching of the INTT-form third polynomial of each trio in ws_trio
+  // into acc ([count][2][ext][N], INTT form, reduced).
+  void key_switch(int count, int moduli_id, const int64_t *key,
+                  int key_moduli_id, int64_t *ws_trio, int64_t *ext,
+                  int64_t *acc) {
+    const int limbs = params.limbs_at(moduli_id);
+    const int ext_limbs = limbs + params.alpha;
+    const int64_t N = n();
+    const int key_chain = params.limbs_at(key_moduli_id) + params.alpha;
+    const int key_offset = params.limbs_at(key_moduli_id) - limbs;
+    const auto parts = params.partitions(moduli_id, key_moduli_id);
+    const int64_t *cst = modup_constants(moduli_id, key_moduli_id);
+    const int stride = params.ks_stride(moduli_id);
+    for (size_t p = 0; p < parts.size(); p++) {
+      const auto &part = parts[p];
+      fhe::launch_modup(tables, ws_trio + 2 * limbs * N, 3 * limbs * N, ext,
+                        ext_limbs * N, count, limbs, moduli_id, part.start,
+                        part.size, cst + p * stride, stream);
+      fhe::ntt_forward(tables, ext, ext, int64_t(count) * ext_limbs,
+                       ext_limbs, moduli_id, nullptr, stream);
+      const int64_t *key0 =
+          key + (int64_t(2 * part.key_index) * key_chain + key_offset) * N;
+      const int64_t *key1 =
+          key + (int64_t(2 * part.key_index + 1) * key_chain + key_offset) * N;
+      fhe::launch_key_inner_product(tables, ext, ext_limbs * N, key0, key1,
+                                    acc, 2 * ext_limbs * N, count, ext_limbs,
+                                    moduli_id, p == 0, stream);
+    }
+    fhe::ntt_inverse(tables, acc, acc, int64_t(count) * 2 * ext_limbs,
+                     ext_limbs, moduli_id, fhe::kInttExitReduce, stream);
+  }
+};
+
+extern "C" {
+
+const char *fhe_last_error(void) { return g_last_error.c_str(); }
+
+static fhe::CkksParams resolve_params(const fhe_params *p) {
+  require(p != nullptr, "null argument");
+  require(p->scale_bits == 40 || p->scale_bits == 54,
+          "CKKS Parameters: unsupported config.");
+  fhe::CkksParams params;
+  if (p->prime_count > 0) {
+    require(p->primes != nullptr, "primes must be given");
+    std::vector<uint64_t> primes(p->primes, p->primes + p->prime_count);
+    params = fhe::CkksParams::from_primes(p->log_coeff_count, primes,
+                                          p->special_prime_count, p->scale_bits,
+                                          p->one_more_level);
+    if (p->max_level >= 0)
+      params.max_level = p->max_level;
+  } else {
+    require(p->small_prime_bits == 40 || p->small_prime_bits == 54,
+            "CKKS Parameters: unsupported config.");
+    // The alternating search reproduces the reference's cached chains only
+    // up to N = 2^16; the N = 2^17 chain must be passed in explicitly.
+    require(p->log_coeff_count <= 16,
+            "CKKS Parameters: N = 2^17 needs an explicit prime chain.");
+    fhe::CkksConfig c;
+    c.log_n = p->log_coeff_count;
+    c.special_prime_count = p->special_prime_count;
+    c.scale_bits = p->scale_bits;
+    c.small_prime_bits = p->small_prime_bits;
+    c.one_more_level = p->one_more_level != 0;
+    c.max_level = p->max_level;
+    params = fhe::CkksParams::from_config(c);
+  }
+  require(params.max_level >= 0 && params.max_level <= params.full_level,
+          "The given max level is out of range.");
+  return params;
+}
+
+int fhe_params_resolve_primes(const fhe_params *p, uint64_t *primes,
+                              int capacity, int *prime_count) {
+  return guarded([&] {
+    require(primes && prime_count, "null argument");
+    const auto params = resolve_params(p);
+    require(capacity >= params.prime_count, "capacity too small");
+    std::memcpy(primes, params.q.data(), sizeof(uint64_t) * params.prime_count);
+    *prime_count = params.prime_count;
+  });
+}
+
+int fhe_context_create(const fhe_params *p, fhe_context **out) {
+  return guarded([&] {
+    require(out != nullptr, "null argument");
+    *out = nullptr;
+    auto params = resolve_params(p);
+    *out = new fhe_context(std::move(params));
+  });
+}
+
+void fhe_context_destroy(fhe_context *ctx) { delete ctx; }
+
+int fhe_context_info(const fhe_context *ctx, int64_t *info, int count) {
+  return guarded([&] {
+    require(ctx && info, "null argument");
+    const auto &p = ctx->params;
+    const int64_t v[FHE_INFO_COUNT] = {p.n,
+                                       p.prime_count,
+                                       p.chain_count,
+                                       p.alpha,
+                                       p.full_level,
+                                       p.max_level,
+                                       p.max_key_switching_level(),
+                                       p.n / 2,
+                                       p.scale_bits};
+    for (int i = 0; i < count && i < FHE_INFO_COUNT; i++)
+      info[i] = v[i];
+  });
+}
+
+int fhe_context_primes(const fhe_context *ctx, uint64_t *out, int capacity) {
+  return guarded([&] {
+    require(ctx && out, "null argument");
+    require(capacity >= ctx->params.prime_count, "capacity too small");
+    std::memcpy(out, ctx->params.q.data(),
+                sizeof(uint64_t) * ctx->params.prime_count);
+  });
+}
+
+int fhe_context_prime_constants(const fhe_context *ctx, int i,
+                                int64_t *constants3, int64_t *root,
+                                int64_t *root_inverse) {
+  return guarded([&] {
+    require(ctx && constants3, "null argument");
+    const auto &p = ctx->params;
+    require(i >= 0 && i < p.prime_count, "prime index out of range");
+    constants3[0] = int64_t(p.k[i]);
+    constants3[1] = p.r2[i];
+    constants3[2] = p.ninv[i];
+    if (root)
+      std::memcpy(root, p.root.data() + size_t(i) * p.n,
+                  sizeof(int64_t) * (p.n - 1));
+    if (root_inverse)
+      std::memcpy(root_inverse, p.root_inverse.data() + size_t(i) * p.n,
+                  sizeof(int64_t) * (p.n - 1));
+  });
+}
+
+void *fhe_context_stream(fhe_context *ctx) {
+  return ctx ? static_cast<void *>(ctx->stream) : nullptr;
+}
+
+int fhe_synchronize(fhe_context *ctx) {
+  return guarded([&] {
+    require(ctx, "null argument");
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+  });
+}
+
+int fhe_device_alloc(fhe_context *ctx, size_t bytes, void **out) {
+  return guarded([&] {
+    require(ctx && out, "null argument");
+    *out = nullptr;
+    hip_check(hipMalloc(out, std::max<size_t>(bytes, 1)), "device allocation");
+  });
+}
+
+int fhe_device_free(fhe_context *ctx, void *ptr) {
+  return guarded([&] {
+    require(ctx, "null argument");
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    hip_check(hipFree(ptr), "device free");
+  });
+}
+
+int fhe_copy_to_device(fhe_context *ctx, void *dst, const void *src,
+                       size_t bytes) {
+  return guarded([&] {
+    require(ctx && dst && src, "null argument");
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                             ctx->stream),
+              "copy to device");
+    hip_check(hipStreamSynchronize(ctx->stream), "copy to device");
+  });
+}
+
+int fhe_copy_to_host(fhe_context *ctx, void *dst, const void *src,
+                     size_t bytes) {
+  return guarded([&] {
+    require(ctx && dst && src, "null argument");
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost,
+                             ctx->stream),
+              "copy to host");
+    hip_check(hipStreamSynchronize(ctx->stream), "copy to host");
+  });
+}
+
+int fhe_ntt(fhe_context *ctx, int64_t *data, int polys, int limbs,
+            int moduli_id, int enter) {
+  return guarded([&] {
+    require(ctx && data, "null argument");
+    require(polys > 0 && limbs > 0, "NTT: empty shape");
+    require(moduli_id >= 0 && moduli_id + limbs <= ctx->params.prime_count,
+            "NTT: limbs exceed the prime chain");
+    fhe::ntt_forward(ctx->tables, data, data, int64_t(polys) * limbs, limbs,
+                     moduli_id, enter ? ctx->tables.r2 : nullptr, ctx->stream);
+    ctx->check_launch("NTT");
+  });
+}
+
+int fhe_intt(fhe_context *ctx, int64_t *data, int polys, int limbs,
+             int moduli_id, int post) {
+  return guarded([&] {
+    require(ctx && data, "null argument");
+    require(polys > 0 && limbs > 0, "INTT: empty shape");
+    require(post >= 0 && post <= 2, "INTT: unknown epilogue");
+    require(moduli_id >= 0 && moduli_id + limbs <= ctx->params.prime_count,
+            "INTT: limbs exceed the prime chain");
+    fhe::ntt_inverse(ctx->tables, data, data, int64_t(polys) * limbs, limbs,
+                     moduli_id, post, ctx->stream);
+    ctx->check_launch("INTT");
+  });
+}
+
+int fhe_mont_mult(fhe_context *ctx, const int64_t *a, const int64_t *b,
+                  int64_t *out, int polys, int limbs, int moduli_id) {
+  return guarded([&] {
+    require(ctx && a && b && out, "null argument");
+    require(polys > 0 && limbs > 0, "mont_mult: empty shape");
+    require(moduli_id >= 0 && moduli_id + limbs <= ctx->params.prime_count,
+            "mont_mult: limbs exceed the prime chain");
+    fhe::launch_mont_mult(ctx->tables, a, b, out, int64_t(polys) * limbs,
+                          limbs, moduli_id, ctx->stream);
+    ctx->check_launch("mont_mult");
+  });
+}
+
+int fhe_rescale(fhe_context *ctx, const int64_t *in, int64_t *out, int polys,
+                int moduli_id) {
+  return guarded([&] {
+    require(ctx && in && out, "null argument");
+    require(polys > 0, "Rescale: empty shape");
+    ctx->check_moduli(moduli_id, 2, "Rescale");
+    fhe::launch_rescale(ctx->tables, in, out, polys,
+                        ctx->params.limbs_at(moduli_id), moduli_id,
+                        ctx->stream);
+    ctx->check_launch("Rescale");
+  });
+}
+
+int fhe_relinearize(fhe_context *ctx, int count, int moduli_id,
+                    const int64_t *trios, const int64_t *key,
+                    int key_moduli_id, int64_t *out) {
+  return guarded([&] {
+    require(ctx && trios && key && out, "null argument");
+    require(count > 0, "Relinearize: empty batch");
+    ctx->check_moduli(moduli_id, 1, "Relinearize");
+    ctx->check_key(moduli_id, key_moduli_id, "Relinearize");
+    const auto &P = ctx->params;
+    const int limbs = P.limbs_at(moduli_id);
+    const int ext_limbs = limbs + P.alpha;
+    const int64_t N = ctx->n();
+    const int64_t trio_l = int64_t(count) * 3 * limbs;
+    const int64_t ext_l = int64_t(count) * ext_limbs;
+    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
+    int64_t *ws = ctx->scratch(size_t(trio_l + ext_l + acc_l) * N * 8);
+    int64_t *trio = ws, *ext = trio + trio_l * N, *acc = ext + ext_l * N;
+    fhe::ntt_inverse(ctx->tables, trios, trio, trio_l, limbs, moduli_id,
+                     fhe::kInttExit, ctx->stream);
+    ctx->key_switch(count, moduli_id, key, key_moduli_id, trio, ext, acc);
+    fhe::launch_moddown_relin(ctx->tables, acc, trio, out, count, limbs,
+                              moduli_id, ctx->stream);
+    ctx->check_launch("Relinearize");
+  });
+}
+
+int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
+                            const int64_t *a, const int64_t *b,
+                            const int64_t *key, int key_moduli_id,
+                            int64_t *out) {
+  return guarded([&] {
+    require(ctx && a && b && key && out, "null argument");
+    require(count > 0, "Multiply: empty batch");
+    ctx->check_moduli(moduli_id, 3, "Multiply");
+    ctx->check_key(moduli_id, key_moduli_id, "Multiply");
+    const auto &P = ctx->params;
+    const int limbs = P.limbs_at(moduli_id);
+    const int ext_limbs = limbs + P.alpha;
+    const int64_t N = ctx->n();
+    const int64_t duo_l = int64_t(count) * 2 * limbs;
+    const int64_t trio_l = int64_t(count) * 3 * limbs;
+    const int64_t ext_l = int64_t(count) * ext_limbs;
+    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
+    int64_t *ws =
+        ctx->scratch(size_t(2 * duo_l + trio_l + ext_l + acc_l) * N * 8);
+    int64_t *an = ws, *bn = an + duo_l * N, *trio = bn + duo_l * N;
+    int64_t *ext = trio + trio_l * N, *acc = ext + ext_l * N;
+    // prepare_for_multiply: to NTT form (src/ckks/engine_multiply.cpp:13-27)
+    fhe::ntt_forward(ctx->tables, a, an, duo_l, limbs, moduli_id,
+                     ctx->tables.r2, ctx->stream);
+    fhe::ntt_forward(ctx->tables, b, bn, duo_l, limbs, moduli_id,
+                     ctx->tables.r2, ctx->stream);
+    fhe::launch_tensor(ctx->tables, an, bn, trio, count, limbs, moduli_id,
+                       ctx->stream);
+    // relinearize: to_intt of all three, key switch of the third
+    fhe::ntt_inverse(ctx->tables, trio, trio, trio_l, limbs, moduli_id,
+                     fhe::kInttExit, ctx->stream);
+    ctx->key_switch(count, moduli_id, key, key_moduli_id, trio, ext, acc);
+    fhe::launch_moddown_relin_rescale(ctx->tables, acc, trio, out, count,
+                                      limbs, moduli_id, ctx->stream);
+    ctx->check_launch("Multiply");
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,44 @@
+// Device-side tables shared by every kernel, and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "modarith.h"
+
+namespace fhe {
+
+// Per-prime tables indexed by ABSOLUTE prime index in the chain (limb l of an
+// object at moduli_id m uses prime m + l).  All pointers are device memory
+// owned by the context.
+struct DeviceTables {
+  const uint64_t *q;
+  const uint64_t *k;
+  const uint64_t *q2;
+  const uint64_t *qh;
+  const int64_t *r2;
+  const int64_t *ninv;
+  const int64_t *root;          // [prime][N]
+  const int64_t *root_inverse;  // [prime][N]
+  const int64_t *p_inverse;     // [alpha][prime_count]
+  const int64_t *rescale;       // [prime_count][prime_count]
+  int prime_count;
+  int log_n;
+  int alpha;
+};
+
+// Post-operation applied by the final inverse-NTT pass.
+enum InttPost : int {
+  kInttRaw = 0,       // plain INTT output, lazy in [0, 2q)
+  kInttExit = 1,      // * N^-1, leave Montgomery form (to_intt)
+  kInttExitReduce = 2 // as kInttExit, then reduce [0, 2q) -> [0, q)
+};
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t work, int per_block = kBlock) {
+  return unsigned((work + per_block - 1) / per_block);
+}
+
+} // namespace fhe

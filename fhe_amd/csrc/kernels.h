@@ -1,0 +1,69 @@
+// Host-side launchers for the HIP kernels.  All limb arrays are device int64
+// in [polynomial][limb][coefficient] order; a limb at flat index f of a batch
+// whose polynomials have `limbs` limbs uses prime moduli_id + f % limbs.
+#pragma once
+
+#include "device.h"
+
+namespace fhe {
+
+// In-place capable forward NTT of total_limbs limbs.  pre (device, indexed by
+// absolute prime) multiplies each input value in Montgomery form first, e.g.
+// R^2 for the reference's to_ntt (mont_enter then ntt); nullptr for none.
+void ntt_forward(const DeviceTables &t, const int64_t *src, int64_t *dst,
+                 int64_t total_limbs, int limbs, int moduli_id,
+                 const int64_t *pre, hipStream_t stream);
+
+// In-place capable inverse NTT with the InttPost epilogue.
+void ntt_inverse(const DeviceTables &t, const int64_t *src, int64_t *dst,
+                 int64_t total_limbs, int limbs, int moduli_id, int post,
+                 hipStream_t stream);
+
+// out = mont_mult(a, b) elementwise (src/montgomery/unit.cpp:5-10).
+void launch_mont_mult(const DeviceTables &t, const int64_t *a,
+                      const int64_t *b, int64_t *out, int64_t total_limbs,
+                      int limbs, int moduli_id, hipStream_t stream);
+
+// NTT-form tensor of `count` duo pairs -> trios (src/ckks/core.cpp:284-314).
+void launch_tensor(const DeviceTables &t, const int64_t *a, const int64_t *b,
+                   int64_t *trio, int count, int limbs, int moduli_id,
+                   hipStream_t stream);
+
+// Garner digits of one partition and exact basis extension to all
+// limbs + alpha primes (src/ckks/core_key_switch.cpp:21-108).  x: INTT-form
+// polynomial of each ciphertext at x + c * x_stride; ext: [count][ext][N].
+void launch_modup(const DeviceTables &t, const int64_t *x, int64_t x_stride,
+                  int64_t *ext, int64_t ext_stride, int count, int limbs,
+                  int moduli_id, int start, int size, const int64_t *constants,
+                  hipStream_t stream);
+
+// Key inner product for one partition: acc[c][h] (+)= ext[c] * key_h
+// (src/ckks/core_key_switch.cpp:318-336).  Each key limb is read once for
+// the whole batch.  key0/key1 point at the key polynomials' first limb
+// aligned with the ciphertext's moduli_id.
+void launch_key_inner_product(const DeviceTables &t, const int64_t *ext,
+                              int64_t ext_stride, const int64_t *key0,
+                              const int64_t *key1, int64_t *acc,
+                              int64_t acc_stride, int count, int ext_limbs,
+                              int moduli_id, bool first, hipStream_t stream);
+
+// ModDown of both key-switched halves, addition of the relinearized d0/d1 and
+// rescale, fused per coefficient (src/ckks/core_key_switch.cpp:110-160,
+// src/ckks/core.cpp:736-737, src/ckks/core_rescale.cpp:5-25).
+//   acc:  [count][2][limbs+alpha][N] INTT form, reduced to [0, q)
+//   trio: [count][3][limbs][N], polynomials 0/1 in INTT form
+//   out:  [count][2][limbs-1][N]
+void launch_moddown_relin_rescale(const DeviceTables &t, const int64_t *acc,
+                                  const int64_t *trio, int64_t *out, int count,
+                                  int limbs, int moduli_id, hipStream_t stream);
+
+// ModDown + add without rescale: out[c][h] = guard(d_h + ModDown(acc_h)).
+void launch_moddown_relin(const DeviceTables &t, const int64_t *acc,
+                          const int64_t *trio, int64_t *out, int count,
+                          int limbs, int moduli_id, hipStream_t stream);
+
+// Rescale of `polys` INTT-form polynomials (src/ckks/core_rescale.cpp:5-25).
+void launch_rescale(const DeviceTables &t, const int64_t *in, int64_t *out,
+                    int polys, int limbs, int moduli_id, hipStream_t stream);
+
+} // namespace fhe

@@ -1,0 +1,192 @@
+// Batched negacyclic NTT / INTT for gfx950.
+//
+// Butterflies are exactly the reference's radix-2 formulas
+// (src/operation/ntt.cpp:46-134 Cooley-Tukey, :177-272 Gentleman-Sande), so
+// every lazy representative in [0, 2q) matches.  The transform of one limb is
+// split into at most two passes over HBM: each pass owns a contiguous range of
+// index bits, loads a 4096-element tile of independent sub-transforms into
+// LDS and runs all of that range's radix-2 stages there.
+//
+//   forward  stage s flips index bit log_n-1-s; group of element i uses
+//            root[2^s - 1 + (i >> (log_n - s))]
+//   inverse  stage t flips bit t; group uses
+//            root_inverse[N - N/2^t + (i >> (t + 1))]
+//
+// A pass covering bits [b, b+L) sees sub-transform elements
+//   i = (hi << (b+L)) | (j << b) | lo,   j in [0, 2^L)
+// and in both directions the twiddle offset is i >> (b + u + 1) for local bit u.
+#include "kernels.h"
+
+namespace fhe {
+
+namespace {
+
+constexpr int kTile = 4096; // elements per workgroup (32 KiB of LDS)
+
+struct PassArgs {
+  const int64_t *src;
+  int64_t *dst;
+  int64_t total_subs; // sub-transforms over the whole batch
+  int limbs;          // limbs per polynomial: limb f uses prime mid + f % limbs
+  int moduli_id;
+  int low_bit;        // b
+  int bits;           // L <= 12
+  const int64_t *pre; // forward first pass: multiply by pre[prime] on load
+  int post;           // inverse last pass: InttPost
+};
+
+template <bool INV>
+__global__ __launch_bounds__(kBlock) void ntt_pass_kernel(PassArgs a,
+                                                          DeviceTables t) {
+  __shared__ int64_t lds[kTile];
+  const int log_n = t.log_n;
+  const int64_t n = int64_t(1) << log_n;
+  const int L = a.bits, b = a.low_bit;
+  const int sub = 1 << L;
+  const int C = kTile >> L;              // sub-transforms per tile
+  const int sub_log = log_n - L;         // log2(sub-transforms per limb)
+  const int64_t tile0 = int64_t(blockIdx.x) * C;
+  const bool columns = b > 0;            // strided sub-transforms
+  const int tid = threadIdx.x;
+
+  // sub-transform sp, element j -> limb-local index i, flat limb f
+  auto locate = [&](int64_t sp, int j, int64_t &f) -> int64_t {
+    f = sp >> sub_log;
+    const int64_t r = sp & ((int64_t(1) << sub_log) - 1);
+    const int64_t lo = r & ((int64_t(1) << b) - 1);
+    const int64_t hi = r >> b;
+    return (hi << (b + L)) | (int64_t(j) << b) | lo;
+  };
+
+  for (int e = tid; e < kTile; e += kBlock) {
+    const int c = columns ? e % C : e / sub;
+    const int j = columns ? e / C : e % sub;
+    const int64_t sp = tile0 + c;
+    int64_t v = 0;
+    if (sp < a.total_subs) {
+      int64_t f;
+      const int64_t i = locate(sp, j, f);
+      v = a.src[f * n + i];
+      if (a.pre) {
+        const int p = a.moduli_id + int(f % a.limbs);
+        v = mont_mult_u(v, a.pre[p], t.q[p], t.k[p]);
+      }
+    }
+    lds[j * C + c] = v;
+  }
+  __syncthreads();
+
+  for (int step = 0; step < L; step++) {
+    const int u = INV ? step : L - 1 - step;
+    for (int bf = tid; bf < kTile / 2; bf += kBlock) {
+      const int c = bf % C;
+      const int kk = bf / C;
+      const int64_t sp = tile0 + c;
+      if (sp >= a.total_subs)
+        continue;
+      const int jl = ((kk >> u) << (u + 1)) | (kk & ((1 << u) - 1));
+      const int jh = jl | (1 << u);
+      int64_t f;
+      const int64_t i = locate(sp, jl, f);
+      const int p = a.moduli_id + int(f % a.limbs);
+      const int64_t q = int64_t(t.q[p]);
+      const int64_t q2 = int64_t(t.q2[p]);
+      const uint64_t k = t.k[p];
+      const int64_t x = lds[jl * C + c];
+      const int64_t y = lds[jh * C + c];
+      const int64_t off = i >> (b + u + 1);
+      if (!INV) {
+        const int s = log_n - 1 - (b + u);
+        const int64_t w = t.root[int64_t(p) * n + (int64_t(1) << s) - 1 + off];
+        const int64_t prod = mont_mult_u(w, y, q, k);
+        lds[jl * C + c] = guard(x + prod, q2);
+        lds[jh * C + c] = guard(x + q2 - prod, q2);
+      } else {
+        const int tt = b + u;
+        const int64_t w =
+            t.root_inverse[int64_t(p) * n + n - (n >> tt) + off];
+        lds[jl * C + c] = guard(x + y, q2);
+        lds[jh * C + c] = mont_mult_u(w, guard(x + q2 - y, q2), q, k);
+      }
+    }
+    __syncthreads();
+  }
+
+  for (int e = tid; e < kTile; e += kBlock) {
+    const int c = columns ? e % C : e / sub;
+    const int j = columns ? e / C : e % sub;
+    const int64_t sp = tile0 + c;
+    if (sp >= a.total_subs)
+      continue;
+    int64_t f;
+    const int64_t i = locate(sp, j, f);
+    int64_t v = lds[j * C + c];
+    if (INV && a.post != kInttRaw) {
+      const int p = a.moduli_id + int(f % a.limbs);
+      const uint64_t q = t.q[p], k = t.k[p];
+      v = mont_exit(mont_mult_u(v, t.ninv[p], q, k), q, k);
+      if (a.post == kInttExitReduce)
+        v = guard(v, int64_t(q));
+    }
+    a.dst[f * n + i] = v;
+  }
+}
+
+// Split log_n into the bit ranges of at most two passes (each <= 12 bits).
+void split_bits(int log_n, int &first, int &second) {
+  if (log_n <= 12) {
+    first = log_n;
+    second = 0;
+  } else {
+    first = log_n / 2;
+    second = log_n - first;
+  }
+}
+
+void launch_pass(bool inverse, const PassArgs &a, const DeviceTables &t,
+                 hipStream_t stream) {
+  const int C = kTile >> a.bits;
+  const unsigned blocks = unsigned((a.total_subs + C - 1) / C);
+  if (inverse)
+    hipLaunchKernelGGL(ntt_pass_kernel<true>, dim3(blocks), dim3(kBlock), 0,
+                       stream, a, t);
+  else
+    hipLaunchKernelGGL(ntt_pass_kernel<false>, dim3(blocks), dim3(kBlock), 0,
+                       stream, a, t);
+}
+
+} // namespace
+
+void ntt_forward(const DeviceTables &t, const int64_t *src, int64_t *dst,
+                 int64_t total_limbs, int limbs, int moduli_id,
+                 const int64_t *pre, hipStream_t stream) {
+  int l1, l2;
+  split_bits(t.log_n, l1, l2);
+  // first pass: the high bits (stages 0 .. l1-1)
+  PassArgs a{src, dst, total_limbs << (t.log_n - l1), limbs, moduli_id,
+             t.log_n - l1, l1, pre, kInttRaw};
+  launch_pass(false, a, t, stream);
+  if (l2 > 0) {
+    PassArgs b2{dst, dst, total_limbs << (t.log_n - l2), limbs, moduli_id,
+                0, l2, nullptr, kInttRaw};
+    launch_pass(false, b2, t, stream);
+  }
+}
+
+void ntt_inverse(const DeviceTables &t, const int64_t *src, int64_t *dst,
+                 int64_t total_limbs, int limbs, int moduli_id, int post,
+                 hipStream_t stream) {
+  int l1, l2;
+  split_bits(t.log_n, l1, l2);
+  // first pass: the low bits (stages with span 1 .. 2^(l1-1))
+  PassArgs a{src, dst, total_limbs << (t.log_n - l1), limbs, moduli_id,
+             0, l1, nullptr, l2 > 0 ? int(kInttRaw) : post};
+  launch_pass(true, a, t, stream);
+  if (l2 > 0) {
+    PassArgs b2{dst, dst, total_limbs << (t.log_n - l2), limbs, moduli_id,
+                l1, l2, nullptr, post};
+    launch_pass(true, b2, t, stream);
+  }
+}
+
+} // namespace fhe

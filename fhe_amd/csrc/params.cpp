@@ -1,0 +1,380 @@
+// Host-side CKKS parameter set (see params.h for the reference mapping).
+#include "params.h"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+#include "modarith.h"
+
+namespace fhe {
+
+using i128 = __int128;
+using u128 = unsigned __int128;
+
+int64_t mod_mul(int64_t a, int64_t b, int64_t m) {
+  i128 p = i128(a) * i128(b);
+  p %= i128(m);
+  return int64_t(p);
+}
+
+int64_t mod_pow(int64_t base, int64_t e, int64_t m) {
+  int64_t r = 1 % m;
+  base %= m;
+  for (; e > 0; e >>= 1) {
+    if (e & 1)
+      r = mod_mul(r, base, m);
+    base = mod_mul(base, base, m);
+  }
+  return r;
+}
+
+// Extended Euclid on int64 with the reference's conventions
+// (src/arithmetic/arithmetic.cpp:10-31): a result in [0, m), 0 when x has no
+// inverse modulo m.
+int64_t mod_inverse(int64_t x, int64_t m) {
+  int64_t r0 = x, r1 = m, s0 = 1, s1 = 0;
+  while (r1 > 0) {
+    const int64_t quot = r0 / r1;
+    int64_t t = r0 - quot * r1;
+    r0 = r1;
+    r1 = t;
+    t = s0 - quot * s1;
+    s0 = s1;
+    s1 = t;
+  }
+  if (r0 != 1)
+    return 0;
+  return s0 >= 0 ? s0 : s0 + m;
+}
+
+static uint64_t pow_u64(uint64_t b, uint64_t e, uint64_t m) {
+  u128 r = 1, x = b % m;
+  for (; e; e >>= 1) {
+    if (e & 1)
+      r = (r * x) % m;
+    x = (x * x) % m;
+  }
+  return uint64_t(r);
+}
+
+// Deterministic Miller-Rabin for 64-bit integers (the first 12 prime bases
+// are a proven witness set below 3.3e24).
+bool is_prime_u64(uint64_t n) {
+  if (n < 2)
+    return false;
+  static const uint64_t bases[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+  for (uint64_t p : bases) {
+    if (n % p == 0)
+      return n == p;
+  }
+  uint64_t d = n - 1;
+  int s = 0;
+  while ((d & 1) == 0) {
+    d >>= 1;
+    s++;
+  }
+  for (uint64_t a : bases) {
+    uint64_t x = pow_u64(a, d, n);
+    if (x == 1 || x == n - 1)
+      continue;
+    bool composite = true;
+    for (int i = 1; i < s; i++) {
+      x = uint64_t((u128(x) * x) % n);
+      if (x == n - 1) {
+        composite = false;
+        break;
+      }
+    }
+    if (composite)
+      return false;
+  }
+  return true;
+}
+
+// Same selection rule as src/arithmetic/arithmetic.cpp:98-113: the first base
+// b = 2, 3, ... whose (q-1)/2n-th power has order exactly 2n.
+int64_t find_ntt_root(int64_t q, int n) {
+  const int64_t unit = (q - 1) / (2 * int64_t(n));
+  int64_t e = 0;
+  for (int64_t b = 2; b < q; b++) {
+    e = mod_pow(b, unit, q);
+    if (mod_pow(e, n, q) != 1)
+      break;
+  }
+  return e;
+}
+
+// k with R * R^-1 - q * k = 1 (src/arithmetic/arithmetic.cpp:159-167).
+uint64_t mont_k(uint64_t q) {
+  const int64_t rinv = mod_inverse(kMontR, int64_t(q));
+  i128 t = i128(kMontR) * i128(rinv) - 1;
+  t /= i128(q);
+  return uint64_t(t);
+}
+
+static bool is_ntt_prime(uint64_t x, int n) {
+  return (x - 1) % (2 * uint64_t(n)) == 0 && is_prime_u64(x);
+}
+
+static uint64_t next_ntt_prime(uint64_t start, int n, bool up) {
+  uint64_t x = start;
+  while (!is_ntt_prime(x, n))
+    x = up ? x + 2 : x - 2;
+  return x;
+}
+
+// src/arithmetic/prime.cpp:913-930: walk down from 2^bits in steps of 2n.
+std::vector<uint64_t> decreasing_ntt_primes(int bits, int n, int count) {
+  std::vector<uint64_t> out;
+  const uint64_t step = 2 * uint64_t(n);
+  uint64_t x = ((uint64_t(1) << bits) / step) * step + 1;
+  while (int(out.size()) < count) {
+    if (is_ntt_prime(x, n))
+      out.push_back(x);
+    x -= step;
+  }
+  return out;
+}
+
+// Primes alternating above and below 2^bits so the accumulated scale drift
+// stays near one (src/arithmetic/prime.cpp:765-843, the multi-precision
+// search the reference's cached tables come from).  The drift is tracked in
+// binary128 with each prime first rounded to double, which reproduces every
+// prime of the reference's cached chains for N = 2^3 .. 2^16.
+std::vector<uint64_t> alternating_ntt_primes(int bits, int n, int count) {
+  std::vector<uint64_t> out;
+  const uint64_t scale = uint64_t(1) << bits;
+  uint64_t hi_start = scale + 1, lo_start = scale - 1;
+  const uint64_t first_hi = next_ntt_prime(hi_start, n, true);
+  const uint64_t first_lo = next_ntt_prime(lo_start, n, false);
+  bool up = first_hi - scale >= scale - first_lo;
+  __float128 drift = 1;
+  const __float128 fscale = __float128(double(scale));
+  while (int(out.size()) < count) {
+    const uint64_t p = next_ntt_prime(up ? hi_start : lo_start, n, up);
+    drift = drift * (fscale / __float128(double(p)));
+    drift = drift * drift;
+    const uint64_t search = uint64_t(int64_t((drift * fscale) / 2));
+    if (up) {
+      hi_start = p + 2;
+      const uint64_t searched = search * 2 - 1;
+      if (searched < lo_start)
+        lo_start = searched;
+    } else {
+      lo_start = p - 2;
+      const uint64_t searched = search * 2 + 1;
+      if (searched > hi_start)
+        hi_start = searched;
+    }
+    up = !up;
+    out.push_back(p);
+  }
+  return out;
+}
+
+// Largest total modulus bit count per ring degree (src/ckks/config.cpp:49-84).
+static int max_q_bits(int log_n) {
+  switch (log_n) {
+  case 3:
+  case 4:
+    return 1000;
+  case 13:
+    return 214;
+  case 14:
+    return 430;
+  case 15:
+    return 868;
+  case 16:
+    return 1747;
+  case 17:
+    return 3523;
+  default:
+    throw std::invalid_argument("CKKS Parameters: unsupported config.");
+  }
+}
+
+CkksParams CkksParams::from_config(const CkksConfig &c) {
+  if (c.special_prime_count < 1 || c.special_prime_count > kMaxAlpha)
+    throw std::invalid_argument("The given special prime count is invalid.");
+  const int n = 1 << c.log_n;
+  const int small_count =
+      (max_q_bits(c.log_n) - 60 * (1 + c.special_prime_count)) /
+      c.small_prime_bits;
+  if (small_count < 2)
+    throw std::invalid_argument("CKKS Parameters: unsupported config.");
+  // src/ckks/parameters.cpp:9-51: small primes, then base + specials, with a
+  // copy of large_primes[1] prepended for the one-more-level chain.
+  std::vector<uint64_t> primes =
+      alternating_ntt_primes(c.small_prime_bits, n, small_count);
+  const auto large =
+      decreasing_ntt_primes(c.large_prime_bits, n, c.special_prime_count + 1);
+  primes.insert(primes.end(), large.begin(), large.end());
+  if (c.one_more_level)
+    primes.insert(primes.begin(), large[1]);
+  CkksParams p = from_primes(c.log_n, primes, c.special_prime_count,
+                             c.scale_bits, c.one_more_level);
+  if (c.max_level >= 0)
+    p.max_level = c.max_level;
+  return p;
+}
+
+CkksParams CkksParams::from_primes(int log_n, const std::vector<uint64_t> &primes,
+                                   int special_prime_count, int scale_bits,
+                                   bool one_more_level) {
+  CkksParams p;
+  p.log_n = log_n;
+  p.n = 1 << log_n;
+  p.q = primes;
+  p.prime_count = int(primes.size());
+  p.alpha = special_prime_count;
+  p.chain_count = p.prime_count - special_prime_count;
+  if (p.chain_count < 2 || special_prime_count < 1 ||
+      special_prime_count > kMaxAlpha)
+    throw std::invalid_argument("CKKS Parameters: unsupported config.");
+  for (auto q : primes)
+    if (q >= (uint64_t(1) << 61) || (q - 1) % (2 * uint64_t(p.n)) != 0)
+      throw std::invalid_argument("CKKS Parameters: unsupported prime.");
+  p.full_level = p.chain_count - 2;
+  p.max_level = p.full_level;
+  p.scale_bits = scale_bits;
+  p.one_more_level = one_more_level;
+  p.derive();
+  return p;
+}
+
+int CkksParams::max_key_switching_level() const {
+  if (one_more_level && max_level == full_level)
+    return full_level - 1;
+  return max_level;
+}
+
+void CkksParams::derive() {
+  const int P = prime_count;
+  k.resize(P);
+  q2.resize(P);
+  qh.resize(P);
+  r2.resize(P);
+  ninv.resize(P);
+  root.assign(size_t(P) * n, 0);
+  root_inverse.assign(size_t(P) * n, 0);
+  std::vector<int64_t> power(n), shuffled(n);
+  auto bitrev = [&](int x) {
+    int y = 0;
+    for (int b = 0; b < log_n; b++)
+      y |= ((x >> b) & 1) << (log_n - 1 - b);
+    return y;
+  };
+  for (int i = 0; i < P; i++) {
+    const int64_t qi = int64_t(q[i]);
+    q2[i] = q[i] << 1;
+    qh[i] = q[i] >> 1;
+    k[i] = mont_k(q[i]);
+    r2[i] = mod_mul(kMontR, kMontR, qi);
+    ninv[i] = mod_mul(mod_inverse(n, qi), kMontR, qi);
+    const int64_t psi = find_ntt_root(qi, n);
+    for (int pass = 0; pass < 2; pass++) {
+      const int64_t base = pass == 0 ? psi : mod_inverse(psi, qi);
+      power[0] = 1;
+      for (int j = 1; j < n; j++)
+        power[j] = mod_mul(power[j - 1], base, qi);
+      for (int j = 0; j < n; j++)
+        shuffled[bitrev(j)] = power[j];
+      int64_t *dst = (pass == 0 ? root.data() : root_inverse.data()) +
+                     size_t(i) * n;
+      if (pass == 0) {
+        // forward: bit-reversed powers 1..N-1 (CT stage s, group g uses
+        // entry 2^s - 1 + g)
+        for (int j = 1; j < n; j++)
+          dst[j - 1] = shuffled[j];
+      } else {
+        // inverse: stage t reads the N/2^(t+1) entries starting at
+        // N - N/2^t (src/ckks/precomputed.cpp:15-31)
+        int o = 0;
+        for (int lg = log_n - 1; lg >= 0; lg--)
+          for (int g = 0; g < (1 << lg); g++)
+            dst[o++] = shuffled[(1 << lg) + g];
+      }
+      for (int j = 0; j < n - 1; j++)
+        dst[j] = mont_mult_u(dst[j], r2[i], q[i], k[i]);
+    }
+  }
+  // src/ckks/precomputed.cpp:97-137
+  p_product_mult_r.assign(chain_count, 0);
+  for (int i = 0; i < chain_count; i++) {
+    int64_t prod = kMontR;
+    for (int s = chain_count; s < P; s++)
+      prod = mod_mul(prod, int64_t(q[s]), int64_t(q[i]));
+    p_product_mult_r[i] = prod;
+  }
+  p_inverse_mult_r.assign(size_t(alpha) * P, 0);
+  for (int sp = 0; sp < alpha; sp++) {
+    const int pi = P - 1 - sp;
+    for (int i = 0; i < pi; i++)
+      p_inverse_mult_r[size_t(sp) * P + i] = mod_mul(
+          mod_inverse(int64_t(q[pi]), int64_t(q[i])), kMontR, int64_t(q[i]));
+  }
+  rescale_scale.assign(size_t(P) * P, 0);
+  for (int from = 0; from < P; from++)
+    for (int to = from + 1; to < P; to++)
+      rescale_scale[size_t(from) * P + to] =
+          mod_mul(mod_inverse(int64_t(q[from]), int64_t(q[to])), kMontR,
+                  int64_t(q[to]));
+}
+
+std::vector<Partition> CkksParams::partitions(int moduli_id,
+                                              int key_moduli_id) const {
+  const int chain = limbs_at(moduli_id);
+  const int count = partition_count(moduli_id);
+  int key_index = partition_count(key_moduli_id) - count;
+  std::vector<Partition> out;
+  for (int start = chain - count * alpha; start < chain;
+       start += alpha, key_index++) {
+    const int lo = std::max(0, start);
+    const int hi = std::min(chain, start + alpha);
+    out.push_back({lo, hi - lo, key_index});
+  }
+  return out;
+}
+
+// (prod_{t<j} q[moduli_id + start + t]) mod m
+static int64_t prefix_product_mod(const CkksParams &p, int moduli_id, int start,
+                                  int j, int64_t m) {
+  int64_t r = 1 % m;
+  for (int t = 0; t < j; t++)
+    r = mod_mul(r, int64_t(p.q[moduli_id + start + t] % uint64_t(m)), m);
+  return r;
+}
+
+std::vector<int64_t> CkksParams::modup_constants(int moduli_id,
+                                                 int key_moduli_id) const {
+  const auto parts = partitions(moduli_id, key_moduli_id);
+  const int ext = limbs_at(moduli_id) + alpha;
+  const int stride = ks_stride(moduli_id);
+  std::vector<int64_t> c(size_t(stride) * parts.size(), 0);
+  for (size_t pi = 0; pi < parts.size(); pi++) {
+    const auto &part = parts[pi];
+    int64_t *blk = c.data() + pi * stride;
+    for (int j = 1; j < part.size; j++) {
+      const int64_t m = int64_t(q[moduli_id + part.start + j]);
+      const int64_t prod = prefix_product_mod(*this, moduli_id, part.start, j, m);
+      blk[kKsInvOffset + j] = mod_mul(mod_inverse(prod, m), kMontR, m);
+      for (int t = j + 1; t < part.size; t++) {
+        const int64_t mt = int64_t(q[moduli_id + part.start + t]);
+        blk[kKsTileOffset + j * kMaxAlpha + t] = mod_mul(
+            prefix_product_mod(*this, moduli_id, part.start, j, mt), kMontR, mt);
+      }
+    }
+    for (int j = 0; j < part.size; j++)
+      for (int i = 0; i < ext; i++) {
+        const int64_t m = int64_t(q[moduli_id + i]);
+        blk[kKsExtOffset + j * ext + i] =
+            j == 0 ? r2[moduli_id + i]
+                   : mod_mul(prefix_product_mod(*this, moduli_id, part.start, j, m),
+                             r2[moduli_id + i], m);
+      }
+  }
+  return c;
+}
+
+} // namespace fhe

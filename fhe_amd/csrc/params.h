@@ -1,0 +1,103 @@
+// Host-side CKKS parameter set: prime chain, per-prime Montgomery constants,
+// NTT twiddles, level bookkeeping and key-switching partitions.
+//
+// The chain layout and every constant follow the reference's definitions so
+// that device results match its limbs bit for bit:
+//   prime chain       src/ckks/parameters.cpp:9-105, src/ckks/config.cpp:49-84,
+//                     src/arithmetic/prime.cpp:840-1015
+//   per-prime         src/ckks/derived_parameters.cpp:19-50
+//   chain constants   src/ckks/precomputed.cpp:33-137
+//   partitions        src/ckks/parameters.cpp:325-360,
+//                     src/ckks/precomputed.cpp:325-382
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fhe {
+
+// Number theory (host, exact with 128-bit intermediates).
+int64_t mod_mul(int64_t a, int64_t b, int64_t m);
+int64_t mod_pow(int64_t base, int64_t e, int64_t m);
+int64_t mod_inverse(int64_t x, int64_t m);  // 0 when not invertible
+bool is_prime_u64(uint64_t n);
+int64_t find_ntt_root(int64_t q, int n);    // primitive 2n-th root
+uint64_t mont_k(uint64_t q);                // -q^-1 mod 2^62
+
+// Prime searches that reproduce the reference's chains.
+std::vector<uint64_t> decreasing_ntt_primes(int bits, int n, int count);
+std::vector<uint64_t> alternating_ntt_primes(int bits, int n, int count);
+
+struct CkksConfig {
+  int log_n = 14;
+  int special_prime_count = 1;
+  int scale_bits = 40;       // 40 (default preset) or 54 (bootstrap preset)
+  int small_prime_bits = 40; // 40 or 54
+  int large_prime_bits = 60;
+  bool one_more_level = true;
+  int max_level = -1;
+};
+
+// One key-switching partition (digit) of a ciphertext at some moduli_id.
+struct Partition {
+  int start;      // first limb, relative to the ciphertext's moduli_id
+  int size;       // limbs in the digit (<= special prime count)
+  int key_index;  // partition index inside the key
+};
+
+class CkksParams {
+public:
+  static CkksParams from_config(const CkksConfig &config);
+  static CkksParams from_primes(int log_n, const std::vector<uint64_t> &primes,
+                                int special_prime_count, int scale_bits,
+                                bool one_more_level);
+
+  int log_n = 0, n = 0;
+  int prime_count = 0;   // Q chain + specials
+  int chain_count = 0;   // Q chain only
+  int alpha = 0;         // special prime count
+  int full_level = 0;
+  int max_level = 0;
+  int scale_bits = 0;
+  bool one_more_level = false;
+
+  std::vector<uint64_t> q, k, q2, qh;
+  std::vector<int64_t> r2, ninv;
+  // Montgomery-form twiddles, N-1 used entries per prime, padded to N:
+  // root[p*N + i] = psi^bitrev(i+1) * R, root_inverse in backward-paint order.
+  std::vector<int64_t> root, root_inverse;
+  std::vector<int64_t> p_product_mult_r;   // [chain_count]
+  std::vector<int64_t> p_inverse_mult_r;   // [alpha][prime_count]
+  std::vector<int64_t> rescale_scale;      // [prime_count][prime_count]
+
+  int limbs_at(int moduli_id) const { return chain_count - moduli_id; }
+  int moduli_id_of_level(int level) const { return full_level - level; }
+  int level_of(int moduli_id, bool rescaled) const {
+    return rescaled ? full_level - moduli_id + 1 : full_level - moduli_id;
+  }
+  int max_key_switching_level() const;
+  int partition_count(int moduli_id) const {
+    return (limbs_at(moduli_id) + alpha - 1) / alpha;
+  }
+  std::vector<Partition> partitions(int moduli_id, int key_moduli_id) const;
+
+  // Device constants for ModUp of a ciphertext at moduli_id: for each
+  // partition p, a block of kKsStride int64 entries
+  //   [0, 16)            Garner inverses (prod q_<j)^-1 * R mod q_j, j>=1
+  //   [16, 16+256)       Garner tile constants (prod q_<=j) * R mod q_t
+  //   [272, 272+16*ext)  basis-extension constants (prod q_<j) * R^2 mod q_i
+  static constexpr int kMaxAlpha = 16;
+  static constexpr int kKsInvOffset = 0;
+  static constexpr int kKsTileOffset = 16;
+  static constexpr int kKsExtOffset = 16 + 16 * 16;
+  int ks_stride(int moduli_id) const {
+    return kKsExtOffset + kMaxAlpha * (limbs_at(moduli_id) + alpha);
+  }
+  std::vector<int64_t> modup_constants(int moduli_id, int key_moduli_id) const;
+
+private:
+  void derive();
+};
+
+} // namespace fhe

@@ -334,6 +334,19 @@ void oracle_to_intt(const oracle_ctx *c, int mid, int limbs, int64_t *x) {
   }
 }
 
+/* SingleProcessor::mont_mult, src/processor/single.cpp:60-74 */
+void oracle_mont_mult_limbs(const oracle_ctx *c, int mid, int limbs,
+                            const int64_t *a, const int64_t *b, int64_t *out) {
+  const int n = c->n;
+  for (int l = 0; l < limbs; l++) {
+    const int pi = mid + l;
+    for (int j = 0; j < n; j++) {
+      const size_t o = (size_t)l * n + j;
+      out[o] = MONT(a[o], b[o], c->q[pi], c->k[pi]);
+    }
+  }
+}
+
 /* src/ckks/core.cpp:284-314 */
 void oracle_tensor(const oracle_ctx *c, int mid, int limbs, const int64_t *a,
                    const int64_t *b, int64_t *trio) {
@@ -572,7 +585,7 @@ void oracle_rescale(const oracle_ctx *c, int mid, int polys, const int64_t *x,
 }
 
 /* src/ckks/engine_multiply.cpp:149-162 on rescaled INTT inputs, followed by
- * src/ckks/engine_rescale.cpp:38-49 */
+ * src/ckks/engine_rescale.cpp:50-60 */
 void oracle_hmult_relin_rescale(const oracle_ctx *c, int mid, const int64_t *a,
                                 const int64_t *b, const int64_t *key,
                                 int key_mid, int64_t *out) {

@@ -55,6 +55,10 @@ void oracle_to_ntt(const oracle_ctx *ctx, int moduli_id, int limbs,
 void oracle_to_intt(const oracle_ctx *ctx, int moduli_id, int limbs,
                     int64_t *x);
 
+/* elementwise Montgomery product of `limbs` limbs starting at moduli_id */
+void oracle_mont_mult_limbs(const oracle_ctx *ctx, int moduli_id, int limbs,
+                            const int64_t *a, const int64_t *b, int64_t *out);
+
 /* NTT-form tensor of two duos -> trio */
 void oracle_tensor(const oracle_ctx *ctx, int moduli_id, int limbs,
                    const int64_t *a, const int64_t *b, int64_t *trio);

@@ -299,7 +299,7 @@ void *ref_rescale(void *handle, void *ct) {
 
 // One HMult+relin+rescale unit of work through the reference engine:
 // multiply (src/ckks/engine_multiply.cpp:149-162) then rescale
-// (src/ckks/engine_rescale.cpp:38-49).
+// (src/ckks/engine_rescale.cpp:50-60).
 void *ref_hmult_relin_rescale(void *handle, void *a, void *b, void *rlk) {
   return guarded([&]() -> void * {
     auto ref = static_cast<RefEngine *>(handle);

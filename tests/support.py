@@ -1,0 +1,282 @@
+"""Test-only loaders for the checkers under oracle/.
+
+  Oracle     ctypes binding of oracle/liboracle.so, the repo's C restatement
+             of the reference's hot-path arithmetic.
+  Reference  ctypes binding of oracle/_ref/libdesilo_ref.so, the reference
+             engine compiled from its own sources (present only where it was
+             built; tests that need it skip otherwise).
+
+Only tests/, the smoke check and bench.py's cpu_baseline leg use these.
+"""
+import ctypes
+import json
+import os
+import sys
+
+import numpy as np
+
+TESTS = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(TESTS)
+GOLDEN = os.path.join(TESTS, "golden")
+ORACLE_LIB = os.path.join(REPO, "oracle", "liboracle.so")
+REF_LIB = os.path.join(REPO, "oracle", "_ref", "libdesilo_ref.so")
+
+sys.path.insert(0, GOLDEN)
+from seeded_inputs import hmult_inputs, uniform_limbs  # noqa: E402,F401
+
+VP = ctypes.c_void_p
+
+
+def ptr(a):
+    return a.ctypes.data_as(VP)
+
+
+def golden(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+class Oracle:
+    """One oracle context over a prime chain."""
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            if not os.path.exists(ORACLE_LIB):
+                raise RuntimeError("oracle/liboracle.so is not built "
+                                   "(make -C oracle oracle)")
+            lib = ctypes.CDLL(ORACLE_LIB)
+            lib.oracle_ctx_create.restype = VP
+            lib.oracle_ctx_create.argtypes = [ctypes.c_int, VP, ctypes.c_int,
+                                              ctypes.c_int]
+            lib.oracle_ctx_destroy.argtypes = [VP]
+            i64, u64 = ctypes.c_int64, ctypes.c_uint64
+            lib.oracle_mont_mult.restype = i64
+            lib.oracle_mont_mult.argtypes = [i64, i64, u64, u64]
+            lib.oracle_mont_exit.restype = i64
+            lib.oracle_mont_exit.argtypes = [i64, u64, u64]
+            lib.oracle_calculate_k.restype = u64
+            lib.oracle_calculate_k.argtypes = [u64]
+            lib.oracle_ntt_raw.argtypes = [VP, VP, u64, u64, ctypes.c_int]
+            lib.oracle_intt_raw.argtypes = [VP, VP, u64, u64, ctypes.c_int]
+            for name in ["oracle_ntt_limb", "oracle_intt_limb"]:
+                getattr(lib, name).argtypes = [VP, ctypes.c_int, VP]
+            lib.oracle_prime_constants.argtypes = [VP, ctypes.c_int, VP, VP,
+                                                   VP]
+            for name in ["oracle_to_ntt", "oracle_to_intt"]:
+                getattr(lib, name).argtypes = [VP, ctypes.c_int, ctypes.c_int,
+                                               VP]
+            lib.oracle_mont_mult_limbs.argtypes = [VP, ctypes.c_int,
+                                                   ctypes.c_int, VP, VP, VP]
+            lib.oracle_tensor.argtypes = [VP, ctypes.c_int, ctypes.c_int, VP,
+                                          VP, VP]
+            lib.oracle_key_switch.argtypes = [VP, ctypes.c_int, VP, VP,
+                                              ctypes.c_int, VP]
+            lib.oracle_relinearize.argtypes = [VP, ctypes.c_int, VP, VP,
+                                               ctypes.c_int, VP]
+            lib.oracle_rescale.argtypes = [VP, ctypes.c_int, ctypes.c_int, VP,
+                                           VP]
+            lib.oracle_hmult_relin_rescale.argtypes = [VP, ctypes.c_int, VP,
+                                                       VP, VP, ctypes.c_int,
+                                                       VP]
+            lib.oracle_partitions.argtypes = [VP, ctypes.c_int, ctypes.c_int,
+                                              VP, VP, VP]
+            cls._lib = lib
+        return cls._lib
+
+    def __init__(self, log_n, primes, alpha):
+        self.log_n = log_n
+        self.n = 1 << log_n
+        self.primes = np.array([int(q) for q in primes], dtype=np.uint64)
+        self.alpha = alpha
+        self.chain_count = len(primes) - alpha
+        self.h = self.lib().oracle_ctx_create(log_n, ptr(self.primes),
+                                              len(primes), alpha)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib().oracle_ctx_destroy(self.h)
+            self.h = None
+
+    def limbs_at(self, mid):
+        return self.chain_count - mid
+
+    def constants(self, i):
+        c = np.zeros(3, np.int64)
+        r = np.zeros(self.n - 1, np.int64)
+        ri = np.zeros(self.n - 1, np.int64)
+        self.lib().oracle_prime_constants(self.h, i, ptr(c), ptr(r), ptr(ri))
+        return c, r, ri
+
+    def ntt(self, x, mid, enter=True):
+        """[polys][limbs][N] -> NTT form (copy)."""
+        y = np.array(x, dtype=np.int64, copy=True)
+        polys, limbs = _pl(y)
+        flat = y.reshape(polys, limbs, self.n)
+        for p in range(polys):
+            if enter:
+                self.lib().oracle_to_ntt(self.h, mid, limbs, ptr(flat[p]))
+            else:
+                for l in range(limbs):
+                    self.lib().oracle_ntt_limb(self.h, mid + l, ptr(flat[p, l]))
+        return y
+
+    def intt(self, x, mid, post=1):
+        y = np.array(x, dtype=np.int64, copy=True)
+        polys, limbs = _pl(y)
+        flat = y.reshape(polys, limbs, self.n)
+        for p in range(polys):
+            if post == 0:
+                for l in range(limbs):
+                    self.lib().oracle_intt_limb(self.h, mid + l, ptr(flat[p, l]))
+            else:
+                self.lib().oracle_to_intt(self.h, mid, limbs, ptr(flat[p]))
+                if post == 2:
+                    q = self.primes[mid:mid + limbs].astype(np.int64)[:, None]
+                    flat[p] = np.where(flat[p] < q, flat[p], flat[p] - q)
+        return y
+
+    def mont_mult(self, a, b, mid):
+        a = np.ascontiguousarray(a, dtype=np.int64)
+        b = np.ascontiguousarray(b, dtype=np.int64)
+        polys, limbs = _pl(a)
+        out = np.empty_like(a)
+        fa, fb, fo = (v.reshape(polys, limbs, self.n) for v in (a, b, out))
+        for p in range(polys):
+            self.lib().oracle_mont_mult_limbs(self.h, mid, limbs, ptr(fa[p]),
+                                              ptr(fb[p]), ptr(fo[p]))
+        return out
+
+    def key_switch(self, x, key, mid, key_mid):
+        limbs = self.limbs_at(mid)
+        out = np.zeros((2, limbs, self.n), np.int64)
+        self.lib().oracle_key_switch(self.h, mid, ptr(np.ascontiguousarray(x)),
+                                     ptr(np.ascontiguousarray(key)), key_mid,
+                                     ptr(out))
+        return out
+
+    def relinearize(self, trio, key, mid, key_mid):
+        limbs = self.limbs_at(mid)
+        out = np.zeros((2, limbs, self.n), np.int64)
+        self.lib().oracle_relinearize(self.h, mid,
+                                      ptr(np.ascontiguousarray(trio)),
+                                      ptr(np.ascontiguousarray(key)), key_mid,
+                                      ptr(out))
+        return out
+
+    def rescale(self, x, mid):
+        limbs = self.limbs_at(mid)
+        x = np.ascontiguousarray(x, dtype=np.int64)
+        polys = x.size // (limbs * self.n)
+        out = np.zeros((polys, limbs - 1, self.n), np.int64)
+        self.lib().oracle_rescale(self.h, mid, polys, ptr(x), ptr(out))
+        return out
+
+    def hmult(self, a, b, key, mid, key_mid):
+        limbs = self.limbs_at(mid)
+        out = np.zeros((2, limbs - 1, self.n), np.int64)
+        self.lib().oracle_hmult_relin_rescale(
+            self.h, mid, ptr(np.ascontiguousarray(a)),
+            ptr(np.ascontiguousarray(b)), ptr(np.ascontiguousarray(key)),
+            key_mid, ptr(out))
+        return out
+
+    def partitions(self, mid, key_mid):
+        s = np.zeros(128, np.int32)
+        z = np.zeros(128, np.int32)
+        k = np.zeros(128, np.int32)
+        c = self.lib().oracle_partitions(self.h, mid, key_mid, ptr(s), ptr(z),
+                                         ptr(k))
+        return [(int(s[i]), int(z[i]), int(k[i])) for i in range(c)]
+
+
+def _pl(y):
+    limbs = y.shape[-2]
+    polys = int(np.prod(y.shape[:-2])) if y.ndim > 2 else 1
+    return polys, limbs
+
+
+class Reference:
+    """The reference engine compiled from its sources (oracle/_ref)."""
+
+    @staticmethod
+    def available():
+        return os.path.exists(REF_LIB)
+
+    def __init__(self, log_n, alpha, bootstrap=0, threads=1):
+        lib = ctypes.CDLL(REF_LIB)
+        i32 = ctypes.c_int
+        # (restype, argtypes) of every shim entry point used by the tests
+        sigs = {
+            "ref_engine_create": (VP, [i32, i32, i32, i32]),
+            "ref_engine_destroy": (None, [VP]),
+            "ref_engine_info": (i32, [VP, VP]),
+            "ref_engine_primes": (i32, [VP, VP]),
+            "ref_secret_key": (VP, [VP]),
+            "ref_public_key": (VP, [VP, VP]),
+            "ref_relin_key": (VP, [VP, VP]),
+            "ref_encrypt": (VP, [VP, VP, i32, VP, i32]),
+            "ref_decrypt": (i32, [VP, VP, VP, VP]),
+            "ref_rescale": (VP, [VP, VP]),
+            "ref_multiply_relinearize": (VP, [VP, VP, VP, VP]),
+            "ref_hmult_relin_rescale": (VP, [VP, VP, VP, VP]),
+            "ref_ct_shape": (i32, [VP, VP]),
+            "ref_ct_get": (i32, [VP, VP]),
+            "ref_ct_set": (i32, [VP, VP]),
+            "ref_ct_scale": (ctypes.c_double, [VP]),
+            "ref_relin_key_shape": (i32, [VP, VP]),
+            "ref_relin_key_get": (i32, [VP, VP]),
+            "ref_relin_key_set": (i32, [VP, VP]),
+            "ref_free_ct": (None, [VP]),
+            "ref_free_relin_key": (None, [VP]),
+            "ref_free_public_key": (None, [VP]),
+            "ref_free_secret_key": (None, [VP]),
+            "ref_ntt_limb": (i32, [VP, i32, VP, i32]),
+            "ref_prime_constants": (i32, [VP, i32, VP, VP, VP]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        lib.ref_last_error.restype = ctypes.c_char_p
+        self.lib = lib
+        self.e = lib.ref_engine_create(log_n, alpha, bootstrap, threads)
+        if not self.e:
+            raise RuntimeError(lib.ref_last_error().decode())
+        info = np.zeros(9, np.int64)
+        lib.ref_engine_info(self.e, ptr(info))
+        self.n = int(info[0])
+        self.prime_count = int(info[1])
+        self.chain_count = int(info[2])
+        self.slot_count = int(info[6])
+        q = np.zeros(self.prime_count, np.uint64)
+        lib.ref_engine_primes(self.e, ptr(q))
+        self.primes = q
+
+    def close(self):
+        if self.e:
+            self.lib.ref_engine_destroy(self.e)
+            self.e = None
+
+    def shape(self, ct):
+        s = np.zeros(6, np.int64)
+        self.lib.ref_ct_shape(ct, ptr(s))
+        return [int(v) for v in s]
+
+    def get(self, ct):
+        mid, polys, chain, n, _, _ = self.shape(ct)
+        out = np.zeros((polys, chain, n), np.int64)
+        self.lib.ref_ct_get(ct, ptr(out))
+        return out
+
+    def set(self, ct, data):
+        self.lib.ref_ct_set(ct, ptr(np.ascontiguousarray(data,
+                                                         dtype=np.int64)))
+
+    def key(self, rlk):
+        s = np.zeros(4, np.int64)
+        self.lib.ref_relin_key_shape(rlk, ptr(s))
+        out = np.zeros((int(s[1]), int(s[2]), int(s[3])), np.int64)
+        self.lib.ref_relin_key_get(rlk, ptr(out))
+        return int(s[0]), out
