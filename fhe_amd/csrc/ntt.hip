@@ -132,6 +132,194 @@ __global__ __launch_bounds__(kBlock) void ntt_pass_kernel(PassArgs a,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident pass over an 8-bit range: each of the 16 sub-transforms
+// of a tile (256 points) is owned by 16 threads holding 16 points each.  The
+// eight radix-2 stages run as two radix-16 rounds in registers (4 stages
+// each) with one LDS exchange between them, plus one more exchange where the
+// global side of the pass is contiguous and the register ownership is not
+// (coalescing).  Prime, twiddle base and addresses are per-block scalars.
+//
+// Round ownership (16 values v[m] per thread, r = thread's row):
+//   forward  round A: j = r + 16 m  (bits 7..4 vary), round B: j = 16 r + m
+//   inverse  round A: j = 16 r + m  (bits 3..0 vary), round B: j = r + 16 m
+// so within a round a thread's butterflies pair v[m] with v[m ^ 2^x].
+// ---------------------------------------------------------------------------
+constexpr int kR16Subs = 16;
+constexpr int kR16Row = 273; // LDS row stride: 16 subs hit distinct banks
+
+__device__ __forceinline__ int r16_lds(int c, int j) {
+  return c * kR16Row + j + (j >> 4);
+}
+
+template <bool INV, bool COLS>
+__global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
+    PassArgs a, DeviceTables t, int polys, int tiles_per_limb, int swizzle) {
+  __shared__ int64_t lds[kR16Subs * kR16Row];
+  const int log_n = t.log_n;
+  const int64_t n = int64_t(1) << log_n;
+  const int b = a.low_bit;
+
+  // block -> (limb l, tile, polynomial), polynomial fastest so blocks sharing
+  // a prime's twiddles run together; with swizzle each of the 8 XCDs (blocks
+  // are dealt round-robin to XCDs) gets one contiguous range of that order.
+  int64_t w = blockIdx.x;
+  if (swizzle) {
+    const int64_t per = int64_t(gridDim.x) >> 3;
+    w = int64_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  const int64_t poly = w % polys;
+  const int64_t rest = w / polys;
+  const int tile = int(rest % tiles_per_limb);
+  const int l = int(rest / tiles_per_limb);
+  const int p = a.moduli_id + l;
+  const uint64_t q = t.q[p], k = t.k[p];
+  const int64_t q2 = int64_t(t.q2[p]);
+
+  const int tid = threadIdx.x;
+  const int c = COLS ? (tid & 15) : (tid >> 4);
+  const int r = COLS ? (tid >> 4) : (tid & 15);
+  const int64_t sp = int64_t(tile) * kR16Subs + c;
+  const int64_t lo = sp & ((int64_t(1) << b) - 1);
+  const int64_t hi = sp >> b;
+  const int64_t base = (poly * a.limbs + l) * n + (hi << (b + 8)) + lo;
+  const int64_t *src = a.src + base;
+  int64_t *dst = a.dst + base;
+  const int64_t *tw = (INV ? t.root_inverse : t.root) + int64_t(p) * n;
+
+  int64_t v[16];
+
+  // ---- load (round A ownership); the inverse contiguous pass loads rows
+  // j = r + 16 m (coalesced) and transposes through LDS
+  constexpr bool kLoadT = INV && !COLS;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int j = (INV && !kLoadT) ? 16 * r + m : r + 16 * m;
+    v[m] = src[int64_t(j) << b];
+  }
+  if (a.pre) {
+    const int64_t cst = a.pre[p];
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      v[m] = mont_mult_u(v[m], cst, q, k);
+  }
+  if (kLoadT) {
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      lds[r16_lds(c, r + 16 * m)] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      v[m] = lds[r16_lds(c, 16 * r + m)];
+    __syncthreads();
+  }
+
+  // ---- round A
+  if (!INV) {
+#pragma unroll
+    for (int u = 7; u >= 4; u--) {
+      const int x = u - 4;
+      const int s = log_n - 1 - b - u;
+      const int64_t tb = (int64_t(1) << s) - 1 + (hi << (7 - u));
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        if (m & (1 << x))
+          continue;
+        const int64_t wv = tw[tb + (m >> (u - 3))];
+        const int64_t prod = mont_mult_u(wv, v[m + (1 << x)], q, k);
+        const int64_t xv = v[m];
+        v[m] = guard(xv + prod, q2);
+        v[m + (1 << x)] = guard(xv + q2 - prod, q2);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u <= 3; u++) {
+      const int64_t tb =
+          n - (n >> (b + u)) + (hi << (7 - u)) + (int64_t(r) << (3 - u));
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        if (m & (1 << u))
+          continue;
+        const int64_t wv = tw[tb + (m >> (u + 1))];
+        const int64_t xv = v[m], yv = v[m + (1 << u)];
+        v[m] = guard(xv + yv, q2);
+        v[m + (1 << u)] = mont_mult_u(wv, guard(xv + q2 - yv, q2), q, k);
+      }
+    }
+  }
+
+  // ---- exchange A -> B ownership
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    lds[r16_lds(c, INV ? 16 * r + m : r + 16 * m)] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    v[m] = lds[r16_lds(c, INV ? r + 16 * m : 16 * r + m)];
+
+  // ---- round B
+  if (!INV) {
+#pragma unroll
+    for (int u = 3; u >= 0; u--) {
+      const int s = log_n - 1 - b - u;
+      const int64_t tb = (int64_t(1) << s) - 1 + (hi << (7 - u)) +
+                         (int64_t(r) << (3 - u));
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        if (m & (1 << u))
+          continue;
+        const int64_t wv = tw[tb + (m >> (u + 1))];
+        const int64_t prod = mont_mult_u(wv, v[m + (1 << u)], q, k);
+        const int64_t xv = v[m];
+        v[m] = guard(xv + prod, q2);
+        v[m + (1 << u)] = guard(xv + q2 - prod, q2);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int u = 4; u <= 7; u++) {
+      const int x = u - 4;
+      const int64_t tb = n - (n >> (b + u)) + (hi << (7 - u));
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        if (m & (1 << x))
+          continue;
+        const int64_t wv = tw[tb + (m >> (u - 3))];
+        const int64_t xv = v[m], yv = v[m + (1 << x)];
+        v[m] = guard(xv + yv, q2);
+        v[m + (1 << x)] = mont_mult_u(wv, guard(xv + q2 - yv, q2), q, k);
+      }
+    }
+    if (a.post != kInttRaw) {
+      const int64_t ninv = t.ninv[p];
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        int64_t y = mont_exit(mont_mult_u(v[m], ninv, q, k), q, k);
+        v[m] = a.post == kInttExitReduce ? guard(y, int64_t(q)) : y;
+      }
+    }
+  }
+
+  // ---- store; the forward contiguous pass transposes back to rows first
+  constexpr bool kStoreT = !INV && !COLS;
+  if (kStoreT) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      lds[r16_lds(c, 16 * r + m)] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      v[m] = lds[r16_lds(c, r + 16 * m)];
+  }
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int j = (INV || kStoreT) ? r + 16 * m : 16 * r + m;
+    dst[int64_t(j) << b] = v[m];
+  }
+}
+
 // Split log_n into the bit ranges of at most two passes (each <= 12 bits).
 void split_bits(int log_n, int &first, int &second) {
   if (log_n <= 12) {
@@ -145,6 +333,30 @@ void split_bits(int log_n, int &first, int &second) {
 
 void launch_pass(bool inverse, const PassArgs &a, const DeviceTables &t,
                  hipStream_t stream) {
+  if (a.bits == 8 && t.log_n >= 12) {
+    const int64_t total_limbs = a.total_subs >> (t.log_n - 8);
+    const int polys = int(total_limbs / a.limbs);
+    const int tiles = int(int64_t(1) << (t.log_n - 12));
+    const int64_t blocks = total_limbs * tiles;
+    const int swizzle = (blocks % 8 == 0) ? 1 : 0;
+    const bool cols = a.low_bit > 0;
+#define FHE_R16(I, C)                                                          \
+  hipLaunchKernelGGL((ntt_radix16_kernel<I, C>), dim3(unsigned(blocks)),      \
+                     dim3(kBlock), 0, stream, a, t, polys, tiles, swizzle)
+    if (inverse) {
+      if (cols)
+        FHE_R16(true, true);
+      else
+        FHE_R16(true, false);
+    } else {
+      if (cols)
+        FHE_R16(false, true);
+      else
+        FHE_R16(false, false);
+    }
+#undef FHE_R16
+    return;
+  }
   const int C = kTile >> a.bits;
   const unsigned blocks = unsigned((a.total_subs + C - 1) / C);
   if (inverse)
