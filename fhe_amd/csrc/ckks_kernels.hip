@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kBlock) void tensor_kernel(
   z[0] = mont_mult_u(a0, b0, q, k);
   const int64_t z1 = mont_mult_u(a0, b1, q, k);
   const int64_t z3 = mont_mult_u(a1, b0, q, k);
-  z[poly] = guard(guard(z1 + z3, q2), int64_t(q));
+  z[poly] = guard_fast(guard_fast(add64(z1, z3), q2), int64_t(q));
   z[2 * poly] = mont_mult_u(a1, b1, q, k);
 }
 
@@ -132,8 +132,8 @@ __global__ __launch_bounds__(kBlock) void key_inner_product_kernel(
       *a0 = p0;
       *a1 = p1;
     } else {
-      *a0 = guard(guard(p0 + *a0, q2), int64_t(q));
-      *a1 = guard(guard(p1 + *a1, q2), int64_t(q));
+      *a0 = guard_fast(guard_fast(add64(p0, *a0), q2), int64_t(q));
+      *a1 = guard_fast(guard_fast(add64(p1, *a1), q2), int64_t(q));
     }
   }
 }

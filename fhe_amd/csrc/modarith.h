@@ -51,10 +51,95 @@ FHE_HD uint64_t mont_reduce(U128 x, uint64_t q, uint64_t k) {
   return (hi << 2) | (lo >> kMontBits);
 }
 
-// Montgomery product of two NON-NEGATIVE operands (the common case: lazy
-// residues in [0, 2q) and table constants in [0, q)).
+#if defined(__HIP_DEVICE_COMPILE__)
+// 32-bit add with carry in/out: v_add_co_u32 / v_addc_co_u32 on gfx950,
+// each a full-rate op, where a 64-bit add (v_lshl_add_u64) is quarter rate.
+__device__ __forceinline__ uint32_t add_carry(uint32_t a, uint32_t b,
+                                              uint32_t cin, uint32_t &cout) {
+  unsigned int co;
+  const uint32_t r = __builtin_addc(a, b, cin, &co);
+  cout = co;
+  return r;
+}
+
+// 128-bit product of a, b < 2^63 as four 32-bit words.  The two cross
+// products are summed inside one v_mad_u64_u32 chain: a1*b0 + a0*b1 < 2^64
+// because both high halves are below 2^31.
+struct Words4 {
+  uint32_t w0, w1, w2, w3;
+};
+__device__ __forceinline__ Words4 mul_words(uint64_t a, uint64_t b) {
+  const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32);
+  const uint32_t b0 = uint32_t(b), b1 = uint32_t(b >> 32);
+  const uint64_t p00 = uint64_t(a0) * b0;
+  const uint64_t mid = uint64_t(a1) * b0 + uint64_t(a0) * b1;
+  const uint64_t p11 = uint64_t(a1) * b1;
+  uint32_t c;
+  Words4 x;
+  x.w0 = uint32_t(p00);
+  x.w1 = add_carry(uint32_t(p00 >> 32), uint32_t(mid), 0, c);
+  x.w2 = add_carry(uint32_t(p11), uint32_t(mid >> 32), c, c);
+  x.w3 = uint32_t(p11 >> 32) + c;
+  return x;
+}
+#endif
+
+// Montgomery product of two NON-NEGATIVE operands below 2^63 (lazy residues
+// in [0, 2q) and table constants in [0, q)).  Same value as mont_mult.
 FHE_HD int64_t mont_mult_u(int64_t a, int64_t b, uint64_t q, uint64_t k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const Words4 x = mul_words(uint64_t(a), uint64_t(b));
+  // s = (x mod 2^64) * k mod 2^62
+  const uint32_t k0 = uint32_t(k), k1 = uint32_t(k >> 32);
+  const uint64_t t = uint64_t(x.w0) * k0;
+  const uint32_t s0 = uint32_t(t);
+  const uint32_t s1 =
+      (uint32_t(t >> 32) + x.w0 * k1 + x.w1 * k0) & ((1u << 30) - 1);
+  const Words4 y = mul_words((uint64_t(s1) << 32) | s0, q);
+  // x + s*q has 62 zero low bits; keep bits 62..125
+  uint32_t c;
+  (void)add_carry(x.w0, y.w0, 0, c);
+  const uint32_t z1 = add_carry(x.w1, y.w1, c, c);
+  const uint32_t z2 = add_carry(x.w2, y.w2, c, c);
+  const uint32_t z3 = x.w3 + y.w3 + c;
+  const uint32_t lo = __builtin_amdgcn_alignbit(z2, z1, 30);
+  const uint32_t hi = __builtin_amdgcn_alignbit(z3, z2, 30);
+  return int64_t((uint64_t(hi) << 32) | lo);
+#else
   return int64_t(mont_reduce(mul_wide(uint64_t(a), uint64_t(b)), q, k));
+#endif
+}
+
+// 64-bit add / subtract as two 32-bit carry ops on the device.
+FHE_HD int64_t add64(int64_t a, int64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t c;
+  const uint32_t lo = add_carry(uint32_t(a), uint32_t(b), 0, c);
+  const uint32_t hi =
+      add_carry(uint32_t(uint64_t(a) >> 32), uint32_t(uint64_t(b) >> 32), c, c);
+  return int64_t((uint64_t(hi) << 32) | lo);
+#else
+  return int64_t(uint64_t(a) + uint64_t(b));
+#endif
+}
+
+FHE_HD int64_t sub64(int64_t a, int64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  unsigned int bo;
+  const uint32_t lo = __builtin_subc(uint32_t(a), uint32_t(b), 0u, &bo);
+  const uint32_t hi = __builtin_subc(uint32_t(uint64_t(a) >> 32),
+                                     uint32_t(uint64_t(b) >> 32), bo, &bo);
+  return int64_t((uint64_t(hi) << 32) | lo);
+#else
+  return int64_t(uint64_t(a) - uint64_t(b));
+#endif
+}
+
+// guard() for |x|, |q| < 2^62 (no signed overflow in x - q): the sign of
+// x - q decides, so a 64-bit subtract and a select replace the compare.
+FHE_HD int64_t guard_fast(int64_t x, int64_t q) {
+  const int64_t d = sub64(x, q);
+  return int32_t(uint64_t(d) >> 32) < 0 ? x : d;
 }
 
 // Montgomery product of arbitrary int64 operands: the 128-bit product of the

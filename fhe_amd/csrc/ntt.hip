@@ -23,6 +23,27 @@ namespace {
 
 constexpr int kTile = 4096; // elements per workgroup (32 KiB of LDS)
 
+// Radix-2 butterflies of src/operation/ntt.cpp: Cooley-Tukey
+//   p = mont(w, y); x' = guard(x + p, 2q); y' = guard(x + 2q - p, 2q)
+// and Gentleman-Sande
+//   x' = guard(x + y, 2q); y' = mont(w, guard(x + 2q - y, 2q)).
+__device__ __forceinline__ void ct_butterfly(int64_t &x, int64_t &y, int64_t w,
+                                             int64_t q2, uint64_t q,
+                                             uint64_t k) {
+  const int64_t p = mont_mult_u(w, y, q, k);
+  const int64_t x0 = x;
+  x = guard_fast(add64(x0, p), q2);
+  y = guard_fast(sub64(add64(x0, q2), p), q2);
+}
+
+__device__ __forceinline__ void gs_butterfly(int64_t &x, int64_t &y, int64_t w,
+                                             int64_t q2, uint64_t q,
+                                             uint64_t k) {
+  const int64_t x0 = x, y0 = y;
+  x = guard_fast(add64(x0, y0), q2);
+  y = mont_mult_u(w, guard_fast(sub64(add64(x0, q2), y0), q2), q, k);
+}
+
 struct PassArgs {
   const int64_t *src;
   int64_t *dst;
@@ -98,15 +119,18 @@ __global__ __launch_bounds__(kBlock) void ntt_pass_kernel(PassArgs a,
       if (!INV) {
         const int s = log_n - 1 - (b + u);
         const int64_t w = t.root[int64_t(p) * n + (int64_t(1) << s) - 1 + off];
-        const int64_t prod = mont_mult_u(w, y, q, k);
-        lds[jl * C + c] = guard(x + prod, q2);
-        lds[jh * C + c] = guard(x + q2 - prod, q2);
+        int64_t xa = x, ya = y;
+        ct_butterfly(xa, ya, w, q2, q, k);
+        lds[jl * C + c] = xa;
+        lds[jh * C + c] = ya;
       } else {
         const int tt = b + u;
         const int64_t w =
             t.root_inverse[int64_t(p) * n + n - (n >> tt) + off];
-        lds[jl * C + c] = guard(x + y, q2);
-        lds[jh * C + c] = mont_mult_u(w, guard(x + q2 - y, q2), q, k);
+        int64_t xa = x, ya = y;
+        gs_butterfly(xa, ya, w, q2, q, k);
+        lds[jl * C + c] = xa;
+        lds[jh * C + c] = ya;
       }
     }
     __syncthreads();
@@ -226,10 +250,7 @@ __global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
         if (m & (1 << x))
           continue;
         const int64_t wv = tw[tb + (m >> (u - 3))];
-        const int64_t prod = mont_mult_u(wv, v[m + (1 << x)], q, k);
-        const int64_t xv = v[m];
-        v[m] = guard(xv + prod, q2);
-        v[m + (1 << x)] = guard(xv + q2 - prod, q2);
+        ct_butterfly(v[m], v[m + (1 << x)], wv, q2, q, k);
       }
     }
   } else {
@@ -242,9 +263,7 @@ __global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
         if (m & (1 << u))
           continue;
         const int64_t wv = tw[tb + (m >> (u + 1))];
-        const int64_t xv = v[m], yv = v[m + (1 << u)];
-        v[m] = guard(xv + yv, q2);
-        v[m + (1 << u)] = mont_mult_u(wv, guard(xv + q2 - yv, q2), q, k);
+        gs_butterfly(v[m], v[m + (1 << u)], wv, q2, q, k);
       }
     }
   }
@@ -270,10 +289,7 @@ __global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
         if (m & (1 << u))
           continue;
         const int64_t wv = tw[tb + (m >> (u + 1))];
-        const int64_t prod = mont_mult_u(wv, v[m + (1 << u)], q, k);
-        const int64_t xv = v[m];
-        v[m] = guard(xv + prod, q2);
-        v[m + (1 << u)] = guard(xv + q2 - prod, q2);
+        ct_butterfly(v[m], v[m + (1 << u)], wv, q2, q, k);
       }
     }
   } else {
@@ -286,9 +302,7 @@ __global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
         if (m & (1 << x))
           continue;
         const int64_t wv = tw[tb + (m >> (u - 3))];
-        const int64_t xv = v[m], yv = v[m + (1 << x)];
-        v[m] = guard(xv + yv, q2);
-        v[m + (1 << x)] = mont_mult_u(wv, guard(xv + q2 - yv, q2), q, k);
+        gs_butterfly(v[m], v[m + (1 << x)], wv, q2, q, k);
       }
     }
     if (a.post != kInttRaw) {
