@@ -113,6 +113,13 @@ struct fhe_context {
 
   int64_t n() const { return params.n; }
 
+  // extended polynomials held at once by key switching: every partition on
+  // the fused path, one at a time otherwise
+  int ext_slots(int moduli_id) const {
+    return fhe::ks_fused_supported(tables) ? params.partition_count(moduli_id)
+                                           : 1;
+  }
+
   int64_t *scratch(size_t bytes) {
     if (bytes > workspace_bytes) {
       hip_check(hipStreamSynchronize(stream), "workspace sync");
@@ -175,6 +182,26 @@ struct fhe_context {
     const auto parts = params.partitions(moduli_id, key_moduli_id);
     const int64_t *cst = modup_constants(moduli_id, key_moduli_id);
     const int stride = params.ks_stride(moduli_id);
+    if (fhe::ks_fused_supported(tables)) {
+      // ext holds every partition: [count][parts][ext_limbs][N]
+      const int np = int(parts.size());
+      for (int p = 0; p < np; p++)
+        fhe::launch_modup(tables, ws_trio + 2 * limbs * N, 3 * limbs * N,
+                          ext + int64_t(p) * ext_limbs * N,
+                          int64_t(np) * ext_limbs * N, count, limbs, moduli_id,
+                          parts[p].start, parts[p].size, cst + p * stride,
+                          stream);
+      fhe::ntt_forward_first(tables, ext, ext, int64_t(count) * np * ext_limbs,
+                             ext_limbs, moduli_id, nullptr, stream);
+      fhe::launch_ks_inner_product_fused(tables, ext, key, key_chain,
+                                         key_offset, parts[0].key_index, np,
+                                         acc, count, ext_limbs, moduli_id,
+                                         stream);
+      fhe::ntt_inverse_second(tables, acc, acc, int64_t(count) * 2 * ext_limbs,
+                              ext_limbs, moduli_id, fhe::kInttExitReduce,
+                              stream);
+      return;
+    }
     for (size_t p = 0; p < parts.size(); p++) {
       const auto &part = parts[p];
       fhe::launch_modup(tables, ws_trio + 2 * limbs * N, 3 * limbs * N, ext,
@@ -416,7 +443,8 @@ int fhe_relinearize(fhe_context *ctx, int count, int moduli_id,
     const int ext_limbs = limbs + P.alpha;
     const int64_t N = ctx->n();
     const int64_t trio_l = int64_t(count) * 3 * limbs;
-    const int64_t ext_l = int64_t(count) * ext_limbs;
+    const int64_t ext_l =
+        int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
     const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
     int64_t *ws = ctx->scratch(size_t(trio_l + ext_l + acc_l) * N * 8);
     int64_t *trio = ws, *ext = trio + trio_l * N, *acc = ext + ext_l * N;
@@ -444,7 +472,8 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
     const int64_t N = ctx->n();
     const int64_t duo_l = int64_t(count) * 2 * limbs;
     const int64_t trio_l = int64_t(count) * 3 * limbs;
-    const int64_t ext_l = int64_t(count) * ext_limbs;
+    const int64_t ext_l =
+        int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
     const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
     int64_t *ws =
         ctx->scratch(size_t(2 * duo_l + trio_l + ext_l + acc_l) * N * 8);

@@ -19,6 +19,30 @@ void ntt_inverse(const DeviceTables &t, const int64_t *src, int64_t *dst,
                  int64_t total_limbs, int limbs, int moduli_id, int post,
                  hipStream_t stream);
 
+// Single passes of the two-pass transform (N >= 2^13): the forward pass over
+// the high index bits, and the inverse pass over the high index bits with
+// the InttPost epilogue.  The remaining passes run inside fused kernels.
+void ntt_forward_first(const DeviceTables &t, const int64_t *src, int64_t *dst,
+                       int64_t total_limbs, int limbs, int moduli_id,
+                       const int64_t *pre, hipStream_t stream);
+void ntt_inverse_second(const DeviceTables &t, const int64_t *src,
+                        int64_t *dst, int64_t total_limbs, int limbs,
+                        int moduli_id, int post, hipStream_t stream);
+
+// Fused key-switching core for N = 2^16: for each tile of each extended limb
+// and ciphertext, the low-bit forward NTT pass of every partition's
+// extended polynomial, the key inner product over all partitions
+// (src/ckks/core_key_switch.cpp:318-336) and the low-bit inverse NTT pass of
+// both accumulators.  ext: [count][parts][ext_limbs][N] after
+// ntt_forward_first; acc: [count][2][ext_limbs][N] before ntt_inverse_second.
+bool ks_fused_supported(const DeviceTables &t);
+void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
+                                   const int64_t *key, int key_chain,
+                                   int key_offset, int key_partition0,
+                                   int parts, int64_t *acc, int count,
+                                   int ext_limbs, int moduli_id,
+                                   hipStream_t stream);
+
 // out = mont_mult(a, b) elementwise (src/montgomery/unit.cpp:5-10).
 void launch_mont_mult(const DeviceTables &t, const int64_t *a,
                       const int64_t *b, int64_t *out, int64_t total_limbs,

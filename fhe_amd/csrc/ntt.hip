@@ -16,33 +16,13 @@
 //   i = (hi << (b+L)) | (j << b) | lo,   j in [0, 2^L)
 // and in both directions the twiddle offset is i >> (b + u + 1) for local bit u.
 #include "kernels.h"
+#include "ntt_r16.h"
 
 namespace fhe {
 
 namespace {
 
 constexpr int kTile = 4096; // elements per workgroup (32 KiB of LDS)
-
-// Radix-2 butterflies of src/operation/ntt.cpp: Cooley-Tukey
-//   p = mont(w, y); x' = guard(x + p, 2q); y' = guard(x + 2q - p, 2q)
-// and Gentleman-Sande
-//   x' = guard(x + y, 2q); y' = mont(w, guard(x + 2q - y, 2q)).
-__device__ __forceinline__ void ct_butterfly(int64_t &x, int64_t &y, int64_t w,
-                                             int64_t q2, uint64_t q,
-                                             uint64_t k) {
-  const int64_t p = mont_mult_u(w, y, q, k);
-  const int64_t x0 = x;
-  x = guard_fast(add64(x0, p), q2);
-  y = guard_fast(sub64(add64(x0, q2), p), q2);
-}
-
-__device__ __forceinline__ void gs_butterfly(int64_t &x, int64_t &y, int64_t w,
-                                             int64_t q2, uint64_t q,
-                                             uint64_t k) {
-  const int64_t x0 = x, y0 = y;
-  x = guard_fast(add64(x0, y0), q2);
-  y = mont_mult_u(w, guard_fast(sub64(add64(x0, q2), y0), q2), q, k);
-}
 
 struct PassArgs {
   const int64_t *src;
@@ -158,28 +138,17 @@ __global__ __launch_bounds__(kBlock) void ntt_pass_kernel(PassArgs a,
 
 // ---------------------------------------------------------------------------
 // Register-resident pass over an 8-bit range: each of the 16 sub-transforms
-// of a tile (256 points) is owned by 16 threads holding 16 points each.  The
-// eight radix-2 stages run as two radix-16 rounds in registers (4 stages
-// each) with one LDS exchange between them, plus one more exchange where the
-// global side of the pass is contiguous and the register ownership is not
-// (coalescing).  Prime, twiddle base and addresses are per-block scalars.
-//
-// Round ownership (16 values v[m] per thread, r = thread's row):
-//   forward  round A: j = r + 16 m  (bits 7..4 vary), round B: j = 16 r + m
-//   inverse  round A: j = 16 r + m  (bits 3..0 vary), round B: j = r + 16 m
-// so within a round a thread's butterflies pair v[m] with v[m ^ 2^x].
+// of a tile (256 points) is owned by 16 threads holding 16 points each, and
+// the eight radix-2 stages run as two radix-16 rounds in registers with one
+// LDS exchange between them (ntt_r16.h).  A second exchange is added where
+// the global side of the pass is contiguous and the register ownership is
+// not, so that every global access is coalesced.  Prime, twiddle base and
+// addresses are per-block scalars.
 // ---------------------------------------------------------------------------
-constexpr int kR16Subs = 16;
-constexpr int kR16Row = 273; // LDS row stride: 16 subs hit distinct banks
-
-__device__ __forceinline__ int r16_lds(int c, int j) {
-  return c * kR16Row + j + (j >> 4);
-}
-
 template <bool INV, bool COLS>
 __global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
     PassArgs a, DeviceTables t, int polys, int tiles_per_limb, int swizzle) {
-  __shared__ int64_t lds[kR16Subs * kR16Row];
+  __shared__ int64_t lds[kR16Lds];
   const int log_n = t.log_n;
   const int64_t n = int64_t(1) << log_n;
   const int b = a.low_bit;
@@ -197,141 +166,68 @@ __global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
   const int tile = int(rest % tiles_per_limb);
   const int l = int(rest / tiles_per_limb);
   const int p = a.moduli_id + l;
-  const uint64_t q = t.q[p], k = t.k[p];
-  const int64_t q2 = int64_t(t.q2[p]);
 
   const int tid = threadIdx.x;
   const int c = COLS ? (tid & 15) : (tid >> 4);
   const int r = COLS ? (tid >> 4) : (tid & 15);
-  const int64_t sp = int64_t(tile) * kR16Subs + c;
+  const int64_t sp = int64_t(tile) * 16 + c;
   const int64_t lo = sp & ((int64_t(1) << b) - 1);
   const int64_t hi = sp >> b;
   const int64_t base = (poly * a.limbs + l) * n + (hi << (b + 8)) + lo;
   const int64_t *src = a.src + base;
   int64_t *dst = a.dst + base;
-  const int64_t *tw = (INV ? t.root_inverse : t.root) + int64_t(p) * n;
+
+  R16Pass s;
+  s.tw = (INV ? t.root_inverse : t.root) + int64_t(p) * n;
+  s.n = n;
+  s.hi = hi;
+  s.log_n = log_n;
+  s.b = b;
+  s.r = r;
+  s.q = t.q[p];
+  s.k = t.k[p];
+  s.q2 = int64_t(t.q2[p]);
 
   int64_t v[16];
-
-  // ---- load (round A ownership); the inverse contiguous pass loads rows
-  // j = r + 16 m (coalesced) and transposes through LDS
-  constexpr bool kLoadT = INV && !COLS;
+  // load in the ownership of the first round; the contiguous inverse pass
+  // loads coalesced ROWS and re-owns through LDS
+  const bool cols_load = INV && COLS;
 #pragma unroll
-  for (int m = 0; m < 16; m++) {
-    const int j = (INV && !kLoadT) ? 16 * r + m : r + 16 * m;
-    v[m] = src[int64_t(j) << b];
-  }
+  for (int m = 0; m < 16; m++)
+    v[m] = src[int64_t(cols_load ? 16 * r + m : r + 16 * m) << b];
   if (a.pre) {
     const int64_t cst = a.pre[p];
 #pragma unroll
     for (int m = 0; m < 16; m++)
-      v[m] = mont_mult_u(v[m], cst, q, k);
+      v[m] = mont_mult_u(v[m], cst, s.q, s.k);
   }
-  if (kLoadT) {
-#pragma unroll
-    for (int m = 0; m < 16; m++)
-      lds[r16_lds(c, r + 16 * m)] = v[m];
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < 16; m++)
-      v[m] = lds[r16_lds(c, 16 * r + m)];
-    __syncthreads();
-  }
+  if (INV && !COLS)
+    r16_exchange(v, lds, c, r, true);
 
-  // ---- round A
   if (!INV) {
-#pragma unroll
-    for (int u = 7; u >= 4; u--) {
-      const int x = u - 4;
-      const int s = log_n - 1 - b - u;
-      const int64_t tb = (int64_t(1) << s) - 1 + (hi << (7 - u));
-#pragma unroll
-      for (int m = 0; m < 16; m++) {
-        if (m & (1 << x))
-          continue;
-        const int64_t wv = tw[tb + (m >> (u - 3))];
-        ct_butterfly(v[m], v[m + (1 << x)], wv, q2, q, k);
-      }
-    }
+    r16_forward_high(v, s);
+    r16_exchange(v, lds, c, r, true);
+    r16_forward_low(v, s);
+    if (!COLS)
+      r16_exchange(v, lds, c, r, false); // coalesced contiguous store
   } else {
-#pragma unroll
-    for (int u = 0; u <= 3; u++) {
-      const int64_t tb =
-          n - (n >> (b + u)) + (hi << (7 - u)) + (int64_t(r) << (3 - u));
-#pragma unroll
-      for (int m = 0; m < 16; m++) {
-        if (m & (1 << u))
-          continue;
-        const int64_t wv = tw[tb + (m >> (u + 1))];
-        gs_butterfly(v[m], v[m + (1 << u)], wv, q2, q, k);
-      }
-    }
-  }
-
-  // ---- exchange A -> B ownership
-#pragma unroll
-  for (int m = 0; m < 16; m++)
-    lds[r16_lds(c, INV ? 16 * r + m : r + 16 * m)] = v[m];
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < 16; m++)
-    v[m] = lds[r16_lds(c, INV ? r + 16 * m : 16 * r + m)];
-
-  // ---- round B
-  if (!INV) {
-#pragma unroll
-    for (int u = 3; u >= 0; u--) {
-      const int s = log_n - 1 - b - u;
-      const int64_t tb = (int64_t(1) << s) - 1 + (hi << (7 - u)) +
-                         (int64_t(r) << (3 - u));
-#pragma unroll
-      for (int m = 0; m < 16; m++) {
-        if (m & (1 << u))
-          continue;
-        const int64_t wv = tw[tb + (m >> (u + 1))];
-        ct_butterfly(v[m], v[m + (1 << u)], wv, q2, q, k);
-      }
-    }
-  } else {
-#pragma unroll
-    for (int u = 4; u <= 7; u++) {
-      const int x = u - 4;
-      const int64_t tb = n - (n >> (b + u)) + (hi << (7 - u));
-#pragma unroll
-      for (int m = 0; m < 16; m++) {
-        if (m & (1 << x))
-          continue;
-        const int64_t wv = tw[tb + (m >> (u - 3))];
-        gs_butterfly(v[m], v[m + (1 << x)], wv, q2, q, k);
-      }
-    }
+    r16_inverse_low(v, s);
+    r16_exchange(v, lds, c, r, false);
+    r16_inverse_high(v, s);
     if (a.post != kInttRaw) {
       const int64_t ninv = t.ninv[p];
 #pragma unroll
       for (int m = 0; m < 16; m++) {
-        int64_t y = mont_exit(mont_mult_u(v[m], ninv, q, k), q, k);
-        v[m] = a.post == kInttExitReduce ? guard(y, int64_t(q)) : y;
+        const int64_t y = mont_exit(mont_mult_u(v[m], ninv, s.q, s.k), s.q, s.k);
+        v[m] = a.post == kInttExitReduce ? guard(y, int64_t(s.q)) : y;
       }
     }
   }
-
-  // ---- store; the forward contiguous pass transposes back to rows first
-  constexpr bool kStoreT = !INV && !COLS;
-  if (kStoreT) {
-    __syncthreads();
+  // store: ROWS ownership except the strided forward pass (COLS)
+  const bool cols_store = !INV && COLS;
 #pragma unroll
-    for (int m = 0; m < 16; m++)
-      lds[r16_lds(c, 16 * r + m)] = v[m];
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < 16; m++)
-      v[m] = lds[r16_lds(c, r + 16 * m)];
-  }
-#pragma unroll
-  for (int m = 0; m < 16; m++) {
-    const int j = (INV || kStoreT) ? r + 16 * m : 16 * r + m;
-    dst[int64_t(j) << b] = v[m];
-  }
+  for (int m = 0; m < 16; m++)
+    dst[int64_t(cols_store ? 16 * r + m : r + 16 * m) << b] = v[m];
 }
 
 // Split log_n into the bit ranges of at most two passes (each <= 12 bits).
@@ -413,6 +309,26 @@ void ntt_inverse(const DeviceTables &t, const int64_t *src, int64_t *dst,
                 l1, l2, nullptr, post};
     launch_pass(true, b2, t, stream);
   }
+}
+
+void ntt_forward_first(const DeviceTables &t, const int64_t *src, int64_t *dst,
+                       int64_t total_limbs, int limbs, int moduli_id,
+                       const int64_t *pre, hipStream_t stream) {
+  int l1, l2;
+  split_bits(t.log_n, l1, l2);
+  PassArgs a{src, dst, total_limbs << (t.log_n - l1), limbs, moduli_id,
+             t.log_n - l1, l1, pre, kInttRaw};
+  launch_pass(false, a, t, stream);
+}
+
+void ntt_inverse_second(const DeviceTables &t, const int64_t *src,
+                        int64_t *dst, int64_t total_limbs, int limbs,
+                        int moduli_id, int post, hipStream_t stream) {
+  int l1, l2;
+  split_bits(t.log_n, l1, l2);
+  PassArgs a{src, dst, total_limbs << (t.log_n - l2), limbs, moduli_id,
+             l1, l2, nullptr, post};
+  launch_pass(true, a, t, stream);
 }
 
 } // namespace fhe
