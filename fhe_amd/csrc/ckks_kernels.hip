@@ -47,9 +47,19 @@ __global__ __launch_bounds__(kBlock) void tensor_kernel(
   z[2 * poly] = mont_mult_u(a1, b1, q, k);
 }
 
+// Montgomery product for ModUp: the signed form where a digit can be
+// negative, the fast unsigned form where the host has shown it cannot.
+template <bool SIGNED>
+__device__ __forceinline__ int64_t modup_mult(int64_t a, int64_t b,
+                                              uint64_t q, uint64_t k) {
+  return SIGNED ? mont_mult(a, b, q, k) : mont_mult_u(a, b, q, k);
+}
+
 // One thread per (ciphertext, coefficient): Garner digits of the partition
-// in registers, then every limb of the extended basis.
-template <int MAXZ>
+// in registers, then every limb of the extended basis.  With SIGNED false
+// the partition's first prime is at most twice each of its other primes, so
+// every difference and digit below is non-negative and under 2^62.
+template <int MAXZ, bool SIGNED>
 __global__ __launch_bounds__(kBlock) void modup_kernel(
     const int64_t *x, int64_t x_stride, int64_t *ext, int64_t ext_stride,
     int count, int limbs, int moduli_id, int start, int size,
@@ -74,19 +84,21 @@ __global__ __launch_bounds__(kBlock) void modup_kernel(
     if (j < size) {
       const int pj = moduli_id + start + j;
       const int64_t m = int64_t(t.q[pj]), m2 = int64_t(t.q2[pj]);
-      const int64_t diff = guard(xs[int64_t(start + j) * n] + m2 - d[j], m2);
-      d[j] = guard(mont_mult(diff, cst[CkksParams::kKsInvOffset + j], t.q[pj],
-                             t.k[pj]),
-                   m);
+      const int64_t diff = guard_fast(
+          sub64(add64(xs[int64_t(start + j) * n], m2), d[j]), m2);
+      d[j] = guard_fast(modup_mult<SIGNED>(diff,
+                                           cst[CkksParams::kKsInvOffset + j],
+                                           t.q[pj], t.k[pj]),
+                        m);
 #pragma unroll
       for (int tt = j + 1; tt < MAXZ; tt++) {
         if (tt < size) {
           const int pt = moduli_id + start + tt;
-          const int64_t prod = mont_mult(
+          const int64_t prod = modup_mult<SIGNED>(
               d[j], cst[CkksParams::kKsTileOffset + j * kMaxAlpha + tt],
               t.q[pt], t.k[pt]);
-          d[tt] = guard(guard(prod + d[tt], int64_t(t.q2[pt])),
-                        int64_t(t.q[pt]));
+          d[tt] = guard_fast(guard_fast(add64(prod, d[tt]), int64_t(t.q2[pt])),
+                             int64_t(t.q[pt]));
         }
       }
     }
@@ -98,11 +110,13 @@ __global__ __launch_bounds__(kBlock) void modup_kernel(
     const int p = moduli_id + i;
     const uint64_t q = t.q[p], k = t.k[p];
     const int64_t q2 = int64_t(t.q2[p]);
-    int64_t y = mont_mult(d[0], ec[i], q, k);
+    int64_t y = modup_mult<SIGNED>(d[0], ec[i], q, k);
 #pragma unroll
     for (int j = 1; j < MAXZ; j++)
       if (j < size)
-        y = guard(mont_mult(d[j], ec[j * ext_limbs + i], q, k) + y, q2);
+        y = guard_fast(
+            add64(modup_mult<SIGNED>(d[j], ec[j * ext_limbs + i], q, k), y),
+            q2);
     out[int64_t(i) * n] = y;
   }
 }
@@ -287,22 +301,30 @@ void launch_tensor(const DeviceTables &t, const int64_t *a, const int64_t *b,
 void launch_modup(const DeviceTables &t, const int64_t *x, int64_t x_stride,
                   int64_t *ext, int64_t ext_stride, int count, int limbs,
                   int moduli_id, int start, int size, const int64_t *constants,
-                  hipStream_t stream) {
+                  bool non_negative, hipStream_t stream) {
   const dim3 grid(grid_for(int64_t(count) << t.log_n));
-#define FHE_MODUP(W)                                                           \
-  hipLaunchKernelGGL(modup_kernel<W>, grid, dim3(kBlock), 0, stream, x,        \
+#define FHE_MODUP(W, S)                                                        \
+  hipLaunchKernelGGL((modup_kernel<W, S>), grid, dim3(kBlock), 0, stream, x,  \
                      x_stride, ext, ext_stride, count, limbs, moduli_id,       \
                      start, size, constants, t)
+#define FHE_MODUP_W(W)                                                         \
+  do {                                                                         \
+    if (non_negative)                                                          \
+      FHE_MODUP(W, false);                                                     \
+    else                                                                       \
+      FHE_MODUP(W, true);                                                      \
+  } while (0)
   if (size <= 1)
-    FHE_MODUP(1);
+    FHE_MODUP_W(1);
   else if (size <= 2)
-    FHE_MODUP(2);
+    FHE_MODUP_W(2);
   else if (size <= 4)
-    FHE_MODUP(4);
+    FHE_MODUP_W(4);
   else if (size <= 8)
-    FHE_MODUP(8);
+    FHE_MODUP_W(8);
   else
-    FHE_MODUP(16);
+    FHE_MODUP_W(16);
+#undef FHE_MODUP_W
 #undef FHE_MODUP
 }
 

@@ -190,6 +190,7 @@ struct fhe_context {
                           ext + int64_t(p) * ext_limbs * N,
                           int64_t(np) * ext_limbs * N, count, limbs, moduli_id,
                           parts[p].start, parts[p].size, cst + p * stride,
+                          params.modup_non_negative(moduli_id, parts[p]),
                           stream);
       fhe::ntt_forward_first(tables, ext, ext, int64_t(count) * np * ext_limbs,
                              ext_limbs, moduli_id, nullptr, stream);
@@ -206,7 +207,8 @@ struct fhe_context {
       const auto &part = parts[p];
       fhe::launch_modup(tables, ws_trio + 2 * limbs * N, 3 * limbs * N, ext,
                         ext_limbs * N, count, limbs, moduli_id, part.start,
-                        part.size, cst + p * stride, stream);
+                        part.size, cst + p * stride,
+                        params.modup_non_negative(moduli_id, part), stream);
       fhe::ntt_forward(tables, ext, ext, int64_t(count) * ext_limbs,
                        ext_limbs, moduli_id, nullptr, stream);
       const int64_t *key0 =

@@ -56,10 +56,13 @@ void launch_tensor(const DeviceTables &t, const int64_t *a, const int64_t *b,
 // Garner digits of one partition and exact basis extension to all
 // limbs + alpha primes (src/ckks/core_key_switch.cpp:21-108).  x: INTT-form
 // polynomial of each ciphertext at x + c * x_stride; ext: [count][ext][N].
+// non_negative: the partition's first prime is at most twice each of its
+// other primes (CkksParams::modup_non_negative), which keeps every Garner
+// difference non-negative and allows the unsigned Montgomery product.
 void launch_modup(const DeviceTables &t, const int64_t *x, int64_t x_stride,
                   int64_t *ext, int64_t ext_stride, int count, int limbs,
                   int moduli_id, int start, int size, const int64_t *constants,
-                  hipStream_t stream);
+                  bool non_negative, hipStream_t stream);
 
 // Key inner product for one partition: acc[c][h] (+)= ext[c] * key_h
 // (src/ckks/core_key_switch.cpp:318-336).  Each key limb is read once for

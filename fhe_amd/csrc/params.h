@@ -95,6 +95,16 @@ public:
     return kKsExtOffset + kMaxAlpha * (limbs_at(moduli_id) + alpha);
   }
   std::vector<int64_t> modup_constants(int moduli_id, int key_moduli_id) const;
+  // True when the first prime of the partition is at most twice each of its
+  // other primes: then the Garner differences of rns_to_mrs
+  // (src/ckks/core_key_switch.cpp:21-56) are never negative.
+  bool modup_non_negative(int moduli_id, const Partition &part) const {
+    const uint64_t first = q[moduli_id + part.start];
+    for (int j = 1; j < part.size; j++)
+      if (first > 2 * q[moduli_id + part.start + j])
+        return false;
+    return true;
+  }
 
 private:
   void derive();
