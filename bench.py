@@ -153,6 +153,59 @@ def cpu_baseline(budget_s=20.0):
             "kind": "port", "sample": f"{done} oracle HMult at Gold L=24"}
 
 
+# Algorithmic HBM bytes of one launch of each single-kernel stage (read each
+# input limb once, write each output limb once; N * 8 bytes per limb) and
+# the modular products it performs (one per butterfly, one per pointwise
+# product).  Shape keys: n, count, limbs, ext (= limbs + alpha), parts.
+def stage_model(name, s):
+    limb = 8 * s["n"]
+    c, l, e, p, n = s["count"], s["limbs"], s["ext"], s["parts"], s["n"]
+    half_pass = 4 * n  # butterflies of one 8-bit pass over one limb (N/2 * 8)
+    if name == "ks_inner_product_r16":
+        # ext (all partitions) and the key in, both accumulators out; the low
+        # forward pass and two key products per ext limb, the low inverse
+        # pass per accumulator limb
+        return (limb * (c * p * e + 2 * p * e + 2 * c * e),
+                c * p * e * (half_pass + 2 * n) + 2 * c * e * half_pass)
+    if name == "modup":
+        return limb * c * (ALPHA + e), None
+    if name == "ntt_pass_high(ext)":
+        return 2 * limb * c * p * e, c * p * e * half_pass
+    if name == "intt_pass_high(acc)":
+        return 2 * limb * 2 * c * e, 2 * c * e * (half_pass + n)
+    if name == "moddown_relin_rescale":
+        return limb * c * (2 * e + 2 * l + 2 * (l - 1)), None
+    if name == "tensor":
+        return limb * c * 7 * l, 4 * c * l * n
+    return None, None
+
+
+def stage_report(stages, total_ms, shape):
+    """Per-stage share of the step and the roofline of the dominant
+    single-kernel stage (one kernel launch per stage call)."""
+    breakdown = {}
+    best = None
+    for name, (ms, launches) in sorted(stages.items(), key=lambda kv: -kv[1][0]):
+        avg = ms / launches
+        breakdown[name] = {"avg_launch_ms": round(avg, 4),
+                           "launches": launches,
+                           "share": round(ms / total_ms, 4)}
+        nbytes, _ = stage_model(name, shape)
+        if nbytes is not None and (best is None or ms > best[1]):
+            best = (name, ms, avg)
+    name, _, avg = best
+    nbytes, mults = stage_model(name, shape)
+    gbs = nbytes / (avg / 1e3) / 1e9
+    roofline = {"bound": "hbm", "kernel": name, "achieved": round(gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(avg, 4),
+                "algorithmic_bytes": nbytes}
+    if mults is not None:
+        roofline["modmul_per_s"] = round(mults / (avg / 1e3), 0)
+    return breakdown, roofline
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,19 +239,21 @@ def main():
             ctx, ev, lambda: ctx.hmult_relin_rescale(pa, pb, dk, key_mid, pout,
                                                      MODULI_ID),
             max(5, args.steps), 2)
-        # dominant kernel: forward NTT of the extended basis (two passes),
-        # timed alone on the same stream over a batch-sized buffer
-        ext = limbs + ALPHA
-        ntt_buf = ctx.array((args.batch, ext, n))
-        ntt_ms = time_steps(ctx, ev, lambda: ctx.ntt(ntt_buf, MODULI_ID,
-                                                     enter=False),
-                            args.steps, 2)
-        for arr in (da, db, dk, out, pa, pb, pout, ntt_buf):
+        # per-stage launch times, HIP events on the library's stream
+        ctx.profile(True)
+        prof_steps = max(3, min(args.steps, 10))
+        for _ in range(prof_steps):
+            ctx.hmult_relin_rescale(da, db, dk, key_mid, out, MODULI_ID)
+        stages = ctx.profile_entries()
+        ctx.profile(False)
+        parts = -(-limbs // ALPHA)
+        for arr in (da, db, dk, out, pa, pb, pout):
             arr.free()
 
     value = args.batch / (step_ms / 1e3)
-    ntt_bytes = 2 * 8 * n * ext * args.batch  # read + write once (algorithmic)
-    ntt_gbs = ntt_bytes / (ntt_ms / 1e3) / 1e9
+    shape = dict(n=n, count=args.batch, limbs=limbs, ext=limbs + ALPHA,
+                 parts=parts)
+    breakdown, roofline = stage_report(stages, step_ms * prof_steps, shape)
     line = {
         "metric": "HMult+relin+rescale ciphertexts/s at N=2^16,L=24 (batched)",
         "value": round(value, 2),
@@ -218,11 +273,8 @@ def main():
                    "batch": args.batch, "log_n": LOG_N, "moduli_id": MODULI_ID},
         "plain_ciphertexts_per_s": round(1 / (plain_ms / 1e3), 2),
         "batched_vs_plain": round(value * plain_ms / 1e3, 3),
-        "roofline": {"bound": "hbm", "kernel": "ntt_pass_kernel (forward, 2 "
-                     "passes, extended basis)",
-                     "achieved": round(ntt_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ntt_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+        "roofline": roofline,
+        "stages": breakdown,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()

@@ -26,6 +26,8 @@ EXPORTED_SYMBOLS = (
     "fhe_device_free", "fhe_copy_to_device", "fhe_copy_to_host", "fhe_ntt",
     "fhe_intt", "fhe_mont_mult", "fhe_rescale", "fhe_relinearize",
     "fhe_hmult_relin_rescale", "fhe_params_resolve_primes",
+    "fhe_automorphism", "fhe_rotate", "fhe_conjugate", "fhe_profile_enable",
+    "fhe_profile_count", "fhe_profile_entry",
 )
 
 
@@ -82,6 +84,14 @@ def library():
         lib.fhe_relinearize.argtypes = [vp, i32, i32, vp, vp, i32, vp]
         lib.fhe_hmult_relin_rescale.argtypes = [vp, i32, i32, vp, vp, vp, i32,
                                                 vp]
+        lib.fhe_automorphism.argtypes = [vp, vp, vp, i32, i32, i32, i32, i32]
+        lib.fhe_rotate.argtypes = [vp, i32, i32, vp, vp, i32, i32, vp]
+        lib.fhe_conjugate.argtypes = [vp, i32, i32, vp, vp, i32, vp]
+        lib.fhe_profile_enable.argtypes = [vp, i32]
+        lib.fhe_profile_count.argtypes = [vp, ctypes.POINTER(i32)]
+        lib.fhe_profile_entry.argtypes = [
+            vp, i32, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
+            ctypes.POINTER(ctypes.c_int64)]
         _lib = lib
     return _lib
 
@@ -247,6 +257,41 @@ class Context:
         _check(library().fhe_hmult_relin_rescale(self.handle, count, moduli_id,
                                                  a.ptr, b.ptr, key.ptr,
                                                  key_moduli_id, out.ptr))
+
+    def automorphism(self, src, dst, moduli_id, delta, ntt_form=False):
+        polys, limbs = _polys_limbs(src, self.n)
+        _check(library().fhe_automorphism(self.handle, src.ptr, dst.ptr, polys,
+                                          limbs, moduli_id, delta,
+                                          1 if ntt_form else 0))
+
+    def rotate(self, cts, key, key_moduli_id, delta, out, moduli_id):
+        count = int(np.prod(cts.shape[:-3]))
+        _check(library().fhe_rotate(self.handle, count, moduli_id, cts.ptr,
+                                    key.ptr, key_moduli_id, delta, out.ptr))
+
+    def conjugate(self, cts, key, key_moduli_id, out, moduli_id):
+        count = int(np.prod(cts.shape[:-3]))
+        _check(library().fhe_conjugate(self.handle, count, moduli_id, cts.ptr,
+                                       key.ptr, key_moduli_id, out.ptr))
+
+    def profile(self, on):
+        """Start (clearing totals) or stop stage timing."""
+        _check(library().fhe_profile_enable(self.handle, 1 if on else 0))
+
+    def profile_entries(self):
+        """{stage name: (total ms, launches)} since profile(True)."""
+        lib = library()
+        count = ctypes.c_int()
+        _check(lib.fhe_profile_count(self.handle, ctypes.byref(count)))
+        out = {}
+        for i in range(count.value):
+            name = ctypes.create_string_buffer(64)
+            ms, launches = ctypes.c_double(), ctypes.c_int64()
+            _check(lib.fhe_profile_entry(self.handle, i, name, 64,
+                                         ctypes.byref(ms),
+                                         ctypes.byref(launches)))
+            out[name.value.decode()] = (ms.value, launches.value)
+        return out
 
 
 def _polys_limbs(arr, n):

@@ -21,7 +21,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["params.cpp"]
-HIP_SOURCES = ["ntt.hip", "ckks_kernels.hip", "ks_fused.hip", "context.hip"]
+HIP_SOURCES = ["ntt.hip", "ckks_kernels.hip", "ks_fused.hip", "automorphism.hip",
+               "context.hip"]
 
 INCLUDES = ["-I" + os.path.join(REPO, "include"), "-I" + CSRC]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kBlock) void tensor_kernel(
   z[0] = mont_mult_u(a0, b0, q, k);
   const int64_t z1 = mont_mult_u(a0, b1, q, k);
   const int64_t z3 = mont_mult_u(a1, b0, q, k);
-  z[poly] = guard_fast(guard_fast(add64(z1, z3), q2), int64_t(q));
+  z[poly] = guard_u(guard_u(add64(z1, z3), q2), int64_t(q));
   z[2 * poly] = mont_mult_u(a1, b1, q, k);
 }
 
@@ -146,8 +146,8 @@ __global__ __launch_bounds__(kBlock) void key_inner_product_kernel(
       *a0 = p0;
       *a1 = p1;
     } else {
-      *a0 = guard_fast(guard_fast(add64(p0, *a0), q2), int64_t(q));
-      *a1 = guard_fast(guard_fast(add64(p1, *a1), q2), int64_t(q));
+      *a0 = guard_u(guard_u(add64(p0, *a0), q2), int64_t(q));
+      *a1 = guard_u(guard_u(add64(p1, *a1), q2), int64_t(q));
     }
   }
 }
@@ -202,10 +202,12 @@ template <int MAXA> struct ModDown {
   }
 };
 
+// The addend of half h of ciphertext c is d + c * d_stride + h * limbs * N
+// for h < d_polys and zero otherwise.
 template <int MAXA, bool RESCALE>
-__global__ __launch_bounds__(kBlock) void moddown_relin_kernel(
-    const int64_t *acc, const int64_t *trio, int64_t *out, int count,
-    int limbs, int moduli_id, DeviceTables t) {
+__global__ __launch_bounds__(kBlock) void moddown_add_kernel(
+    const int64_t *acc, const int64_t *dbase, int64_t d_stride, int d_polys,
+    int64_t *out, int count, int limbs, int moduli_id, DeviceTables t) {
   const int64_t n = int64_t(1) << t.log_n;
   const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (idx >= int64_t(count) * 2 * n)
@@ -215,7 +217,8 @@ __global__ __launch_bounds__(kBlock) void moddown_relin_kernel(
   const int64_t c = ch >> 1, h = ch & 1;
   const int ext_limbs = limbs + t.alpha;
   const int64_t *a = acc + ch * ext_limbs * n + coef;
-  const int64_t *d = trio + (c * 3 + h) * limbs * n + coef;
+  const int64_t *d =
+      h < d_polys ? dbase + c * d_stride + h * limbs * n + coef : nullptr;
 
   ModDown<MAXA> md;
   md.alpha = t.alpha;
@@ -223,12 +226,13 @@ __global__ __launch_bounds__(kBlock) void moddown_relin_kernel(
   md.moduli_id = moduli_id;
   md.prepare(a, n, t);
 
-  // relinearize: y_i = reduce(d_i + ks_i) (src/ckks/core.cpp:736-737)
+  // relinearize: y_i = reduce(d_i + ks_i) (src/ckks/core.cpp:736-737); a
+  // canonical ks_i passes the two guards unchanged when there is no d
   auto relin = [&](int i) {
     const int p = moduli_id + i;
     const int64_t ks = md.limb(a[int64_t(i) * n], i, t);
-    return guard(guard(d[int64_t(i) * n] + ks, int64_t(t.q2[p])),
-                 int64_t(t.q[p]));
+    const int64_t di = d ? d[int64_t(i) * n] : 0;
+    return guard(guard(di + ks, int64_t(t.q2[p])), int64_t(t.q[p]));
   };
 
   if (!RESCALE) {
@@ -341,12 +345,14 @@ void launch_key_inner_product(const DeviceTables &t, const int64_t *ext,
 
 template <bool RESCALE>
 static void launch_moddown(const DeviceTables &t, const int64_t *acc,
-                           const int64_t *trio, int64_t *out, int count,
-                           int limbs, int moduli_id, hipStream_t stream) {
+                           const int64_t *d, int64_t d_stride, int d_polys,
+                           int64_t *out, int count, int limbs, int moduli_id,
+                           hipStream_t stream) {
   const dim3 grid(grid_for((int64_t(count) * 2) << t.log_n));
 #define FHE_MODDOWN(W)                                                         \
-  hipLaunchKernelGGL((moddown_relin_kernel<W, RESCALE>), grid, dim3(kBlock),  \
-                     0, stream, acc, trio, out, count, limbs, moduli_id, t)
+  hipLaunchKernelGGL((moddown_add_kernel<W, RESCALE>), grid, dim3(kBlock), 0, \
+                     stream, acc, d, d_stride, d_polys, out, count, limbs,     \
+                     moduli_id, t)
   if (t.alpha <= 1)
     FHE_MODDOWN(1);
   else if (t.alpha <= 2)
@@ -364,13 +370,23 @@ void launch_moddown_relin_rescale(const DeviceTables &t, const int64_t *acc,
                                   const int64_t *trio, int64_t *out, int count,
                                   int limbs, int moduli_id,
                                   hipStream_t stream) {
-  launch_moddown<true>(t, acc, trio, out, count, limbs, moduli_id, stream);
+  launch_moddown<true>(t, acc, trio, int64_t(3) * limbs << t.log_n, 2, out,
+                       count, limbs, moduli_id, stream);
 }
 
 void launch_moddown_relin(const DeviceTables &t, const int64_t *acc,
                           const int64_t *trio, int64_t *out, int count,
                           int limbs, int moduli_id, hipStream_t stream) {
-  launch_moddown<false>(t, acc, trio, out, count, limbs, moduli_id, stream);
+  launch_moddown<false>(t, acc, trio, int64_t(3) * limbs << t.log_n, 2, out,
+                        count, limbs, moduli_id, stream);
+}
+
+void launch_moddown_add(const DeviceTables &t, const int64_t *acc,
+                        const int64_t *d, int64_t d_stride, int d_polys,
+                        int64_t *out, int count, int limbs, int moduli_id,
+                        hipStream_t stream) {
+  launch_moddown<false>(t, acc, d, d_stride, d_polys, out, count, limbs,
+                        moduli_id, stream);
 }
 
 void launch_rescale(const DeviceTables &t, const int64_t *in, int64_t *out,

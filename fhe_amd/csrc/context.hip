@@ -2,6 +2,7 @@
 // declared in include/fhe_amd.h.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -34,6 +35,23 @@ void hip_check(hipError_t e, const char *what) {
 void require(bool ok, const std::string &message) {
   if (!ok)
     throw FheError(FHE_ERR_INVALID, message);
+}
+
+// Exponent g of the automorphism X -> X^g for a rotation by delta slots:
+// 5^-delta mod 2N, or -1 for conjugation (delta = -1 mod 2N); the
+// reference's get_jump (src/ckks/encode.cpp:513-526).
+uint32_t automorphism_exponent(int log_n, int delta) {
+  const int64_t two_n = int64_t(2) << log_n;
+  const int64_t shift = ((int64_t(delta) % two_n) + two_n) % two_n;
+  if (shift == two_n - 1)
+    return uint32_t(shift);
+  int64_t g = 1, b = 5;
+  for (int64_t e = two_n - shift; e > 0; e >>= 1) {
+    if (e & 1)
+      g = (g * b) & (two_n - 1);
+    b = (b * b) & (two_n - 1);
+  }
+  return uint32_t(g);
 }
 
 template <class F> int guarded(F &&f) {
@@ -77,6 +95,17 @@ struct fhe_context {
   // growable scratch for the fused pipelines
   void *workspace = nullptr;
   size_t workspace_bytes = 0;
+  // stage timing: HIP events recorded on `stream` around each stage while
+  // enabled, folded into per-stage totals when read
+  bool timing = false;
+  struct PendingStage {
+    const char *name;
+    hipEvent_t start, stop;
+  };
+  std::vector<PendingStage> pending;
+  std::vector<std::string> stage_names;
+  std::vector<double> stage_ms;
+  std::vector<int64_t> stage_launches;
 
   explicit fhe_context(fhe::CkksParams p) : params(std::move(p)) {
     hip_check(hipGetDevice(&device), "hipGetDevice");
@@ -101,6 +130,10 @@ struct fhe_context {
   ~fhe_context() {
     if (stream)
       hipStreamSynchronize(stream);
+    for (auto &st : pending) {
+      hipEventDestroy(st.start);
+      hipEventDestroy(st.stop);
+    }
     for (auto &kv : modup_cache)
       hipFree(kv.second);
     for (void *p : owned)
@@ -112,6 +145,43 @@ struct fhe_context {
   }
 
   int64_t n() const { return params.n; }
+
+  // Runs f (kernel launches on `stream`), bracketed by events when timing.
+  template <class F> void stage(const char *name, F &&f) {
+    if (!timing) {
+      f();
+      return;
+    }
+    PendingStage st{name, nullptr, nullptr};
+    hip_check(hipEventCreate(&st.start), "hipEventCreate");
+    hip_check(hipEventCreate(&st.stop), "hipEventCreate");
+    pending.push_back(st);
+    hip_check(hipEventRecord(st.start, stream), "hipEventRecord");
+    f();
+    hip_check(hipEventRecord(st.stop, stream), "hipEventRecord");
+  }
+
+  void fold_timings() {
+    hip_check(hipStreamSynchronize(stream), "timing sync");
+    for (auto &st : pending) {
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, st.start, st.stop),
+                "hipEventElapsedTime");
+      size_t i = 0;
+      while (i < stage_names.size() && stage_names[i] != st.name)
+        i++;
+      if (i == stage_names.size()) {
+        stage_names.push_back(st.name);
+        stage_ms.push_back(0);
+        stage_launches.push_back(0);
+      }
+      stage_ms[i] += ms;
+      stage_launches[i]++;
+      hipEventDestroy(st.start);
+      hipEventDestroy(st.stop);
+    }
+    pending.clear();
+  }
 
   // extended polynomials held at once by key switching: every partition on
   // the fused path, one at a time otherwise
@@ -169,11 +239,11 @@ struct fhe_context {
                               "level.");
   }
 
-  // Key switching of the INTT-form third polynomial of each trio in ws_trio
-  // into acc ([count][2][ext][N], INTT form, reduced).
+  // Key switching of the INTT-form polynomial src + c * src_stride of each
+  // ciphertext c into acc ([count][2][ext][N], INTT form, reduced).
   void key_switch(int count, int moduli_id, const int64_t *key,
-                  int key_moduli_id, int64_t *ws_trio, int64_t *ext,
-                  int64_t *acc) {
+                  int key_moduli_id, const int64_t *src, int64_t src_stride,
+                  int64_t *ext, int64_t *acc) {
     const int limbs = params.limbs_at(moduli_id);
     const int ext_limbs = limbs + params.alpha;
     const int64_t N = n();
@@ -186,27 +256,37 @@ struct fhe_context {
       // ext holds every partition: [count][parts][ext_limbs][N]
       const int np = int(parts.size());
       for (int p = 0; p < np; p++)
-        fhe::launch_modup(tables, ws_trio + 2 * limbs * N, 3 * limbs * N,
-                          ext + int64_t(p) * ext_limbs * N,
-                          int64_t(np) * ext_limbs * N, count, limbs, moduli_id,
-                          parts[p].start, parts[p].size, cst + p * stride,
-                          params.modup_non_negative(moduli_id, parts[p]),
-                          stream);
-      fhe::ntt_forward_first(tables, ext, ext, int64_t(count) * np * ext_limbs,
-                             ext_limbs, moduli_id, nullptr, stream);
-      fhe::launch_ks_inner_product_fused(tables, ext, key, key_chain,
-                                         key_offset, parts[0].key_index, np,
-                                         acc, count, ext_limbs, moduli_id,
-                                         stream);
-      fhe::ntt_inverse_second(tables, acc, acc, int64_t(count) * 2 * ext_limbs,
-                              ext_limbs, moduli_id, fhe::kInttExitReduce,
-                              stream);
+        stage("modup", [&] {
+          fhe::launch_modup(tables, src, src_stride,
+                            ext + int64_t(p) * ext_limbs * N,
+                            int64_t(np) * ext_limbs * N, count, limbs,
+                            moduli_id, parts[p].start, parts[p].size,
+                            cst + p * stride,
+                            params.modup_non_negative(moduli_id, parts[p]),
+                            stream);
+        });
+      stage("ntt_pass_high(ext)", [&] {
+        fhe::ntt_forward_first(tables, ext, ext,
+                               int64_t(count) * np * ext_limbs, ext_limbs,
+                               moduli_id, nullptr, stream);
+      });
+      stage("ks_inner_product_r16", [&] {
+        fhe::launch_ks_inner_product_fused(tables, ext, key, key_chain,
+                                           key_offset, parts[0].key_index, np,
+                                           acc, count, ext_limbs, moduli_id,
+                                           stream);
+      });
+      stage("intt_pass_high(acc)", [&] {
+        fhe::ntt_inverse_second(tables, acc, acc,
+                                int64_t(count) * 2 * ext_limbs, ext_limbs,
+                                moduli_id, fhe::kInttExitReduce, stream);
+      });
       return;
     }
     for (size_t p = 0; p < parts.size(); p++) {
       const auto &part = parts[p];
-      fhe::launch_modup(tables, ws_trio + 2 * limbs * N, 3 * limbs * N, ext,
-                        ext_limbs * N, count, limbs, moduli_id, part.start,
+      fhe::launch_modup(tables, src, src_stride, ext, ext_limbs * N, count,
+                        limbs, moduli_id, part.start,
                         part.size, cst + p * stride,
                         params.modup_non_negative(moduli_id, part), stream);
       fhe::ntt_forward(tables, ext, ext, int64_t(count) * ext_limbs,
@@ -450,11 +530,16 @@ int fhe_relinearize(fhe_context *ctx, int count, int moduli_id,
     const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
     int64_t *ws = ctx->scratch(size_t(trio_l + ext_l + acc_l) * N * 8);
     int64_t *trio = ws, *ext = trio + trio_l * N, *acc = ext + ext_l * N;
-    fhe::ntt_inverse(ctx->tables, trios, trio, trio_l, limbs, moduli_id,
-                     fhe::kInttExit, ctx->stream);
-    ctx->key_switch(count, moduli_id, key, key_moduli_id, trio, ext, acc);
-    fhe::launch_moddown_relin(ctx->tables, acc, trio, out, count, limbs,
-                              moduli_id, ctx->stream);
+    ctx->stage("intt(trio)", [&] {
+      fhe::ntt_inverse(ctx->tables, trios, trio, trio_l, limbs, moduli_id,
+                       fhe::kInttExit, ctx->stream);
+    });
+    ctx->key_switch(count, moduli_id, key, key_moduli_id, trio + 2 * limbs * N,
+                    3 * limbs * N, ext, acc);
+    ctx->stage("moddown_relin", [&] {
+      fhe::launch_moddown_relin(ctx->tables, acc, trio, out, count, limbs,
+                                moduli_id, ctx->stream);
+    });
     ctx->check_launch("Relinearize");
   });
 }
@@ -482,19 +567,156 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
     int64_t *an = ws, *bn = an + duo_l * N, *trio = bn + duo_l * N;
     int64_t *ext = trio + trio_l * N, *acc = ext + ext_l * N;
     // prepare_for_multiply: to NTT form (src/ckks/engine_multiply.cpp:13-27)
-    fhe::ntt_forward(ctx->tables, a, an, duo_l, limbs, moduli_id,
-                     ctx->tables.r2, ctx->stream);
-    fhe::ntt_forward(ctx->tables, b, bn, duo_l, limbs, moduli_id,
-                     ctx->tables.r2, ctx->stream);
-    fhe::launch_tensor(ctx->tables, an, bn, trio, count, limbs, moduli_id,
-                       ctx->stream);
+    ctx->stage("ntt(inputs)", [&] {
+      fhe::ntt_forward(ctx->tables, a, an, duo_l, limbs, moduli_id,
+                       ctx->tables.r2, ctx->stream);
+      fhe::ntt_forward(ctx->tables, b, bn, duo_l, limbs, moduli_id,
+                       ctx->tables.r2, ctx->stream);
+    });
+    ctx->stage("tensor", [&] {
+      fhe::launch_tensor(ctx->tables, an, bn, trio, count, limbs, moduli_id,
+                         ctx->stream);
+    });
     // relinearize: to_intt of all three, key switch of the third
-    fhe::ntt_inverse(ctx->tables, trio, trio, trio_l, limbs, moduli_id,
-                     fhe::kInttExit, ctx->stream);
-    ctx->key_switch(count, moduli_id, key, key_moduli_id, trio, ext, acc);
-    fhe::launch_moddown_relin_rescale(ctx->tables, acc, trio, out, count,
-                                      limbs, moduli_id, ctx->stream);
+    ctx->stage("intt(trio)", [&] {
+      fhe::ntt_inverse(ctx->tables, trio, trio, trio_l, limbs, moduli_id,
+                       fhe::kInttExit, ctx->stream);
+    });
+    ctx->key_switch(count, moduli_id, key, key_moduli_id, trio + 2 * limbs * N,
+                    3 * limbs * N, ext, acc);
+    ctx->stage("moddown_relin_rescale", [&] {
+      fhe::launch_moddown_relin_rescale(ctx->tables, acc, trio, out, count,
+                                        limbs, moduli_id, ctx->stream);
+    });
     ctx->check_launch("Multiply");
+  });
+}
+
+int fhe_profile_enable(fhe_context *ctx, int on) {
+  return guarded([&] {
+    require(ctx != nullptr, "null argument");
+    ctx->fold_timings();
+    if (on) {
+      ctx->stage_names.clear();
+      ctx->stage_ms.clear();
+      ctx->stage_launches.clear();
+    }
+    ctx->timing = on != 0;
+  });
+}
+
+int fhe_profile_entry(fhe_context *ctx, int index, char *name,
+                      int name_capacity, double *total_ms,
+                      int64_t *launches) {
+  return guarded([&] {
+    require(ctx && name && total_ms && launches && name_capacity > 0,
+            "null argument");
+    ctx->fold_timings();
+    require(index >= 0 && index < int(ctx->stage_names.size()),
+            "profile: index out of range");
+    std::snprintf(name, size_t(name_capacity), "%s",
+                  ctx->stage_names[index].c_str());
+    *total_ms = ctx->stage_ms[index];
+    *launches = ctx->stage_launches[index];
+  });
+}
+
+int fhe_profile_count(fhe_context *ctx, int *count) {
+  return guarded([&] {
+    require(ctx && count, "null argument");
+    ctx->fold_timings();
+    *count = int(ctx->stage_names.size());
+  });
+}
+
+int fhe_automorphism(fhe_context *ctx, const int64_t *in, int64_t *out,
+                     int polys, int limbs, int moduli_id, int delta,
+                     int ntt_form) {
+  return guarded([&] {
+    require(ctx && in && out, "null argument");
+    require(polys > 0 && limbs > 0, "Rotate: empty shape");
+    require(moduli_id >= 0 && moduli_id + limbs <= ctx->params.prime_count,
+            "Rotate: limbs exceed the prime chain");
+    require(in != out, "Rotate: the output must not alias the input");
+    const uint32_t g = automorphism_exponent(ctx->params.log_n, delta);
+    if (ntt_form)
+      fhe::launch_ntt_automorphism(ctx->tables, in, out,
+                                   int64_t(polys) * limbs, g, ctx->stream);
+    else
+      fhe::launch_automorphism(ctx->tables, in, out, int64_t(polys) * limbs,
+                               limbs, moduli_id, g, ctx->stream);
+    ctx->check_launch("Rotate");
+  });
+}
+
+int fhe_rotate(fhe_context *ctx, int count, int moduli_id, const int64_t *in,
+               const int64_t *key, int key_moduli_id, int delta,
+               int64_t *out) {
+  return guarded([&] {
+    require(ctx && in && key && out, "null argument");
+    require(count > 0, "Rotate: empty batch");
+    ctx->check_moduli(moduli_id, 1, "Rotate");
+    ctx->check_key(moduli_id, key_moduli_id, "Rotate");
+    const auto &P = ctx->params;
+    const int limbs = P.limbs_at(moduli_id);
+    const int ext_limbs = limbs + P.alpha;
+    const int64_t N = ctx->n();
+    const int64_t duo_l = int64_t(count) * 2 * limbs;
+    const int64_t ext_l =
+        int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
+    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
+    int64_t *ws = ctx->scratch(size_t(duo_l + ext_l + acc_l) * N * 8);
+    int64_t *sw = ws, *ext = sw + duo_l * N, *acc = ext + ext_l * N;
+    // key switch of c1; c0 joins after ModDown, where the reference adds
+    // c0 * P before it (src/ckks/core_key_switch.cpp:509-548): both give
+    // floor((X + c0 P) / P) = floor(X / P) + c0
+    ctx->key_switch(count, moduli_id, key, key_moduli_id, in + limbs * N,
+                    2 * limbs * N, ext, acc);
+    ctx->stage("moddown_add", [&] {
+      fhe::launch_moddown_add(ctx->tables, acc, in, 2 * limbs * N, 1, sw,
+                              count, limbs, moduli_id, ctx->stream);
+    });
+    ctx->stage("automorphism", [&] {
+      fhe::launch_automorphism(ctx->tables, sw, out, duo_l, limbs, moduli_id,
+                               automorphism_exponent(P.log_n, delta),
+                               ctx->stream);
+    });
+    ctx->check_launch("Rotate");
+  });
+}
+
+int fhe_conjugate(fhe_context *ctx, int count, int moduli_id,
+                  const int64_t *in, const int64_t *key, int key_moduli_id,
+                  int64_t *out) {
+  return guarded([&] {
+    require(ctx && in && key && out, "null argument");
+    require(count > 0, "Conjugate: empty batch");
+    ctx->check_moduli(moduli_id, 1, "Conjugate");
+    ctx->check_key(moduli_id, key_moduli_id, "Conjugate");
+    const auto &P = ctx->params;
+    const int limbs = P.limbs_at(moduli_id);
+    const int ext_limbs = limbs + P.alpha;
+    const int64_t N = ctx->n();
+    const int64_t duo_l = int64_t(count) * 2 * limbs;
+    const int64_t ext_l =
+        int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
+    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
+    int64_t *ws = ctx->scratch(size_t(duo_l + ext_l + acc_l) * N * 8);
+    int64_t *perm = ws, *ext = perm + duo_l * N, *acc = ext + ext_l * N;
+    // CKKSCore::conjugate (src/ckks/core.cpp:1035-1056): automorphism
+    // first, then key switching of the permuted c1 plus the permuted c0
+    ctx->stage("automorphism", [&] {
+      fhe::launch_automorphism(ctx->tables, in, perm, duo_l, limbs, moduli_id,
+                               automorphism_exponent(P.log_n, -1),
+                               ctx->stream);
+    });
+    ctx->key_switch(count, moduli_id, key, key_moduli_id, perm + limbs * N,
+                    2 * limbs * N, ext, acc);
+    ctx->stage("moddown_add", [&] {
+      fhe::launch_moddown_add(ctx->tables, acc, perm, 2 * limbs * N, 1, out,
+                              count, limbs, moduli_id, ctx->stream);
+    });
+    ctx->check_launch("Conjugate");
   });
 }
 

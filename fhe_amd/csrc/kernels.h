@@ -89,6 +89,31 @@ void launch_moddown_relin(const DeviceTables &t, const int64_t *acc,
                           const int64_t *trio, int64_t *out, int count,
                           int limbs, int moduli_id, hipStream_t stream);
 
+// ModDown of both halves plus an optional addend: out[c][h] =
+// guard(d_h + ModDown(acc_h)) where d_h = d + c * d_stride + h * limbs * N
+// for h < d_polys and zero otherwise (the c0 term of a rotation,
+// src/ckks/core_key_switch.cpp:509-548, and the copy of the second half in
+// src/ckks/core.cpp:1035-1056).
+void launch_moddown_add(const DeviceTables &t, const int64_t *acc,
+                        const int64_t *d, int64_t d_stride, int d_polys,
+                        int64_t *out, int count, int limbs, int moduli_id,
+                        hipStream_t stream);
+
+// Coefficient-domain automorphism X -> X^g of total_limbs INTT-form limbs
+// (g odd, below 2N), as the reference's rotate_ with its permutation and
+// sign (src/ckks/core.cpp:845-869, src/ckks/encode.cpp:513-526):
+// out[(g i mod 2N) mod N] = +-in[i], then make_unsigned_reduce_2q_to_q.
+void launch_automorphism(const DeviceTables &t, const int64_t *in,
+                         int64_t *out, int64_t total_limbs, int limbs,
+                         int moduli_id, uint32_t g, hipStream_t stream);
+
+// NTT-domain automorphism: the same map on the evaluation points of NTT-form
+// limbs, a pure gather out[i] = in[perm(i)] (src/ckks/core.cpp:886-910,
+// src/ckks/encode.cpp:528-557).
+void launch_ntt_automorphism(const DeviceTables &t, const int64_t *in,
+                             int64_t *out, int64_t total_limbs, uint32_t g,
+                             hipStream_t stream);
+
 // Rescale of `polys` INTT-form polynomials (src/ckks/core_rescale.cpp:5-25).
 void launch_rescale(const DeviceTables &t, const int64_t *in, int64_t *out,
                     int polys, int limbs, int moduli_id, hipStream_t stream);

@@ -91,13 +91,11 @@ __global__ __launch_bounds__(kBlock) void ks_inner_product_r16_kernel(
     } else {
 #pragma unroll
       for (int m = 0; m < 16; m++) {
-        acc0[m] = guard_fast(
-            guard_fast(add64(mont_mult_u(v[m], k0[m], f.q, f.k), acc0[m]),
-                       f.q2),
+        acc0[m] = guard_u(
+            guard_u(add64(mont_mult_u(v[m], k0[m], f.q, f.k), acc0[m]), f.q2),
             qq);
-        acc1[m] = guard_fast(
-            guard_fast(add64(mont_mult_u(v[m], k1[m], f.q, f.k), acc1[m]),
-                       f.q2),
+        acc1[m] = guard_u(
+            guard_u(add64(mont_mult_u(v[m], k1[m], f.q, f.k), acc1[m]), f.q2),
             qq);
       }
     }

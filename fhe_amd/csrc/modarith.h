@@ -142,6 +142,14 @@ FHE_HD int64_t guard_fast(int64_t x, int64_t q) {
   return int32_t(uint64_t(d) >> 32) < 0 ? x : d;
 }
 
+// guard() for NON-NEGATIVE x and q: the borrow out of the unsigned x - q is
+// the condition x < q, so the device sequence is sub, subb and two selects.
+FHE_HD int64_t guard_u(int64_t x, int64_t q) {
+  uint64_t d;
+  const bool borrow = __builtin_sub_overflow(uint64_t(x), uint64_t(q), &d);
+  return borrow ? x : int64_t(d);
+}
+
 // Montgomery product of arbitrary int64 operands: the 128-bit product of the
 // sign-extended values, i.e. the unsigned product corrected in its high word.
 FHE_HD int64_t mont_mult(int64_t a, int64_t b, uint64_t q, uint64_t k) {

@@ -16,14 +16,16 @@
 
 namespace fhe {
 
+// Operands are lazy residues in [0, 2q), so every guard sees a non-negative
+// value and takes the borrow form.
 // Cooley-Tukey: p = mont(w, y); x' = guard(x + p, 2q); y' = guard(x + 2q - p, 2q)
 __device__ __forceinline__ void ct_butterfly(int64_t &x, int64_t &y, int64_t w,
                                              int64_t q2, uint64_t q,
                                              uint64_t k) {
   const int64_t p = mont_mult_u(w, y, q, k);
   const int64_t x0 = x;
-  x = guard_fast(add64(x0, p), q2);
-  y = guard_fast(sub64(add64(x0, q2), p), q2);
+  x = guard_u(add64(x0, p), q2);
+  y = guard_u(sub64(add64(x0, q2), p), q2);
 }
 
 // Gentleman-Sande: x' = guard(x + y, 2q); y' = mont(w, guard(x + 2q - y, 2q))
@@ -31,8 +33,8 @@ __device__ __forceinline__ void gs_butterfly(int64_t &x, int64_t &y, int64_t w,
                                              int64_t q2, uint64_t q,
                                              uint64_t k) {
   const int64_t x0 = x, y0 = y;
-  x = guard_fast(add64(x0, y0), q2);
-  y = mont_mult_u(w, guard_fast(sub64(add64(x0, q2), y0), q2), q, k);
+  x = guard_u(add64(x0, y0), q2);
+  y = mont_mult_u(w, guard_u(sub64(add64(x0, q2), y0), q2), q, k);
 }
 
 // Everything a thread needs to address twiddles in one 8-bit pass: the pass

@@ -134,6 +134,45 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
                             const int64_t *key, int key_moduli_id,
                             int64_t *out);
 
+/* Automorphism X -> X^g, g = 5^-delta mod 2N (g = -1 for delta = -1, the
+ * conjugation), of polys x limbs limbs at moduli_id into out (no aliasing).
+ * ntt_form = 0: coefficient-domain permutation with signs and
+ * make_unsigned_reduce_2q_to_q (CKKSCore::rotate_intt_form,
+ * src/ckks/core.cpp:871-884, tables src/ckks/encode.cpp:513-526);
+ * ntt_form = 1: the slot permutation of NTT-form data
+ * (CKKSCore::rotate_ntt_form, src/ckks/core.cpp:886-914, table
+ * src/ckks/encode.cpp:528-557). */
+int fhe_automorphism(fhe_context *ctx, const int64_t *in, int64_t *out,
+                     int polys, int limbs, int moduli_id, int delta,
+                     int ntt_form);
+
+/* Rotation of `count` INTT-form duos [count][2][limbs][N] at moduli_id by
+ * the fixed rotation key made for `delta` ([key_polys][key_limbs][N] at
+ * key_moduli_id, device) into INTT-form duos at moduli_id
+ * (CKKSEngine::rotate(UnitCiphertext*, FixedRotationKey*),
+ * src/ckks/engine_rotate.cpp:55-76; CKKSCore::rotate,
+ * src/ckks/core.cpp:938-962).  out may alias in. */
+int fhe_rotate(fhe_context *ctx, int count, int moduli_id, const int64_t *in,
+               const int64_t *key, int key_moduli_id, int delta,
+               int64_t *out);
+
+/* Conjugation of `count` INTT-form duos with a conjugation key
+ * (CKKSEngine::conjugate, src/ckks/engine_conjugate.cpp:53-73;
+ * CKKSCore::conjugate, src/ckks/core.cpp:1035-1056).  out may alias in. */
+int fhe_conjugate(fhe_context *ctx, int count, int moduli_id,
+                  const int64_t *in, const int64_t *key, int key_moduli_id,
+                  int64_t *out);
+
+/* Stage timing (no counterpart in the reference, which has no device
+ * stages): while enabled, every pipeline stage that a call launches on the
+ * context's stream is bracketed by HIP events; fhe_profile_count / _entry
+ * synchronize the stream and report each stage's name, total milliseconds
+ * and launch count since the last fhe_profile_enable(ctx, 1). */
+int fhe_profile_enable(fhe_context *ctx, int on);
+int fhe_profile_count(fhe_context *ctx, int *count);
+int fhe_profile_entry(fhe_context *ctx, int index, char *name,
+                      int name_capacity, double *total_ms, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -610,3 +610,86 @@ void oracle_hmult_relin_rescale(const oracle_ctx *c, int mid, const int64_t *a,
   free(trio);
   free(rel);
 }
+
+/* src/ckks/encode.cpp:44-56: 5^(2N - shift) mod 2N, or shift itself for the
+ * conjugation shift 2N - 1 */
+static int64_t automorphism_jump(int n, int delta) {
+  const int64_t dsize = 2 * (int64_t)n;
+  const int64_t shift = ((int64_t)delta + dsize) % dsize;
+  if (shift == dsize - 1)
+    return shift;
+  int64_t jump = 1;
+  for (int64_t p = 0; p < dsize - shift; p++) {
+    jump *= 5;
+    jump &= dsize - 1;
+  }
+  return jump;
+}
+
+/* CKKSCore::rotate_ (src/ckks/core.cpp:845-869) with the permutation and
+ * sign of src/ckks/encode.cpp:513-526: y[(j i mod 2N) mod N] = +-x[i], then
+ * make_unsigned_reduce_2q_to_q */
+void oracle_automorphism(const oracle_ctx *c, int mid, int limbs, int polys,
+                         int delta, const int64_t *x, int64_t *y) {
+  const int n = c->n;
+  const int64_t dsize = 2 * (int64_t)n;
+  const int64_t jump = automorphism_jump(n, delta);
+  for (int p = 0; p < polys; p++)
+    for (int l = 0; l < limbs; l++) {
+      const int64_t q = (int64_t)c->q[mid + l];
+      const int64_t *xl = x + ((size_t)p * limbs + l) * n;
+      int64_t *yl = y + ((size_t)p * limbs + l) * n;
+      for (int i = 0; i < n; i++) {
+        const int64_t pi = jump * i % dsize;
+        int64_t v = ((pi / n) & 1) ? -xl[i] : xl[i];
+        yl[pi % n] = v < 0 ? v + q : guard(v, q);
+      }
+    }
+}
+
+/* key switching of x followed by ModDown and the add of d (INTT forms):
+ * out0 = reduce(d + ModDown(ks0)), out1 = ModDown(ks1) */
+static void switch_and_add(const oracle_ctx *c, int mid, const int64_t *x,
+                           const int64_t *d, const int64_t *key, int key_mid,
+                           int64_t *out) {
+  const int n = c->n;
+  const int chain = c->chain_count - mid;
+  oracle_key_switch(c, mid, x, key, key_mid, out);
+  for (int l = 0; l < chain; l++) {
+    const int64_t q = (int64_t)c->q[mid + l], q2 = (int64_t)c->q2[mid + l];
+    int64_t *y = out + (size_t)l * n;
+    const int64_t *dl = d + (size_t)l * n;
+    for (int t = 0; t < n; t++)
+      y[t] = guard(guard(y[t] + dl[t], q2), q);
+  }
+}
+
+/* CKKSCore::rotate (src/ckks/core.cpp:954-962): decompose c1, key switcher
+ * with b*P added in the NTT domain (src/ckks/core_key_switch.cpp:509-548),
+ * ModDown (src/ckks/core.cpp:916-936), then the automorphism.  The b*P term
+ * divided by P is exactly b, so it is added after ModDown here; the
+ * canonical outputs are identical. */
+void oracle_rotate(const oracle_ctx *c, int mid, const int64_t *ct,
+                   const int64_t *key, int key_mid, int delta, int64_t *out) {
+  const int n = c->n;
+  const int chain = c->chain_count - mid;
+  const size_t poly = (size_t)chain * n;
+  int64_t *sw = malloc(sizeof(int64_t) * 2 * poly);
+  switch_and_add(c, mid, ct + poly, ct, key, key_mid, sw);
+  oracle_automorphism(c, mid, chain, 2, delta, sw, out);
+  free(sw);
+}
+
+/* CKKSCore::conjugate (src/ckks/core.cpp:1035-1056): automorphism with
+ * delta = -1 first, then key switching of the permuted c1 and the add of
+ * the permuted c0. */
+void oracle_conjugate(const oracle_ctx *c, int mid, const int64_t *ct,
+                      const int64_t *key, int key_mid, int64_t *out) {
+  const int n = c->n;
+  const int chain = c->chain_count - mid;
+  const size_t poly = (size_t)chain * n;
+  int64_t *perm = malloc(sizeof(int64_t) * 2 * poly);
+  oracle_automorphism(c, mid, chain, 2, -1, ct, perm);
+  switch_and_add(c, mid, perm + poly, perm, key, key_mid, out);
+  free(perm);
+}

@@ -84,6 +84,19 @@ void oracle_hmult_relin_rescale(const oracle_ctx *ctx, int moduli_id,
                                 const int64_t *key, int key_moduli_id,
                                 int64_t *out);
 
+/* coefficient-domain automorphism X -> X^(5^-delta) (delta = -1:
+ * conjugation) of polys x limbs INTT-form limbs */
+void oracle_automorphism(const oracle_ctx *ctx, int moduli_id, int limbs,
+                         int polys, int delta, const int64_t *x, int64_t *y);
+
+/* rotation of an INTT-form duo with a fixed rotation key made for `delta`,
+ * and conjugation with a conjugation key; outputs INTT-form duos */
+void oracle_rotate(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
+                   const int64_t *key, int key_moduli_id, int delta,
+                   int64_t *out);
+void oracle_conjugate(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
+                      const int64_t *key, int key_moduli_id, int64_t *out);
+
 /* partition layout for a ciphertext at moduli_id against a key at
  * key_moduli_id: returns partition count; starts/sizes/key_index filled */
 int oracle_partitions(const oracle_ctx *ctx, int moduli_id, int key_moduli_id,

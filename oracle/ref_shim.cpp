@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "ckks/config.h"
+#include "ckks/encode.h"
 #include "ckks/engine.h"
 #include "ckks/precomputed.h"
 #include "engine/engine.h"
@@ -294,6 +295,84 @@ void *ref_rescale(void *handle, void *ct) {
   return guarded([&]() -> void * {
     auto ref = static_cast<RefEngine *>(handle);
     return ref->engine->rescale(static_cast<UnitCiphertext *>(ct));
+  });
+}
+
+// Rotation / conjugation keys (src/ckks/engine.cpp:82-124) and their
+// KeySwitchingKey limbs.
+void *ref_fixed_rotation_key(void *handle, void *sk, int delta) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->create_fixed_rotation_key(
+        static_cast<SecretKey *>(sk), delta);
+  });
+}
+void *ref_conjugation_key(void *handle, void *sk) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->create_conjugation_key(
+        static_cast<SecretKey *>(sk));
+  });
+}
+void ref_free_fixed_rotation_key(void *p) {
+  delete static_cast<FixedRotationKey *>(p);
+}
+void ref_free_conjugation_key(void *p) {
+  delete static_cast<ConjugationKey *>(p);
+}
+int ref_fixed_rotation_key_delta(void *frk) {
+  return static_cast<FixedRotationKey *>(frk)->delta_;
+}
+static KeySwitchingKey *ksk_of(void *key, int kind) {
+  if (kind == 0)
+    return static_cast<RelinearizationKey *>(key)->key_switching_key_;
+  if (kind == 1)
+    return static_cast<FixedRotationKey *>(key)->key_switching_key_;
+  return static_cast<ConjugationKey *>(key)->key_switching_key_;
+}
+// kind: 0 relinearization, 1 fixed rotation, 2 conjugation
+int ref_key_shape(void *key, int kind, int64_t *shape) {
+  data_shape(ksk_of(key, kind), shape);
+  return 0;
+}
+int ref_key_get(void *key, int kind, int64_t *out) {
+  copy_out(ksk_of(key, kind), out);
+  return 0;
+}
+int ref_key_set(void *key, int kind, const int64_t *in) {
+  copy_in(ksk_of(key, kind), in);
+  return 0;
+}
+
+// CKKSEngine::rotate(ct, FixedRotationKey) (src/ckks/engine_rotate.cpp:78-101)
+void *ref_rotate(void *handle, void *ct, void *frk) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    return ref->engine->rotate(static_cast<UnitCiphertext *>(ct),
+                               static_cast<FixedRotationKey *>(frk));
+  });
+}
+
+// CKKSEngine::conjugate(ct, ConjugationKey) (src/ckks/engine_conjugate.cpp:53-73)
+void *ref_conjugate(void *handle, void *ct, void *cjk) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    return ref->engine->conjugate(static_cast<UnitCiphertext *>(ct),
+                                  static_cast<ConjugationKey *>(cjk));
+  });
+}
+
+// Automorphism tables of src/ckks/encode.cpp:513-557 for N coefficients.
+int ref_permutation_and_sign(int n, int delta, int32_t *permutation,
+                             int32_t *sign) {
+  return guarded([&]() -> int {
+    get_permutation_and_sign(permutation, sign, n, delta);
+    return 0;
+  });
+}
+
+int ref_ntt_permutation(int n, int delta, int32_t *permutation) {
+  return guarded([&]() -> int {
+    get_ntt_permutation(permutation, n, delta);
+    return 0;
   });
 }
 

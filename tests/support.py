@@ -82,6 +82,10 @@ class Oracle:
                                                        VP]
             lib.oracle_partitions.argtypes = [VP, ctypes.c_int, ctypes.c_int,
                                               VP, VP, VP]
+            i32 = ctypes.c_int
+            lib.oracle_automorphism.argtypes = [VP, i32, i32, i32, i32, VP, VP]
+            lib.oracle_rotate.argtypes = [VP, i32, VP, VP, i32, i32, VP]
+            lib.oracle_conjugate.argtypes = [VP, i32, VP, VP, i32, VP]
             cls._lib = lib
         return cls._lib
 
@@ -182,6 +186,29 @@ class Oracle:
             key_mid, ptr(out))
         return out
 
+    def automorphism(self, x, mid, delta):
+        """Coefficient-domain automorphism of [polys][limbs][N] limbs."""
+        x = np.ascontiguousarray(x, dtype=np.int64)
+        polys, limbs = _pl(x)
+        out = np.empty_like(x)
+        self.lib().oracle_automorphism(self.h, mid, limbs, polys, delta,
+                                       ptr(x), ptr(out))
+        return out
+
+    def rotate(self, ct, key, mid, key_mid, delta):
+        out = np.zeros((2, self.limbs_at(mid), self.n), np.int64)
+        self.lib().oracle_rotate(self.h, mid, ptr(np.ascontiguousarray(ct)),
+                                 ptr(np.ascontiguousarray(key)), key_mid,
+                                 delta, ptr(out))
+        return out
+
+    def conjugate(self, ct, key, mid, key_mid):
+        out = np.zeros((2, self.limbs_at(mid), self.n), np.int64)
+        self.lib().oracle_conjugate(self.h, mid, ptr(np.ascontiguousarray(ct)),
+                                    ptr(np.ascontiguousarray(key)), key_mid,
+                                    ptr(out))
+        return out
+
     def partitions(self, mid, key_mid):
         s = np.zeros(128, np.int32)
         z = np.zeros(128, np.int32)
@@ -189,6 +216,28 @@ class Oracle:
         c = self.lib().oracle_partitions(self.h, mid, key_mid, ptr(s), ptr(z),
                                          ptr(k))
         return [(int(s[i]), int(z[i]), int(k[i])) for i in range(c)]
+
+
+def automorphism_exponent(n, delta):
+    """g of X -> X^g for a rotation by delta: 5^-delta mod 2N, 2N - 1 for
+    the conjugation (delta = -1 mod 2N)."""
+    two_n = 2 * n
+    shift = delta % two_n
+    return two_n - 1 if shift == two_n - 1 else pow(5, two_n - shift, two_n)
+
+
+def ntt_permutation(n, delta):
+    """out[i] = in[perm[i]] for NTT-form data: slot i holds the value at
+    psi^(2 bitrev(i) + 1), so perm[i] = bitrev((g (2 bitrev(i) + 1) mod 2N
+    - 1) / 2)."""
+    bits = n.bit_length() - 1
+    idx = np.arange(n, dtype=np.int64)
+    rev = np.zeros(n, np.int64)
+    for b in range(bits):
+        rev |= ((idx >> b) & 1) << (bits - 1 - b)
+    g = automorphism_exponent(n, delta)
+    e = (g * (2 * rev + 1)) % (2 * n)
+    return rev[(e - 1) // 2]
 
 
 def _pl(y):
@@ -204,7 +253,8 @@ class Reference:
     def available():
         return os.path.exists(REF_LIB)
 
-    def __init__(self, log_n, alpha, bootstrap=0, threads=1):
+    @staticmethod
+    def load_lib():
         lib = ctypes.CDLL(REF_LIB)
         i32 = ctypes.c_int
         # (restype, argtypes) of every shim entry point used by the tests
@@ -234,12 +284,27 @@ class Reference:
             "ref_free_secret_key": (None, [VP]),
             "ref_ntt_limb": (i32, [VP, i32, VP, i32]),
             "ref_prime_constants": (i32, [VP, i32, VP, VP, VP]),
+            "ref_fixed_rotation_key": (VP, [VP, VP, i32]),
+            "ref_conjugation_key": (VP, [VP, VP]),
+            "ref_free_fixed_rotation_key": (None, [VP]),
+            "ref_free_conjugation_key": (None, [VP]),
+            "ref_key_shape": (i32, [VP, i32, VP]),
+            "ref_key_get": (i32, [VP, i32, VP]),
+            "ref_key_set": (i32, [VP, i32, VP]),
+            "ref_rotate": (VP, [VP, VP, VP]),
+            "ref_conjugate": (VP, [VP, VP, VP]),
+            "ref_permutation_and_sign": (i32, [i32, i32, VP, VP]),
+            "ref_ntt_permutation": (i32, [i32, i32, VP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
             fn.restype = res
             fn.argtypes = args
         lib.ref_last_error.restype = ctypes.c_char_p
+        return lib
+
+    def __init__(self, log_n, alpha, bootstrap=0, threads=1):
+        lib = self.load_lib()
         self.lib = lib
         self.e = lib.ref_engine_create(log_n, alpha, bootstrap, threads)
         if not self.e:

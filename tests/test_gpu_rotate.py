@@ -1,0 +1,228 @@
+"""GPU parity of rotations, conjugation and the automorphism kernels,
+bit-exact.
+
+  * every case of tests/golden/rotate_golden.json (digests of the reference
+    engine's own rotate / conjugate output limbs, Gold included), inputs
+    regenerated from the recorded seeds;
+  * batched launches equal one-ciphertext launches and the oracle;
+  * coefficient- and NTT-domain automorphisms equal the oracle and the
+    closed-form NTT permutation;
+  * with the compiled reference present (oracle/_ref), real rotation and
+    conjugation keys: limbs equal the reference's and decrypt to the rotated
+    / conjugated message.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import fhe_amd
+from support import (Oracle, Reference, golden, hmult_inputs,
+                     ntt_permutation, ptr, uniform_limbs)
+
+pytestmark = pytest.mark.gpu
+
+CASES = golden("rotate_golden.json")["cases"]
+
+
+def digest(arr):
+    return [[hashlib.sha256(np.ascontiguousarray(arr[p, l]).tobytes())
+             .hexdigest()[:32] for l in range(arr.shape[1])]
+            for p in range(arr.shape[0])]
+
+
+_contexts = {}
+
+
+def context_for(case):
+    key = (case["log_n"], case["special_prime_count"], tuple(case["primes"]))
+    if key not in _contexts:
+        _contexts[key] = fhe_amd.Context(case["log_n"],
+                                         case["special_prime_count"],
+                                         primes=case["primes"])
+    return _contexts[key]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _close_contexts():
+    yield
+    for ctx in _contexts.values():
+        ctx.close()
+    _contexts.clear()
+
+
+def case_inputs(case):
+    n = 1 << case["log_n"]
+    A, _, K = hmult_inputs(case["primes"], n, case["moduli_id"],
+                           case["limbs"], case["key_moduli_id"],
+                           case["key_polys"], case["key_chain"], case["seed"])
+    return (A.reshape(2, case["limbs"], n),
+            K.reshape(case["key_polys"], case["key_chain"], n))
+
+
+def run(ctx, case, cts, dk):
+    out = ctx.array(cts.shape)
+    if case["op"] == "rotate":
+        ctx.rotate(cts, dk, case["key_moduli_id"], case["delta"], out,
+                   case["moduli_id"])
+    else:
+        ctx.conjugate(cts, dk, case["key_moduli_id"], out, case["moduli_id"])
+    got = out.download()
+    out.free()
+    return got
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c["name"])
+def test_rotate_matches_reference_digests(case, gpu_lib):
+    ctx = context_for(case)
+    A, K = case_inputs(case)
+    assert ctx.key_shape(case["key_moduli_id"]) == K.shape
+    da, dk = ctx.to_device(A[None]), ctx.to_device(K)
+    got = run(ctx, case, da, dk)[0]
+    assert (da.download()[0] == A).all()  # input untouched
+    da.free()
+    dk.free()
+    assert digest(got) == case["out_sha256"]
+
+
+@pytest.mark.parametrize("name", ["iron_mid1_rot7", "silver_mid9_conj",
+                                  "gold_mid13_rot12345"])
+def test_batched_equals_single_and_oracle(name, gpu_lib):
+    case = [c for c in CASES if c["name"] == name][0]
+    ctx = context_for(case)
+    n, mid, limbs = ctx.n, case["moduli_id"], case["limbs"]
+    _, K = case_inputs(case)
+    rng = np.random.Generator(np.random.PCG64(77))
+    batch = 3
+    A = np.stack([np.stack([uniform_limbs(rng, case["primes"], mid, limbs, n)
+                            for _ in range(2)]) for _ in range(batch)])
+    dk, da = ctx.to_device(K), ctx.to_device(A)
+    batched = run(ctx, case, da, dk)
+    for c in range(batch):
+        sa = ctx.to_device(A[c:c + 1])
+        assert (run(ctx, case, sa, dk)[0] == batched[c]).all(), c
+        sa.free()
+    if case["log_n"] <= 15:
+        orc = Oracle(case["log_n"], case["primes"],
+                     case["special_prime_count"])
+        for c in range(batch):
+            if case["op"] == "rotate":
+                ref = orc.rotate(A[c], K, mid, case["key_moduli_id"],
+                                 case["delta"])
+            else:
+                ref = orc.conjugate(A[c], K, mid, case["key_moduli_id"])
+            assert (batched[c] == ref).all(), c
+    # in place: out aliases in
+    if case["op"] == "rotate":
+        ctx.rotate(da, dk, case["key_moduli_id"], case["delta"], da, mid)
+    else:
+        ctx.conjugate(da, dk, case["key_moduli_id"], da, mid)
+    assert (da.download() == batched).all()
+    da.free()
+    dk.free()
+
+
+@pytest.mark.parametrize("log_n,delta", [(13, 1), (13, -1), (15, 3),
+                                         (16, 12345), (16, -1)])
+def test_automorphism_kernels(log_n, delta, gpu_lib):
+    case = [c for c in CASES if c["log_n"] == log_n][0]
+    ctx = context_for(case)
+    n, mid = ctx.n, 1
+    limbs = 4
+    rng = np.random.Generator(np.random.PCG64(log_n * 100 + delta % 97))
+    x = np.stack([uniform_limbs(rng, case["primes"], mid, limbs, n)
+                  for _ in range(3)])
+    x[0, 0, :5] = case["primes"][mid]  # lazy zeros
+    dx, dy = ctx.to_device(x), ctx.array(x.shape)
+    ctx.automorphism(dx, dy, mid, delta, ntt_form=False)
+    got = dy.download()
+    if log_n <= 15:
+        orc = Oracle(log_n, case["primes"], case["special_prime_count"])
+        assert (got == orc.automorphism(x, mid, delta)).all()
+    else:
+        from support import automorphism_exponent
+        g = automorphism_exponent(n, delta)
+        dst = (g * np.arange(n, dtype=np.int64)) % (2 * n)
+        q = np.array(case["primes"][mid:mid + limbs], np.int64)[None, :, None]
+        expect = np.empty_like(x)
+        signed = np.where(dst >= n, -x, x)
+        expect[..., dst % n] = np.where(signed < 0, signed + q,
+                                        np.where(signed < q, signed,
+                                                 signed - q))
+        assert (got == expect).all()
+    ctx.automorphism(dx, dy, mid, delta, ntt_form=True)
+    assert (dy.download() == x[..., ntt_permutation(n, delta)]).all()
+    dx.free()
+    dy.free()
+
+
+def test_invalid_calls_are_rejected(gpu_lib):
+    case = CASES[0]
+    ctx = context_for(case)
+    lib = fhe_amd.library()
+    d = ctx.array((1, 2, case["limbs"], ctx.n))
+    k = ctx.array(ctx.key_shape(case["key_moduli_id"]))
+    assert lib.fhe_rotate(ctx.handle, 0, 1, d.ptr, k.ptr, 1, 1, d.ptr) == 1
+    assert lib.fhe_rotate(ctx.handle, 1, 0, d.ptr, k.ptr, 1, 1, d.ptr) == 1
+    assert lib.fhe_conjugate(ctx.handle, 1, 1, d.ptr, None, 1, d.ptr) == 1
+    assert lib.fhe_automorphism(ctx.handle, d.ptr, d.ptr, 2, 2, 1, 1, 0) == 1
+    d.free()
+    k.free()
+
+
+@pytest.mark.reference
+@pytest.mark.skipif(not Reference.available(),
+                    reason="oracle/_ref (compiled reference) not built")
+@pytest.mark.parametrize("log_n,alpha,level", [(14, 1, 5), (16, 4, 24)])
+def test_real_keys_match_reference_and_decrypt(log_n, alpha, level, gpu_lib):
+    ref = Reference(log_n, alpha, threads=8)
+    lib, e = ref.lib, ref.e
+    sk = lib.ref_secret_key(e)
+    pk = lib.ref_public_key(e, sk)
+    slots = ref.slot_count
+    x = np.random.default_rng(7).uniform(-1, 1, slots)
+    ct0 = lib.ref_encrypt(e, ptr(x), slots, pk, level)
+    ct = lib.ref_rescale(e, ct0)
+    mid = ref.shape(ct)[0]
+    ctx = fhe_amd.Context(log_n, alpha, primes=ref.primes)
+    dct = ctx.to_device(ref.get(ct)[None])
+    for delta in (1, 5, -1):
+        if delta == -1:
+            key_h, kind = lib.ref_conjugation_key(e, sk), 2
+            out_h = lib.ref_conjugate(e, ct, key_h)
+            expect_msg = x
+        else:
+            key_h, kind = lib.ref_fixed_rotation_key(e, sk, delta), 1
+            out_h = lib.ref_rotate(e, ct, key_h)
+            expect_msg = np.roll(x, delta)
+        shape = np.zeros(4, np.int64)
+        lib.ref_key_shape(key_h, kind, ptr(shape))
+        key = np.zeros(tuple(int(v) for v in shape[1:]), np.int64)
+        lib.ref_key_get(key_h, kind, ptr(key))
+        expect = ref.get(out_h)
+        dk = ctx.to_device(key)
+        out = ctx.array((1,) + expect.shape)
+        if delta == -1:
+            ctx.conjugate(dct, dk, int(shape[0]), out, mid)
+        else:
+            ctx.rotate(dct, dk, int(shape[0]), delta, out, mid)
+        got = out.download()[0]
+        assert (got == expect).all(), delta
+        ref.set(out_h, got)
+        dec = np.zeros(slots)
+        lib.ref_decrypt(e, out_h, sk, ptr(dec))
+        assert np.abs(dec - expect_msg).max() < 1e-5, delta
+        out.free()
+        dk.free()
+        lib.ref_free_ct(out_h)
+        if delta == -1:
+            lib.ref_free_conjugation_key(key_h)
+        else:
+            lib.ref_free_fixed_rotation_key(key_h)
+    dct.free()
+    ctx.close()
+    for h in (ct0, ct):
+        lib.ref_free_ct(h)
+    lib.ref_free_public_key(pk)
+    lib.ref_free_secret_key(sk)
+    ref.close()

@@ -15,7 +15,7 @@ import random
 import numpy as np
 import pytest
 
-from support import Oracle, golden, hmult_inputs
+from support import Oracle, Reference, golden, hmult_inputs
 
 
 # ---------------------------------------------------------------- 1. KATs
@@ -291,3 +291,137 @@ def test_rescale_closed_form():
         expect = (X - r) // q0 + (1 if r > q0 // 2 else 0)
         for i in range(1, chain):
             assert y[i - 1, coef] == expect % primes[i]
+
+
+# ------------------------------------------- rotations and conjugation
+
+def rotate_cases(max_log_n):
+    return [c for c in golden("rotate_golden.json")["cases"]
+            if c["log_n"] <= max_log_n]
+
+
+@pytest.mark.parametrize("case", rotate_cases(15), ids=lambda c: c["name"])
+def test_oracle_rotate_matches_reference_digests(case):
+    orc = Oracle(case["log_n"], case["primes"], case["special_prime_count"])
+    n = 1 << case["log_n"]
+    A, _, K = hmult_inputs(case["primes"], n, case["moduli_id"], case["limbs"],
+                           case["key_moduli_id"], case["key_polys"],
+                           case["key_chain"], case["seed"])
+    if case["op"] == "rotate":
+        out = orc.rotate(A, K, case["moduli_id"], case["key_moduli_id"],
+                         case["delta"])
+    else:
+        out = orc.conjugate(A, K, case["moduli_id"], case["key_moduli_id"])
+    assert _digest(out) == case["out_sha256"]
+
+
+def _automorphism_exponent(n, delta):
+    """5^-delta mod 2N, or 2N - 1 for the conjugation."""
+    two_n = 2 * n
+    shift = delta % two_n
+    return two_n - 1 if shift == two_n - 1 else pow(5, two_n - shift, two_n)
+
+
+@pytest.mark.parametrize("delta", [-1, 0, 1, 2, 5, 11])
+def test_automorphism_closed_form(delta):
+    """y(X) = x(X^g) in Z_q[X]/(X^N + 1), with g = 5^-delta mod 2N: the
+    coefficient of X^(g i mod 2N) picks up x_i with sign (-1)^floor(g i / N).
+    Checked by evaluation: y(psi^e) = x(psi^(g e)) for every odd e."""
+    log_n = 4
+    n = 1 << log_n
+    primes = small_ring(log_n, 1, 3, 40)
+    orc = Oracle(log_n, primes, 1)
+    rng = random.Random(delta + 100)
+    x = np.array([[rng.randrange(q) for _ in range(n)] for q in primes[:2]],
+                 np.int64)
+    # coefficient q, a lazy zero, must come out as 0
+    x[0, 3] = primes[0]
+    y = orc.automorphism(x[None], 0, delta)[0]
+    g = _automorphism_exponent(n, delta)
+    for l, q in enumerate(primes[:2]):
+        assert ((0 <= y[l]) & (y[l] < q)).all()
+        psi = _primitive_root_2n(q, n)
+        for e in range(1, 2 * n, 2):
+            lhs = sum(int(c) * pow(psi, e * i, q) for i, c in enumerate(y[l]))
+            rhs = sum(int(c) * pow(psi, g * e * i, q)
+                      for i, c in enumerate(x[l]))
+            assert lhs % q == rhs % q, (l, e)
+
+
+def _primitive_root_2n(q, n):
+    for a in range(2, q):
+        r = pow(a, (q - 1) // (2 * n), q)
+        if pow(r, n, q) == q - 1:
+            return r
+    raise AssertionError("no root")
+
+
+@pytest.mark.parametrize("alpha,chain,delta", [(1, 3, 1), (2, 5, 3),
+                                               (3, 7, -1)])
+def test_rotate_closed_form(alpha, chain, delta):
+    """rotate = automorphism of (c0 + floor(sum_p c_p X_p / P), ...): the
+    rotation with an arbitrary key equals the key switch of c1 plus c0 and
+    then the signed permutation; conjugation permutes first."""
+    log_n = 3
+    n = 1 << log_n
+    primes = small_ring(log_n, alpha, chain + alpha, 40, seed=alpha)
+    orc = Oracle(log_n, primes, alpha)
+    rng = np.random.Generator(np.random.PCG64(alpha * 10 + chain))
+    key_mid = 0
+    parts = -(-chain // alpha)
+    key = np.stack([np.stack([rng.integers(0, 2 * q, n, dtype=np.uint64)
+                              .astype(np.int64) for q in primes])
+                    for _ in range(2 * parts)])
+    ct = np.stack([np.stack([rng.integers(0, q, n, dtype=np.uint64)
+                             .astype(np.int64) for q in primes[:chain]])
+                   for _ in range(2)])
+    qs = np.array(primes[:chain], np.int64)[:, None]
+    if delta == -1:
+        got = orc.conjugate(ct, key, 0, key_mid)
+        perm = orc.automorphism(ct, 0, -1)
+        ks = orc.key_switch(perm[1], key, 0, key_mid)
+        expect = np.stack([(ks[0] + perm[0]) % qs, ks[1]])
+    else:
+        got = orc.rotate(ct, key, 0, key_mid, delta)
+        ks = orc.key_switch(ct[1], key, 0, key_mid)
+        expect = orc.automorphism(np.stack([(ks[0] + ct[0]) % qs, ks[1]]), 0,
+                                  delta)
+    assert (got == expect).all()
+
+
+def test_ntt_permutation_commutes_with_automorphism():
+    """NTT(x(X^g)) = NTT(x) permuted by support.ntt_permutation, mod q."""
+    from support import ntt_permutation
+    log_n = 5
+    n = 1 << log_n
+    primes = small_ring(log_n, 1, 3, 40)
+    orc = Oracle(log_n, primes, 1)
+    rng = random.Random(3)
+    x = np.array([[rng.randrange(q) for _ in range(n)] for q in primes[:2]],
+                 np.int64)[None]
+    q = np.array(primes[:2], np.int64)[:, None]
+    for delta in (-1, 1, 3, 6):
+        lhs = orc.ntt(orc.automorphism(x, 0, delta), 0)[0] % q
+        rhs = orc.ntt(x, 0)[0][:, ntt_permutation(n, delta)] % q
+        assert (lhs == rhs).all(), delta
+
+
+@pytest.mark.reference
+@pytest.mark.skipif(not Reference.available(),
+                    reason="oracle/_ref (compiled reference) not built")
+def test_permutation_tables_match_reference():
+    """src/ckks/encode.cpp:513-557 tables against the closed forms."""
+    from support import automorphism_exponent, ntt_permutation, ptr
+    lib = Reference.load_lib()
+    for n in (16, 1024, 1 << 16):
+        for delta in (-1, 0, 1, 2, 7, n // 4, n // 2 - 1):
+            perm = np.zeros(n, np.int32)
+            sign = np.zeros(n, np.int32)
+            assert lib.ref_permutation_and_sign(n, delta, ptr(perm),
+                                                ptr(sign)) == 0
+            g = automorphism_exponent(n, delta)
+            dst = (g * np.arange(n, dtype=np.int64)) % (2 * n)
+            assert (perm == dst % n).all()
+            assert (sign == np.where(dst >= n, -1, 1)).all()
+            assert lib.ref_ntt_permutation(n, delta, ptr(perm)) == 0
+            assert (perm == ntt_permutation(n, delta)).all(), (n, delta)
