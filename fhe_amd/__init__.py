@@ -27,8 +27,23 @@ EXPORTED_SYMBOLS = (
     "fhe_intt", "fhe_mont_mult", "fhe_rescale", "fhe_relinearize",
     "fhe_hmult_relin_rescale", "fhe_params_resolve_primes",
     "fhe_automorphism", "fhe_rotate", "fhe_conjugate", "fhe_profile_enable",
-    "fhe_profile_count", "fhe_profile_entry",
+    "fhe_profile_count", "fhe_profile_entry", "fhe_elementwise",
+    "fhe_multiplier", "fhe_params_multiplier", "fhe_multiply_double",
+    "fhe_multiply_int", "fhe_add_double", "fhe_level_down",
 )
+
+
+# fhe_elementwise operation codes (FHE_EW_* in include/fhe_amd.h)
+EW = dict(MONT_MULT=0, MONT_ENTER=1, REDUCE_2Q_TO_Q=2, MAKE_SIGNED=3,
+          NEGATE=4, ADD=5, SUBTRACT=6, ENTER_ADD=7, ENTER_ADD_REDUCE=8,
+          MULT_ADD=9, SUB_ENTER_REDUCE=10, SUB_ENTER=11, ENTER_EXIT=12,
+          ENTER_EXIT_REDUCE=13, ADD_REDUCE=14, SUB_REDUCE=15,
+          MAKE_UNSIGNED_REDUCE=16, NEGATE_MAKE_UNSIGNED=17, ADD_SCALAR=18,
+          ROUNDER=19, ENTER_REDUCE=20, ADD_SCALAR_REDUCE=21)
+
+# fhe_multiplier kinds (FHE_MULTIPLIER_*)
+MULTIPLIER = dict(ADD=0, MULT=1, MULT_INT=2, LEVEL_DOWN=3,
+                  LEVEL_DOWN_AFTER_RESCALE=4)
 
 
 class FheError(RuntimeError):
@@ -88,6 +103,19 @@ def library():
         lib.fhe_rotate.argtypes = [vp, i32, i32, vp, vp, i32, i32, vp]
         lib.fhe_conjugate.argtypes = [vp, i32, i32, vp, vp, i32, vp]
         lib.fhe_profile_enable.argtypes = [vp, i32]
+        lib.fhe_elementwise.argtypes = [vp, i32, vp, vp, vp, i32, i32, i32, vp]
+        lib.fhe_multiplier.argtypes = [vp, i32, ctypes.c_double,
+                                       ctypes.c_int64, i32, i32, vp, i32,
+                                       ctypes.POINTER(i32)]
+        lib.fhe_multiply_double.argtypes = [vp, vp, vp, i32, i32,
+                                            ctypes.c_double]
+        lib.fhe_multiply_int.argtypes = [vp, vp, vp, i32, i32, ctypes.c_int64]
+        lib.fhe_add_double.argtypes = [vp, vp, vp, i32, i32, i32,
+                                       ctypes.c_double]
+        lib.fhe_level_down.argtypes = [vp, vp, vp, i32, i32, i32, i32]
+        lib.fhe_params_multiplier.argtypes = [
+            ctypes.POINTER(Params), i32, ctypes.c_double, ctypes.c_int64, i32,
+            i32, vp, i32, ctypes.POINTER(i32)]
         lib.fhe_profile_count.argtypes = [vp, ctypes.POINTER(i32)]
         lib.fhe_profile_entry.argtypes = [
             vp, i32, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
@@ -123,6 +151,20 @@ def resolve_primes(log_n, special_prime_count, scale_bits=40,
     _check(library().fhe_params_resolve_primes(ctypes.byref(p), out, 512,
                                                ctypes.byref(count)))
     return np.array(out[:count.value], dtype=np.uint64)
+
+
+def params_multiplier(log_n, special_prime_count, kind, level, value=0.0,
+                      int_value=0, level2=0, scale_bits=40,
+                      small_prime_bits=40, one_more_level=True, primes=None):
+    """Host-only fhe_params_multiplier (no GPU needed)."""
+    p = make_params(log_n, special_prime_count, scale_bits, small_prime_bits,
+                    one_more_level, -1, primes)
+    out = np.zeros(512, np.int64)
+    count = ctypes.c_int(0)
+    _check(library().fhe_params_multiplier(
+        ctypes.byref(p), MULTIPLIER[kind], float(value), int(int_value),
+        level, level2, out.ctypes.data, 512, ctypes.byref(count)))
+    return out[:count.value]
 
 
 class DeviceArray:
@@ -273,6 +315,51 @@ class Context:
         count = int(np.prod(cts.shape[:-3]))
         _check(library().fhe_conjugate(self.handle, count, moduli_id, cts.ptr,
                                        key.ptr, key_moduli_id, out.ptr))
+
+    def elementwise(self, op, x1, y, moduli_id, x2=None, constants=None):
+        """y = op(x1, x2, constants[limb]) per coefficient (EW_* codes)."""
+        polys, limbs = _polys_limbs(x1, self.n)
+        c = None
+        if constants is not None:
+            c = np.ascontiguousarray(constants, dtype=np.int64)
+            assert c.shape == (limbs,)
+        _check(library().fhe_elementwise(
+            self.handle, op, x1.ptr, x2.ptr if x2 is not None else None,
+            y.ptr, polys, limbs, moduli_id,
+            c.ctypes.data if c is not None else None))
+
+    def multiplier(self, kind, level, value=0.0, int_value=0, level2=0):
+        """Per-limb scalars of a CKKSPrecomputed multiplier (MULTIPLIER
+        kinds); host-only."""
+        out = np.zeros(self.prime_count, np.int64)
+        count = ctypes.c_int()
+        _check(library().fhe_multiplier(self.handle, MULTIPLIER[kind],
+                                        float(value), int(int_value), level,
+                                        level2, out.ctypes.data,
+                                        self.prime_count, ctypes.byref(count)))
+        return out[:count.value]
+
+    def multiply_double(self, src, dst, level, value):
+        polys = int(np.prod(src.shape[:-2]))
+        _check(library().fhe_multiply_double(self.handle, src.ptr, dst.ptr,
+                                             polys, level, float(value)))
+
+    def multiply_int(self, src, dst, moduli_id, value):
+        polys = int(np.prod(src.shape[:-2]))
+        _check(library().fhe_multiply_int(self.handle, src.ptr, dst.ptr,
+                                          polys, moduli_id, int(value)))
+
+    def add_double(self, src, dst, level, value):
+        """src: [count][polys][limbs][N]"""
+        count, polys = int(np.prod(src.shape[:-3])), src.shape[-3]
+        _check(library().fhe_add_double(self.handle, src.ptr, dst.ptr, count,
+                                        polys, level, float(value)))
+
+    def level_down(self, src, dst, level, dst_level, rescaled=False):
+        polys = int(np.prod(src.shape[:-2]))
+        _check(library().fhe_level_down(self.handle, src.ptr, dst.ptr, polys,
+                                        level, 1 if rescaled else 0,
+                                        dst_level))
 
     def profile(self, on):
         """Start (clearing totals) or stop stage timing."""

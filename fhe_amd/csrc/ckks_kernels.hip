@@ -284,7 +284,72 @@ __global__ __launch_bounds__(kBlock) void rescale_kernel(
   }
 }
 
+// Scalar multiply then rescale (CKKSCore::multiply(ct, double),
+// src/ckks/core.cpp:196-226, and CKKSCore::level_down, :792-838):
+// y_i = guard(mont(x_i, c_i), q_i) on every source limb, then
+// rescale_once_ (src/ckks/core_rescale.cpp:5-25) dropping limb 0, and the
+// first `skip` limbs of the rescaled result are dropped as well.
+struct ScalarRescaleArgs {
+  const int64_t *in;
+  int64_t *out;
+  int polys;
+  int limbs; // source limbs
+  int moduli_id;
+  int skip;
+  int64_t c[kMaxEwLimbs];
+};
+
+__global__ __launch_bounds__(kBlock) void scalar_rescale_kernel(
+    ScalarRescaleArgs a, DeviceTables t) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= int64_t(a.polys) * n)
+    return;
+  const int64_t poly = idx >> t.log_n;
+  const int64_t coef = idx & (n - 1);
+  const int64_t *x = a.in + poly * a.limbs * n + coef;
+  const int out_limbs = a.limbs - 1 - a.skip;
+  int64_t *o = a.out + poly * out_limbs * n + coef;
+  auto entered = [&](int i) {
+    const int p = a.moduli_id + i;
+    return guard(mont_mult(x[int64_t(i) * n], a.c[i], t.q[p], t.k[p]),
+                 int64_t(t.q[p]));
+  };
+  const int64_t y0 = entered(0);
+  const int64_t rounder = y0 > int64_t(t.qh[a.moduli_id]) ? 1 : 0;
+  for (int i = 1 + a.skip; i < a.limbs; i++) {
+    const int p = a.moduli_id + i;
+    const uint64_t q = t.q[p], k = t.k[p];
+    const int64_t q2 = int64_t(t.q2[p]);
+    int64_t v = entered(i) - y0;
+    v = v >= 0 ? v : v + q2;
+    v = guard(v, int64_t(q));
+    v = mont_mult(v, t.rescale[int64_t(a.moduli_id) * t.prime_count + p], q,
+                  k);
+    v = guard(v + rounder, q2);
+    o[int64_t(i - 1 - a.skip) * n] = guard(v, int64_t(q));
+  }
+}
+
 } // namespace
+
+void launch_scalar_rescale(const DeviceTables &t, const int64_t *in,
+                           int64_t *out, int polys, int limbs, int moduli_id,
+                           const int64_t *constants, int skip,
+                           hipStream_t stream) {
+  ScalarRescaleArgs a{};
+  a.in = in;
+  a.out = out;
+  a.polys = polys;
+  a.limbs = limbs;
+  a.moduli_id = moduli_id;
+  a.skip = skip;
+  for (int l = 0; l < limbs; l++)
+    a.c[l] = constants[l];
+  const dim3 grid(grid_for(int64_t(polys) << t.log_n));
+  hipLaunchKernelGGL(scalar_rescale_kernel, grid, dim3(kBlock), 0, stream, a,
+                     t);
+}
 
 void launch_mont_mult(const DeviceTables &t, const int64_t *a,
                       const int64_t *b, int64_t *out, int64_t total_limbs,

@@ -308,7 +308,8 @@ extern "C" {
 
 const char *fhe_last_error(void) { return g_last_error.c_str(); }
 
-static fhe::CkksParams resolve_params(const fhe_params *p) {
+static fhe::CkksParams resolve_params(const fhe_params *p,
+                                      bool twiddles = true) {
   require(p != nullptr, "null argument");
   require(p->scale_bits == 40 || p->scale_bits == 54,
           "CKKS Parameters: unsupported config.");
@@ -318,7 +319,7 @@ static fhe::CkksParams resolve_params(const fhe_params *p) {
     std::vector<uint64_t> primes(p->primes, p->primes + p->prime_count);
     params = fhe::CkksParams::from_primes(p->log_coeff_count, primes,
                                           p->special_prime_count, p->scale_bits,
-                                          p->one_more_level);
+                                          p->one_more_level, twiddles);
     if (p->max_level >= 0)
       params.max_level = p->max_level;
   } else {
@@ -335,7 +336,7 @@ static fhe::CkksParams resolve_params(const fhe_params *p) {
     c.small_prime_bits = p->small_prime_bits;
     c.one_more_level = p->one_more_level != 0;
     c.max_level = p->max_level;
-    params = fhe::CkksParams::from_config(c);
+    params = fhe::CkksParams::from_config(c, twiddles);
   }
   require(params.max_level >= 0 && params.max_level <= params.full_level,
           "The given max level is out of range.");
@@ -346,7 +347,7 @@ int fhe_params_resolve_primes(const fhe_params *p, uint64_t *primes,
                               int capacity, int *prime_count) {
   return guarded([&] {
     require(primes && prime_count, "null argument");
-    const auto params = resolve_params(p);
+    const auto params = resolve_params(p, false);
     require(capacity >= params.prime_count, "capacity too small");
     std::memcpy(primes, params.q.data(), sizeof(uint64_t) * params.prime_count);
     *prime_count = params.prime_count;
@@ -589,6 +590,202 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
                                         limbs, moduli_id, ctx->stream);
     });
     ctx->check_launch("Multiply");
+  });
+}
+
+static void multiplier_of(const fhe::CkksParams &P, int kind, double value,
+                          int64_t int_value, int level, int level2,
+                          int64_t *out, int capacity, int *count) {
+  {
+    auto level_ok = [&](int l) { return l >= 0 && l <= P.full_level; };
+    std::vector<int64_t> v;
+    switch (kind) {
+    case FHE_MULTIPLIER_ADD:
+      require(level_ok(level), "multiplier: level out of range");
+      v = P.add_multiplier(value, level);
+      break;
+    case FHE_MULTIPLIER_MULT:
+      require(level_ok(level) && level >= 1, "multiplier: level out of range");
+      v = P.mult_multiplier(value, level);
+      break;
+    case FHE_MULTIPLIER_MULT_INT:
+      require(level >= 0 && level < P.chain_count,
+              "multiplier: moduli_id out of range");
+      v = P.mult_multiplier_int(int_value, level);
+      break;
+    case FHE_MULTIPLIER_LEVEL_DOWN:
+      require(level_ok(level) && level_ok(level2) && level2 < level,
+              "multiplier: levels out of range");
+      v = P.level_down_multiplier(level, level2);
+      break;
+    case FHE_MULTIPLIER_LEVEL_DOWN_AFTER_RESCALE:
+      require(level_ok(level) && level >= 1 && level_ok(level2) &&
+                  level2 < level,
+              "multiplier: levels out of range");
+      v = P.level_down_multiplier_after_rescale(level, level2);
+      break;
+    default:
+      require(false, "multiplier: unknown kind");
+    }
+    require(capacity >= int(v.size()), "multiplier: capacity too small");
+    std::memcpy(out, v.data(), v.size() * sizeof(int64_t));
+    *count = int(v.size());
+  }
+}
+
+int fhe_multiplier(const fhe_context *ctx, int kind, double value,
+                   int64_t int_value, int level, int level2, int64_t *out,
+                   int capacity, int *count) {
+  return guarded([&] {
+    require(ctx && out && count, "null argument");
+    multiplier_of(ctx->params, kind, value, int_value, level, level2, out,
+                  capacity, count);
+  });
+}
+
+int fhe_params_multiplier(const fhe_params *p, int kind, double value,
+                          int64_t int_value, int level, int level2,
+                          int64_t *out, int capacity, int *count) {
+  return guarded([&] {
+    require(out && count, "null argument");
+    multiplier_of(resolve_params(p, false), kind, value, int_value, level,
+                  level2, out, capacity, count);
+  });
+}
+
+int fhe_elementwise(fhe_context *ctx, int op, const int64_t *x1,
+                    const int64_t *x2, int64_t *y, int polys, int limbs,
+                    int moduli_id, const int64_t *constants) {
+  return guarded([&] {
+    require(ctx && x1 && y, "null argument");
+    require(op >= 0 && op < fhe::kEwOpCount, "elementwise: unknown operation");
+    require(polys > 0 && limbs > 0, "elementwise: empty shape");
+    require(moduli_id >= 0 && moduli_id + limbs <= ctx->params.prime_count,
+            "elementwise: limbs exceed the prime chain");
+    require(limbs <= fhe::kMaxEwLimbs, "elementwise: too many limbs");
+    require(!fhe::ew_needs_x2(op) || x2 != nullptr,
+            "elementwise: the operation needs a second operand");
+    require(!fhe::ew_needs_constants(op) || constants != nullptr,
+            "elementwise: the operation needs per-limb constants");
+    ctx->stage("elementwise", [&] {
+      fhe::launch_elementwise(ctx->tables, op, x1, x2, y,
+                              int64_t(polys) * limbs, limbs, moduli_id,
+                              constants, ctx->stream);
+    });
+    ctx->check_launch("elementwise");
+  });
+}
+
+// ---- scalar operations on INTT-form, not-rescaled data at a level
+
+int fhe_multiply_double(fhe_context *ctx, const int64_t *in, int64_t *out,
+                        int polys, int level, double value) {
+  return guarded([&] {
+    require(ctx && in && out, "null argument");
+    require(polys > 0, "Multiply: empty shape");
+    const auto &P = ctx->params;
+    require(level >= 1 && level <= P.full_level,
+            "Multiply: the level must be positive.");
+    const int mid = P.moduli_id_of_level(level);
+    const auto c = P.mult_multiplier(value, level);
+    require(int(c.size()) <= fhe::kMaxEwLimbs, "Multiply: too many limbs");
+    ctx->stage("scalar_rescale", [&] {
+      fhe::launch_scalar_rescale(ctx->tables, in, out, polys, int(c.size()),
+                                 mid, c.data(), 0, ctx->stream);
+    });
+    ctx->check_launch("Multiply");
+  });
+}
+
+int fhe_multiply_int(fhe_context *ctx, const int64_t *in, int64_t *out,
+                     int polys, int moduli_id, int64_t value) {
+  return guarded([&] {
+    require(ctx && in && out, "null argument");
+    require(polys > 0, "Multiply: empty shape");
+    ctx->check_moduli(moduli_id, 1, "Multiply");
+    const auto c = ctx->params.mult_multiplier_int(value, moduli_id);
+    require(int(c.size()) <= fhe::kMaxEwLimbs, "Multiply: too many limbs");
+    ctx->stage("elementwise", [&] {
+      fhe::launch_elementwise(ctx->tables, fhe::kEwEnterReduce, in, nullptr,
+                              out, int64_t(polys) * int64_t(c.size()),
+                              int(c.size()), moduli_id, c.data(),
+                              ctx->stream);
+    });
+    ctx->check_launch("Multiply");
+  });
+}
+
+int fhe_add_double(fhe_context *ctx, const int64_t *in, int64_t *out,
+                   int count, int polys, int level, double value) {
+  return guarded([&] {
+    require(ctx && in && out, "null argument");
+    require(count > 0 && polys > 0, "Add: empty shape");
+    const auto &P = ctx->params;
+    require(level >= 0 && level <= P.full_level, "Add: level out of range");
+    const int mid = P.moduli_id_of_level(level);
+    const auto c = P.add_multiplier(value, level);
+    const int limbs = int(c.size());
+    require(limbs <= fhe::kMaxEwLimbs, "Add: too many limbs");
+    const int64_t poly = int64_t(limbs) << P.log_n;
+    ctx->stage("elementwise", [&] {
+      for (int ct = 0; ct < count; ct++) {
+        const int64_t *x = in + ct * polys * poly;
+        int64_t *y = out + ct * polys * poly;
+        // polynomial 0 takes the scalar, the others are copied
+        fhe::launch_elementwise(ctx->tables, fhe::kEwAddScalarReduce, x,
+                                nullptr, y, limbs, limbs, mid, c.data(),
+                                ctx->stream);
+        if (polys > 1 && x != y)
+          hip_check(hipMemcpyAsync(y + poly, x + poly,
+                                   size_t(polys - 1) * poly * 8,
+                                   hipMemcpyDeviceToDevice, ctx->stream),
+                    "copy");
+      }
+    });
+    ctx->check_launch("Add");
+  });
+}
+
+int fhe_level_down(fhe_context *ctx, const int64_t *in, int64_t *out,
+                   int polys, int level, int rescaled, int dst_level) {
+  return guarded([&] {
+    require(ctx && in && out, "null argument");
+    require(polys > 0, "Level down: empty shape");
+    const auto &P = ctx->params;
+    require(level >= 0 && level <= P.full_level && dst_level >= 0 &&
+                dst_level < level,
+            "Level down: the destination level must be below the level.");
+    const int dst_mid = P.moduli_id_of_level(dst_level);
+    if (rescaled) {
+      // CKKSCore::level_down_after_rescale (src/ckks/core.cpp:743-779): the
+      // rescaled input sits at moduli_id_of_level(level) + 1
+      const int src_mid = P.moduli_id_of_level(level) + 1;
+      const auto c = P.level_down_multiplier_after_rescale(level, dst_level);
+      const int limbs = int(c.size());
+      require(limbs <= fhe::kMaxEwLimbs, "Level down: too many limbs");
+      const int64_t src_poly = int64_t(P.limbs_at(src_mid)) << P.log_n;
+      const int64_t dst_poly = int64_t(limbs) << P.log_n;
+      const int64_t offset = int64_t(dst_mid - src_mid) << P.log_n;
+      ctx->stage("elementwise", [&] {
+        for (int p = 0; p < polys; p++)
+          fhe::launch_elementwise(ctx->tables, fhe::kEwEnterReduce,
+                                  in + p * src_poly + offset, nullptr,
+                                  out + p * dst_poly, limbs, limbs, dst_mid,
+                                  c.data(), ctx->stream);
+      });
+    } else {
+      // CKKSCore::level_down (src/ckks/core.cpp:792-838)
+      const int src_mid = P.moduli_id_of_level(level);
+      const int next_mid = P.moduli_id_of_level(level - 1);
+      const auto c = P.level_down_multiplier(level, dst_level);
+      require(int(c.size()) <= fhe::kMaxEwLimbs, "Level down: too many limbs");
+      ctx->stage("scalar_rescale", [&] {
+        fhe::launch_scalar_rescale(ctx->tables, in, out, polys, int(c.size()),
+                                   src_mid, c.data(), dst_mid - next_mid,
+                                   ctx->stream);
+      });
+    }
+    ctx->check_launch("Level down");
   });
 }
 

@@ -114,6 +114,52 @@ void launch_ntt_automorphism(const DeviceTables &t, const int64_t *in,
                              int64_t *out, int64_t total_limbs, uint32_t g,
                              hipStream_t stream);
 
+// Per-coefficient primitives of src/montgomery/unit.cpp:5-86,
+// src/montgomery/compound.cpp:6-109 and src/operation/scalar.cpp:5-11; the
+// codes equal the FHE_EW_* values of the C ABI.  x2 is the second operand
+// (or the accumulator of the enter-add forms), c the per-limb constant.
+enum EwOp : int {
+  kEwMontMult = 0,            // mont(x1, x2)
+  kEwMontEnter = 1,           // mont(x1, c)
+  kEwReduce = 2,              // guard(x1, q)
+  kEwMakeSigned = 3,          // x1 <= q/2 (unsigned compare) ? x1 : x1 - q
+  kEwNegate = 4,              // -x1
+  kEwAdd = 5,                 // guard(x1 + x2, 2q)
+  kEwSubtract = 6,            // x1 - x2, + 2q when negative
+  kEwEnterAdd = 7,            // guard(mont(x1, c) + x2, 2q)
+  kEwEnterAddReduce = 8,      // guard(guard(mont(x1, c) + x2, 2q), q)
+  kEwMultAdd = 9,             // guard(mont(x1, x2) + y, 2q), y in place
+  kEwSubEnterReduce = 10,     // guard(mont(guard(x1 + 2q - x2, 2q), c), q)
+  kEwSubEnter = 11,           // mont(guard(x1 + 2q - x2, 2q), c)
+  kEwEnterExit = 12,          // exit(mont(x1, c))
+  kEwEnterExitReduce = 13,    // guard(exit(mont(x1, c)), q)
+  kEwAddReduce = 14,          // guard(guard(x1 + x2, 2q), q)
+  kEwSubReduce = 15,          // guard(guard(x1 + 2q - x2, 2q), q)
+  kEwMakeUnsignedReduce = 16, // x1 < 0 ? x1 + q : guard(x1, q)
+  kEwNegateMakeUnsigned = 17, // q - x1
+  kEwAddScalar = 18,          // coefficient 0: x1 + c, others x1
+  kEwRounder = 19,            // x1 > q/2 (unsigned compare) ? 1 : 0
+  kEwEnterReduce = 20,        // guard(mont(x1, c), q)
+  kEwAddScalarReduce = 21,    // guard(coefficient 0: x1 + c, others x1, q)
+  kEwOpCount = 22
+};
+constexpr int kMaxEwLimbs = 128;
+bool ew_needs_x2(int op);
+bool ew_needs_constants(int op);
+// constants: host array of `limbs` values (or nullptr)
+void launch_elementwise(const DeviceTables &t, int op, const int64_t *x1,
+                        const int64_t *x2, int64_t *y, int64_t total_limbs,
+                        int limbs, int moduli_id, const int64_t *constants,
+                        hipStream_t stream);
+
+// y = rescale(guard(mont(x, c), q)) per polynomial, dropping `skip` more
+// limbs after the rescale (scalar multiply, level down).  constants: host,
+// `limbs` values; limbs <= kMaxEwLimbs.
+void launch_scalar_rescale(const DeviceTables &t, const int64_t *in,
+                           int64_t *out, int polys, int limbs, int moduli_id,
+                           const int64_t *constants, int skip,
+                           hipStream_t stream);
+
 // Rescale of `polys` INTT-form polynomials (src/ckks/core_rescale.cpp:5-25).
 void launch_rescale(const DeviceTables &t, const int64_t *in, int64_t *out,
                     int polys, int limbs, int moduli_id, hipStream_t stream);

@@ -194,7 +194,7 @@ static int max_q_bits(int log_n) {
   }
 }
 
-CkksParams CkksParams::from_config(const CkksConfig &c) {
+CkksParams CkksParams::from_config(const CkksConfig &c, bool twiddles) {
   if (c.special_prime_count < 1 || c.special_prime_count > kMaxAlpha)
     throw std::invalid_argument("The given special prime count is invalid.");
   const int n = 1 << c.log_n;
@@ -213,7 +213,7 @@ CkksParams CkksParams::from_config(const CkksConfig &c) {
   if (c.one_more_level)
     primes.insert(primes.begin(), large[1]);
   CkksParams p = from_primes(c.log_n, primes, c.special_prime_count,
-                             c.scale_bits, c.one_more_level);
+                             c.scale_bits, c.one_more_level, twiddles);
   if (c.max_level >= 0)
     p.max_level = c.max_level;
   return p;
@@ -221,7 +221,7 @@ CkksParams CkksParams::from_config(const CkksConfig &c) {
 
 CkksParams CkksParams::from_primes(int log_n, const std::vector<uint64_t> &primes,
                                    int special_prime_count, int scale_bits,
-                                   bool one_more_level) {
+                                   bool one_more_level, bool twiddles) {
   CkksParams p;
   p.log_n = log_n;
   p.n = 1 << log_n;
@@ -239,7 +239,7 @@ CkksParams CkksParams::from_primes(int log_n, const std::vector<uint64_t> &prime
   p.max_level = p.full_level;
   p.scale_bits = scale_bits;
   p.one_more_level = one_more_level;
-  p.derive();
+  p.derive(twiddles);
   return p;
 }
 
@@ -249,15 +249,16 @@ int CkksParams::max_key_switching_level() const {
   return max_level;
 }
 
-void CkksParams::derive() {
+void CkksParams::derive(bool twiddles) {
+  derive_deviations();
   const int P = prime_count;
   k.resize(P);
   q2.resize(P);
   qh.resize(P);
   r2.resize(P);
   ninv.resize(P);
-  root.assign(size_t(P) * n, 0);
-  root_inverse.assign(size_t(P) * n, 0);
+  root.assign(twiddles ? size_t(P) * n : 0, 0);
+  root_inverse.assign(twiddles ? size_t(P) * n : 0, 0);
   std::vector<int64_t> power(n), shuffled(n);
   auto bitrev = [&](int x) {
     int y = 0;
@@ -272,6 +273,8 @@ void CkksParams::derive() {
     k[i] = mont_k(q[i]);
     r2[i] = mod_mul(kMontR, kMontR, qi);
     ninv[i] = mod_mul(mod_inverse(n, qi), kMontR, qi);
+    if (!twiddles)
+      continue;
     const int64_t psi = find_ntt_root(qi, n);
     for (int pass = 0; pass < 2; pass++) {
       const int64_t base = pass == 0 ? psi : mod_inverse(psi, qi);
@@ -375,6 +378,124 @@ std::vector<int64_t> CkksParams::modup_constants(int moduli_id,
       }
   }
   return c;
+}
+
+
+// ---------------------------------------------------------------- scales
+
+namespace {
+
+// floor(x) mod q for a finite x >= 0, where floor(x) is the integer the
+// reference builds digit by digit in base 2^64 (BigInt::from_double,
+// src/arithmetic/big_int.cpp:29-42): each digit is the double remainder
+// truncated to an integer.
+uint64_t floor_double_mod(double x, uint64_t q) {
+  const double base = std::pow(2, 64);
+  std::vector<unsigned __int128> digits;
+  while (x > 0) {
+    const double quotient = std::floor(x / base);
+    const double remainder = x - quotient * base;
+    x = quotient;
+    digits.push_back(static_cast<unsigned __int128>(remainder));
+  }
+  unsigned __int128 acc = 0;
+  const unsigned __int128 base_mod = ((unsigned __int128)1 << 64) % q;
+  for (size_t i = digits.size(); i-- > 0;)
+    acc = (acc * base_mod + digits[i] % q) % q;
+  return uint64_t(acc);
+}
+
+// scale_modulo / scale_enter_modulo (src/arithmetic/arithmetic.cpp:60-95):
+// the double `product` (operation order as there), its floor reduced per
+// prime, times R when entering, negated as q - s for a negative product.
+std::vector<int64_t> signed_scalar(double product, bool enter,
+                                   const uint64_t *q, int count) {
+  std::vector<int64_t> out(count);
+  for (int i = 0; i < count; i++) {
+    uint64_t s = floor_double_mod(std::abs(product), q[i]);
+    if (enter)
+      s = uint64_t((unsigned __int128)s * (uint64_t(kMontR) % q[i]) % q[i]);
+    int64_t v = int64_t(s);
+    if (product < 0)
+      v = int64_t(q[i]) - v;
+    out[i] = v;
+  }
+  return out;
+}
+
+} // namespace
+
+void CkksParams::derive_deviations() {
+  scale = std::pow(2, scale_bits);
+  // levels full_level .. 0 in order, as src/ckks/parameters.cpp:88-97
+  std::vector<long double> dev;
+  for (int level = full_level, moduli_id = 0; level >= 0; level--, moduli_id++) {
+    if (level == full_level || (one_more_level && level == full_level - 1)) {
+      dev.push_back(1.0);
+    } else {
+      const long double level_q = static_cast<long double>(q[moduli_id - 1]);
+      const auto alpha_sq = std::pow(scale / level_q, 2);
+      dev.push_back(std::pow(dev.back(), 2) * alpha_sq);
+    }
+  }
+  deviation_ld.assign(dev.rbegin(), dev.rend());
+  deviation.resize(deviation_ld.size());
+  for (size_t i = 0; i < deviation_ld.size(); i++)
+    deviation[i] = static_cast<double>(deviation_ld[i]);
+}
+
+std::vector<int64_t> CkksParams::add_multiplier(double value, int level) const {
+  const int mid = moduli_id_of_level(level);
+  double dev = deviation.at(level);
+  if (one_more_level && mid == 0)
+    dev = q[0] / scale;
+  const double product = (value * scale * dev + 0.5) * scale;
+  return signed_scalar(product, false, q.data() + mid, limbs_at(mid));
+}
+
+std::vector<int64_t> CkksParams::mult_multiplier(double value,
+                                                 int level) const {
+  const int mid = moduli_id_of_level(level);
+  const double dev = deviation.at(level - 1);
+  const double product = value * scale * std::sqrt(dev) + 0.5;
+  return signed_scalar(product, true, q.data() + mid, limbs_at(mid));
+}
+
+std::vector<int64_t> CkksParams::mult_multiplier_int(int64_t value,
+                                                     int moduli_id) const {
+  // modulus_multiply (src/arithmetic/arithmetic.cpp:33-38): a truncated
+  // 128-bit remainder, negative for a negative value
+  std::vector<int64_t> out(limbs_at(moduli_id));
+  for (size_t i = 0; i < out.size(); i++) {
+    const __int128 prod = (__int128)value * (__int128)kMontR;
+    out[i] = int64_t(prod % (__int128)q[moduli_id + i]);
+  }
+  return out;
+}
+
+std::vector<int64_t>
+CkksParams::level_down_multiplier(int source_level,
+                                  int destination_level) const {
+  const int src_mid = moduli_id_of_level(source_level);
+  const double diff = deviation.at(destination_level) /
+                      deviation.at(source_level);
+  double delta;
+  if (one_more_level && src_mid == 0)
+    delta = scale * diff;
+  else
+    delta = q[src_mid] * diff;
+  return signed_scalar(std::round(delta), true, q.data() + src_mid,
+                       limbs_at(src_mid));
+}
+
+std::vector<int64_t>
+CkksParams::level_down_multiplier_after_rescale(int source_level,
+                                                int destination_level) const {
+  const double diff = deviation.at(destination_level) /
+                      std::sqrt(deviation.at(source_level - 1));
+  const int dst_mid = moduli_id_of_level(destination_level);
+  return signed_scalar(std::round(scale * diff), true, q.data() + dst_mid,
+                       limbs_at(dst_mid));
 }
 
 } // namespace fhe

@@ -48,10 +48,12 @@ struct Partition {
 
 class CkksParams {
 public:
-  static CkksParams from_config(const CkksConfig &config);
+  // twiddles = false skips the NTT tables (host-only queries)
+  static CkksParams from_config(const CkksConfig &config,
+                                bool twiddles = true);
   static CkksParams from_primes(int log_n, const std::vector<uint64_t> &primes,
                                 int special_prime_count, int scale_bits,
-                                bool one_more_level);
+                                bool one_more_level, bool twiddles = true);
 
   int log_n = 0, n = 0;
   int prime_count = 0;   // Q chain + specials
@@ -67,6 +69,11 @@ public:
   // Montgomery-form twiddles, N-1 used entries per prime, padded to N:
   // root[p*N + i] = psi^bitrev(i+1) * R, root_inverse in backward-paint order.
   std::vector<int64_t> root, root_inverse;
+  // Scale 2^scale_bits and the per-level scale deviations, index = level
+  // 0..full_level (src/ckks/parameters.cpp:54-104).
+  double scale = 0;
+  std::vector<long double> deviation_ld;
+  std::vector<double> deviation;
   std::vector<int64_t> p_product_mult_r;   // [chain_count]
   std::vector<int64_t> p_inverse_mult_r;   // [alpha][prime_count]
   std::vector<int64_t> rescale_scale;      // [prime_count][prime_count]
@@ -106,8 +113,26 @@ public:
     return true;
   }
 
+  // Per-limb scalars of CKKSPrecomputed's multipliers
+  // (src/ckks/precomputed.cpp:418-533), limb l of the object at the
+  // multiplier's moduli_id:
+  //   add_multiplier(value, level)          make_add_multiplier
+  //   mult_multiplier(value, level)         make_mult_multiplier(double)
+  //   mult_multiplier_int(value, moduli_id) make_mult_multiplier(int64_t)
+  //   level_down_multiplier(src, dst)       make_level_down_multiplier
+  //   level_down_multiplier_after_rescale   ..._after_rescale
+  std::vector<int64_t> add_multiplier(double value, int level) const;
+  std::vector<int64_t> mult_multiplier(double value, int level) const;
+  std::vector<int64_t> mult_multiplier_int(int64_t value, int moduli_id) const;
+  std::vector<int64_t> level_down_multiplier(int source_level,
+                                             int destination_level) const;
+  std::vector<int64_t>
+  level_down_multiplier_after_rescale(int source_level,
+                                      int destination_level) const;
+
 private:
-  void derive();
+  void derive(bool twiddles);
+  void derive_deviations();
 };
 
 } // namespace fhe

@@ -134,6 +134,84 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
                             const int64_t *key, int key_moduli_id,
                             int64_t *out);
 
+/* Per-coefficient primitives over polys x limbs device limbs at moduli_id:
+ * y = op(x1, x2, c[limb]) for the reference's limb operations
+ * (src/montgomery/unit.cpp:5-86, src/montgomery/compound.cpp:6-109,
+ * src/operation/scalar.cpp:5-11).  x2 is required by the two-operand codes,
+ * `constants` (host, `limbs` values, limb l of every polynomial) by the codes
+ * that use c.  y may alias x1 or x2; FHE_EW_MULT_ADD reads y as well. */
+enum {
+  FHE_EW_MONT_MULT = 0,              /* mont(x1, x2)                   */
+  FHE_EW_MONT_ENTER = 1,             /* mont(x1, c)                    */
+  FHE_EW_REDUCE_2Q_TO_Q = 2,         /* guard(x1, q)                   */
+  FHE_EW_MAKE_SIGNED = 3,            /* x1 <= q/2 (unsigned) ? x1 : x1 - q */
+  FHE_EW_NEGATE = 4,                 /* -x1                            */
+  FHE_EW_ADD = 5,                    /* guard(x1 + x2, 2q)             */
+  FHE_EW_SUBTRACT = 6,               /* x1 - x2 (+2q if negative)      */
+  FHE_EW_ENTER_ADD = 7,              /* guard(mont(x1, c) + x2, 2q)    */
+  FHE_EW_ENTER_ADD_REDUCE = 8,       /* ... then guard(., q)           */
+  FHE_EW_MULT_ADD = 9,               /* guard(mont(x1, x2) + y, 2q)    */
+  FHE_EW_SUB_ENTER_REDUCE = 10,      /* guard(mont(guard(x1+2q-x2, 2q), c), q) */
+  FHE_EW_SUB_ENTER = 11,             /* mont(guard(x1 + 2q - x2, 2q), c) */
+  FHE_EW_ENTER_EXIT = 12,            /* exit(mont(x1, c))              */
+  FHE_EW_ENTER_EXIT_REDUCE = 13,     /* guard(exit(mont(x1, c)), q)    */
+  FHE_EW_ADD_REDUCE = 14,            /* guard(guard(x1 + x2, 2q), q)   */
+  FHE_EW_SUB_REDUCE = 15,            /* guard(guard(x1 + 2q - x2, 2q), q) */
+  FHE_EW_MAKE_UNSIGNED_REDUCE = 16,  /* x1 < 0 ? x1 + q : guard(x1, q) */
+  FHE_EW_NEGATE_MAKE_UNSIGNED = 17,  /* q - x1                         */
+  FHE_EW_ADD_SCALAR = 18,            /* coefficient 0 only: x1 + c     */
+  FHE_EW_ROUNDER = 19,               /* x1 > q/2 (unsigned) ? 1 : 0    */
+  FHE_EW_ENTER_REDUCE = 20,          /* guard(mont(x1, c), q)          */
+  FHE_EW_ADD_SCALAR_REDUCE = 21      /* guard(x1 (+ c at coefficient 0), q) */
+};
+int fhe_elementwise(fhe_context *ctx, int op, const int64_t *x1,
+                    const int64_t *x2, int64_t *y, int polys, int limbs,
+                    int moduli_id, const int64_t *constants);
+
+/* Host only (no device work): the per-limb scalars of CKKSPrecomputed's
+ * multipliers (src/ckks/precomputed.cpp:418-533), from the reference's
+ * scale deviations (src/ckks/parameters.cpp:88-104) and BigInt reductions
+ * (src/arithmetic/arithmetic.cpp:60-95, src/arithmetic/big_int.cpp:29-42).
+ * The scalars feed fhe_elementwise: limb l of the object at the multiplier's
+ * moduli_id.  `level` is a level, or a moduli_id for MULT_INT. */
+enum {
+  FHE_MULTIPLIER_ADD = 0,        /* make_add_multiplier(value, level)      */
+  FHE_MULTIPLIER_MULT = 1,       /* make_mult_multiplier(value, level)     */
+  FHE_MULTIPLIER_MULT_INT = 2,   /* make_mult_multiplier(int_value, mid)   */
+  FHE_MULTIPLIER_LEVEL_DOWN = 3, /* make_level_down_multiplier(level, level2) */
+  FHE_MULTIPLIER_LEVEL_DOWN_AFTER_RESCALE = 4
+};
+int fhe_multiplier(const fhe_context *ctx, int kind, double value,
+                   int64_t int_value, int level, int level2, int64_t *out,
+                   int capacity, int *count);
+/* The same from parameters alone, without a context or device. */
+int fhe_params_multiplier(const fhe_params *params, int kind, double value,
+                          int64_t int_value, int level, int level2,
+                          int64_t *out, int capacity, int *count);
+
+/* Scalar operations on INTT-form, not-rescaled polynomials at a level
+ * (moduli_id = full_level - level), `polys` polynomials of that level's
+ * limbs, with the reference's multiplier constants:
+ *  fhe_multiply_double: CKKSCore::multiply(ct, double)
+ *    (src/ckks/core.cpp:196-226): scalar enter, reduce, rescale; out has one
+ *    limb less (moduli_id + 1).
+ *  fhe_multiply_int: CKKSCore::multiply(ct, int64_t) (src/ckks/core.cpp:
+ *    228-251) at any moduli_id; out has the same limbs.
+ *  fhe_add_double: CKKSCore::add(ct, double) (src/ckks/core.cpp:340-373) on
+ *    `count` ciphertexts of `polys` polynomials: the scalar joins the
+ *    constant coefficient of polynomial 0, the others are copied.
+ *  fhe_level_down: CKKSCore::level_down (src/ckks/core.cpp:792-838) or, with
+ *    rescaled = 1, level_down_after_rescale (:743-779), whose input sits at
+ *    moduli_id + 1; out has the limbs of dst_level. */
+int fhe_multiply_double(fhe_context *ctx, const int64_t *in, int64_t *out,
+                        int polys, int level, double value);
+int fhe_multiply_int(fhe_context *ctx, const int64_t *in, int64_t *out,
+                     int polys, int moduli_id, int64_t value);
+int fhe_add_double(fhe_context *ctx, const int64_t *in, int64_t *out,
+                   int count, int polys, int level, double value);
+int fhe_level_down(fhe_context *ctx, const int64_t *in, int64_t *out,
+                   int polys, int level, int rescaled, int dst_level);
+
 /* Automorphism X -> X^g, g = 5^-delta mod 2N (g = -1 for delta = -1, the
  * conjugation), of polys x limbs limbs at moduli_id into out (no aliasing).
  * ntt_form = 0: coefficient-domain permutation with signs and

@@ -693,3 +693,51 @@ void oracle_conjugate(const oracle_ctx *c, int mid, const int64_t *ct,
   switch_and_add(c, mid, perm + poly, perm, key, key_mid, out);
   free(perm);
 }
+
+/* Per-coefficient primitives of one limb, indexed like the C ABI's FHE_EW_*
+ * codes: src/montgomery/unit.cpp:5-86 (mont_mult, mont_enter,
+ * reduce_2q_to_q, make_signed, negate, mont_add, mont_subtract,
+ * get_rounder), src/montgomery/compound.cpp:6-109 (the enter / sub / exit /
+ * reduce compounds, make_unsigned_reduce_2q_to_q, negate_make_unsigned) and
+ * src/operation/scalar.cpp:5-11 (add_scalar).  y may alias x1 or x2; the
+ * mult-add form accumulates into y. */
+void oracle_elementwise_limb(int op, const int64_t *x1, const int64_t *x2,
+                             int64_t *y, int n, uint64_t q, uint64_t k,
+                             int64_t c) {
+  const int64_t qq = (int64_t)q, q2 = (int64_t)(2 * q);
+  const int64_t qh = (int64_t)(q >> 1);
+  for (int i = 0; i < n; i++) {
+    const int64_t a = x1[i], b = x2 ? x2[i] : 0;
+    int64_t r;
+    switch (op) {
+    case 0: r = MONT(a, b, q, k); break;
+    case 1: r = MONT(a, c, q, k); break;
+    case 2: r = guard(a, qq); break;
+    /* make_signed and get_rounder compare against an unsigned q/2, so a
+     * negative int64 operand counts as large */
+    case 3: r = (uint64_t)a <= (uint64_t)qh ? a : a - qq; break;
+    case 4: r = -a; break;
+    case 5: r = guard(a + b, q2); break;
+    case 6: r = a - b; r = r >= 0 ? r : r + q2; break;
+    case 7: r = guard(MONT(a, c, q, k) + b, q2); break;
+    case 8: r = guard(guard(MONT(a, c, q, k) + b, q2), qq); break;
+    case 9: r = guard(MONT(a, b, q, k) + y[i], q2); break;
+    case 10: r = guard(MONT(guard(a + q2 - b, q2), c, q, k), qq); break;
+    case 11: r = MONT(guard(a + q2 - b, q2), c, q, k); break;
+    case 12: r = oracle_mont_exit(MONT(a, c, q, k), q, k); break;
+    case 13: r = guard(oracle_mont_exit(MONT(a, c, q, k), q, k), qq); break;
+    case 14: r = guard(guard(a + b, q2), qq); break;
+    case 15: r = guard(guard(a + q2 - b, q2), qq); break;
+    case 16: r = a < 0 ? a + qq : guard(a, qq); break;
+    case 17: r = qq - a; break;
+    case 18: r = i == 0 ? a + c : a; break;
+    case 19: r = (uint64_t)a > (uint64_t)qh ? 1 : 0; break;
+    /* mont_enter_scalar_ then reduce_2q_to_q_ (src/ckks/core.cpp:228-251);
+     * add_scalar_ then reduce_2q_to_q_ (src/ckks/core.cpp:340-373) */
+    case 20: r = guard(MONT(a, c, q, k), qq); break;
+    case 21: r = guard(i == 0 ? a + c : a, qq); break;
+    default: r = 0;
+    }
+    y[i] = r;
+  }
+}

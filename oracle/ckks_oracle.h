@@ -97,6 +97,11 @@ void oracle_rotate(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
 void oracle_conjugate(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
                       const int64_t *key, int key_moduli_id, int64_t *out);
 
+/* one limb of a per-coefficient primitive (codes of FHE_EW_*) */
+void oracle_elementwise_limb(int op, const int64_t *x1, const int64_t *x2,
+                             int64_t *y, int n, uint64_t q, uint64_t k,
+                             int64_t c);
+
 /* partition layout for a ciphertext at moduli_id against a key at
  * key_moduli_id: returns partition count; starts/sizes/key_index filled */
 int oracle_partitions(const oracle_ctx *ctx, int moduli_id, int key_moduli_id,

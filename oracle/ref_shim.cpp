@@ -21,6 +21,10 @@
 #include "ckks/engine.h"
 #include "ckks/precomputed.h"
 #include "engine/engine.h"
+#include "montgomery/compound.h"
+#include "montgomery/unit.h"
+#include "operation/scalar.h"
+#include "parameter/multiplier.h"
 #include "operation/ntt.h"
 
 using namespace desilo::fhe;
@@ -373,6 +377,122 @@ int ref_ntt_permutation(int n, int delta, int32_t *permutation) {
   return guarded([&]() -> int {
     get_ntt_permutation(permutation, n, delta);
     return 0;
+  });
+}
+
+// One limb of a per-coefficient primitive through the reference's own limb
+// functions (src/montgomery/unit.cpp, src/montgomery/compound.cpp,
+// src/operation/scalar.cpp), with the op codes of the C ABI's FHE_EW_*.
+// The accumulate forms start from y = x2 (enter-add) or the given y.
+int ref_elementwise(int op, const int64_t *x1_in, const int64_t *x2_in,
+                    int64_t *y, int n, uint64_t q, uint64_t k, int64_t c) {
+  return guarded([&]() -> int {
+    std::vector<int64_t> x1(x1_in, x1_in + n), x2(n, 0);
+    if (x2_in)
+      std::copy(x2_in, x2_in + n, x2.begin());
+    const uint64_t q2 = q << 1, qh = q >> 1;
+    int64_t *a = x1.data(), *b = x2.data();
+    switch (op) {
+    case 0: mont_mult(a, b, y, q, k, n); break;
+    case 1: mont_enter(a, y, c, q, k, n); break;
+    case 2: reduce_2q_to_q(a, y, q, n); break;
+    case 3: make_signed(a, y, q, qh, n); break;
+    case 4: negate(a, y, 1, n); break;
+    case 5: mont_add(a, b, y, q2, n); break;
+    case 6: mont_subtract(a, b, y, q2, n); break;
+    case 7: std::copy(b, b + n, y); mont_enter_add(a, y, c, q2, q, k, n); break;
+    case 8:
+      std::copy(b, b + n, y);
+      mont_enter_add_reduce_2q_to_q(a, y, c, q2, q, k, n);
+      break;
+    case 9: mont_mult_add(a, b, y, q2, q, k, n); break;
+    case 10: mont_sub_enter_reduce_2q_to_q(a, b, y, c, q2, q, k, n); break;
+    case 11: mont_sub_enter(a, b, y, c, q2, q, k, n); break;
+    case 12: mont_enter_exit(a, y, c, q, k, n); break;
+    case 13: mont_enter_exit_reduce_2q_to_q(a, y, c, q, k, n); break;
+    case 14: mont_add_reduce_2q_to_q(a, b, y, q2, q, n); break;
+    case 15: mont_sub_reduce_2q_to_q(a, b, y, q2, q, n); break;
+    case 16: make_unsigned_reduce_2q_to_q(a, y, q, n); break;
+    case 17: negate_make_unsigned(a, y, q, n); break;
+    case 18: add_scalar(a, y, c, n); break;
+    case 19: get_rounder(a, y, qh, n); break;
+    case 20:
+      mont_enter(a, y, c, q, k, n);
+      reduce_2q_to_q(y, y, q, n);
+      break;
+    case 21:
+      add_scalar(a, y, c, n);
+      reduce_2q_to_q(y, y, q, n);
+      break;
+    default: throw std::invalid_argument("unknown op");
+    }
+    return 0;
+  });
+}
+
+// CKKSPrecomputed's multiplier scalars (src/ckks/precomputed.cpp:418-533).
+// kind 0 add(value, level), 1 mult(value, level), 2 mult(int_value,
+// moduli_id = level), 3 level_down(level, level2), 4
+// level_down_after_rescale(level, level2).  Returns the scalar count.
+int ref_multiplier(void *handle, int kind, double value, int64_t int_value,
+                   int level, int level2, int64_t *out) {
+  return guarded([&]() -> int {
+    auto pre = static_cast<RefEngine *>(handle)->precomputed;
+    Multiplier *m = nullptr;
+    switch (kind) {
+    case 0: m = pre->make_add_multiplier(value, level); break;
+    case 1: m = pre->make_mult_multiplier(value, level); break;
+    case 2: m = pre->make_mult_multiplier(int_value, level); break;
+    case 3: m = pre->make_level_down_multiplier(level, level2); break;
+    case 4:
+      m = pre->make_level_down_multiplier_after_rescale(level, level2);
+      break;
+    default: throw std::invalid_argument("unknown multiplier kind");
+    }
+    const auto values = m->get();
+    const int count = int(m->multiplier_.size());
+    for (int i = 0; i < count; i++)
+      out[i] = values[i];
+    delete m;
+    return count;
+  });
+}
+
+// Scalar operations and level down through the reference engine
+// (src/ckks/engine_multiply.cpp:418-457, src/ckks/engine_add.cpp:518-536,
+// src/ckks/engine_level_down.cpp:5-37, src/ckks/engine_negate.cpp).
+void *ref_multiply_double(void *handle, void *ct, double value) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->multiply(
+        static_cast<UnitCiphertext *>(ct), value);
+  });
+}
+
+void *ref_multiply_int(void *handle, void *ct, int64_t value) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->multiply(
+        static_cast<UnitCiphertext *>(ct), value);
+  });
+}
+
+void *ref_add_double(void *handle, void *ct, double value) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->add(
+        static_cast<UnitCiphertext *>(ct), value);
+  });
+}
+
+void *ref_level_down(void *handle, void *ct, int level) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->level_down(
+        static_cast<UnitCiphertext *>(ct), level);
+  });
+}
+
+void *ref_negate(void *handle, void *ct) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->negate(
+        static_cast<UnitCiphertext *>(ct));
   });
 }
 

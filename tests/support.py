@@ -295,6 +295,11 @@ class Reference:
             "ref_conjugate": (VP, [VP, VP, VP]),
             "ref_permutation_and_sign": (i32, [i32, i32, VP, VP]),
             "ref_ntt_permutation": (i32, [i32, i32, VP]),
+            "ref_multiply_double": (VP, [VP, VP, ctypes.c_double]),
+            "ref_multiply_int": (VP, [VP, VP, ctypes.c_int64]),
+            "ref_add_double": (VP, [VP, VP, ctypes.c_double]),
+            "ref_level_down": (VP, [VP, VP, i32]),
+            "ref_negate": (VP, [VP, VP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)

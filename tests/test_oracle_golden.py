@@ -425,3 +425,89 @@ def test_permutation_tables_match_reference():
             assert (sign == np.where(dst >= n, -1, 1)).all()
             assert lib.ref_ntt_permutation(n, delta, ptr(perm)) == 0
             assert (perm == ntt_permutation(n, delta)).all(), (n, delta)
+
+
+# ------------------------------------------------ per-coefficient primitives
+
+EW_OPS = list(range(22))
+
+
+def _ew_operands(q, n, rng):
+    """Canonical, lazy and negative residues plus the boundary values."""
+    edges = [0, 1, q - 1, q, q + 1, 2 * q - 1, -1, -q + 1, q // 2, q // 2 + 1]
+    pool = np.concatenate([
+        rng.integers(0, q, n // 3, dtype=np.int64),
+        rng.integers(0, 2 * q, n // 3, dtype=np.int64),
+        rng.integers(-q + 1, 0, n - 2 * (n // 3), dtype=np.int64)])
+    pool[:len(edges)] = edges
+    return rng.permutation(pool)
+
+
+@pytest.mark.reference
+@pytest.mark.skipif(not Reference.available(),
+                    reason="oracle/_ref (compiled reference) not built")
+@pytest.mark.parametrize("op", EW_OPS)
+def test_elementwise_matches_reference_functions(op):
+    """Each oracle primitive equals the reference's own limb function on the
+    same operands (src/montgomery/unit.cpp, compound.cpp, scalar.cpp)."""
+    from support import ptr
+    import ctypes
+    lib = Reference.load_lib()
+    lib.ref_elementwise.argtypes = [ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_int, ctypes.c_uint64,
+                                    ctypes.c_uint64, ctypes.c_int64]
+    olib = Oracle.lib()
+    olib.oracle_elementwise_limb.argtypes = [
+        ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64]
+    rng = np.random.default_rng(op)
+    for q in (1099516280833, 1152921504606584833):  # 40-bit and 60-bit primes
+        k = olib.oracle_calculate_k(q)
+        n = 600
+        x1, x2 = _ew_operands(q, n, rng), _ew_operands(q, n, rng)
+        y0 = rng.integers(0, 2 * q, n, dtype=np.int64)
+        for c in (int(rng.integers(0, q)), 0, q - 1):
+            expect, got = y0.copy(), y0.copy()
+            assert lib.ref_elementwise(op, ptr(x1), ptr(x2), ptr(expect), n,
+                                       q, k, c) == 0
+            olib.oracle_elementwise_limb(op, ptr(x1), ptr(x2), ptr(got), n, q,
+                                         k, c)
+            assert (got == expect).all(), (op, q, c)
+
+
+def test_elementwise_closed_forms():
+    """Value-level identities of the primitives, independent of the
+    reference: products are a*b*2^-62 mod q, sums and differences are the
+    residues of a+b and a-b, exit is x*2^-62 mod q."""
+    from support import ptr
+    import ctypes
+    olib = Oracle.lib()
+    olib.oracle_elementwise_limb.argtypes = [
+        ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64]
+    q = 1099516280833
+    k = olib.oracle_calculate_k(q)
+    rinv = pow(1 << 62, -1, q)
+    rng = np.random.default_rng(5)
+    n = 256
+    a = rng.integers(0, 2 * q, n, dtype=np.int64)
+    b = rng.integers(0, 2 * q, n, dtype=np.int64)
+    c = int(rng.integers(0, q))
+
+    def run(op):
+        y = np.zeros(n, np.int64)
+        olib.oracle_elementwise_limb(op, ptr(a), ptr(b), ptr(y), n, q, k, c)
+        return [int(v) for v in y]
+
+    A, B = [int(v) for v in a], [int(v) for v in b]
+    assert all(v % q == x * y * rinv % q for v, x, y in zip(run(0), A, B))
+    assert all(v % q == x * c * rinv % q for v, x in zip(run(1), A))
+    for op in (5, 14):
+        assert all(v % q == (x + y) % q for v, x, y in zip(run(op), A, B))
+    for op in (6, 15):
+        assert all(v % q == (x - y) % q for v, x, y in zip(run(op), A, B))
+    assert all(0 <= v < q for v in run(14) + run(15) + run(13))
+    assert all(v % q == x * c * rinv * rinv % q for v, x in zip(run(12), A))
+    assert all(v % q == ((x - y) * c * rinv) % q
+               for v, x, y in zip(run(10), A, B))
