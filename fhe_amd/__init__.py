@@ -30,6 +30,7 @@ EXPORTED_SYMBOLS = (
     "fhe_profile_count", "fhe_profile_entry", "fhe_elementwise",
     "fhe_multiplier", "fhe_params_multiplier", "fhe_multiply_double",
     "fhe_multiply_int", "fhe_add_double", "fhe_level_down",
+    "fhe_encrypt_pk", "fhe_decrypt",
 )
 
 
@@ -113,6 +114,9 @@ def library():
         lib.fhe_add_double.argtypes = [vp, vp, vp, i32, i32, i32,
                                        ctypes.c_double]
         lib.fhe_level_down.argtypes = [vp, vp, vp, i32, i32, i32, i32]
+        lib.fhe_encrypt_pk.argtypes = [vp, i32, i32, vp, vp, i32, vp, vp, vp,
+                                       vp]
+        lib.fhe_decrypt.argtypes = [vp, i32, i32, vp, vp, i32, vp]
         lib.fhe_params_multiplier.argtypes = [
             ctypes.POINTER(Params), i32, ctypes.c_double, ctypes.c_int64, i32,
             i32, vp, i32, ctypes.POINTER(i32)]
@@ -360,6 +364,17 @@ class Context:
         _check(library().fhe_level_down(self.handle, src.ptr, dst.ptr, polys,
                                         level, 1 if rescaled else 0,
                                         dst_level))
+
+    def encrypt_pk(self, pt, pk, pk_moduli_id, u, e0, e1, out, moduli_id):
+        count = int(np.prod(pt.shape[:-2]))
+        _check(library().fhe_encrypt_pk(self.handle, count, moduli_id, pt.ptr,
+                                        pk.ptr, pk_moduli_id, u.ptr, e0.ptr,
+                                        e1.ptr, out.ptr))
+
+    def decrypt(self, cts, sk, sk_moduli_id, out, moduli_id):
+        count = int(np.prod(cts.shape[:-3]))
+        _check(library().fhe_decrypt(self.handle, count, moduli_id, cts.ptr,
+                                     sk.ptr, sk_moduli_id, out.ptr))
 
     def profile(self, on):
         """Start (clearing totals) or stop stage timing."""

@@ -789,6 +789,79 @@ int fhe_level_down(fhe_context *ctx, const int64_t *in, int64_t *out,
   });
 }
 
+int fhe_encrypt_pk(fhe_context *ctx, int count, int moduli_id,
+                   const int64_t *pt, const int64_t *pk, int pk_moduli_id,
+                   const int64_t *u, const int64_t *e0, const int64_t *e1,
+                   int64_t *out) {
+  return guarded([&] {
+    require(ctx && pt && pk && u && e0 && e1 && out, "null argument");
+    require(count > 0, "Encrypt: empty batch");
+    ctx->check_moduli(moduli_id, 1, "Encrypt");
+    require(pk_moduli_id >= 0 && pk_moduli_id <= moduli_id,
+            "Encrypt: the key level is lower than the plaintext level.");
+    const auto &P = ctx->params;
+    const int limbs = P.limbs_at(moduli_id);
+    const int pk_limbs = P.limbs_at(pk_moduli_id);
+    const int64_t N = ctx->n();
+    const int64_t poly = int64_t(limbs) * N;
+    int64_t *un = ctx->scratch(size_t(count) * poly * 8);
+    ctx->stage("encrypt", [&] {
+      fhe::launch_tile_unsigned(ctx->tables, u, un, count, limbs, moduli_id,
+                                ctx->stream);
+      fhe::ntt_forward(ctx->tables, un, un, int64_t(count) * limbs, limbs,
+                       moduli_id, ctx->tables.r2, ctx->stream);
+      for (int c = 0; c < count; c++)
+        for (int h = 0; h < 2; h++)
+          fhe::launch_mont_mult(
+              ctx->tables, un + c * poly,
+              pk + (int64_t(h) * pk_limbs + (moduli_id - pk_moduli_id)) * N,
+              out + (2 * int64_t(c) + h) * poly, limbs, limbs, moduli_id,
+              ctx->stream);
+      fhe::ntt_inverse(ctx->tables, out, out, int64_t(count) * 2 * limbs,
+                       limbs, moduli_id, fhe::kInttExit, ctx->stream);
+      fhe::launch_encrypt_finish(ctx->tables, pt, e0, e1, out, count, limbs,
+                                 moduli_id, ctx->stream);
+    });
+    ctx->check_launch("Encrypt");
+  });
+}
+
+int fhe_decrypt(fhe_context *ctx, int count, int moduli_id, const int64_t *ct,
+                const int64_t *sk, int sk_moduli_id, int64_t *out) {
+  return guarded([&] {
+    require(ctx && ct && sk && out, "null argument");
+    require(count > 0, "Decrypt: empty batch");
+    ctx->check_moduli(moduli_id, 1, "Decrypt");
+    require(sk_moduli_id >= 0 && sk_moduli_id <= moduli_id,
+            "Decrypt: the key level is lower than the ciphertext level.");
+    require(static_cast<const void *>(out) != static_cast<const void *>(ct),
+            "Decrypt: the output must not alias the ciphertext");
+    const auto &P = ctx->params;
+    const int limbs = P.limbs_at(moduli_id);
+    const int64_t N = ctx->n();
+    const int64_t poly = int64_t(limbs) * N;
+    const int64_t *s = sk + int64_t(moduli_id - sk_moduli_id) * N;
+    ctx->stage("decrypt", [&] {
+      // out_c = to_intt(to_ntt(c1) * s) (src/ckks/core.cpp:171-194)
+      for (int c = 0; c < count; c++) {
+        fhe::ntt_forward(ctx->tables, ct + (2 * int64_t(c) + 1) * poly,
+                         out + c * poly, limbs, limbs, moduli_id,
+                         ctx->tables.r2, ctx->stream);
+        fhe::launch_mont_mult(ctx->tables, out + c * poly, s, out + c * poly,
+                              limbs, limbs, moduli_id, ctx->stream);
+      }
+      fhe::ntt_inverse(ctx->tables, out, out, int64_t(count) * limbs, limbs,
+                       moduli_id, fhe::kInttExit, ctx->stream);
+      for (int c = 0; c < count; c++)
+        fhe::launch_elementwise(ctx->tables, fhe::kEwAddReduce,
+                                ct + 2 * int64_t(c) * poly, out + c * poly,
+                                out + c * poly, limbs, limbs, moduli_id,
+                                nullptr, ctx->stream);
+    });
+    ctx->check_launch("Decrypt");
+  });
+}
+
 int fhe_profile_enable(fhe_context *ctx, int on) {
   return guarded([&] {
     require(ctx != nullptr, "null argument");

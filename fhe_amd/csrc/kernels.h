@@ -160,6 +160,17 @@ void launch_scalar_rescale(const DeviceTables &t, const int64_t *in,
                            const int64_t *constants, int skip,
                            hipStream_t stream);
 
+// Encryption arithmetic: samples [count][N] lifted to [count][limbs][N]
+// with make_unsigned_reduce_2q_to_q, and the error / plaintext fold of
+// CKKSCore::encrypt (src/ckks/core.cpp:35-82) into out [count][2][limbs][N].
+void launch_tile_unsigned(const DeviceTables &t, const int64_t *samples,
+                          int64_t *out, int count, int limbs, int moduli_id,
+                          hipStream_t stream);
+void launch_encrypt_finish(const DeviceTables &t, const int64_t *pt,
+                           const int64_t *e0, const int64_t *e1, int64_t *out,
+                           int count, int limbs, int moduli_id,
+                           hipStream_t stream);
+
 // Rescale of `polys` INTT-form polynomials (src/ckks/core_rescale.cpp:5-25).
 void launch_rescale(const DeviceTables &t, const int64_t *in, int64_t *out,
                     int polys, int limbs, int moduli_id, hipStream_t stream);

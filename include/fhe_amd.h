@@ -212,6 +212,24 @@ int fhe_add_double(fhe_context *ctx, const int64_t *in, int64_t *out,
 int fhe_level_down(fhe_context *ctx, const int64_t *in, int64_t *out,
                    int polys, int level, int rescaled, int dst_level);
 
+/* Public-key encryption of `count` INTT-form plaintexts pt
+ * [count][limbs][N] at moduli_id with the public key pk [2][pk_limbs][N]
+ * (NTT form, from pk_moduli_id, device) and the host-drawn samples u
+ * (binary), e0, e1 [count][N] (small signed integers, device) into INTT-form
+ * duos out [count][2][limbs][N] (CKKSCore::encrypt(pt, pk),
+ * src/ckks/core.cpp:35-82; the samples are what get_binary and
+ * get_single_error draw, src/ckks/core_random.cpp:7-74). */
+int fhe_encrypt_pk(fhe_context *ctx, int count, int moduli_id,
+                   const int64_t *pt, const int64_t *pk, int pk_moduli_id,
+                   const int64_t *u, const int64_t *e0, const int64_t *e1,
+                   int64_t *out);
+
+/* Decryption of `count` INTT-form duos to plaintext limbs [count][limbs][N]
+ * with the secret key sk [sk_limbs][N] (NTT form, from sk_moduli_id)
+ * (CKKSCore::decrypt_duo, src/ckks/core.cpp:171-194). */
+int fhe_decrypt(fhe_context *ctx, int count, int moduli_id, const int64_t *ct,
+                const int64_t *sk, int sk_moduli_id, int64_t *out);
+
 /* Automorphism X -> X^g, g = 5^-delta mod 2N (g = -1 for delta = -1, the
  * conjugation), of polys x limbs limbs at moduli_id into out (no aliasing).
  * ntt_form = 0: coefficient-domain permutation with signs and

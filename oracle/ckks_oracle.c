@@ -741,3 +741,68 @@ void oracle_elementwise_limb(int op, const int64_t *x1, const int64_t *x2,
     y[i] = r;
   }
 }
+
+/* make_unsigned_reduce_2q_to_q of one signed value (src/montgomery/
+ * compound.cpp:85-91), as tile_unsigned_reduce_2q_to_q_ applies it to a
+ * single-limb sample on every limb (src/ckks/core_calculate.cpp:485-531) */
+static int64_t unsigned_reduce(int64_t x, int64_t q) {
+  return x < 0 ? x + q : guard(x, q);
+}
+
+/* CKKSCore::encrypt(pt, pk) (src/ckks/core.cpp:35-82) with the host's
+ * samples u (binary), e0, e1 (N signed values each) given explicitly:
+ * c_h = to_intt(to_ntt(tile(u)) * pk_h) + err_h, err0 = pt + tile(e0),
+ * err1 = tile(e1), every sum through mont_add_reduce_2q_to_q.  pk is the
+ * public key's [2][pk_limbs][N] from pk_moduli_id; out is [2][limbs][N]. */
+void oracle_encrypt_pk(const oracle_ctx *c, int mid, const int64_t *pt,
+                       const int64_t *pk, int pk_mid, const int64_t *u,
+                       const int64_t *e0, const int64_t *e1, int64_t *out) {
+  const int n = c->n;
+  const int limbs = c->chain_count - mid;
+  const int pk_limbs = c->chain_count - pk_mid;
+  const size_t poly = (size_t)limbs * n;
+  int64_t *un = malloc(sizeof(int64_t) * poly);
+  for (int l = 0; l < limbs; l++)
+    for (int j = 0; j < n; j++)
+      un[(size_t)l * n + j] = unsigned_reduce(u[j], (int64_t)c->q[mid + l]);
+  oracle_to_ntt(c, mid, limbs, un);
+  for (int h = 0; h < 2; h++) {
+    const int64_t *pkh = pk + ((size_t)h * pk_limbs + (mid - pk_mid)) * n;
+    oracle_mont_mult_limbs(c, mid, limbs, un, pkh, out + h * poly);
+    oracle_to_intt(c, mid, limbs, out + h * poly);
+  }
+  for (int l = 0; l < limbs; l++) {
+    const int64_t q = (int64_t)c->q[mid + l], q2 = (int64_t)c->q2[mid + l];
+    for (int j = 0; j < n; j++) {
+      const size_t o = (size_t)l * n + j;
+      const int64_t err0 =
+          guard(guard(pt[o] + unsigned_reduce(e0[j], q), q2), q);
+      const int64_t err1 = unsigned_reduce(e1[j], q);
+      out[o] = guard(guard(out[o] + err0, q2), q);
+      out[poly + o] = guard(guard(out[poly + o] + err1, q2), q);
+    }
+  }
+  free(un);
+}
+
+/* CKKSCore::decrypt_duo (src/ckks/core.cpp:171-194): pt = c0 +
+ * to_intt(to_ntt(c1) * s), the sum through mont_add_reduce_2q_to_q.  sk is
+ * the secret key's [sk_limbs][N] from sk_moduli_id. */
+void oracle_decrypt(const oracle_ctx *c, int mid, const int64_t *ct,
+                    const int64_t *sk, int sk_mid, int64_t *out) {
+  const int n = c->n;
+  const int limbs = c->chain_count - mid;
+  const size_t poly = (size_t)limbs * n;
+  memcpy(out, ct + poly, sizeof(int64_t) * poly);
+  oracle_to_ntt(c, mid, limbs, out);
+  oracle_mont_mult_limbs(c, mid, limbs, out, sk + (size_t)(mid - sk_mid) * n,
+                         out);
+  oracle_to_intt(c, mid, limbs, out);
+  for (int l = 0; l < limbs; l++) {
+    const int64_t q = (int64_t)c->q[mid + l], q2 = (int64_t)c->q2[mid + l];
+    for (int j = 0; j < n; j++) {
+      const size_t o = (size_t)l * n + j;
+      out[o] = guard(guard(ct[o] + out[o], q2), q);
+    }
+  }
+}

@@ -97,6 +97,14 @@ void oracle_rotate(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
 void oracle_conjugate(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
                       const int64_t *key, int key_moduli_id, int64_t *out);
 
+/* public-key encryption with explicit samples u, e0, e1 (N values each)
+ * and duo decryption to plaintext limbs */
+void oracle_encrypt_pk(const oracle_ctx *ctx, int moduli_id, const int64_t *pt,
+                       const int64_t *pk, int pk_moduli_id, const int64_t *u,
+                       const int64_t *e0, const int64_t *e1, int64_t *out);
+void oracle_decrypt(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
+                    const int64_t *sk, int sk_moduli_id, int64_t *out);
+
 /* one limb of a per-coefficient primitive (codes of FHE_EW_*) */
 void oracle_elementwise_limb(int op, const int64_t *x1, const int64_t *x2,
                              int64_t *y, int n, uint64_t q, uint64_t k,

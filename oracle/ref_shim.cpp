@@ -325,14 +325,19 @@ void ref_free_conjugation_key(void *p) {
 int ref_fixed_rotation_key_delta(void *frk) {
   return static_cast<FixedRotationKey *>(frk)->delta_;
 }
-static KeySwitchingKey *ksk_of(void *key, int kind) {
+static Data *ksk_of(void *key, int kind) {
   if (kind == 0)
     return static_cast<RelinearizationKey *>(key)->key_switching_key_;
   if (kind == 1)
     return static_cast<FixedRotationKey *>(key)->key_switching_key_;
-  return static_cast<ConjugationKey *>(key)->key_switching_key_;
+  if (kind == 2)
+    return static_cast<ConjugationKey *>(key)->key_switching_key_;
+  if (kind == 3)
+    return static_cast<SecretKey *>(key);
+  return static_cast<PublicKey *>(key);
 }
-// kind: 0 relinearization, 1 fixed rotation, 2 conjugation
+// kind: 0 relinearization, 1 fixed rotation, 2 conjugation, 3 secret key,
+// 4 public key
 int ref_key_shape(void *key, int kind, int64_t *shape) {
   data_shape(ksk_of(key, kind), shape);
   return 0;
@@ -493,6 +498,22 @@ void *ref_negate(void *handle, void *ct) {
   return guarded([&]() -> void * {
     return static_cast<RefEngine *>(handle)->engine->negate(
         static_cast<UnitCiphertext *>(ct));
+  });
+}
+
+// CKKSEngine::decrypt_to_unit_plaintext (src/ckks/engine_decrypt.cpp):
+// the plaintext limbs before decoding.  shape as ref_ct_shape.
+int ref_decrypt_limbs(void *handle, void *ct, void *sk, int64_t *shape,
+                      int64_t *out) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    auto pt = ref->engine->decrypt_to_unit_plaintext(
+        static_cast<UnitCiphertext *>(ct), static_cast<SecretKey *>(sk));
+    data_shape(pt, shape);
+    if (out)
+      copy_out(pt, out);
+    delete pt;
+    return 0;
   });
 }
 

@@ -83,6 +83,9 @@ class Oracle:
             lib.oracle_partitions.argtypes = [VP, ctypes.c_int, ctypes.c_int,
                                               VP, VP, VP]
             i32 = ctypes.c_int
+            lib.oracle_encrypt_pk.argtypes = [VP, i32, VP, VP, i32, VP, VP,
+                                              VP, VP]
+            lib.oracle_decrypt.argtypes = [VP, i32, VP, VP, i32, VP]
             lib.oracle_automorphism.argtypes = [VP, i32, i32, i32, i32, VP, VP]
             lib.oracle_rotate.argtypes = [VP, i32, VP, VP, i32, i32, VP]
             lib.oracle_conjugate.argtypes = [VP, i32, VP, VP, i32, VP]
@@ -209,6 +212,22 @@ class Oracle:
                                     ptr(out))
         return out
 
+    def encrypt_pk(self, pt, pk, pk_mid, u, e0, e1, mid):
+        out = np.zeros((2, self.limbs_at(mid), self.n), np.int64)
+        c = [np.ascontiguousarray(v, dtype=np.int64)
+             for v in (pt, pk, u, e0, e1)]
+        self.lib().oracle_encrypt_pk(self.h, mid, ptr(c[0]), ptr(c[1]), pk_mid,
+                                     ptr(c[2]), ptr(c[3]), ptr(c[4]),
+                                     ptr(out))
+        return out
+
+    def decrypt(self, ct, sk, sk_mid, mid):
+        out = np.zeros((self.limbs_at(mid), self.n), np.int64)
+        self.lib().oracle_decrypt(self.h, mid, ptr(np.ascontiguousarray(ct)),
+                                  ptr(np.ascontiguousarray(sk)), sk_mid,
+                                  ptr(out))
+        return out
+
     def partitions(self, mid, key_mid):
         s = np.zeros(128, np.int32)
         z = np.zeros(128, np.int32)
@@ -300,6 +319,7 @@ class Reference:
             "ref_add_double": (VP, [VP, VP, ctypes.c_double]),
             "ref_level_down": (VP, [VP, VP, i32]),
             "ref_negate": (VP, [VP, VP]),
+            "ref_decrypt_limbs": (i32, [VP, VP, VP, VP, VP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -343,6 +363,22 @@ class Reference:
     def set(self, ct, data):
         self.lib.ref_ct_set(ct, ptr(np.ascontiguousarray(data,
                                                          dtype=np.int64)))
+
+    def key_data(self, key, kind):
+        """(moduli_id, limbs array) of a key object (kind as ref_key_shape:
+        0 relin, 1 rotation, 2 conjugation, 3 secret, 4 public)."""
+        s = np.zeros(4, np.int64)
+        self.lib.ref_key_shape(key, kind, ptr(s))
+        out = np.zeros((int(s[1]), int(s[2]), int(s[3])), np.int64)
+        self.lib.ref_key_get(key, kind, ptr(out))
+        return int(s[0]), out
+
+    def decrypt_limbs(self, ct, sk):
+        s = np.zeros(4, np.int64)
+        self.lib.ref_decrypt_limbs(self.e, ct, sk, ptr(s), None)
+        out = np.zeros((int(s[2]), int(s[3])), np.int64)
+        self.lib.ref_decrypt_limbs(self.e, ct, sk, ptr(s), ptr(out))
+        return int(s[0]), out
 
     def key(self, rlk):
         s = np.zeros(4, np.int64)

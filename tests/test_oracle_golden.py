@@ -15,7 +15,7 @@ import random
 import numpy as np
 import pytest
 
-from support import Oracle, Reference, golden, hmult_inputs
+from support import Oracle, Reference, golden, hmult_inputs, ptr
 
 
 # ---------------------------------------------------------------- 1. KATs
@@ -511,3 +511,54 @@ def test_elementwise_closed_forms():
     assert all(v % q == x * c * rinv * rinv % q for v, x in zip(run(12), A))
     assert all(v % q == ((x - y) * c * rinv) % q
                for v, x, y in zip(run(10), A, B))
+
+
+# ------------------------------------------------ encryption / decryption
+
+def _small_signed(rng, n, bound):
+    return rng.integers(-bound, bound + 1, n, dtype=np.int64)
+
+
+@pytest.mark.reference
+@pytest.mark.skipif(not Reference.available(),
+                    reason="oracle/_ref (compiled reference) not built")
+@pytest.mark.parametrize("log_n,alpha,level", [(13, 1, 2), (14, 2, 5)])
+def test_oracle_decrypt_and_encrypt_against_reference(log_n, alpha, level):
+    """Oracle decrypt equals the reference's decrypt_to_unit_plaintext limb
+    for limb on reference encryptions; an oracle encryption of a message m
+    with samples (u, e0, e1) decrypts, through the reference, to m plus a
+    noise polynomial whose centered coefficients are small and equal across
+    limbs (a dropped or mis-signed term would not be)."""
+    ref = Reference(log_n, alpha, threads=4)
+    lib, e = ref.lib, ref.e
+    sk = lib.ref_secret_key(e)
+    pk = lib.ref_public_key(e, sk)
+    sk_mid, skd = ref.key_data(sk, 3)
+    pk_mid, pkd = ref.key_data(pk, 4)
+    orc = Oracle(log_n, ref.primes, alpha)
+    x = np.random.default_rng(4).uniform(-1, 1, ref.slot_count)
+    ct = lib.ref_encrypt(e, ptr(x), ref.slot_count, pk, level)
+    mid = ref.shape(ct)[0]
+    pmid, expect = ref.decrypt_limbs(ct, sk)
+    assert pmid == mid
+    assert (orc.decrypt(ref.get(ct), skd[0], sk_mid, mid) == expect).all()
+
+    rng = np.random.default_rng(9)
+    n = ref.n
+    m = _small_signed(rng, n, 1 << 30)
+    u = rng.integers(0, 2, n, dtype=np.int64)
+    e0, e1 = _small_signed(rng, n, 20), _small_signed(rng, n, 20)
+    qs = ref.primes[mid:mid + orc.limbs_at(mid)].astype(np.int64)[:, None]
+    pt = np.where(m < 0, m + qs, m)
+    oct_ = orc.encrypt_pk(pt, pkd, pk_mid, u, e0, e1, mid)
+    ref.set(ct, oct_)
+    _, dec = ref.decrypt_limbs(ct, sk)
+    noise = (dec - pt) % qs
+    noise = np.where(noise > qs // 2, noise - qs, noise)
+    assert (noise == noise[0]).all()
+    assert np.abs(noise[0]).max() < 1 << 20
+    assert np.abs(noise[0]).max() > 0
+    lib.ref_free_ct(ct)
+    lib.ref_free_public_key(pk)
+    lib.ref_free_secret_key(sk)
+    ref.close()
