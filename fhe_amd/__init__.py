@@ -335,12 +335,13 @@ class Context:
     def multiplier(self, kind, level, value=0.0, int_value=0, level2=0):
         """Per-limb scalars of a CKKSPrecomputed multiplier (MULTIPLIER
         kinds); host-only."""
-        out = np.zeros(self.prime_count, np.int64)
+        prime_count = self.info["prime_count"]
+        out = np.zeros(prime_count, np.int64)
         count = ctypes.c_int()
         _check(library().fhe_multiplier(self.handle, MULTIPLIER[kind],
                                         float(value), int(int_value), level,
                                         level2, out.ctypes.data,
-                                        self.prime_count, ctypes.byref(count)))
+                                        prime_count, ctypes.byref(count)))
         return out[:count.value]
 
     def multiply_double(self, src, dst, level, value):

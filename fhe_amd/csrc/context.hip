@@ -95,6 +95,8 @@ struct fhe_context {
   // growable scratch for the fused pipelines
   void *workspace = nullptr;
   size_t workspace_bytes = 0;
+  // lazy transforms where results are canonicalized before use
+  bool lazy_key_switch = true;
   // stage timing: HIP events recorded on `stream` around each stage while
   // enabled, folded into per-stage totals when read
   bool timing = false;
@@ -120,6 +122,8 @@ struct fhe_context {
     t.ninv = upload(params.ninv, owned);
     t.root = upload(params.root, owned);
     t.root_inverse = upload(params.root_inverse, owned);
+    t.root_shoup = upload(params.root_shoup, owned);
+    t.root_inverse_shoup = upload(params.root_inverse_shoup, owned);
     t.p_inverse = upload(params.p_inverse_mult_r, owned);
     t.rescale = upload(params.rescale_scale, owned);
     t.prime_count = params.prime_count;
@@ -265,21 +269,26 @@ struct fhe_context {
                             params.modup_non_negative(moduli_id, parts[p]),
                             stream);
         });
+      // The extended polynomials, their products with the key and the
+      // accumulators are only read by ModDown after the canonicalizing
+      // exit, so their transforms use the lazy arithmetic.
+      const fhe::LazySpec lazy = lazy_key_switch ? fhe::kAllLazy
+                                                 : fhe::LazySpec();
       stage("ntt_pass_high(ext)", [&] {
         fhe::ntt_forward_first(tables, ext, ext,
                                int64_t(count) * np * ext_limbs, ext_limbs,
-                               moduli_id, nullptr, stream);
+                               moduli_id, nullptr, stream, lazy);
       });
       stage("ks_inner_product_r16", [&] {
         fhe::launch_ks_inner_product_fused(tables, ext, key, key_chain,
                                            key_offset, parts[0].key_index, np,
                                            acc, count, ext_limbs, moduli_id,
-                                           stream);
+                                           lazy_key_switch, stream);
       });
       stage("intt_pass_high(acc)", [&] {
         fhe::ntt_inverse_second(tables, acc, acc,
                                 int64_t(count) * 2 * ext_limbs, ext_limbs,
-                                moduli_id, fhe::kInttExitReduce, stream);
+                                moduli_id, fhe::kInttExitReduce, stream, lazy);
       });
       return;
     }
@@ -531,9 +540,11 @@ int fhe_relinearize(fhe_context *ctx, int count, int moduli_id,
     const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
     int64_t *ws = ctx->scratch(size_t(trio_l + ext_l + acc_l) * N * 8);
     int64_t *trio = ws, *ext = trio + trio_l * N, *acc = ext + ext_l * N;
+    const fhe::LazySpec d01 =
+        ctx->lazy_key_switch ? fhe::LazySpec{3, 2} : fhe::LazySpec();
     ctx->stage("intt(trio)", [&] {
       fhe::ntt_inverse(ctx->tables, trios, trio, trio_l, limbs, moduli_id,
-                       fhe::kInttExit, ctx->stream);
+                       fhe::kInttExit, ctx->stream, d01);
     });
     ctx->key_switch(count, moduli_id, key, key_moduli_id, trio + 2 * limbs * N,
                     3 * limbs * N, ext, acc);
@@ -568,20 +579,27 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
     int64_t *an = ws, *bn = an + duo_l * N, *trio = bn + duo_l * N;
     int64_t *ext = trio + trio_l * N, *acc = ext + ext_l * N;
     // prepare_for_multiply: to NTT form (src/ckks/engine_multiply.cpp:13-27)
+    // Only a1 * b1 reaches the representative-sensitive ModUp (through the
+    // exact INTT of d2), so the transforms of a0 and b0 may be lazy.
+    const fhe::LazySpec first_poly =
+        ctx->lazy_key_switch ? fhe::LazySpec{2, 1} : fhe::LazySpec();
     ctx->stage("ntt(inputs)", [&] {
       fhe::ntt_forward(ctx->tables, a, an, duo_l, limbs, moduli_id,
-                       ctx->tables.r2, ctx->stream);
+                       ctx->tables.r2, ctx->stream, first_poly);
       fhe::ntt_forward(ctx->tables, b, bn, duo_l, limbs, moduli_id,
-                       ctx->tables.r2, ctx->stream);
+                       ctx->tables.r2, ctx->stream, first_poly);
     });
     ctx->stage("tensor", [&] {
       fhe::launch_tensor(ctx->tables, an, bn, trio, count, limbs, moduli_id,
                          ctx->stream);
     });
-    // relinearize: to_intt of all three, key switch of the third
+    // relinearize: to_intt of all three, key switch of the third; d0 and d1
+    // are canonicalized by the relinearization add, d2 feeds ModUp exactly
+    const fhe::LazySpec d01 =
+        ctx->lazy_key_switch ? fhe::LazySpec{3, 2} : fhe::LazySpec();
     ctx->stage("intt(trio)", [&] {
       fhe::ntt_inverse(ctx->tables, trio, trio, trio_l, limbs, moduli_id,
-                       fhe::kInttExit, ctx->stream);
+                       fhe::kInttExit, ctx->stream, d01);
     });
     ctx->key_switch(count, moduli_id, key, key_moduli_id, trio + 2 * limbs * N,
                     3 * limbs * N, ext, acc);

@@ -21,6 +21,9 @@ struct DeviceTables {
   const int64_t *ninv;
   const int64_t *root;          // [prime][N]
   const int64_t *root_inverse;  // [prime][N]
+  // standard-form twiddles with Shoup quotients, [prime][N][2]
+  const uint64_t *root_shoup;
+  const uint64_t *root_inverse_shoup;
   const int64_t *p_inverse;     // [alpha][prime_count]
   const int64_t *rescale;       // [prime_count][prime_count]
   int prime_count;

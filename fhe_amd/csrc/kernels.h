@@ -7,27 +7,41 @@
 
 namespace fhe {
 
+// Which polynomials of a batched transform may use the lazy arithmetic
+// (ntt_r16.h): those whose index modulo `period` is below `count`.  Lazy
+// outputs are other representatives of the same residues: forward outputs
+// below 2q, or below 2^48 for primes under 2^42; inverse outputs after the
+// exit as in the exact form.  The default is exact everywhere.
+struct LazySpec {
+  int period = 0;
+  int count = 0;
+};
+constexpr LazySpec kAllLazy{1, 1};
+
 // In-place capable forward NTT of total_limbs limbs.  pre (device, indexed by
 // absolute prime) multiplies each input value in Montgomery form first, e.g.
 // R^2 for the reference's to_ntt (mont_enter then ntt); nullptr for none.
 void ntt_forward(const DeviceTables &t, const int64_t *src, int64_t *dst,
                  int64_t total_limbs, int limbs, int moduli_id,
-                 const int64_t *pre, hipStream_t stream);
+                 const int64_t *pre, hipStream_t stream,
+                 LazySpec lazy = LazySpec());
 
 // In-place capable inverse NTT with the InttPost epilogue.
 void ntt_inverse(const DeviceTables &t, const int64_t *src, int64_t *dst,
                  int64_t total_limbs, int limbs, int moduli_id, int post,
-                 hipStream_t stream);
+                 hipStream_t stream, LazySpec lazy = LazySpec());
 
 // Single passes of the two-pass transform (N >= 2^13): the forward pass over
 // the high index bits, and the inverse pass over the high index bits with
 // the InttPost epilogue.  The remaining passes run inside fused kernels.
 void ntt_forward_first(const DeviceTables &t, const int64_t *src, int64_t *dst,
                        int64_t total_limbs, int limbs, int moduli_id,
-                       const int64_t *pre, hipStream_t stream);
+                       const int64_t *pre, hipStream_t stream,
+                       LazySpec lazy = LazySpec());
 void ntt_inverse_second(const DeviceTables &t, const int64_t *src,
                         int64_t *dst, int64_t total_limbs, int limbs,
-                        int moduli_id, int post, hipStream_t stream);
+                        int moduli_id, int post, hipStream_t stream,
+                        LazySpec lazy = LazySpec());
 
 // Fused key-switching core for N = 2^16: for each tile of each extended limb
 // and ciphertext, the low-bit forward NTT pass of every partition's
@@ -40,7 +54,7 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
                                    const int64_t *key, int key_chain,
                                    int key_offset, int key_partition0,
                                    int parts, int64_t *acc, int count,
-                                   int ext_limbs, int moduli_id,
+                                   int ext_limbs, int moduli_id, bool lazy,
                                    hipStream_t stream);
 
 // out = mont_mult(a, b) elementwise (src/montgomery/unit.cpp:5-10).

@@ -29,25 +29,19 @@ struct KsArgs {
   int key_offset;      // first key limb aligned with the ciphertext's limb 0
   int key_partition0;  // key partition index of the ciphertext's partition 0
   int swizzle;
+  int lazy;            // lazy transforms (the ext pass before was lazy too)
 };
 
-__global__ __launch_bounds__(kBlock) void ks_inner_product_r16_kernel(
-    KsArgs a, DeviceTables t) {
-  __shared__ int64_t lds[kR16Lds];
+// One (extended limb, tile, ciphertext) work item with arithmetic A for the
+// two transforms (the key products and the accumulation are the same in
+// every form: Montgomery products of operands whose product stays below
+// q * 2^62, accumulated with the reference's guards).
+template <int A>
+__device__ __forceinline__ void ks_tile(const KsArgs &a, const DeviceTables &t,
+                                        int64_t *lds, int ct, int tile,
+                                        int i) {
   const int64_t n = int64_t(1) << t.log_n;
-  const int tiles = int(n >> 12);
-
-  int64_t w = blockIdx.x;
-  if (a.swizzle) {
-    const int64_t per = int64_t(gridDim.x) >> 3;
-    w = int64_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  }
-  const int ct = int(w % a.count);
-  const int64_t rest = w / a.count;
-  const int tile = int(rest % tiles);
-  const int i = int(rest / tiles);
   const int p = a.moduli_id + i;
-
   const int tid = threadIdx.x;
   const int r = tid & 15;
   const int c = tid >> 4;
@@ -56,6 +50,7 @@ __global__ __launch_bounds__(kBlock) void ks_inner_product_r16_kernel(
 
   R16Pass f;
   f.tw = t.root + int64_t(p) * n;
+  f.ts = t.root_shoup + int64_t(p) * n * 2;
   f.n = n;
   f.hi = sp;
   f.log_n = t.log_n;
@@ -73,9 +68,10 @@ __global__ __launch_bounds__(kBlock) void ks_inner_product_r16_kernel(
 #pragma unroll
     for (int m = 0; m < 16; m++)
       v[m] = src[r + 16 * m];
-    r16_forward_high(v, f);
+    r16_forward_high<A>(v, f);
     r16_exchange(v, lds, c, r, true);
-    r16_forward_low(v, f);
+    r16_forward_low<A>(v, f);
+    r16_forward_settle<A>(v, f);
     // v[m] is now the NTT value at coefficient off + 16 r + m
     const int kp = a.key_partition0 + pp;
     const int64_t *k0 =
@@ -103,20 +99,45 @@ __global__ __launch_bounds__(kBlock) void ks_inner_product_r16_kernel(
 
   R16Pass g = f;
   g.tw = t.root_inverse + int64_t(p) * n;
+  g.ts = t.root_inverse_shoup + int64_t(p) * n * 2;
   int64_t *d0 = a.acc + (int64_t(ct) * 2 * a.ext_limbs + i) * n + off;
   int64_t *d1 = d0 + int64_t(a.ext_limbs) * n;
-  r16_inverse_low(acc0, g);
+  r16_inverse_low<A>(acc0, g);
   r16_exchange(acc0, lds, c, r, false);
-  r16_inverse_high(acc0, g);
+  r16_inverse_high<A>(acc0, g);
 #pragma unroll
   for (int m = 0; m < 16; m++)
     d0[r + 16 * m] = acc0[m];
-  r16_inverse_low(acc1, g);
+  r16_inverse_low<A>(acc1, g);
   r16_exchange(acc1, lds, c, r, false);
-  r16_inverse_high(acc1, g);
+  r16_inverse_high<A>(acc1, g);
 #pragma unroll
   for (int m = 0; m < 16; m++)
     d1[r + 16 * m] = acc1[m];
+}
+
+__global__ __launch_bounds__(kBlock) void ks_inner_product_r16_kernel(
+    KsArgs a, DeviceTables t) {
+  __shared__ int64_t lds[kR16Lds];
+  const int64_t n = int64_t(1) << t.log_n;
+  const int tiles = int(n >> 12);
+
+  int64_t w = blockIdx.x;
+  if (a.swizzle) {
+    const int64_t per = int64_t(gridDim.x) >> 3;
+    w = int64_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  const int ct = int(w % a.count);
+  const int64_t rest = w / a.count;
+  const int tile = int(rest % tiles);
+  const int i = int(rest / tiles);
+  const uint64_t q = t.q[a.moduli_id + i];
+  if (!a.lazy)
+    ks_tile<kExact>(a, t, lds, ct, tile, i);
+  else if (q < kLazySmallBound)
+    ks_tile<kLazySmall>(a, t, lds, ct, tile, i);
+  else
+    ks_tile<kLazyLarge>(a, t, lds, ct, tile, i);
 }
 
 } // namespace
@@ -127,13 +148,14 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
                                    const int64_t *key, int key_chain,
                                    int key_offset, int key_partition0,
                                    int parts, int64_t *acc, int count,
-                                   int ext_limbs, int moduli_id,
+                                   int ext_limbs, int moduli_id, bool lazy,
                                    hipStream_t stream) {
   const int tiles = int((int64_t(1) << t.log_n) >> 12);
   const int64_t blocks = int64_t(ext_limbs) * tiles * count;
   KsArgs a{ext,        key,       acc,        count,
            parts,      ext_limbs, moduli_id,  key_chain,
-           key_offset, key_partition0, blocks % 8 == 0 ? 1 : 0};
+           key_offset, key_partition0, blocks % 8 == 0 ? 1 : 0,
+           lazy ? 1 : 0};
   hipLaunchKernelGGL(ks_inner_product_r16_kernel, dim3(unsigned(blocks)),
                      dim3(kBlock), 0, stream, a, t);
 }

@@ -142,12 +142,11 @@ FHE_HD int64_t guard_fast(int64_t x, int64_t q) {
   return int32_t(uint64_t(d) >> 32) < 0 ? x : d;
 }
 
-// guard() for NON-NEGATIVE x and q: the borrow out of the unsigned x - q is
-// the condition x < q, so the device sequence is sub, subb and two selects.
+// guard() for NON-NEGATIVE x and q (an unsigned compare).  Written as a
+// plain compare-and-select: on gfx950 this measured faster inside the
+// butterflies than a borrow-based form.
 FHE_HD int64_t guard_u(int64_t x, int64_t q) {
-  uint64_t d;
-  const bool borrow = __builtin_sub_overflow(uint64_t(x), uint64_t(q), &d);
-  return borrow ? x : int64_t(d);
+  return uint64_t(x) < uint64_t(q) ? x : x - q;
 }
 
 // Montgomery product of arbitrary int64 operands: the 128-bit product of the

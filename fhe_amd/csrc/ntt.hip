@@ -34,6 +34,9 @@ struct PassArgs {
   int bits;           // L <= 12
   const int64_t *pre; // forward first pass: multiply by pre[prime] on load
   int post;           // inverse last pass: InttPost
+  int last;           // the pass completes the transform
+  int lazy_period;    // lazy arithmetic for polynomial index % period <
+  int lazy_count;     //   count (radix-16 passes only; 0: all exact)
 };
 
 template <bool INV>
@@ -145,26 +148,13 @@ __global__ __launch_bounds__(kBlock) void ntt_pass_kernel(PassArgs a,
 // not, so that every global access is coalesced.  Prime, twiddle base and
 // addresses are per-block scalars.
 // ---------------------------------------------------------------------------
-template <bool INV, bool COLS>
-__global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
-    PassArgs a, DeviceTables t, int polys, int tiles_per_limb, int swizzle) {
-  __shared__ int64_t lds[kR16Lds];
+template <bool INV, bool COLS, int A>
+__device__ __forceinline__ void r16_pass(const PassArgs &a,
+                                         const DeviceTables &t, int64_t *lds,
+                                         int64_t poly, int tile, int l) {
   const int log_n = t.log_n;
   const int64_t n = int64_t(1) << log_n;
   const int b = a.low_bit;
-
-  // block -> (limb l, tile, polynomial), polynomial fastest so blocks sharing
-  // a prime's twiddles run together; with swizzle each of the 8 XCDs (blocks
-  // are dealt round-robin to XCDs) gets one contiguous range of that order.
-  int64_t w = blockIdx.x;
-  if (swizzle) {
-    const int64_t per = int64_t(gridDim.x) >> 3;
-    w = int64_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  }
-  const int64_t poly = w % polys;
-  const int64_t rest = w / polys;
-  const int tile = int(rest % tiles_per_limb);
-  const int l = int(rest / tiles_per_limb);
   const int p = a.moduli_id + l;
 
   const int tid = threadIdx.x;
@@ -179,6 +169,7 @@ __global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
 
   R16Pass s;
   s.tw = (INV ? t.root_inverse : t.root) + int64_t(p) * n;
+  s.ts = (INV ? t.root_inverse_shoup : t.root_shoup) + int64_t(p) * n * 2;
   s.n = n;
   s.hi = hi;
   s.log_n = log_n;
@@ -205,15 +196,17 @@ __global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
     r16_exchange(v, lds, c, r, true);
 
   if (!INV) {
-    r16_forward_high(v, s);
+    r16_forward_high<A>(v, s);
     r16_exchange(v, lds, c, r, true);
-    r16_forward_low(v, s);
+    r16_forward_low<A>(v, s);
+    if (a.last)
+      r16_forward_settle<A>(v, s);
     if (!COLS)
       r16_exchange(v, lds, c, r, false); // coalesced contiguous store
   } else {
-    r16_inverse_low(v, s);
+    r16_inverse_low<A>(v, s);
     r16_exchange(v, lds, c, r, false);
-    r16_inverse_high(v, s);
+    r16_inverse_high<A>(v, s);
     if (a.post != kInttRaw) {
       const int64_t ninv = t.ninv[p];
 #pragma unroll
@@ -228,6 +221,34 @@ __global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
 #pragma unroll
   for (int m = 0; m < 16; m++)
     dst[int64_t(cols_store ? 16 * r + m : r + 16 * m) << b] = v[m];
+}
+
+// Register-resident pass kernel.  Polynomials whose index modulo
+// a.lazy_period is below a.lazy_count take the lazy arithmetic of their
+// prime's size class; the others the exact one.
+template <bool INV, bool COLS>
+__global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
+    PassArgs a, DeviceTables t, int polys, int tiles_per_limb, int swizzle) {
+  __shared__ int64_t lds[kR16Lds];
+  // block -> (limb l, tile, polynomial), polynomial fastest so blocks sharing
+  // a prime's twiddles run together; with swizzle each of the 8 XCDs (blocks
+  // are dealt round-robin to XCDs) gets one contiguous range of that order.
+  int64_t w = blockIdx.x;
+  if (swizzle) {
+    const int64_t per = int64_t(gridDim.x) >> 3;
+    w = int64_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  const int64_t poly = w % polys;
+  const int64_t rest = w / polys;
+  const int tile = int(rest % tiles_per_limb);
+  const int l = int(rest / tiles_per_limb);
+  const bool lazy = a.lazy_period > 0 && (poly % a.lazy_period) < a.lazy_count;
+  if (!lazy)
+    r16_pass<INV, COLS, kExact>(a, t, lds, poly, tile, l);
+  else if (t.q[a.moduli_id + l] < kLazySmallBound)
+    r16_pass<INV, COLS, kLazySmall>(a, t, lds, poly, tile, l);
+  else
+    r16_pass<INV, COLS, kLazyLarge>(a, t, lds, poly, tile, l);
 }
 
 // Split log_n into the bit ranges of at most two passes (each <= 12 bits).
@@ -281,53 +302,67 @@ void launch_pass(bool inverse, const PassArgs &a, const DeviceTables &t,
 
 void ntt_forward(const DeviceTables &t, const int64_t *src, int64_t *dst,
                  int64_t total_limbs, int limbs, int moduli_id,
-                 const int64_t *pre, hipStream_t stream) {
+                 const int64_t *pre, hipStream_t stream, LazySpec lazy) {
   int l1, l2;
   split_bits(t.log_n, l1, l2);
   // first pass: the high bits (stages 0 .. l1-1)
-  PassArgs a{src, dst, total_limbs << (t.log_n - l1), limbs, moduli_id,
-             t.log_n - l1, l1, pre, kInttRaw};
+  PassArgs a{src,     dst,      total_limbs << (t.log_n - l1),
+             limbs,   moduli_id, t.log_n - l1,
+             l1,      pre,       kInttRaw,
+             l2 == 0, lazy.period, lazy.count};
   launch_pass(false, a, t, stream);
   if (l2 > 0) {
-    PassArgs b2{dst, dst, total_limbs << (t.log_n - l2), limbs, moduli_id,
-                0, l2, nullptr, kInttRaw};
+    PassArgs b2{dst,   dst,      total_limbs << (t.log_n - l2),
+                limbs, moduli_id, 0,
+                l2,    nullptr,   kInttRaw,
+                1,     lazy.period, lazy.count};
     launch_pass(false, b2, t, stream);
   }
 }
 
 void ntt_inverse(const DeviceTables &t, const int64_t *src, int64_t *dst,
                  int64_t total_limbs, int limbs, int moduli_id, int post,
-                 hipStream_t stream) {
+                 hipStream_t stream, LazySpec lazy) {
   int l1, l2;
   split_bits(t.log_n, l1, l2);
   // first pass: the low bits (stages with span 1 .. 2^(l1-1))
-  PassArgs a{src, dst, total_limbs << (t.log_n - l1), limbs, moduli_id,
-             0, l1, nullptr, l2 > 0 ? int(kInttRaw) : post};
+  PassArgs a{src,     dst,      total_limbs << (t.log_n - l1),
+             limbs,   moduli_id, 0,
+             l1,      nullptr,   l2 > 0 ? int(kInttRaw) : post,
+             l2 == 0, lazy.period, lazy.count};
   launch_pass(true, a, t, stream);
   if (l2 > 0) {
-    PassArgs b2{dst, dst, total_limbs << (t.log_n - l2), limbs, moduli_id,
-                l1, l2, nullptr, post};
+    PassArgs b2{dst,   dst,      total_limbs << (t.log_n - l2),
+                limbs, moduli_id, l1,
+                l2,    nullptr,   post,
+                1,     lazy.period, lazy.count};
     launch_pass(true, b2, t, stream);
   }
 }
 
 void ntt_forward_first(const DeviceTables &t, const int64_t *src, int64_t *dst,
                        int64_t total_limbs, int limbs, int moduli_id,
-                       const int64_t *pre, hipStream_t stream) {
+                       const int64_t *pre, hipStream_t stream,
+                       LazySpec lazy) {
   int l1, l2;
   split_bits(t.log_n, l1, l2);
-  PassArgs a{src, dst, total_limbs << (t.log_n - l1), limbs, moduli_id,
-             t.log_n - l1, l1, pre, kInttRaw};
+  PassArgs a{src,     dst,      total_limbs << (t.log_n - l1),
+             limbs,   moduli_id, t.log_n - l1,
+             l1,      pre,       kInttRaw,
+             l2 == 0, lazy.period, lazy.count};
   launch_pass(false, a, t, stream);
 }
 
 void ntt_inverse_second(const DeviceTables &t, const int64_t *src,
                         int64_t *dst, int64_t total_limbs, int limbs,
-                        int moduli_id, int post, hipStream_t stream) {
+                        int moduli_id, int post, hipStream_t stream,
+                        LazySpec lazy) {
   int l1, l2;
   split_bits(t.log_n, l1, l2);
-  PassArgs a{src, dst, total_limbs << (t.log_n - l2), limbs, moduli_id,
-             l1, l2, nullptr, post};
+  PassArgs a{src,   dst,      total_limbs << (t.log_n - l2),
+             limbs, moduli_id, l1,
+             l2,    nullptr,   post,
+             1,     lazy.period, lazy.count};
   launch_pass(true, a, t, stream);
 }
 

@@ -7,18 +7,39 @@
 //   ROWS  j = r + 16 m   (bits 7..4 of j vary inside a thread)
 //   COLS  j = 16 r + m   (bits 3..0 vary inside a thread)
 // The forward pass runs its local bits 7..4 in ROWS and 3..0 in COLS; the
-// inverse pass runs 0..3 in COLS and 4..7 in ROWS.  Butterflies are the
-// reference's exact radix-2 formulas (src/operation/ntt.cpp:46-134,
-// :177-272); twiddle addressing is derived in the ntt.hip header comment.
+// inverse pass runs 0..3 in COLS and 4..7 in ROWS.  Twiddle addressing is
+// derived in the ntt.hip header comment.
+//
+// Three arithmetics share the loop structure (template parameter A):
+//   kExact      the reference's radix-2 formulas with Montgomery twiddles
+//               (src/operation/ntt.cpp:46-134, :177-272), so every lazy
+//               representative in [0, 2q) matches the reference;
+//   kLazySmall  for q < 2^42: Shoup products with standard-form twiddles and
+//               no reductions inside the transform.  Forward values grow by
+//               less than 2q per stage; inverse sums double per stage and the
+//               difference of global stage t is offset by q << (1 + t) (a
+//               bound of every input of that stage when the transform's input
+//               is below 2q).  Sixteen stages stay below 2^60.
+//   kLazyLarge  Harvey's lazy forms for any q < 2^61: forward values in
+//               [0, 4q) with one conditional subtraction per butterfly,
+//               inverse values in [0, 2q).
+// The lazy forms give other representatives of the same residues; they serve
+// stages whose results are canonicalized before anything representative-
+// dependent reads them.
 #pragma once
 
 #include "device.h"
 
 namespace fhe {
 
+enum NttArith : int { kExact = 0, kLazySmall = 1, kLazyLarge = 2 };
+
+// Primes below this bound take the kLazySmall forms.
+constexpr uint64_t kLazySmallBound = uint64_t(1) << 42;
+
 // Operands are lazy residues in [0, 2q), so every guard sees a non-negative
-// value and takes the borrow form.
-// Cooley-Tukey: p = mont(w, y); x' = guard(x + p, 2q); y' = guard(x + 2q - p, 2q)
+// value.  Cooley-Tukey: p = mont(w, y); x' = guard(x + p, 2q);
+// y' = guard(x + 2q - p, 2q)
 __device__ __forceinline__ void ct_butterfly(int64_t &x, int64_t &y, int64_t w,
                                              int64_t q2, uint64_t q,
                                              uint64_t k) {
@@ -37,11 +58,28 @@ __device__ __forceinline__ void gs_butterfly(int64_t &x, int64_t &y, int64_t w,
   y = mont_mult_u(w, guard_u(sub64(add64(x0, q2), y0), q2), q, k);
 }
 
+// High 64 bits of a 64 x 64-bit product from 32-bit multiply-adds.
+__device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
+  const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32);
+  const uint32_t b0 = uint32_t(b), b1 = uint32_t(b >> 32);
+  const uint64_t ll = uint64_t(a0) * b0;
+  const uint64_t lh = uint64_t(a0) * b1 + (ll >> 32);
+  const uint64_t hl = uint64_t(a1) * b0 + uint32_t(lh);
+  return uint64_t(a1) * b1 + (lh >> 32) + (hl >> 32);
+}
+
+// Shoup: w y mod q in [0, 2q) for any y < 2^64, w < q, wq = floor(w 2^64 / q).
+__device__ __forceinline__ uint64_t shoup_mult(uint64_t y, uint64_t w,
+                                               uint64_t wq, uint64_t q) {
+  return w * y - mulhi64(wq, y) * q;
+}
+
 // Everything a thread needs to address twiddles in one 8-bit pass: the pass
 // covers index bits [b, b+8) of a limb of 2^log_n points; hi is the
 // sub-transform's index bits above b+8 and r the thread's row.
 struct R16Pass {
-  const int64_t *tw; // this prime's twiddle table (forward or inverse)
+  const int64_t *tw;   // Montgomery twiddles of this prime (kExact)
+  const uint64_t *ts;  // {w, w'} pairs of this prime (lazy forms)
   int64_t n;
   int64_t hi;
   int log_n;
@@ -51,7 +89,48 @@ struct R16Pass {
   uint64_t q, k;
 };
 
+// One forward butterfly on (x, y) with twiddle entry idx.
+template <int A>
+__device__ __forceinline__ void fwd_bfly(int64_t &x, int64_t &y,
+                                         const R16Pass &s, int64_t idx) {
+  if (A == kExact) {
+    ct_butterfly(x, y, s.tw[idx], s.q2, s.q, s.k);
+    return;
+  }
+  const uint64_t w = s.ts[2 * idx], wq = s.ts[2 * idx + 1];
+  uint64_t x0 = uint64_t(x);
+  if (A == kLazyLarge && x0 >= uint64_t(s.q2))
+    x0 -= uint64_t(s.q2);
+  const uint64_t t = shoup_mult(uint64_t(y), w, wq, s.q);
+  x = int64_t(x0 + t);
+  y = int64_t(x0 + uint64_t(s.q2) - t);
+}
+
+// One inverse butterfly at global stage t.
+template <int A>
+__device__ __forceinline__ void inv_bfly(int64_t &x, int64_t &y,
+                                         const R16Pass &s, int64_t idx,
+                                         int t) {
+  if (A == kExact) {
+    gs_butterfly(x, y, s.tw[idx], s.q2, s.q, s.k);
+    return;
+  }
+  const uint64_t w = s.ts[2 * idx], wq = s.ts[2 * idx + 1];
+  const uint64_t x0 = uint64_t(x), y0 = uint64_t(y);
+  uint64_t sum = x0 + y0;
+  uint64_t off = uint64_t(s.q2);
+  if (A == kLazyLarge) {
+    if (sum >= uint64_t(s.q2))
+      sum -= uint64_t(s.q2);
+  } else {
+    off = s.q << (1 + t);
+  }
+  x = int64_t(sum);
+  y = int64_t(shoup_mult(x0 + off - y0, w, wq, s.q));
+}
+
 // forward local bits 7..4, ROWS ownership
+template <int A = kExact>
 __device__ __forceinline__ void r16_forward_high(int64_t (&v)[16],
                                                  const R16Pass &s) {
 #pragma unroll
@@ -63,13 +142,13 @@ __device__ __forceinline__ void r16_forward_high(int64_t (&v)[16],
     for (int m = 0; m < 16; m++) {
       if (m & (1 << x))
         continue;
-      ct_butterfly(v[m], v[m + (1 << x)], s.tw[tb + (m >> (u - 3))], s.q2,
-                   s.q, s.k);
+      fwd_bfly<A>(v[m], v[m + (1 << x)], s, tb + (m >> (u - 3)));
     }
   }
 }
 
 // forward local bits 3..0, COLS ownership
+template <int A = kExact>
 __device__ __forceinline__ void r16_forward_low(int64_t (&v)[16],
                                                 const R16Pass &s) {
 #pragma unroll
@@ -81,13 +160,13 @@ __device__ __forceinline__ void r16_forward_low(int64_t (&v)[16],
     for (int m = 0; m < 16; m++) {
       if (m & (1 << u))
         continue;
-      ct_butterfly(v[m], v[m + (1 << u)], s.tw[tb + (m >> (u + 1))], s.q2,
-                   s.q, s.k);
+      fwd_bfly<A>(v[m], v[m + (1 << u)], s, tb + (m >> (u + 1)));
     }
   }
 }
 
 // inverse local bits 0..3, COLS ownership
+template <int A = kExact>
 __device__ __forceinline__ void r16_inverse_low(int64_t (&v)[16],
                                                 const R16Pass &s) {
 #pragma unroll
@@ -98,13 +177,13 @@ __device__ __forceinline__ void r16_inverse_low(int64_t (&v)[16],
     for (int m = 0; m < 16; m++) {
       if (m & (1 << u))
         continue;
-      gs_butterfly(v[m], v[m + (1 << u)], s.tw[tb + (m >> (u + 1))], s.q2,
-                   s.q, s.k);
+      inv_bfly<A>(v[m], v[m + (1 << u)], s, tb + (m >> (u + 1)), s.b + u);
     }
   }
 }
 
 // inverse local bits 4..7, ROWS ownership
+template <int A = kExact>
 __device__ __forceinline__ void r16_inverse_high(int64_t (&v)[16],
                                                  const R16Pass &s) {
 #pragma unroll
@@ -115,10 +194,21 @@ __device__ __forceinline__ void r16_inverse_high(int64_t (&v)[16],
     for (int m = 0; m < 16; m++) {
       if (m & (1 << x))
         continue;
-      gs_butterfly(v[m], v[m + (1 << x)], s.tw[tb + (m >> (u - 3))], s.q2,
-                   s.q, s.k);
+      inv_bfly<A>(v[m], v[m + (1 << x)], s, tb + (m >> (u - 3)), s.b + u);
     }
   }
+}
+
+// Bring lazy forward outputs to [0, 2q) (kLazyLarge values are below 4q;
+// kLazySmall values are left as they are: their consumers accept them).
+template <int A>
+__device__ __forceinline__ void r16_forward_settle(int64_t (&v)[16],
+                                                   const R16Pass &s) {
+  if (A != kLazyLarge)
+    return;
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    v[m] = guard_u(v[m], s.q2);
 }
 
 // LDS exchange buffer: 16 sub-transforms x 256 points, padded so that both

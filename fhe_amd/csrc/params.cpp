@@ -259,6 +259,8 @@ void CkksParams::derive(bool twiddles) {
   ninv.resize(P);
   root.assign(twiddles ? size_t(P) * n : 0, 0);
   root_inverse.assign(twiddles ? size_t(P) * n : 0, 0);
+  root_shoup.assign(twiddles ? size_t(P) * n * 2 : 0, 0);
+  root_inverse_shoup.assign(twiddles ? size_t(P) * n * 2 : 0, 0);
   std::vector<int64_t> power(n), shuffled(n);
   auto bitrev = [&](int x) {
     int y = 0;
@@ -298,8 +300,13 @@ void CkksParams::derive(bool twiddles) {
           for (int g = 0; g < (1 << lg); g++)
             dst[o++] = shuffled[(1 << lg) + g];
       }
-      for (int j = 0; j < n - 1; j++)
+      uint64_t *sh = (pass == 0 ? root_shoup.data() : root_inverse_shoup.data()) +
+                     size_t(i) * n * 2;
+      for (int j = 0; j < n - 1; j++) {
+        sh[2 * j] = uint64_t(dst[j]);
+        sh[2 * j + 1] = uint64_t(((unsigned __int128)dst[j] << 64) / q[i]);
         dst[j] = mont_mult_u(dst[j], r2[i], q[i], k[i]);
+      }
     }
   }
   // src/ckks/precomputed.cpp:97-137

@@ -69,6 +69,11 @@ public:
   // Montgomery-form twiddles, N-1 used entries per prime, padded to N:
   // root[p*N + i] = psi^bitrev(i+1) * R, root_inverse in backward-paint order.
   std::vector<int64_t> root, root_inverse;
+  // The same twiddles in standard form with their Shoup quotients,
+  // interleaved per entry: [prime][N][2] = {w, floor(w * 2^64 / q)}.  They
+  // drive the lazy transforms of stages whose outputs are canonicalized
+  // later, where any representative of the residue is acceptable.
+  std::vector<uint64_t> root_shoup, root_inverse_shoup;
   // Scale 2^scale_bits and the per-level scale deviations, index = level
   // 0..full_level (src/ckks/parameters.cpp:54-104).
   double scale = 0;

@@ -494,6 +494,20 @@ void *ref_level_down(void *handle, void *ct, int level) {
   });
 }
 
+void *ref_add_ct(void *handle, void *a, void *b) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->add(
+        static_cast<UnitCiphertext *>(a), static_cast<UnitCiphertext *>(b));
+  });
+}
+
+void *ref_subtract_ct(void *handle, void *a, void *b) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->subtract(
+        static_cast<UnitCiphertext *>(a), static_cast<UnitCiphertext *>(b));
+  });
+}
+
 void *ref_negate(void *handle, void *ct) {
   return guarded([&]() -> void * {
     return static_cast<RefEngine *>(handle)->engine->negate(

@@ -319,6 +319,8 @@ class Reference:
             "ref_add_double": (VP, [VP, VP, ctypes.c_double]),
             "ref_level_down": (VP, [VP, VP, i32]),
             "ref_negate": (VP, [VP, VP]),
+            "ref_add_ct": (VP, [VP, VP, VP]),
+            "ref_subtract_ct": (VP, [VP, VP, VP]),
             "ref_decrypt_limbs": (i32, [VP, VP, VP, VP, VP]),
         }
         for name, (res, args) in sigs.items():

@@ -170,3 +170,30 @@ def test_scalar_ops_match_oracle_primitives(gpu_lib):
                   expect.shape)
     assert (got == expect).all()
     ctx.close()
+
+
+@needs_ref
+@pytest.mark.parametrize("level", [0, 4])
+def test_add_subtract_negate_match_reference(env, level, gpu_lib):
+    """CKKSEngine::add / subtract (ct, ct) and negate on INTT-form
+    ciphertexts = FHE_EW_ADD_REDUCE / SUB_REDUCE / NEGATE_MAKE_UNSIGNED over
+    both polynomials."""
+    lib, ref, ctx = env["lib"], env["ref"], env["ctx"]
+    a, b = encrypt(env, level, 60 + level), encrypt(env, level, 70 + level)
+    mid = ref.shape(a)[0]
+    da, db = ctx.to_device(ref.get(a)), ctx.to_device(ref.get(b))
+    out = ctx.array(da.shape)
+    for op, ref_fn in (("ADD_REDUCE", lambda: lib.ref_add_ct(env["e"], a, b)),
+                       ("SUB_REDUCE",
+                        lambda: lib.ref_subtract_ct(env["e"], a, b)),
+                       ("NEGATE_MAKE_UNSIGNED",
+                        lambda: lib.ref_negate(env["e"], a))):
+        h = ref_fn()
+        assert h, lib.ref_last_error()
+        ctx.elementwise(fhe_amd.EW[op], da, out, mid, x2=db)
+        assert (out.download() == ref.get(h)).all(), op
+        lib.ref_free_ct(h)
+    for d in (da, db, out):
+        d.free()
+    lib.ref_free_ct(a)
+    lib.ref_free_ct(b)

@@ -1,0 +1,113 @@
+"""Bounds of the lazy transform arithmetic used by the representative-free
+stages (fhe_amd/csrc/ntt_r16.h): a model of the kLazySmall and kLazyLarge
+butterflies in exact integers checks, for worst-case inputs just below 2q,
+that every intermediate stays inside the stated bounds (so no 64-bit
+wrap-around can occur on the device) and that the residues equal the exact
+transform's.  The device results themselves are checked bit-exact against
+the oracle by the GPU tests."""
+import random
+
+import pytest
+
+M64 = (1 << 64) - 1
+
+
+def mulhi64(a, b):
+    a0, a1, b0, b1 = a & 0xffffffff, a >> 32, b & 0xffffffff, b >> 32
+    ll = a0 * b0
+    lh = a0 * b1 + (ll >> 32)
+    hl = a1 * b0 + (lh & 0xffffffff)
+    return (a1 * b1 + (lh >> 32) + (hl >> 32)) & M64
+
+
+def shoup(y, w, wq, q):
+    assert 0 <= y <= M64
+    r = (w * y - mulhi64(wq, y) * q) & M64
+    assert r < 2 * q
+    return r
+
+
+def ntt_prime(bits, n):
+    x = ((1 << bits) // (2 * n)) * (2 * n) + 1
+    while True:
+        if all(pow(a, x - 1, x) == 1 for a in (2, 3, 5, 7)):
+            return x
+        x -= 2 * n
+
+
+def root(q, n):
+    for a in range(2, 1000):
+        r = pow(a, (q - 1) // (2 * n), q)
+        if pow(r, n, q) == q - 1:
+            return r
+
+
+def brev(x, b):
+    return int(format(x, f"0{b}b")[::-1], 2)
+
+
+def forward(x, q, log_n, mode):
+    n = 1 << log_n
+    psi = root(q, n)
+    x, q2, top = list(x), 2 * q, 0
+    for s in range(log_n):
+        half = n >> (s + 1)
+        for g in range(1 << s):
+            w = pow(psi, brev((1 << s) + g, log_n), q)
+            wq = (w << 64) // q
+            for j in range(2 * g * half, 2 * g * half + half):
+                a, b = x[j], x[j + half]
+                if mode == "exact":
+                    t = w * b % q
+                    x[j], x[j + half] = (a + t) % q, (a - t) % q
+                    continue
+                if mode == "large" and a >= q2:
+                    a -= q2
+                t = shoup(b, w, wq, q)
+                x[j], x[j + half] = a + t, a + q2 - t
+                top = max(top, x[j], x[j + half])
+    return x, top
+
+
+def inverse(x, q, log_n, mode):
+    n = 1 << log_n
+    psi_inv = pow(root(q, n), -1, q)
+    x, q2, top = list(x), 2 * q, 0
+    for t in range(log_n):
+        half = 1 << t
+        for g in range(n >> (t + 1)):
+            w = pow(psi_inv, brev((n >> (t + 1)) + g, log_n), q)
+            wq = (w << 64) // q
+            for j in range(2 * g * half, 2 * g * half + half):
+                a, b = x[j], x[j + half]
+                if mode == "exact":
+                    x[j], x[j + half] = (a + b) % q, (a - b) * w % q
+                    continue
+                total, off = a + b, q2
+                if mode == "large":
+                    total = total - q2 if total >= q2 else total
+                else:
+                    off = q << (1 + t)
+                    assert b < off
+                x[j], x[j + half] = total, shoup(a + off - b, w, wq, q)
+                top = max(top, x[j], x[j + half])
+    return x, top
+
+
+@pytest.mark.parametrize("bits,mode", [(40, "small"), (41, "small"),
+                                       (60, "large"), (54, "large")])
+def test_lazy_butterflies_bounds_and_residues(bits, mode):
+    log_n = 8
+    q = ntt_prime(bits, 1 << log_n)
+    rng = random.Random(bits)
+    x = [2 * q - 1 - rng.randrange(64) for _ in range(1 << log_n)]
+    exact, _ = forward([v % q for v in x], q, log_n, "exact")
+    lazy, top = forward(x, q, log_n, mode)
+    assert [v % q for v in lazy] == exact
+    # forward: below 2q + 2q per stage (small), below 4q (large)
+    assert top < ((2 + 2 * log_n) * q if mode == "small" else 4 * q)
+    exact, _ = inverse([v % q for v in x], q, log_n, "exact")
+    lazy, top = inverse(x, q, log_n, mode)
+    assert [v % q for v in lazy] == exact
+    # inverse: below 2q * 2^stages (small; 2^59 after 16 stages for q < 2^42)
+    assert top < ((2 * q) << log_n if mode == "small" else 2 * q)
