@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "fhe_profile_count", "fhe_profile_entry", "fhe_elementwise",
     "fhe_multiplier", "fhe_params_multiplier", "fhe_multiply_double",
     "fhe_multiply_int", "fhe_add_double", "fhe_level_down",
-    "fhe_encrypt_pk", "fhe_decrypt",
+    "fhe_encrypt_pk", "fhe_decrypt", "fhe_rotate_batch",
 )
 
 
@@ -117,6 +117,8 @@ def library():
         lib.fhe_encrypt_pk.argtypes = [vp, i32, i32, vp, vp, i32, vp, vp, vp,
                                        vp]
         lib.fhe_decrypt.argtypes = [vp, i32, i32, vp, vp, i32, vp]
+        lib.fhe_rotate_batch.argtypes = [vp, i32, i32, vp, i32, vp, vp, vp,
+                                         vp]
         lib.fhe_params_multiplier.argtypes = [
             ctypes.POINTER(Params), i32, ctypes.c_double, ctypes.c_int64, i32,
             i32, vp, i32, ctypes.POINTER(i32)]
@@ -314,6 +316,18 @@ class Context:
         count = int(np.prod(cts.shape[:-3]))
         _check(library().fhe_rotate(self.handle, count, moduli_id, cts.ptr,
                                     key.ptr, key_moduli_id, delta, out.ptr))
+
+    def rotate_batch(self, cts, keys, key_moduli_ids, deltas, outs,
+                     moduli_id):
+        """Hoisted rotations of cts by every (key, key_moduli_id, delta)."""
+        count = int(np.prod(cts.shape[:-3]))
+        m = len(keys)
+        kp = (ctypes.c_void_p * m)(*[k.ptr.value for k in keys])
+        op = (ctypes.c_void_p * m)(*[o.ptr.value for o in outs])
+        km = (ctypes.c_int * m)(*key_moduli_ids)
+        dl = (ctypes.c_int * m)(*deltas)
+        _check(library().fhe_rotate_batch(self.handle, count, moduli_id,
+                                          cts.ptr, m, kp, km, dl, op))
 
     def conjugate(self, cts, key, key_moduli_id, out, moduli_id):
         count = int(np.prod(cts.shape[:-3]))

@@ -22,7 +22,7 @@ ARCH = "gfx950"
 
 HOST_SOURCES = ["params.cpp"]
 HIP_SOURCES = ["ntt.hip", "ckks_kernels.hip", "ks_fused.hip", "automorphism.hip",
-               "elementwise.hip", "crypt.hip",
+               "elementwise.hip", "crypt.hip", "tensor_fused.hip",
                "context.hip"]
 
 INCLUDES = ["-I" + os.path.join(REPO, "include"), "-I" + CSRC]

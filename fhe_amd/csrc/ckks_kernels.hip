@@ -6,6 +6,8 @@
 // into one pass over HBM.  Exact-arithmetic modular work: VALU only, no MFMA.
 #include "kernels.h"
 
+#include "modup.h"
+#include "ntt_r16.h"
 #include "params.h"
 
 namespace fhe {
@@ -47,23 +49,15 @@ __global__ __launch_bounds__(kBlock) void tensor_kernel(
   z[2 * poly] = mont_mult_u(a1, b1, q, k);
 }
 
-// Montgomery product for ModUp: the signed form where a digit can be
-// negative, the fast unsigned form where the host has shown it cannot.
-template <bool SIGNED>
-__device__ __forceinline__ int64_t modup_mult(int64_t a, int64_t b,
-                                              uint64_t q, uint64_t k) {
-  return SIGNED ? mont_mult(a, b, q, k) : mont_mult_u(a, b, q, k);
-}
-
 // One thread per (ciphertext, coefficient): Garner digits of the partition
 // in registers, then every limb of the extended basis.  With SIGNED false
 // the partition's first prime is at most twice each of its other primes, so
 // every difference and digit below is non-negative and under 2^62.
-template <int MAXZ, bool SIGNED>
+template <int MAXZ, bool SIGNED, bool LAZY>
 __global__ __launch_bounds__(kBlock) void modup_kernel(
     const int64_t *x, int64_t x_stride, int64_t *ext, int64_t ext_stride,
     int count, int limbs, int moduli_id, int start, int size,
-    const int64_t *cst, DeviceTables t) {
+    const int64_t *cst, int ext_shoup_offset, DeviceTables t) {
   const int64_t n = int64_t(1) << t.log_n;
   const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (idx >= int64_t(count) * n)
@@ -73,51 +67,22 @@ __global__ __launch_bounds__(kBlock) void modup_kernel(
   const int64_t *xs = x + c * x_stride + coef;
   const int ext_limbs = limbs + t.alpha;
 
-  int64_t d[MAXZ];
-  const int64_t first = xs[int64_t(start) * n];
+  int64_t xv[MAXZ], d[MAXZ];
 #pragma unroll
   for (int j = 0; j < MAXZ; j++)
-    d[j] = first;
-  // rns_to_mrs (src/ckks/core_key_switch.cpp:21-56)
-#pragma unroll
-  for (int j = 1; j < MAXZ; j++) {
-    if (j < size) {
-      const int pj = moduli_id + start + j;
-      const int64_t m = int64_t(t.q[pj]), m2 = int64_t(t.q2[pj]);
-      const int64_t diff = guard_fast(
-          sub64(add64(xs[int64_t(start + j) * n], m2), d[j]), m2);
-      d[j] = guard_fast(modup_mult<SIGNED>(diff,
-                                           cst[CkksParams::kKsInvOffset + j],
-                                           t.q[pj], t.k[pj]),
-                        m);
-#pragma unroll
-      for (int tt = j + 1; tt < MAXZ; tt++) {
-        if (tt < size) {
-          const int pt = moduli_id + start + tt;
-          const int64_t prod = modup_mult<SIGNED>(
-              d[j], cst[CkksParams::kKsTileOffset + j * kMaxAlpha + tt],
-              t.q[pt], t.k[pt]);
-          d[tt] = guard_fast(guard_fast(add64(prod, d[tt]), int64_t(t.q2[pt])),
-                             int64_t(t.q[pt]));
-        }
-      }
-    }
-  }
-  // mod_up (src/ckks/core_key_switch.cpp:92-108) before its NTT
+    xv[j] = j < size ? xs[int64_t(start + j) * n] : 0;
+  garner_digits<MAXZ, SIGNED>(xv, d, size, moduli_id + start, cst, t);
   const int64_t *ec = cst + CkksParams::kKsExtOffset;
+  const uint64_t *es = reinterpret_cast<const uint64_t *>(cst + ext_shoup_offset);
   int64_t *out = ext + c * ext_stride + coef;
   for (int i = 0; i < ext_limbs; i++) {
     const int p = moduli_id + i;
-    const uint64_t q = t.q[p], k = t.k[p];
-    const int64_t q2 = int64_t(t.q2[p]);
-    int64_t y = modup_mult<SIGNED>(d[0], ec[i], q, k);
-#pragma unroll
-    for (int j = 1; j < MAXZ; j++)
-      if (j < size)
-        y = guard_fast(
-            add64(modup_mult<SIGNED>(d[j], ec[j * ext_limbs + i], q, k), y),
-            q2);
-    out[int64_t(i) * n] = y;
+    if (LAZY)
+      out[int64_t(i) * n] = extend_limb_lazy<MAXZ>(
+          d, size, i, ext_limbs, es, t.q[p], t.q[p] < kLazySmallBound);
+    else
+      out[int64_t(i) * n] = extend_limb<MAXZ, SIGNED>(
+          d, size, i, ext_limbs, ec, t.q[p], t.k[p], int64_t(t.q2[p]));
   }
 }
 
@@ -188,17 +153,28 @@ template <int MAXA> struct ModDown {
     }
   }
 
-  // the canonical key-switched value of ordinary limb i
-  __device__ int64_t limb(int64_t v, int i, const DeviceTables &t) const {
+  // The canonical key-switched value of ordinary limb i from its
+  // accumulator value v >= 0.  The reference's steps
+  //   v = mont(guard(v + 2q - p_j, 2q), s_j^-1 R)   (signed), then
+  //   make_unsigned_reduce_2q_to_q
+  // give the canonical residue of (..((v - p_0) s_0^-1 - p_1) s_1^-1 ..) mod
+  // q for the exact peeled integers p_j, whatever representatives the steps
+  // carry; here each step is one Shoup product of v + q_top - p_j with the
+  // standard-form inverse.  v + q_top - p_j lies in [0, 2^64) because every
+  // p_j is a signed Montgomery product with a constant below 2^61, hence in
+  // (-2^62, 2^62 + 2^61), and q_top is in [2^63 - q, 2^63).
+  __device__ int64_t limb_lazy(int64_t v, int i, const DeviceTables &t) const {
     const int p = moduli_id + i;
-    const uint64_t q = t.q[p], k = t.k[p];
-    const int64_t q2 = int64_t(t.q2[p]);
+    const uint64_t q = t.q[p], top = t.q_top[p];
+    uint64_t x = uint64_t(v);
 #pragma unroll
     for (int sp = 0; sp < MAXA; sp++)
-      if (sp < alpha)
-        v = mont_mult(guard(v + q2 - peeled[sp], q2),
-                      t.p_inverse[int64_t(sp) * t.prime_count + p], q, k);
-    return v < 0 ? v + int64_t(q) : guard(v, int64_t(q));
+      if (sp < alpha) {
+        const uint64_t *c =
+            t.p_inverse_shoup + (int64_t(sp) * t.prime_count + p) * 2;
+        x = shoup_mult(x + top - uint64_t(peeled[sp]), c[0], c[1], q);
+      }
+    return guard_u(int64_t(x), int64_t(q));
   }
 };
 
@@ -230,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
   // canonical ks_i passes the two guards unchanged when there is no d
   auto relin = [&](int i) {
     const int p = moduli_id + i;
-    const int64_t ks = md.limb(a[int64_t(i) * n], i, t);
+    const int64_t ks = md.limb_lazy(a[int64_t(i) * n], i, t);
     const int64_t di = d ? d[int64_t(i) * n] : 0;
     return guard(guard(di + ks, int64_t(t.q2[p])), int64_t(t.q[p]));
   };
@@ -370,18 +346,21 @@ void launch_tensor(const DeviceTables &t, const int64_t *a, const int64_t *b,
 void launch_modup(const DeviceTables &t, const int64_t *x, int64_t x_stride,
                   int64_t *ext, int64_t ext_stride, int count, int limbs,
                   int moduli_id, int start, int size, const int64_t *constants,
-                  bool non_negative, hipStream_t stream) {
+                  int ext_shoup_offset, bool non_negative, bool lazy,
+                  hipStream_t stream) {
   const dim3 grid(grid_for(int64_t(count) << t.log_n));
-#define FHE_MODUP(W, S)                                                        \
-  hipLaunchKernelGGL((modup_kernel<W, S>), grid, dim3(kBlock), 0, stream, x,  \
-                     x_stride, ext, ext_stride, count, limbs, moduli_id,       \
-                     start, size, constants, t)
+#define FHE_MODUP(W, S, L)                                                     \
+  hipLaunchKernelGGL((modup_kernel<W, S, L>), grid, dim3(kBlock), 0, stream,  \
+                     x, x_stride, ext, ext_stride, count, limbs, moduli_id,    \
+                     start, size, constants, ext_shoup_offset, t)
 #define FHE_MODUP_W(W)                                                         \
   do {                                                                         \
-    if (non_negative)                                                          \
-      FHE_MODUP(W, false);                                                     \
+    if (non_negative && lazy)                                                  \
+      FHE_MODUP(W, false, true);                                               \
+    else if (non_negative)                                                     \
+      FHE_MODUP(W, false, false);                                              \
     else                                                                       \
-      FHE_MODUP(W, true);                                                      \
+      FHE_MODUP(W, true, false);                                               \
   } while (0)
   if (size <= 1)
     FHE_MODUP_W(1);

@@ -97,6 +97,10 @@ struct fhe_context {
   size_t workspace_bytes = 0;
   // lazy transforms where results are canonicalized before use
   bool lazy_key_switch = true;
+  // ModUp fused with the first NTT pass of the extended limbs
+  bool fuse_modup = false;
+  // input NTT second pass + tensor + trio INTT first pass in one kernel
+  bool fuse_tensor = true;
   // stage timing: HIP events recorded on `stream` around each stage while
   // enabled, folded into per-stage totals when read
   bool timing = false;
@@ -125,6 +129,8 @@ struct fhe_context {
     t.root_shoup = upload(params.root_shoup, owned);
     t.root_inverse_shoup = upload(params.root_inverse_shoup, owned);
     t.p_inverse = upload(params.p_inverse_mult_r, owned);
+    t.p_inverse_shoup = upload(params.p_inverse_shoup, owned);
+    t.q_top = upload(params.q_top, owned);
     t.rescale = upload(params.rescale_scale, owned);
     t.prime_count = params.prime_count;
     t.log_n = params.log_n;
@@ -248,6 +254,11 @@ struct fhe_context {
   void key_switch(int count, int moduli_id, const int64_t *key,
                   int key_moduli_id, const int64_t *src, int64_t src_stride,
                   int64_t *ext, int64_t *acc) {
+    if (fhe::ks_fused_supported(tables)) {
+      ks_decompose(count, moduli_id, key_moduli_id, src, src_stride, ext);
+      ks_apply(count, moduli_id, key, key_moduli_id, ext, acc);
+      return;
+    }
     const int limbs = params.limbs_at(moduli_id);
     const int ext_limbs = limbs + params.alpha;
     const int64_t N = n();
@@ -256,48 +267,13 @@ struct fhe_context {
     const auto parts = params.partitions(moduli_id, key_moduli_id);
     const int64_t *cst = modup_constants(moduli_id, key_moduli_id);
     const int stride = params.ks_stride(moduli_id);
-    if (fhe::ks_fused_supported(tables)) {
-      // ext holds every partition: [count][parts][ext_limbs][N]
-      const int np = int(parts.size());
-      for (int p = 0; p < np; p++)
-        stage("modup", [&] {
-          fhe::launch_modup(tables, src, src_stride,
-                            ext + int64_t(p) * ext_limbs * N,
-                            int64_t(np) * ext_limbs * N, count, limbs,
-                            moduli_id, parts[p].start, parts[p].size,
-                            cst + p * stride,
-                            params.modup_non_negative(moduli_id, parts[p]),
-                            stream);
-        });
-      // The extended polynomials, their products with the key and the
-      // accumulators are only read by ModDown after the canonicalizing
-      // exit, so their transforms use the lazy arithmetic.
-      const fhe::LazySpec lazy = lazy_key_switch ? fhe::kAllLazy
-                                                 : fhe::LazySpec();
-      stage("ntt_pass_high(ext)", [&] {
-        fhe::ntt_forward_first(tables, ext, ext,
-                               int64_t(count) * np * ext_limbs, ext_limbs,
-                               moduli_id, nullptr, stream, lazy);
-      });
-      stage("ks_inner_product_r16", [&] {
-        fhe::launch_ks_inner_product_fused(tables, ext, key, key_chain,
-                                           key_offset, parts[0].key_index, np,
-                                           acc, count, ext_limbs, moduli_id,
-                                           lazy_key_switch, stream);
-      });
-      stage("intt_pass_high(acc)", [&] {
-        fhe::ntt_inverse_second(tables, acc, acc,
-                                int64_t(count) * 2 * ext_limbs, ext_limbs,
-                                moduli_id, fhe::kInttExitReduce, stream, lazy);
-      });
-      return;
-    }
     for (size_t p = 0; p < parts.size(); p++) {
       const auto &part = parts[p];
       fhe::launch_modup(tables, src, src_stride, ext, ext_limbs * N, count,
-                        limbs, moduli_id, part.start,
-                        part.size, cst + p * stride,
-                        params.modup_non_negative(moduli_id, part), stream);
+                        limbs, moduli_id, part.start, part.size,
+                        cst + p * stride, params.ks_ext_shoup_offset(moduli_id),
+                        params.modup_non_negative(moduli_id, part), false,
+                        stream);
       fhe::ntt_forward(tables, ext, ext, int64_t(count) * ext_limbs,
                        ext_limbs, moduli_id, nullptr, stream);
       const int64_t *key0 =
@@ -310,6 +286,85 @@ struct fhe_context {
     }
     fhe::ntt_inverse(tables, acc, acc, int64_t(count) * 2 * ext_limbs,
                      ext_limbs, moduli_id, fhe::kInttExitReduce, stream);
+  }
+
+  // The key-independent half of key switching on the fused path (N = 2^16):
+  // ModUp of every partition and the first forward NTT pass of every
+  // extended limb into ext [count][parts][ext][N] (CKKSCore::decompose,
+  // src/ckks/core_key_switch.cpp:434-460, up to its second NTT pass).
+  // The extended polynomials, their products with the key and the
+  // accumulators are only read by ModDown after the canonicalizing exit,
+  // so with lazy_key_switch their arithmetic is lazy.
+  void ks_decompose(int count, int moduli_id, int key_moduli_id,
+                    const int64_t *src, int64_t src_stride, int64_t *ext) {
+    const int limbs = params.limbs_at(moduli_id);
+    const int ext_limbs = limbs + params.alpha;
+    const int64_t N = n();
+    const auto parts = params.partitions(moduli_id, key_moduli_id);
+    const int64_t *cst = modup_constants(moduli_id, key_moduli_id);
+    const int stride = params.ks_stride(moduli_id);
+    const int np = int(parts.size());
+    const fhe::LazySpec lazy = lazy_key_switch ? fhe::kAllLazy
+                                               : fhe::LazySpec();
+    int max_size = 0;
+    bool is_signed[fhe::kMaxFusedParts] = {};
+    std::vector<int> starts(np), sizes(np);
+    for (int p = 0; p < np; p++) {
+      starts[p] = parts[p].start;
+      sizes[p] = parts[p].size;
+      if (p < fhe::kMaxFusedParts)
+        is_signed[p] = !params.modup_non_negative(moduli_id, parts[p]);
+      max_size = std::max(max_size, sizes[p]);
+    }
+    if (fuse_modup && fhe::modup_ntt_fused_supported(tables, max_size, np)) {
+      stage("modup_ntt_high(ext)", [&] {
+        fhe::launch_modup_ntt_fused(tables, src, src_stride, ext, count, limbs,
+                                    moduli_id, starts.data(), sizes.data(),
+                                    is_signed, np, cst, stride,
+                                    lazy_key_switch, stream);
+      });
+      return;
+    }
+    for (int p = 0; p < np; p++)
+      stage("modup", [&] {
+        fhe::launch_modup(tables, src, src_stride,
+                          ext + int64_t(p) * ext_limbs * N,
+                          int64_t(np) * ext_limbs * N, count, limbs, moduli_id,
+                          parts[p].start, parts[p].size, cst + p * stride,
+                          params.ks_ext_shoup_offset(moduli_id),
+                          params.modup_non_negative(moduli_id, parts[p]),
+                          lazy_key_switch, stream);
+      });
+    stage("ntt_pass_high(ext)", [&] {
+      fhe::ntt_forward_first(tables, ext, ext, int64_t(count) * np * ext_limbs,
+                             ext_limbs, moduli_id, nullptr, stream, lazy);
+    });
+  }
+
+  // The key-dependent half on the fused path: second NTT pass, key inner
+  // product over all partitions, both INTT passes (with exit and reduce)
+  // into acc [count][2][ext][N].
+  void ks_apply(int count, int moduli_id, const int64_t *key,
+                int key_moduli_id, const int64_t *ext, int64_t *acc) {
+    const int limbs = params.limbs_at(moduli_id);
+    const int ext_limbs = limbs + params.alpha;
+    const int key_chain = params.limbs_at(key_moduli_id) + params.alpha;
+    const int key_offset = params.limbs_at(key_moduli_id) - limbs;
+    const auto parts = params.partitions(moduli_id, key_moduli_id);
+    const int np = int(parts.size());
+    const fhe::LazySpec lazy = lazy_key_switch ? fhe::kAllLazy
+                                               : fhe::LazySpec();
+    stage("ks_inner_product_r16", [&] {
+      fhe::launch_ks_inner_product_fused(tables, ext, key, key_chain,
+                                         key_offset, parts[0].key_index, np,
+                                         acc, count, ext_limbs, moduli_id,
+                                         lazy_key_switch, stream);
+    });
+    stage("intt_pass_high(acc)", [&] {
+      fhe::ntt_inverse_second(tables, acc, acc, int64_t(count) * 2 * ext_limbs,
+                              ext_limbs, moduli_id, fhe::kInttExitReduce,
+                              stream, lazy);
+    });
   }
 };
 
@@ -583,24 +638,42 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
     // exact INTT of d2), so the transforms of a0 and b0 may be lazy.
     const fhe::LazySpec first_poly =
         ctx->lazy_key_switch ? fhe::LazySpec{2, 1} : fhe::LazySpec();
-    ctx->stage("ntt(inputs)", [&] {
-      fhe::ntt_forward(ctx->tables, a, an, duo_l, limbs, moduli_id,
-                       ctx->tables.r2, ctx->stream, first_poly);
-      fhe::ntt_forward(ctx->tables, b, bn, duo_l, limbs, moduli_id,
-                       ctx->tables.r2, ctx->stream, first_poly);
-    });
-    ctx->stage("tensor", [&] {
-      fhe::launch_tensor(ctx->tables, an, bn, trio, count, limbs, moduli_id,
-                         ctx->stream);
-    });
     // relinearize: to_intt of all three, key switch of the third; d0 and d1
     // are canonicalized by the relinearization add, d2 feeds ModUp exactly
     const fhe::LazySpec d01 =
         ctx->lazy_key_switch ? fhe::LazySpec{3, 2} : fhe::LazySpec();
-    ctx->stage("intt(trio)", [&] {
-      fhe::ntt_inverse(ctx->tables, trio, trio, trio_l, limbs, moduli_id,
-                       fhe::kInttExit, ctx->stream, d01);
-    });
+    if (ctx->fuse_tensor && fhe::tensor_fused_supported(ctx->tables)) {
+      ctx->stage("ntt_pass_high(inputs)", [&] {
+        fhe::ntt_forward_first(ctx->tables, a, an, duo_l, limbs, moduli_id,
+                               ctx->tables.r2, ctx->stream, first_poly);
+        fhe::ntt_forward_first(ctx->tables, b, bn, duo_l, limbs, moduli_id,
+                               ctx->tables.r2, ctx->stream, first_poly);
+      });
+      ctx->stage("tensor_r16", [&] {
+        fhe::launch_tensor_fused(ctx->tables, an, bn, trio, count, limbs,
+                                 moduli_id, ctx->lazy_key_switch,
+                                 ctx->stream);
+      });
+      ctx->stage("intt_pass_high(trio)", [&] {
+        fhe::ntt_inverse_second(ctx->tables, trio, trio, trio_l, limbs,
+                                moduli_id, fhe::kInttExit, ctx->stream, d01);
+      });
+    } else {
+      ctx->stage("ntt(inputs)", [&] {
+        fhe::ntt_forward(ctx->tables, a, an, duo_l, limbs, moduli_id,
+                         ctx->tables.r2, ctx->stream, first_poly);
+        fhe::ntt_forward(ctx->tables, b, bn, duo_l, limbs, moduli_id,
+                         ctx->tables.r2, ctx->stream, first_poly);
+      });
+      ctx->stage("tensor", [&] {
+        fhe::launch_tensor(ctx->tables, an, bn, trio, count, limbs, moduli_id,
+                           ctx->stream);
+      });
+      ctx->stage("intt(trio)", [&] {
+        fhe::ntt_inverse(ctx->tables, trio, trio, trio_l, limbs, moduli_id,
+                         fhe::kInttExit, ctx->stream, d01);
+      });
+    }
     ctx->key_switch(count, moduli_id, key, key_moduli_id, trio + 2 * limbs * N,
                     3 * limbs * N, ext, acc);
     ctx->stage("moddown_relin_rescale", [&] {
@@ -969,6 +1042,59 @@ int fhe_rotate(fhe_context *ctx, int count, int moduli_id, const int64_t *in,
                                automorphism_exponent(P.log_n, delta),
                                ctx->stream);
     });
+    ctx->check_launch("Rotate");
+  });
+}
+
+int fhe_rotate_batch(fhe_context *ctx, int count, int moduli_id,
+                     const int64_t *in, int key_count,
+                     const int64_t *const *keys, const int *key_moduli_ids,
+                     const int *deltas, int64_t *const *outs) {
+  return guarded([&] {
+    require(ctx && in && keys && key_moduli_ids && deltas && outs,
+            "null argument");
+    require(count > 0 && key_count > 0, "Rotate: empty batch");
+    ctx->check_moduli(moduli_id, 1, "Rotate");
+    for (int k = 0; k < key_count; k++) {
+      require(keys[k] && outs[k], "null argument");
+      require(static_cast<const void *>(outs[k]) !=
+                  static_cast<const void *>(in),
+              "Rotate: a batch output must not alias the input");
+      ctx->check_key(moduli_id, key_moduli_ids[k], "Rotate");
+    }
+    const auto &P = ctx->params;
+    const int limbs = P.limbs_at(moduli_id);
+    const int ext_limbs = limbs + P.alpha;
+    const int64_t N = ctx->n();
+    const int64_t duo_l = int64_t(count) * 2 * limbs;
+    const int64_t ext_l =
+        int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
+    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
+    int64_t *ws = ctx->scratch(size_t(duo_l + ext_l + acc_l) * N * 8);
+    int64_t *sw = ws, *ext = sw + duo_l * N, *acc = ext + ext_l * N;
+    const bool hoist = fhe::ks_fused_supported(ctx->tables);
+    // CKKSEngine::rotate_batch (src/ckks/engine_rotate.cpp:262-300): one
+    // decomposition of c1, then the key-dependent part per key
+    if (hoist)
+      ctx->ks_decompose(count, moduli_id, key_moduli_ids[0], in + limbs * N,
+                        2 * limbs * N, ext);
+    for (int k = 0; k < key_count; k++) {
+      if (hoist)
+        ctx->ks_apply(count, moduli_id, keys[k], key_moduli_ids[k], ext, acc);
+      else
+        ctx->key_switch(count, moduli_id, keys[k], key_moduli_ids[k],
+                        in + limbs * N, 2 * limbs * N, ext, acc);
+      ctx->stage("moddown_add", [&] {
+        fhe::launch_moddown_add(ctx->tables, acc, in, 2 * limbs * N, 1, sw,
+                                count, limbs, moduli_id, ctx->stream);
+      });
+      ctx->stage("automorphism", [&] {
+        fhe::launch_automorphism(ctx->tables, sw, outs[k], duo_l, limbs,
+                                 moduli_id,
+                                 automorphism_exponent(P.log_n, deltas[k]),
+                                 ctx->stream);
+      });
+    }
     ctx->check_launch("Rotate");
   });
 }

@@ -25,6 +25,8 @@ struct DeviceTables {
   const uint64_t *root_shoup;
   const uint64_t *root_inverse_shoup;
   const int64_t *p_inverse;     // [alpha][prime_count]
+  const uint64_t *p_inverse_shoup; // [alpha][prime_count][2] standard form
+  const uint64_t *q_top;        // largest multiple of q below 2^63
   const int64_t *rescale;       // [prime_count][prime_count]
   int prime_count;
   int log_n;

@@ -73,10 +73,39 @@ void launch_tensor(const DeviceTables &t, const int64_t *a, const int64_t *b,
 // non_negative: the partition's first prime is at most twice each of its
 // other primes (CkksParams::modup_non_negative), which keeps every Garner
 // difference non-negative and allows the unsigned Montgomery product.
+// lazy: the extended limbs may be any representatives (Shoup products with
+// the standard-form constants at constants + ext_shoup_offset); only for
+// non-negative partitions, whose digits are canonical either way.
 void launch_modup(const DeviceTables &t, const int64_t *x, int64_t x_stride,
                   int64_t *ext, int64_t ext_stride, int count, int limbs,
                   int moduli_id, int start, int size, const int64_t *constants,
-                  bool non_negative, hipStream_t stream);
+                  int ext_shoup_offset, bool non_negative, bool lazy,
+                  hipStream_t stream);
+
+// ModUp of every partition fused with the first forward NTT pass of every
+// extended limb (N = 2^16, partitions of at most 4 limbs): ext
+// [count][parts][ext][N] receives what launch_modup followed by
+// ntt_forward_first would write (the same representatives with lazy = false,
+// the same residues otherwise).  starts / sizes / is_signed: per partition.
+constexpr int kMaxFusedParts = 32;
+bool modup_ntt_fused_supported(const DeviceTables &t, int max_partition_size,
+                               int parts);
+void launch_modup_ntt_fused(const DeviceTables &t, const int64_t *x,
+                            int64_t x_stride, int64_t *ext, int count,
+                            int limbs, int moduli_id, const int *starts,
+                            const int *sizes, const bool *is_signed, int parts,
+                            const int64_t *constants, int constants_stride,
+                            bool lazy, hipStream_t stream);
+
+// Second forward NTT pass of both input duos, tensor and first inverse NTT
+// pass of the trio in one kernel (N = 2^16).  a, b: [count][2][limbs][N]
+// after ntt_forward_first; trio: [count][3][limbs][N] for
+// ntt_inverse_second.  With lazy, a0 / b0 / z0 / z1 use the lazy arithmetic
+// (LazySpec{2, 1} for the first passes, {3, 2} for the last).
+bool tensor_fused_supported(const DeviceTables &t);
+void launch_tensor_fused(const DeviceTables &t, const int64_t *a,
+                         const int64_t *b, int64_t *trio, int count, int limbs,
+                         int moduli_id, bool lazy, hipStream_t stream);
 
 // Key inner product for one partition: acc[c][h] (+)= ext[c] * key_h
 // (src/ckks/core_key_switch.cpp:318-336).  Each key limb is read once for

@@ -11,6 +11,7 @@
 // single write.  Each coefficient sees the reference's exact sequence of
 // butterflies, products and guards.
 #include "kernels.h"
+#include "modup.h"
 #include "ntt_r16.h"
 
 namespace fhe {
@@ -140,9 +141,209 @@ __global__ __launch_bounds__(kBlock) void ks_inner_product_r16_kernel(
     ks_tile<kLazyLarge>(a, t, lds, ct, tile, i);
 }
 
+// ---------------------------------------------------------------------------
+// ModUp fused with the first (high-bit) pass of the forward NTT of every
+// extended limb.  A block owns one 4096-coefficient tile (16 columns of the
+// stride-256 pass) of one ciphertext and one partition: it loads the
+// partition's limbs at those coefficients, keeps the Garner digits of its
+// 16 coefficients per thread in registers, and for each extended limb forms
+// the limb values, runs the 8 high stages in registers and writes the tile
+// once.  The extended polynomials never make the round trip through HBM
+// between ModUp and the NTT.
+struct ModupNttArgs {
+  const int64_t *x;   // INTT-form source polynomial of ciphertext c at x + c*x_stride
+  int64_t x_stride;
+  int64_t *ext;       // [count][parts][ext_limbs][N]
+  const int64_t *cst; // ModUp constants, partition p at cst + p * cst_stride
+  int cst_stride;
+  int count, parts, limbs, ext_limbs, moduli_id;
+  int start[kMaxFusedParts]; // partition start limbs
+  int size[kMaxFusedParts];
+  uint32_t signed_mask; // bit p: partition p needs the signed products
+};
+
+// The digits of a thread's 16 coefficients, one 16-entry array per digit
+// position (separate arrays stay in registers where one 2-D array of 64
+// entries would be placed in scratch memory).
+template <int MAXZ> struct DigitTile {
+  int64_t d0[16], d1[16], d2[16], d3[16];
+  __device__ __forceinline__ void set(int m, const int64_t (&v)[MAXZ]) {
+    d0[m] = v[0];
+    if (MAXZ > 1)
+      d1[m] = v[MAXZ > 1 ? 1 : 0];
+    if (MAXZ > 2) {
+      d2[m] = v[MAXZ > 2 ? 2 : 0];
+      d3[m] = v[MAXZ > 3 ? 3 : 0];
+    }
+  }
+  __device__ __forceinline__ void get(int m, int64_t (&v)[MAXZ]) const {
+    v[0] = d0[m];
+    if (MAXZ > 1)
+      v[MAXZ > 1 ? 1 : 0] = d1[m];
+    if (MAXZ > 2) {
+      v[MAXZ > 2 ? 2 : 0] = d2[m];
+      v[MAXZ > 3 ? 3 : 0] = d3[m];
+    }
+  }
+};
+
+template <int A, int MAXZ, bool SIGNED>
+__device__ __forceinline__ void modup_ntt_limb(const ModupNttArgs &a,
+                                               const DeviceTables &t,
+                                               int64_t *lds,
+                                               const DigitTile<MAXZ> &d,
+                                               int size, const int64_t *ec,
+                                               int i, int64_t *dst, int c,
+                                               int r, int64_t sp) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int p = a.moduli_id + i;
+  R16Pass s;
+  s.tw = t.root + int64_t(p) * n;
+  s.ts = t.root_shoup + int64_t(p) * n * 2;
+  s.n = n;
+  s.hi = 0;
+  s.log_n = t.log_n;
+  s.b = 8;
+  s.r = r;
+  s.q = t.q[p];
+  s.k = t.k[p];
+  s.q2 = int64_t(t.q2[p]);
+  int64_t v[16];
+#pragma clang loop unroll(full)
+  for (int m = 0; m < 16; m++) {
+    int64_t dm[MAXZ];
+    d.get(m, dm);
+    v[m] = extend_limb<MAXZ, SIGNED>(dm, size, i, a.ext_limbs, ec, s.q, s.k,
+                                     s.q2);
+    if (MAXZ > 2)
+      __builtin_amdgcn_sched_barrier(0);
+  }
+  r16_forward_high<A>(v, s);
+  r16_exchange(v, lds, c, r, true);
+  r16_forward_low<A>(v, s);
+  int64_t *out = dst + int64_t(i) * n + sp;
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    out[int64_t(16 * r + m) << 8] = v[m];
+}
+
+template <int MAXZ, bool SIGNED, bool LAZY>
+__device__ __forceinline__ void modup_ntt_tile(const ModupNttArgs &a,
+                                               const DeviceTables &t,
+                                               int64_t *lds, int ct, int part,
+                                               int tile) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int tid = threadIdx.x;
+  const int c = tid & 15, r = tid >> 4;
+  const int64_t sp = int64_t(tile) * 16 + c; // column of the stride-256 pass
+  const int start = a.start[part], size = a.size[part];
+  const int64_t *cst = a.cst + int64_t(part) * a.cst_stride;
+  const int64_t *xs = a.x + int64_t(ct) * a.x_stride + sp;
+
+  // Garner digits of this thread's 16 coefficients (r + 16 m) << 8 | sp
+  DigitTile<MAXZ> d;
+#pragma clang loop unroll(full)
+  for (int m = 0; m < 16; m++) {
+    int64_t xv[MAXZ], dm[MAXZ];
+#pragma unroll
+    for (int j = 0; j < MAXZ; j++)
+      xv[j] = j < size ? xs[int64_t(start + j) * n + (int64_t(r + 16 * m) << 8)]
+                       : 0;
+    garner_digits<MAXZ, SIGNED>(xv, dm, size, a.moduli_id + start, cst, t);
+    d.set(m, dm);
+    // one coefficient's digit chain at a time: interleaving all sixteen
+    // would need far more registers than the tile of digits itself
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const int64_t *ec = cst + CkksParams::kKsExtOffset;
+  int64_t *dst =
+      a.ext + (int64_t(ct) * a.parts + part) * int64_t(a.ext_limbs) * n;
+  for (int i = 0; i < a.ext_limbs; i++) {
+    if (!LAZY)
+      modup_ntt_limb<kExact, MAXZ, SIGNED>(a, t, lds, d, size, ec, i, dst, c,
+                                           r, sp);
+    else if (t.q[a.moduli_id + i] < kLazySmallBound)
+      modup_ntt_limb<kLazySmall, MAXZ, SIGNED>(a, t, lds, d, size, ec, i, dst,
+                                               c, r, sp);
+    else
+      modup_ntt_limb<kLazyLarge, MAXZ, SIGNED>(a, t, lds, d, size, ec, i, dst,
+                                               c, r, sp);
+  }
+}
+
+template <int MAXZ, bool LAZY>
+__global__ __launch_bounds__(kBlock, 1) void modup_ntt_r16_kernel(
+    ModupNttArgs a, DeviceTables t) {
+  __shared__ int64_t lds[kR16Lds];
+  const int tiles = int((int64_t(1) << t.log_n) >> 12);
+  // block -> (ciphertext, partition, tile), tile fastest: consecutive blocks
+  // read neighbouring columns of the same source limbs
+  const int tile = int(blockIdx.x % tiles);
+  const int64_t rest = blockIdx.x / tiles;
+  const int part = int(rest % a.parts);
+  const int ct = int(rest / a.parts);
+  if ((a.signed_mask >> part) & 1)
+    modup_ntt_tile<MAXZ, true, LAZY>(a, t, lds, ct, part, tile);
+  else
+    modup_ntt_tile<MAXZ, false, LAZY>(a, t, lds, ct, part, tile);
+}
+
 } // namespace
 
 bool ks_fused_supported(const DeviceTables &t) { return t.log_n == 16; }
+
+bool modup_ntt_fused_supported(const DeviceTables &t, int max_partition_size,
+                               int parts) {
+  return t.log_n == 16 && max_partition_size <= 4 && parts <= kMaxFusedParts;
+}
+
+void launch_modup_ntt_fused(const DeviceTables &t, const int64_t *x,
+                            int64_t x_stride, int64_t *ext, int count,
+                            int limbs, int moduli_id, const int *starts,
+                            const int *sizes, const bool *is_signed, int parts,
+                            const int64_t *constants, int constants_stride,
+                            bool lazy, hipStream_t stream) {
+  ModupNttArgs a{};
+  a.x = x;
+  a.x_stride = x_stride;
+  a.ext = ext;
+  a.cst = constants;
+  a.cst_stride = constants_stride;
+  a.count = count;
+  a.parts = parts;
+  a.limbs = limbs;
+  a.ext_limbs = limbs + t.alpha;
+  a.moduli_id = moduli_id;
+  int max_size = 0;
+  for (int p = 0; p < parts; p++) {
+    a.start[p] = starts[p];
+    a.size[p] = sizes[p];
+    a.signed_mask |= is_signed[p] ? (uint32_t(1) << p) : 0u;
+    max_size = sizes[p] > max_size ? sizes[p] : max_size;
+  }
+  const int tiles = int((int64_t(1) << t.log_n) >> 12);
+  const dim3 grid(unsigned(int64_t(count) * parts * tiles));
+#define FHE_MODUP_NTT(W, L)                                                    \
+  hipLaunchKernelGGL((modup_ntt_r16_kernel<W, L>), grid, dim3(kBlock), 0,      \
+                     stream, a, t)
+  if (max_size <= 1) {
+    if (lazy)
+      FHE_MODUP_NTT(1, true);
+    else
+      FHE_MODUP_NTT(1, false);
+  } else if (max_size <= 2) {
+    if (lazy)
+      FHE_MODUP_NTT(2, true);
+    else
+      FHE_MODUP_NTT(2, false);
+  } else {
+    if (lazy)
+      FHE_MODUP_NTT(4, true);
+    else
+      FHE_MODUP_NTT(4, false);
+  }
+#undef FHE_MODUP_NTT
+}
 
 void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
                                    const int64_t *key, int key_chain,

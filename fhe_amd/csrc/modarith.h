@@ -149,6 +149,22 @@ FHE_HD int64_t guard_u(int64_t x, int64_t q) {
   return uint64_t(x) < uint64_t(q) ? x : x - q;
 }
 
+// High 64 bits of a 64 x 64-bit product from 32-bit multiply-adds.
+FHE_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
+  const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32);
+  const uint32_t b0 = uint32_t(b), b1 = uint32_t(b >> 32);
+  const uint64_t ll = uint64_t(a0) * b0;
+  const uint64_t lh = uint64_t(a0) * b1 + (ll >> 32);
+  const uint64_t hl = uint64_t(a1) * b0 + uint32_t(lh);
+  return uint64_t(a1) * b1 + (lh >> 32) + (hl >> 32);
+}
+
+// Shoup product (a representative of w y mod q in [0, 2q)) for any
+// y < 2^64, w < q and wq = floor(w 2^64 / q).
+FHE_HD uint64_t shoup_mult(uint64_t y, uint64_t w, uint64_t wq, uint64_t q) {
+  return w * y - mulhi64(wq, y) * q;
+}
+
 // Montgomery product of arbitrary int64 operands: the 128-bit product of the
 // sign-extended values, i.e. the unsigned product corrected in its high word.
 FHE_HD int64_t mont_mult(int64_t a, int64_t b, uint64_t q, uint64_t k) {

@@ -58,22 +58,6 @@ __device__ __forceinline__ void gs_butterfly(int64_t &x, int64_t &y, int64_t w,
   y = mont_mult_u(w, guard_u(sub64(add64(x0, q2), y0), q2), q, k);
 }
 
-// High 64 bits of a 64 x 64-bit product from 32-bit multiply-adds.
-__device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
-  const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32);
-  const uint32_t b0 = uint32_t(b), b1 = uint32_t(b >> 32);
-  const uint64_t ll = uint64_t(a0) * b0;
-  const uint64_t lh = uint64_t(a0) * b1 + (ll >> 32);
-  const uint64_t hl = uint64_t(a1) * b0 + uint32_t(lh);
-  return uint64_t(a1) * b1 + (lh >> 32) + (hl >> 32);
-}
-
-// Shoup: w y mod q in [0, 2q) for any y < 2^64, w < q, wq = floor(w 2^64 / q).
-__device__ __forceinline__ uint64_t shoup_mult(uint64_t y, uint64_t w,
-                                               uint64_t wq, uint64_t q) {
-  return w * y - mulhi64(wq, y) * q;
-}
-
 // Everything a thread needs to address twiddles in one 8-bit pass: the pass
 // covers index bits [b, b+8) of a limb of 2^log_n points; hi is the
 // sub-transform's index bits above b+8 and r the thread's row.

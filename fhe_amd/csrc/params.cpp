@@ -317,12 +317,21 @@ void CkksParams::derive(bool twiddles) {
       prod = mod_mul(prod, int64_t(q[s]), int64_t(q[i]));
     p_product_mult_r[i] = prod;
   }
+  q_top.resize(P);
+  for (int i = 0; i < P; i++)
+    q_top[i] = ((uint64_t(1) << 63) - 1) / q[i] * q[i];
   p_inverse_mult_r.assign(size_t(alpha) * P, 0);
+  p_inverse_shoup.assign(size_t(alpha) * P * 2, 0);
   for (int sp = 0; sp < alpha; sp++) {
     const int pi = P - 1 - sp;
-    for (int i = 0; i < pi; i++)
-      p_inverse_mult_r[size_t(sp) * P + i] = mod_mul(
-          mod_inverse(int64_t(q[pi]), int64_t(q[i])), kMontR, int64_t(q[i]));
+    for (int i = 0; i < pi; i++) {
+      const int64_t inv = mod_inverse(int64_t(q[pi]), int64_t(q[i]));
+      p_inverse_mult_r[size_t(sp) * P + i] =
+          mod_mul(inv, kMontR, int64_t(q[i]));
+      p_inverse_shoup[(size_t(sp) * P + i) * 2] = uint64_t(inv);
+      p_inverse_shoup[(size_t(sp) * P + i) * 2 + 1] =
+          uint64_t(((unsigned __int128)uint64_t(inv) << 64) / q[i]);
+    }
   }
   rescale_scale.assign(size_t(P) * P, 0);
   for (int from = 0; from < P; from++)
@@ -375,13 +384,18 @@ std::vector<int64_t> CkksParams::modup_constants(int moduli_id,
             prefix_product_mod(*this, moduli_id, part.start, j, mt), kMontR, mt);
       }
     }
+    const int shoup = ks_ext_shoup_offset(moduli_id);
     for (int j = 0; j < part.size; j++)
       for (int i = 0; i < ext; i++) {
         const int64_t m = int64_t(q[moduli_id + i]);
+        const int64_t prod =
+            prefix_product_mod(*this, moduli_id, part.start, j, m);
         blk[kKsExtOffset + j * ext + i] =
-            j == 0 ? r2[moduli_id + i]
-                   : mod_mul(prefix_product_mod(*this, moduli_id, part.start, j, m),
-                             r2[moduli_id + i], m);
+            j == 0 ? r2[moduli_id + i] : mod_mul(prod, r2[moduli_id + i], m);
+        const uint64_t w = uint64_t(mod_mul(prod, kMontR, m));
+        blk[shoup + 2 * (j * ext + i)] = int64_t(w);
+        blk[shoup + 2 * (j * ext + i) + 1] =
+            int64_t(uint64_t(((unsigned __int128)w << 64) / uint64_t(m)));
       }
   }
   return c;

@@ -81,6 +81,12 @@ public:
   std::vector<double> deviation;
   std::vector<int64_t> p_product_mult_r;   // [chain_count]
   std::vector<int64_t> p_inverse_mult_r;   // [alpha][prime_count]
+  // the same inverses in standard form with Shoup quotients,
+  // [alpha][prime_count][2] = {p^-1 mod q, floor(p^-1 * 2^64 / q)}
+  std::vector<uint64_t> p_inverse_shoup;
+  // largest multiple of q below 2^63, per prime: adding it makes any
+  // difference x - y with x >= 0 and |y| < 2^62 non-negative and below 2^64
+  std::vector<uint64_t> q_top;
   std::vector<int64_t> rescale_scale;      // [prime_count][prime_count]
 
   int limbs_at(int moduli_id) const { return chain_count - moduli_id; }
@@ -95,16 +101,21 @@ public:
   std::vector<Partition> partitions(int moduli_id, int key_moduli_id) const;
 
   // Device constants for ModUp of a ciphertext at moduli_id: for each
-  // partition p, a block of kKsStride int64 entries
+  // partition p, a block of ks_stride int64 entries
   //   [0, 16)            Garner inverses (prod q_<j)^-1 * R mod q_j, j>=1
   //   [16, 16+256)       Garner tile constants (prod q_<=j) * R mod q_t
   //   [272, 272+16*ext)  basis-extension constants (prod q_<j) * R^2 mod q_i
+  //   [272+16*ext, +32*ext)  the same in standard form (prod q_<j) * R mod
+  //                      q_i with its Shoup quotient, pairs at 2 (j ext + i)
   static constexpr int kMaxAlpha = 16;
   static constexpr int kKsInvOffset = 0;
   static constexpr int kKsTileOffset = 16;
   static constexpr int kKsExtOffset = 16 + 16 * 16;
-  int ks_stride(int moduli_id) const {
+  int ks_ext_shoup_offset(int moduli_id) const {
     return kKsExtOffset + kMaxAlpha * (limbs_at(moduli_id) + alpha);
+  }
+  int ks_stride(int moduli_id) const {
+    return kKsExtOffset + 3 * kMaxAlpha * (limbs_at(moduli_id) + alpha);
   }
   std::vector<int64_t> modup_constants(int moduli_id, int key_moduli_id) const;
   // True when the first prime of the partition is at most twice each of its

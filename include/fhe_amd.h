@@ -252,6 +252,19 @@ int fhe_rotate(fhe_context *ctx, int count, int moduli_id, const int64_t *in,
                const int64_t *key, int key_moduli_id, int delta,
                int64_t *out);
 
+/* Hoisted rotations: the same `count` INTT-form duos rotated by key_count
+ * fixed rotation keys (keys[k] at key_moduli_ids[k], made for deltas[k])
+ * into outs[k] [count][2][limbs][N].  The key-independent decomposition of
+ * c1 (ModUp and the first NTT pass) is done once
+ * (CKKSEngine::rotate_batch, src/ckks/engine_rotate.cpp:262-300, with
+ * CKKSCore::decompose and rotate(ct, decomposed, key),
+ * src/ckks/core_key_switch.cpp:434-460, src/ckks/core.cpp:938-951).  Each
+ * outs[k] equals fhe_rotate with keys[k]; no output may alias the input. */
+int fhe_rotate_batch(fhe_context *ctx, int count, int moduli_id,
+                     const int64_t *in, int key_count,
+                     const int64_t *const *keys, const int *key_moduli_ids,
+                     const int *deltas, int64_t *const *outs);
+
 /* Conjugation of `count` INTT-form duos with a conjugation key
  * (CKKSEngine::conjugate, src/ckks/engine_conjugate.cpp:53-73;
  * CKKSCore::conjugate, src/ckks/core.cpp:1035-1056).  out may alias in. */

@@ -226,3 +226,36 @@ def test_real_keys_match_reference_and_decrypt(log_n, alpha, level, gpu_lib):
     lib.ref_free_public_key(pk)
     lib.ref_free_secret_key(sk)
     ref.close()
+
+
+@pytest.mark.parametrize("name", ["silver_mid9_rot3", "gold_mid13_rot1"])
+def test_rotate_batch_equals_single_rotations(name, gpu_lib):
+    """Hoisted rotations with three keys (different deltas and keys) equal
+    three single rotations, and the single rotation matches the reference
+    digest of the case."""
+    case = [c for c in CASES if c["name"] == name][0]
+    ctx = context_for(case)
+    A, K = case_inputs(case)
+    rng = np.random.Generator(np.random.PCG64(31))
+    mid, key_mid = case["moduli_id"], case["key_moduli_id"]
+    keys = [K] + [np.stack([uniform_limbs(rng, case["primes"], key_mid,
+                                          case["key_chain"], ctx.n, 2)
+                            for _ in range(case["key_polys"])])
+                  for _ in range(2)]
+    deltas = [case["delta"], 5, 77]
+    batch = np.concatenate([A[None], np.stack(
+        [np.stack([uniform_limbs(rng, case["primes"], mid, case["limbs"],
+                                 ctx.n) for _ in range(2)])])])
+    dct = ctx.to_device(batch)
+    dkeys = [ctx.to_device(k) for k in keys]
+    outs = [ctx.array(batch.shape) for _ in keys]
+    ctx.rotate_batch(dct, dkeys, [key_mid] * 3, deltas, outs, mid)
+    got = [o.download() for o in outs]
+    assert digest(got[0][0]) == case["out_sha256"]
+    for k in range(3):
+        single = ctx.array(batch.shape)
+        ctx.rotate(dct, dkeys[k], key_mid, deltas[k], single, mid)
+        assert (single.download() == got[k]).all(), k
+        single.free()
+    for d in [dct] + dkeys + outs:
+        d.free()

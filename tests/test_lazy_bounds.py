@@ -111,3 +111,43 @@ def test_lazy_butterflies_bounds_and_residues(bits, mode):
     assert [v % q for v in lazy] == exact
     # inverse: below 2q * 2^stages (small; 2^59 after 16 stages for q < 2^42)
     assert top < ((2 * q) << log_n if mode == "small" else 2 * q)
+
+
+@pytest.mark.parametrize("bits", [40, 54, 60])
+def test_lazy_moddown_step_matches_reference_formula(bits):
+    """ModDown's per-limb steps v = mont(guard(v + 2q - p_j, 2q), s_j^-1 R)
+    (signed Montgomery products, then make_unsigned_reduce_2q_to_q) equal
+    the Shoup form on v + q_top - p_j, for peeled integers p_j over the range
+    (-2^60, 2^62) that the special-prime chain produces (canonical special
+    limbs below 2^61, then signed products of values in (-2^61, 2^62) with
+    constants below 2^61)."""
+    from support import Oracle
+    lib = Oracle.lib()
+    q = ntt_prime(bits, 1 << 10)
+    k = lib.oracle_calculate_k(q)
+    top = ((1 << 63) - 1) // q * q
+    rng = random.Random(bits)
+    R = 1 << 62
+    for trial in range(300):
+        v = rng.randrange(q)
+        steps = []
+        for _ in range(4):
+            s_inv = rng.randrange(1, q)
+            p = rng.choice([rng.randrange(-(1 << 60) + 1, 1 << 62),
+                            -(1 << 60) + 1, (1 << 62) - 1, 0])
+            steps.append((s_inv, p))
+        # the reference's signed chain
+        x = v
+        for s_inv, p in steps:
+            g = x + 2 * q - p
+            g = g if g < 2 * q else g - 2 * q
+            x = lib.oracle_mont_mult(g, s_inv * R % q, q, k)
+        expect = x + q if x < 0 else (x if x < q else x - q)
+        # the lazy Shoup chain
+        y = v
+        for s_inv, p in steps:
+            a = (y + top - p) & M64
+            assert 0 <= y + top - p < 1 << 64
+            y = shoup(a, s_inv, (s_inv << 64) // q, q)
+        got = y if y < q else y - q
+        assert got == expect, (trial, v, steps)
