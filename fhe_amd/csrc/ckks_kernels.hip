@@ -213,6 +213,7 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
 
   if (!RESCALE) {
     int64_t *o = out + ch * limbs * n + coef;
+#pragma unroll 4
     for (int i = 0; i < limbs; i++)
       o[int64_t(i) * n] = relin(i);
     return;
@@ -221,6 +222,7 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
   const int64_t y0 = relin(0);
   const int64_t rounder = y0 > int64_t(t.qh[moduli_id]) ? 1 : 0;
   int64_t *o = out + ch * (limbs - 1) * n + coef;
+#pragma unroll 4
   for (int i = 1; i < limbs; i++) {
     const int p = moduli_id + i;
     const uint64_t q = t.q[p], k = t.k[p];

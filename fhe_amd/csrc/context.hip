@@ -100,7 +100,13 @@ struct fhe_context {
   // ModUp fused with the first NTT pass of the extended limbs
   bool fuse_modup = false;
   // input NTT second pass + tensor + trio INTT first pass in one kernel
-  bool fuse_tensor = true;
+  bool fuse_tensor = false;
+  // ModUp staged through the Garner digits (digits kernel + extension fused
+  // with the first NTT pass)
+  bool staged_modup = true;
+  // second growable device buffer (Garner digits of the staged ModUp)
+  void *aux = nullptr;
+  size_t aux_bytes = 0;
   // stage timing: HIP events recorded on `stream` around each stage while
   // enabled, folded into per-stage totals when read
   bool timing = false;
@@ -150,6 +156,8 @@ struct fhe_context {
       hipFree(p);
     if (workspace)
       hipFree(workspace);
+    if (aux)
+      hipFree(aux);
     if (stream)
       hipStreamDestroy(stream);
   }
@@ -198,6 +206,19 @@ struct fhe_context {
   int ext_slots(int moduli_id) const {
     return fhe::ks_fused_supported(tables) ? params.partition_count(moduli_id)
                                            : 1;
+  }
+
+  int64_t *aux_scratch(size_t bytes) {
+    if (bytes > aux_bytes) {
+      hip_check(hipStreamSynchronize(stream), "workspace sync");
+      if (aux)
+        hip_check(hipFree(aux), "workspace free");
+      aux = nullptr;
+      aux_bytes = 0;
+      hip_check(hipMalloc(&aux, bytes), "workspace allocation");
+      aux_bytes = bytes;
+    }
+    return static_cast<int64_t *>(aux);
   }
 
   int64_t *scratch(size_t bytes) {
@@ -315,6 +336,21 @@ struct fhe_context {
       if (p < fhe::kMaxFusedParts)
         is_signed[p] = !params.modup_non_negative(moduli_id, parts[p]);
       max_size = std::max(max_size, sizes[p]);
+    }
+    if (staged_modup && fhe::staged_modup_supported(tables, max_size, np)) {
+      int maxz = 1;
+      while (maxz < max_size)
+        maxz <<= 1;
+      int64_t *digits =
+          aux_scratch(size_t(count) * np * maxz * size_t(N) * 8);
+      stage("modup_staged(ext)", [&] {
+        fhe::launch_staged_modup(tables, src, src_stride, digits, ext, count,
+                                 limbs, moduli_id, starts.data(), sizes.data(),
+                                 is_signed, np, cst, stride,
+                                 params.ks_ext_shoup_offset(moduli_id),
+                                 lazy_key_switch, stream);
+      });
+      return;
     }
     if (fuse_modup && fhe::modup_ntt_fused_supported(tables, max_size, np)) {
       stage("modup_ntt_high(ext)", [&] {

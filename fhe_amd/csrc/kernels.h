@@ -97,6 +97,22 @@ void launch_modup_ntt_fused(const DeviceTables &t, const int64_t *x,
                             const int64_t *constants, int constants_stride,
                             bool lazy, hipStream_t stream);
 
+// ModUp staged through the Garner digits (N = 2^16, partitions of at most
+// 4 limbs): digits [count][parts][maxz][N] (maxz = partition size rounded up
+// to a power of two) then ext [count][parts][ext][N] after the first forward
+// NTT pass, as launch_modup + ntt_forward_first would give (the same
+// representatives without lazy; with lazy, for non-negative partitions, the
+// same residues).
+bool staged_modup_supported(const DeviceTables &t, int max_partition_size,
+                            int parts);
+void launch_staged_modup(const DeviceTables &t, const int64_t *x,
+                         int64_t x_stride, int64_t *digits, int64_t *ext,
+                         int count, int limbs, int moduli_id,
+                         const int *starts, const int *sizes,
+                         const bool *is_signed, int parts,
+                         const int64_t *constants, int constants_stride,
+                         int ext_shoup_offset, bool lazy, hipStream_t stream);
+
 // Second forward NTT pass of both input duos, tensor and first inverse NTT
 // pass of the trio in one kernel (N = 2^16).  a, b: [count][2][limbs][N]
 // after ntt_forward_first; trio: [count][3][limbs][N] for

@@ -288,9 +288,223 @@ __global__ __launch_bounds__(kBlock, 1) void modup_ntt_r16_kernel(
     modup_ntt_tile<MAXZ, false, LAZY>(a, t, lds, ct, part, tile);
 }
 
+// ---------------------------------------------------------------------------
+// ModUp in two stages around HBM: the Garner digits of every partition
+// (MAXZ limbs per partition, far fewer than the extended limbs), then one
+// kernel that forms each extended limb from the digits and runs the first
+// (high-bit) forward NTT pass on it before its only write.  The 4096-
+// coefficient digit tile of a block is re-read by the blocks of the other
+// extended limbs; the limb index runs fastest in the block order so those
+// reads hit the XCD's L2.
+struct DigitArgs {
+  const int64_t *x;   // source polynomial of ciphertext c at x + c * x_stride
+  int64_t x_stride;
+  int64_t *digits;    // [count][parts][MAXZ][N]
+  const int64_t *cst;
+  int cst_stride;
+  int count, parts, moduli_id, maxz;
+  int start[kMaxFusedParts];
+  int size[kMaxFusedParts];
+  uint32_t signed_mask;
+};
+
+template <int MAXZ>
+__global__ __launch_bounds__(kBlock) void modup_digits_kernel(DigitArgs a,
+                                                             DeviceTables t) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= int64_t(a.count) * a.parts * n)
+    return;
+  const int64_t coef = idx & (n - 1);
+  const int64_t cp = idx >> t.log_n; // ciphertext * parts + partition
+  const int part = int(cp % a.parts);
+  const int64_t ct = cp / a.parts;
+  const int start = a.start[part], size = a.size[part];
+  const int64_t *xs = a.x + ct * a.x_stride + coef;
+  const int64_t *cst = a.cst + int64_t(part) * a.cst_stride;
+  int64_t xv[MAXZ], d[MAXZ];
+#pragma unroll
+  for (int j = 0; j < MAXZ; j++)
+    xv[j] = j < size ? xs[int64_t(start + j) * n] : 0;
+  if ((a.signed_mask >> part) & 1)
+    garner_digits<MAXZ, true>(xv, d, size, a.moduli_id + start, cst, t);
+  else
+    garner_digits<MAXZ, false>(xv, d, size, a.moduli_id + start, cst, t);
+  int64_t *out = a.digits + cp * MAXZ * n + coef;
+#pragma unroll
+  for (int j = 0; j < MAXZ; j++)
+    if (j < size)
+      out[int64_t(j) * n] = d[j];
+}
+
+struct ExtendArgs {
+  const int64_t *digits; // [count][parts][maxz][N]
+  int64_t *ext;          // [count][parts][ext_limbs][N]
+  const int64_t *cst;
+  int cst_stride, ext_shoup_offset;
+  int count, parts, ext_limbs, moduli_id, maxz, swizzle;
+  int size[kMaxFusedParts];
+  uint32_t signed_mask;
+};
+
+template <int A, int MAXZ, bool SIGNED>
+__device__ __forceinline__ void extend_ntt_tile(const ExtendArgs &a,
+                                                const DeviceTables &t,
+                                                int64_t *lds, int64_t cp,
+                                                int part, int tile, int i) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int p = a.moduli_id + i;
+  const int tid = threadIdx.x;
+  const int c = tid & 15, r = tid >> 4;
+  const int64_t sp = int64_t(tile) * 16 + c;
+  const int size = a.size[part];
+  const int64_t *dg = a.digits + cp * a.maxz * n + sp;
+  const int64_t *cst = a.cst + int64_t(part) * a.cst_stride;
+  R16Pass s;
+  s.tw = t.root + int64_t(p) * n;
+  s.ts = t.root_shoup + int64_t(p) * n * 2;
+  s.n = n;
+  s.hi = 0;
+  s.log_n = t.log_n;
+  s.b = 8;
+  s.r = r;
+  s.q = t.q[p];
+  s.k = t.k[p];
+  s.q2 = int64_t(t.q2[p]);
+  int64_t v[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    int64_t d[MAXZ];
+#pragma unroll
+    for (int j = 0; j < MAXZ; j++)
+      d[j] = j < size ? dg[int64_t(j) * n + (int64_t(r + 16 * m) << 8)] : 0;
+    if (A == kExact)
+      v[m] = extend_limb<MAXZ, SIGNED>(d, size, i, a.ext_limbs,
+                                       cst + CkksParams::kKsExtOffset, s.q,
+                                       s.k, s.q2);
+    else
+      v[m] = extend_limb_lazy<MAXZ>(
+          d, size, i, a.ext_limbs,
+          reinterpret_cast<const uint64_t *>(cst + a.ext_shoup_offset), s.q,
+          A == kLazySmall);
+  }
+  r16_forward_high<A>(v, s);
+  r16_exchange(v, lds, c, r, true);
+  r16_forward_low<A>(v, s);
+  int64_t *out = a.ext + (cp * a.ext_limbs + i) * n + sp;
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    out[int64_t(16 * r + m) << 8] = v[m];
+}
+
+template <int MAXZ, bool LAZY>
+__global__ __launch_bounds__(kBlock) void extend_ntt_r16_kernel(
+    ExtendArgs a, DeviceTables t) {
+  __shared__ int64_t lds[kR16Lds];
+  const int tiles = int((int64_t(1) << t.log_n) >> 12);
+  int64_t w = blockIdx.x;
+  if (a.swizzle) {
+    const int64_t per = int64_t(gridDim.x) >> 3;
+    w = int64_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  // extended limb fastest, then tile, then (ciphertext, partition)
+  const int i = int(w % a.ext_limbs);
+  const int64_t rest = w / a.ext_limbs;
+  const int tile = int(rest % tiles);
+  const int64_t cp = rest / tiles;
+  const int part = int(cp % a.parts);
+  const bool sgn = (a.signed_mask >> part) & 1;
+  if (!LAZY) {
+    if (sgn)
+      extend_ntt_tile<kExact, MAXZ, true>(a, t, lds, cp, part, tile, i);
+    else
+      extend_ntt_tile<kExact, MAXZ, false>(a, t, lds, cp, part, tile, i);
+  } else if (t.q[a.moduli_id + i] < kLazySmallBound) {
+    extend_ntt_tile<kLazySmall, MAXZ, false>(a, t, lds, cp, part, tile, i);
+  } else {
+    extend_ntt_tile<kLazyLarge, MAXZ, false>(a, t, lds, cp, part, tile, i);
+  }
+}
+
 } // namespace
 
 bool ks_fused_supported(const DeviceTables &t) { return t.log_n == 16; }
+
+bool staged_modup_supported(const DeviceTables &t, int max_partition_size,
+                            int parts) {
+  return t.log_n == 16 && max_partition_size <= 4 && parts <= kMaxFusedParts;
+}
+
+void launch_staged_modup(const DeviceTables &t, const int64_t *x,
+                         int64_t x_stride, int64_t *digits, int64_t *ext,
+                         int count, int limbs, int moduli_id,
+                         const int *starts, const int *sizes,
+                         const bool *is_signed, int parts,
+                         const int64_t *constants, int constants_stride,
+                         int ext_shoup_offset, bool lazy, hipStream_t stream) {
+  int maxz = 1;
+  uint32_t mask = 0;
+  bool any_signed = false;
+  for (int p = 0; p < parts; p++) {
+    while (maxz < sizes[p])
+      maxz <<= 1;
+    mask |= is_signed[p] ? (uint32_t(1) << p) : 0u;
+    any_signed = any_signed || is_signed[p];
+  }
+  DigitArgs d{};
+  d.x = x;
+  d.x_stride = x_stride;
+  d.digits = digits;
+  d.cst = constants;
+  d.cst_stride = constants_stride;
+  d.count = count;
+  d.parts = parts;
+  d.moduli_id = moduli_id;
+  d.maxz = maxz;
+  d.signed_mask = mask;
+  ExtendArgs e{};
+  e.digits = digits;
+  e.ext = ext;
+  e.cst = constants;
+  e.cst_stride = constants_stride;
+  e.ext_shoup_offset = ext_shoup_offset;
+  e.count = count;
+  e.parts = parts;
+  e.ext_limbs = limbs + t.alpha;
+  e.moduli_id = moduli_id;
+  e.maxz = maxz;
+  e.signed_mask = mask;
+  for (int p = 0; p < parts; p++) {
+    d.start[p] = starts[p];
+    d.size[p] = sizes[p];
+    e.size[p] = sizes[p];
+  }
+  // the lazy extension needs canonical digits: only for non-negative ones
+  const bool use_lazy = lazy && !any_signed;
+  const int tiles = int((int64_t(1) << t.log_n) >> 12);
+  const int64_t eblocks = int64_t(count) * parts * tiles * e.ext_limbs;
+  e.swizzle = eblocks % 8 == 0 ? 1 : 0;
+  const dim3 dgrid(grid_for((int64_t(count) * parts) << t.log_n));
+  const dim3 egrid{unsigned(eblocks), 1, 1};
+#define FHE_STAGED(W)                                                          \
+  do {                                                                         \
+    hipLaunchKernelGGL(modup_digits_kernel<W>, dgrid, dim3(kBlock), 0, stream, \
+                       d, t);                                                  \
+    if (use_lazy)                                                              \
+      hipLaunchKernelGGL((extend_ntt_r16_kernel<W, true>), egrid, dim3(kBlock), \
+                         0, stream, e, t);                                     \
+    else                                                                       \
+      hipLaunchKernelGGL((extend_ntt_r16_kernel<W, false>), egrid,             \
+                         dim3(kBlock), 0, stream, e, t);                       \
+  } while (0)
+  if (maxz == 1)
+    FHE_STAGED(1);
+  else if (maxz == 2)
+    FHE_STAGED(2);
+  else
+    FHE_STAGED(4);
+#undef FHE_STAGED
+}
 
 bool modup_ntt_fused_supported(const DeviceTables &t, int max_partition_size,
                                int parts) {
