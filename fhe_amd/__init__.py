@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "fhe_profile_count", "fhe_profile_entry", "fhe_elementwise",
     "fhe_multiplier", "fhe_params_multiplier", "fhe_multiply_double",
     "fhe_multiply_int", "fhe_add_double", "fhe_level_down",
-    "fhe_encrypt_pk", "fhe_decrypt", "fhe_rotate_batch",
+    "fhe_encrypt_pk", "fhe_decrypt", "fhe_rotate_batch", "fhe_switch_key",
 )
 
 
@@ -119,6 +119,7 @@ def library():
         lib.fhe_decrypt.argtypes = [vp, i32, i32, vp, vp, i32, vp]
         lib.fhe_rotate_batch.argtypes = [vp, i32, i32, vp, i32, vp, vp, vp,
                                          vp]
+        lib.fhe_switch_key.argtypes = [vp, i32, i32, vp, vp, i32, vp]
         lib.fhe_params_multiplier.argtypes = [
             ctypes.POINTER(Params), i32, ctypes.c_double, ctypes.c_int64, i32,
             i32, vp, i32, ctypes.POINTER(i32)]
@@ -328,6 +329,12 @@ class Context:
         dl = (ctypes.c_int * m)(*deltas)
         _check(library().fhe_rotate_batch(self.handle, count, moduli_id,
                                           cts.ptr, m, kp, km, dl, op))
+
+    def switch_key(self, cts, key, key_moduli_id, out, moduli_id):
+        count = int(np.prod(cts.shape[:-3]))
+        _check(library().fhe_switch_key(self.handle, count, moduli_id,
+                                        cts.ptr, key.ptr, key_moduli_id,
+                                        out.ptr))
 
     def conjugate(self, cts, key, key_moduli_id, out, moduli_id):
         count = int(np.prod(cts.shape[:-3]))

@@ -1135,6 +1135,35 @@ int fhe_rotate_batch(fhe_context *ctx, int count, int moduli_id,
   });
 }
 
+int fhe_switch_key(fhe_context *ctx, int count, int moduli_id,
+                   const int64_t *in, const int64_t *key, int key_moduli_id,
+                   int64_t *out) {
+  return guarded([&] {
+    require(ctx && in && key && out, "null argument");
+    require(count > 0, "Switch key: empty batch");
+    ctx->check_moduli(moduli_id, 1, "Switch key");
+    ctx->check_key(moduli_id, key_moduli_id, "Switch key");
+    const auto &P = ctx->params;
+    const int limbs = P.limbs_at(moduli_id);
+    const int ext_limbs = limbs + P.alpha;
+    const int64_t N = ctx->n();
+    const int64_t ext_l =
+        int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
+    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
+    int64_t *ws = ctx->scratch(size_t(ext_l + acc_l) * N * 8);
+    int64_t *ext = ws, *acc = ext + ext_l * N;
+    // CKKSCore::switch_key (src/ckks/core.cpp:1058-1083): key switching of
+    // c1, then c0 added to the first half and the second half copied
+    ctx->key_switch(count, moduli_id, key, key_moduli_id, in + limbs * N,
+                    2 * limbs * N, ext, acc);
+    ctx->stage("moddown_add", [&] {
+      fhe::launch_moddown_add(ctx->tables, acc, in, 2 * limbs * N, 1, out,
+                              count, limbs, moduli_id, ctx->stream);
+    });
+    ctx->check_launch("Switch key");
+  });
+}
+
 int fhe_conjugate(fhe_context *ctx, int count, int moduli_id,
                   const int64_t *in, const int64_t *key, int key_moduli_id,
                   int64_t *out) {

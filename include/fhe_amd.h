@@ -265,6 +265,15 @@ int fhe_rotate_batch(fhe_context *ctx, int count, int moduli_id,
                      const int64_t *const *keys, const int *key_moduli_ids,
                      const int *deltas, int64_t *const *outs);
 
+/* Key switching of `count` INTT-form duos to the secret of a
+ * key-switching key ([key_polys][key_limbs][N] at key_moduli_id): out =
+ * (c0 + KS(c1)_0, KS(c1)_1) (CKKSCore::switch_key, src/ckks/core.cpp:
+ * 1058-1083, used by mod_raise's dense/sparse switches,
+ * src/ckks/engine_mod_raise.cpp:5-34).  out may alias in. */
+int fhe_switch_key(fhe_context *ctx, int count, int moduli_id,
+                   const int64_t *in, const int64_t *key, int key_moduli_id,
+                   int64_t *out);
+
 /* Conjugation of `count` INTT-form duos with a conjugation key
  * (CKKSEngine::conjugate, src/ckks/engine_conjugate.cpp:53-73;
  * CKKSCore::conjugate, src/ckks/core.cpp:1035-1056).  out may alias in. */

@@ -664,6 +664,13 @@ static void switch_and_add(const oracle_ctx *c, int mid, const int64_t *x,
   }
 }
 
+/* CKKSCore::switch_key (src/ckks/core.cpp:1058-1083) on an INTT-form duo */
+void oracle_switch_key(const oracle_ctx *c, int mid, const int64_t *ct,
+                       const int64_t *key, int key_mid, int64_t *out) {
+  const size_t poly = (size_t)(c->chain_count - mid) * c->n;
+  switch_and_add(c, mid, ct + poly, ct, key, key_mid, out);
+}
+
 /* CKKSCore::rotate (src/ckks/core.cpp:954-962): decompose c1, key switcher
  * with b*P added in the NTT domain (src/ckks/core_key_switch.cpp:509-548),
  * ModDown (src/ckks/core.cpp:916-936), then the automorphism.  The b*P term

@@ -96,6 +96,8 @@ void oracle_rotate(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
                    int64_t *out);
 void oracle_conjugate(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
                       const int64_t *key, int key_moduli_id, int64_t *out);
+void oracle_switch_key(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
+                       const int64_t *key, int key_moduli_id, int64_t *out);
 
 /* public-key encryption with explicit samples u, e0, e1 (N values each)
  * and duo decryption to plaintext limbs */

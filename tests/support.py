@@ -89,6 +89,7 @@ class Oracle:
             lib.oracle_automorphism.argtypes = [VP, i32, i32, i32, i32, VP, VP]
             lib.oracle_rotate.argtypes = [VP, i32, VP, VP, i32, i32, VP]
             lib.oracle_conjugate.argtypes = [VP, i32, VP, VP, i32, VP]
+            lib.oracle_switch_key.argtypes = [VP, i32, VP, VP, i32, VP]
             cls._lib = lib
         return cls._lib
 
@@ -203,6 +204,14 @@ class Oracle:
         self.lib().oracle_rotate(self.h, mid, ptr(np.ascontiguousarray(ct)),
                                  ptr(np.ascontiguousarray(key)), key_mid,
                                  delta, ptr(out))
+        return out
+
+    def switch_key(self, ct, key, mid, key_mid):
+        out = np.zeros((2, self.limbs_at(mid), self.n), np.int64)
+        self.lib().oracle_switch_key(self.h, mid,
+                                     ptr(np.ascontiguousarray(ct)),
+                                     ptr(np.ascontiguousarray(key)), key_mid,
+                                     ptr(out))
         return out
 
     def conjugate(self, ct, key, mid, key_mid):

@@ -259,3 +259,28 @@ def test_rotate_batch_equals_single_rotations(name, gpu_lib):
         single.free()
     for d in [dct] + dkeys + outs:
         d.free()
+
+
+@pytest.mark.parametrize("name", ["bronze_mid4_rot1", "gold_mid13_rot1"])
+def test_switch_key_matches_oracle(name, gpu_lib):
+    """fhe_switch_key = (c0 + KS(c1)_0, KS(c1)_1): the oracle's switch (the
+    rotation's key switch before its automorphism, pinned by the rotation
+    digests) on the small ring; on Gold, the automorphism of the switched
+    duo equals the rotation."""
+    case = [c for c in CASES if c["name"] == name][0]
+    ctx = context_for(case)
+    A, K = case_inputs(case)
+    mid, key_mid = case["moduli_id"], case["key_moduli_id"]
+    da, dk = ctx.to_device(A[None]), ctx.to_device(K)
+    out = ctx.array(da.shape)
+    ctx.switch_key(da, dk, key_mid, out, mid)
+    got = out.download()[0]
+    if case["log_n"] <= 15:
+        orc = Oracle(case["log_n"], case["primes"],
+                     case["special_prime_count"])
+        assert (got == orc.switch_key(A, K, mid, key_mid)).all()
+    rot = ctx.array(da.shape)
+    ctx.automorphism(out, rot, mid, case["delta"])
+    assert digest(rot.download()[0]) == case["out_sha256"]
+    for d in (da, dk, out, rot):
+        d.free()
