@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "fhe_multiplier", "fhe_params_multiplier", "fhe_multiply_double",
     "fhe_multiply_int", "fhe_add_double", "fhe_level_down",
     "fhe_encrypt_pk", "fhe_decrypt", "fhe_rotate_batch", "fhe_switch_key",
+    "fhe_encode", "fhe_params_encode_constants",
 )
 
 
@@ -123,6 +124,11 @@ def library():
         lib.fhe_params_multiplier.argtypes = [
             ctypes.POINTER(Params), i32, ctypes.c_double, ctypes.c_int64, i32,
             i32, vp, i32, ctypes.POINTER(i32)]
+        lib.fhe_encode.argtypes = [vp, vp, i32, i32, i32, vp]
+        lib.fhe_params_encode_constants.argtypes = [
+            ctypes.POINTER(Params), i32, ctypes.POINTER(ctypes.c_double),
+            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+            ctypes.POINTER(i32), vp, i32, ctypes.POINTER(i32)]
         lib.fhe_profile_count.argtypes = [vp, ctypes.POINTER(i32)]
         lib.fhe_profile_entry.argtypes = [
             vp, i32, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
@@ -172,6 +178,28 @@ def params_multiplier(log_n, special_prime_count, kind, level, value=0.0,
         ctypes.byref(p), MULTIPLIER[kind], float(value), int(int_value),
         level, level2, out.ctypes.data, 512, ctypes.byref(count)))
     return out[:count.value]
+
+
+def params_encode_constants(log_n, special_prime_count, level, scale_bits=40,
+                            small_prime_bits=40, one_more_level=True,
+                            primes=None):
+    """Host-only fhe_params_encode_constants: dict of ckks_scale,
+    scale_before_tile, log_scale_after_tile, top_of_one_more_level and
+    r_mult_scale_after_tile."""
+    p = make_params(log_n, special_prime_count, scale_bits, small_prime_bits,
+                    one_more_level, -1, primes)
+    out = np.zeros(512, np.int64)
+    count = ctypes.c_int(0)
+    ckks_scale, sbt = ctypes.c_double(), ctypes.c_double()
+    log_after, top = ctypes.c_int64(), ctypes.c_int()
+    _check(library().fhe_params_encode_constants(
+        ctypes.byref(p), level, ctypes.byref(ckks_scale), ctypes.byref(sbt),
+        ctypes.byref(log_after), ctypes.byref(top), out.ctypes.data, 512,
+        ctypes.byref(count)))
+    return dict(ckks_scale=ckks_scale.value, scale_before_tile=sbt.value,
+                log_scale_after_tile=log_after.value,
+                top_of_one_more_level=bool(top.value),
+                r_mult_scale_after_tile=out[:count.value])
 
 
 class DeviceArray:
@@ -397,6 +425,42 @@ class Context:
         count = int(np.prod(cts.shape[:-3]))
         _check(library().fhe_decrypt(self.handle, count, moduli_id, cts.ptr,
                                      sk.ptr, sk_moduli_id, out.ptr))
+
+    def encode(self, values, level, out=None):
+        """Encode host slot vectors (real or complex, shape [slots] or
+        [count][slots], shorter vectors zero-padded) into plaintext limbs
+        [count][limbs][N] at `level`; returns the device array."""
+        values = np.asarray(values)
+        if values.ndim == 1:
+            values = values[None]
+        slots = self.n // 2
+        if values.shape[-1] > slots:
+            raise ValueError("more values than slots")
+        is_complex = np.iscomplexobj(values)
+        count = values.shape[0]
+        if is_complex:
+            host = np.zeros((count, slots, 2), np.float64)
+            host[:, :values.shape[1], 0] = values.real
+            host[:, :values.shape[1], 1] = values.imag
+        else:
+            host = np.zeros((count, slots), np.float64)
+            host[:, :values.shape[1]] = values
+        dev = self.to_device(host.view(np.int64))
+        limbs = self.info["chain_count"] - (self.info["full_level"] - level)
+        if out is None:
+            out = self.array((count, limbs, self.n))
+        try:
+            self.encode_device(dev, is_complex, count, level, out)
+            self.synchronize()
+        finally:
+            dev.free()
+        return out
+
+    def encode_device(self, values, complex_input, count, level, out):
+        """fhe_encode on a device array of doubles (viewed as int64)."""
+        _check(library().fhe_encode(self.handle, values.ptr,
+                                    1 if complex_input else 0, count, level,
+                                    out.ptr))
 
     def profile(self, on):
         """Start (clearing totals) or stop stage timing."""

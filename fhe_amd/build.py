@@ -23,14 +23,17 @@ ARCH = "gfx950"
 HOST_SOURCES = ["params.cpp"]
 HIP_SOURCES = ["ntt.hip", "ckks_kernels.hip", "ks_fused.hip", "automorphism.hip",
                "elementwise.hip", "crypt.hip", "tensor_fused.hip",
-               "context.hip"]
+               "encode.hip", "context.hip"]
+# per-source flags: the encoder's double arithmetic must round every product
+# and sum separately, as the reference's x86-64 build does
+HIP_SOURCE_FLAGS = {"encode.hip": ["-ffp-contract=off"]}
 
 INCLUDES = ["-I" + os.path.join(REPO, "include"), "-I" + CSRC]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              # no packed-FP32 instruction selection anywhere in this library
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
+HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall"]
 
 
 def _run(cmd):
@@ -55,7 +58,8 @@ def build(verbose=False):
         objects.append(obj)
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        jobs.append([HIPCC, *HIP_FLAGS, *INCLUDES, "-c",
+        jobs.append([HIPCC, *HIP_FLAGS, *HIP_SOURCE_FLAGS.get(src, []),
+                     *INCLUDES, "-c",
                      os.path.join(CSRC, src), "-o", obj])
         objects.append(obj)
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as pool:

@@ -107,6 +107,9 @@ struct fhe_context {
   // second growable device buffer (Garner digits of the staged ModUp)
   void *aux = nullptr;
   size_t aux_bytes = 0;
+  // encoding tables, uploaded on the first encode
+  fhe::EncodeDevice encode{};
+  bool encode_ready = false;
   // stage timing: HIP events recorded on `stream` around each stage while
   // enabled, folded into per-stage totals when read
   bool timing = false;
@@ -232,6 +235,21 @@ struct fhe_context {
       workspace_bytes = bytes;
     }
     return static_cast<int64_t *>(workspace);
+  }
+
+  const fhe::EncodeDevice &encode_tables() {
+    if (!encode_ready) {
+      std::vector<double> fc, fs, tc, ts;
+      params.encode_trig(fc, fs, tc, ts);
+      encode.pre = upload(params.encode_pre_permutation(), owned);
+      encode.fft_cos = upload(fc, owned);
+      encode.fft_sin = upload(fs, owned);
+      encode.twist_cos = upload(tc, owned);
+      encode.twist_sin = upload(ts, owned);
+      encode.log_codec = params.log_n;
+      encode_ready = true;
+    }
+    return encode;
   }
 
   const int64_t *modup_constants(int moduli_id, int key_moduli_id) {
@@ -777,6 +795,60 @@ int fhe_params_multiplier(const fhe_params *p, int kind, double value,
     require(out && count, "null argument");
     multiplier_of(resolve_params(p, false), kind, value, int_value, level,
                   level2, out, capacity, count);
+  });
+}
+
+int fhe_params_encode_constants(const fhe_params *p, int level,
+                                double *ckks_scale, double *scale_before_tile,
+                                int64_t *log_scale_after_tile,
+                                int *top_of_one_more_level, int64_t *r_mult,
+                                int capacity, int *count) {
+  return guarded([&] {
+    require(ckks_scale && scale_before_tile && log_scale_after_tile &&
+                top_of_one_more_level && r_mult && count,
+            "null argument");
+    const auto P = resolve_params(p, false);
+    require(level >= 0 && level <= P.full_level,
+            "encode constants: level out of range");
+    const auto e = P.encode_level(level);
+    const auto &v = e.r_mult_scale_after_tile;
+    require(capacity >= int(v.size()), "encode constants: capacity too small");
+    *ckks_scale = double(e.ckks_scale);
+    *scale_before_tile = e.scale_before_tile;
+    *log_scale_after_tile = e.log_scale_after_tile;
+    *top_of_one_more_level = e.top_of_one_more_level ? 1 : 0;
+    std::memcpy(r_mult, v.data(), v.size() * sizeof(int64_t));
+    *count = int(v.size());
+  });
+}
+
+int fhe_encode(fhe_context *ctx, const double *values, int complex_input,
+               int count, int level, int64_t *out) {
+  return guarded([&] {
+    require(ctx && values && out, "null argument");
+    require(count > 0, "Encode: empty batch");
+    const auto &P = ctx->params;
+    require(level >= 0 && level <= P.max_level,
+            "Encode: the target level cannot be reached with the current "
+            "engine parameters.");
+    const auto e = P.encode_level(level);
+    const int mid = P.moduli_id_of_level(level);
+    const int limbs = P.limbs_at(mid);
+    // the top level of a one-more-level chain leaves limb 0 zero and encodes
+    // onto the primes of the level below (src/ckks/core_encode.cpp:64-78)
+    const int first = e.top_of_one_more_level ? 1 : 0;
+    const int enc_mid =
+        e.top_of_one_more_level ? P.moduli_id_of_level(level - 1) : mid;
+    require(limbs - first <= fhe::kMaxEwLimbs, "Encode: too many limbs");
+    const auto &tables = ctx->encode_tables();
+    void *work = ctx->aux_scratch(fhe::encode_work_bytes(tables, count));
+    ctx->stage("encode", [&] {
+      fhe::launch_encode(ctx->tables, tables, values, complex_input ? 1 : 0,
+                         count, e.scale_before_tile, limbs, first, enc_mid,
+                         e.r_mult_scale_after_tile.data(), out, work,
+                         ctx->stream);
+    });
+    ctx->check_launch("Encode");
   });
 }
 

@@ -1,0 +1,210 @@
+// CKKS encoding on the device (CKKSCore::encode_polynomial,
+// src/ckks/core_encode.cpp:11-150): slot permutation, bit reversal, the
+// complex FFT over 2 * slots points with the host-built twiddles
+// (src/operation/fft.cpp:57-102), normalization and twisting
+// (src/ckks/encode.cpp:264-275), rounding to integers (llround, or the
+// reference's double_to_rns when the largest coefficient exceeds 2^59,
+// src/arithmetic/arithmetic.cpp:242-269), the lift to every limb and the
+// Montgomery entry with the level's scale constants.
+//
+// Bit-exactness with the reference's double arithmetic: every product and
+// sum below is a separate IEEE operation in the reference's order (this file
+// is compiled without floating-point contraction and uses the explicit
+// round-to-nearest intrinsics), the cosine and sine values come from the
+// host's libm exactly as the reference computes them, and complex products
+// are (ac - bd, ad + bc).
+#include "kernels.h"
+
+namespace fhe {
+
+namespace {
+
+struct Cplx {
+  double re, im;
+};
+
+__device__ __forceinline__ Cplx cmul(double ar, double ai, double br,
+                                     double bi) {
+  return {__dsub_rn(__dmul_rn(ar, br), __dmul_rn(ai, bi)),
+          __dadd_rn(__dmul_rn(ar, bi), __dmul_rn(ai, br))};
+}
+
+__device__ __forceinline__ uint32_t bit_reverse(uint32_t x, int bits) {
+  return __builtin_bitreverse32(x) >> (32 - bits);
+}
+
+// x [count][slots] (real) or [count][slots][2] (complex) into the
+// bit-reversed complex array y [count][codec]: slot i goes to codec index
+// p = pre[i] and its mirror codec - 1 - p (conjugated), as pre_permute
+// (src/ckks/encode.cpp:156-185) followed by bit_reversal_permute.
+// Also clears the per-ciphertext maxima of the twist kernel.
+__global__ __launch_bounds__(kBlock) void encode_scatter_kernel(
+    const double *x, Cplx *y, unsigned long long *max_bits, int count,
+    int slots, int log_codec, const uint32_t *pre, int complex_input) {
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx < count)
+    max_bits[idx] = 0;
+  if (idx >= int64_t(count) * slots)
+    return;
+  const int64_t c = idx / slots;
+  const int i = int(idx - c * slots);
+  const int codec = 2 * slots;
+  const double re = complex_input ? x[2 * idx] : x[idx];
+  const double im = complex_input ? x[2 * idx + 1] : 0.0;
+  const uint32_t p = pre[i];
+  Cplx *yc = y + c * codec;
+  yc[bit_reverse(p, log_codec)] = {re, im};
+  yc[bit_reverse(uint32_t(codec) - 1 - p, log_codec)] = {
+      re, complex_input ? -im : 0.0};
+}
+
+// One radix-2 stage of the in-place decimation-in-time FFT: step = 2^(s+1),
+// twiddle index (j - offset) * (codec / step).
+__global__ __launch_bounds__(kBlock) void encode_fft_stage_kernel(
+    Cplx *y, int count, int log_codec, int s, const double *tw_cos,
+    const double *tw_sin) {
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int half_codec = 1 << (log_codec - 1);
+  if (idx >= int64_t(count) * half_codec)
+    return;
+  const int64_t c = idx >> (log_codec - 1);
+  const int b = int(idx & (half_codec - 1));
+  const int half = 1 << s;
+  const int k = b & (half - 1);
+  const int j = ((b >> s) << (s + 1)) + k;
+  const int t = k << (log_codec - 1 - s);
+  Cplx *yc = y + (c << log_codec);
+  const Cplx left = yc[j], r = yc[j + half];
+  const Cplx right = cmul(tw_cos[t], tw_sin[t], r.re, r.im);
+  yc[j] = {__dadd_rn(left.re, right.re), __dadd_rn(left.im, right.im)};
+  yc[j + half] = {__dsub_rn(left.re, right.re), __dsub_rn(left.im, right.im)};
+}
+
+// normalize (src/operation/fft.cpp:50-55) and apply_twister
+// (src/ckks/encode.cpp:264-275): out[i] = (re_i / codec * cos_i - im_i /
+// codec * sin_i) * scale; the per-ciphertext max |out| as ordered bits.
+__global__ __launch_bounds__(kBlock) void encode_twist_kernel(
+    const Cplx *y, double *out, unsigned long long *max_bits, int count,
+    int log_codec, const double *cos_t, const double *sin_t, double scale) {
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int codec = 1 << log_codec;
+  if (idx >= (int64_t(count) << log_codec))
+    return;
+  const int i = int(idx & (codec - 1));
+  const double size = double(codec);
+  const double re = __ddiv_rn(y[idx].re, size);
+  const double im = __ddiv_rn(y[idx].im, size);
+  const double v = __dmul_rn(
+      __dsub_rn(__dmul_rn(re, cos_t[i]), __dmul_rn(im, sin_t[i])), scale);
+  out[idx] = v;
+  // |v| compares as its bit pattern
+  atomicMax(max_bits + (idx >> log_codec),
+            (unsigned long long)__double_as_longlong(fabs(v)));
+}
+
+struct ConvertArgs {
+  const double *v;            // [count][codec]
+  const unsigned long long *max_bits;
+  int64_t *out;               // [count][out_limbs][N]
+  int count, log_n, out_limbs;
+  int first;                  // limbs before the encoded ones (zero-filled)
+  int moduli_id;              // prime of encoded limb 0
+  int64_t c[kMaxEwLimbs];     // r_mult_scale_after_tile per encoded limb
+};
+
+// positive_modulo (src/arithmetic/arithmetic.cpp:271-273)
+__device__ __forceinline__ int64_t positive_mod(int64_t x, int64_t q) {
+  return (q + x % q) % q;
+}
+
+__global__ __launch_bounds__(kBlock) void encode_convert_kernel(
+    ConvertArgs a, DeviceTables t) {
+  const int64_t n = int64_t(1) << a.log_n;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= int64_t(a.count) * a.out_limbs * n)
+    return;
+  const int64_t coef = idx & (n - 1);
+  const int64_t cl = idx >> a.log_n;
+  const int l = int(cl % a.out_limbs);
+  const int64_t c = cl / a.out_limbs;
+  if (l < a.first) {
+    a.out[idx] = 0;
+    return;
+  }
+  const int e = l - a.first;
+  const int p = a.moduli_id + e;
+  const int64_t q = int64_t(t.q[p]);
+  const double x = a.v[(c << a.log_n) + coef];
+  const double max_abs = __longlong_as_double((long long)a.max_bits[c]);
+  int64_t y;
+  if (max_abs > double(uint64_t(1) << 59)) {
+    // double_to_rns
+    if (fabs(x) < double(int64_t(1) << 62)) {
+      y = positive_mod(int64_t(__dadd_rn(x, 0.5)), q);
+    } else {
+      int exponent;
+      const double mantissa = frexp(__dadd_rn(x, 0.5), &exponent);
+      int64_t r = positive_mod(
+          int64_t(__dmul_rn(mantissa, double(int64_t(1) << 52))), q);
+      for (int le = 52; le < exponent; le++) {
+        r <<= 1;
+        r %= q;
+      }
+      y = r;
+    }
+  } else {
+    // llround, then tile_unsigned_reduce_2q_to_q
+    const int64_t r = llround(x);
+    y = r < 0 ? r + q : guard(r, q);
+  }
+  a.out[idx] = mont_mult(y, a.c[e], t.q[p], t.k[p]);
+}
+
+} // namespace
+
+void launch_encode(const DeviceTables &t, const EncodeDevice &e,
+                   const double *x, int complex_input, int count,
+                   double scale_before_tile, int out_limbs, int first,
+                   int moduli_id, const int64_t *r_mult, int64_t *out,
+                   void *work, hipStream_t stream) {
+  const int codec = 1 << e.log_codec;
+  const int slots = codec / 2;
+  Cplx *y = static_cast<Cplx *>(work);
+  double *v = reinterpret_cast<double *>(y + int64_t(count) * codec);
+  auto *max_bits =
+      reinterpret_cast<unsigned long long *>(v + int64_t(count) * codec);
+  hipLaunchKernelGGL(encode_scatter_kernel,
+                     dim3(grid_for(int64_t(count) * slots)), dim3(kBlock), 0,
+                     stream, x, y, max_bits, count, slots, e.log_codec, e.pre,
+                     complex_input);
+  for (int s = 0; s < e.log_codec; s++)
+    hipLaunchKernelGGL(encode_fft_stage_kernel,
+                       dim3(grid_for(int64_t(count) * (codec / 2))),
+                       dim3(kBlock), 0, stream, y, count, e.log_codec, s,
+                       e.fft_cos, e.fft_sin);
+  hipLaunchKernelGGL(encode_twist_kernel,
+                     dim3(grid_for(int64_t(count) * codec)), dim3(kBlock), 0,
+                     stream, y, v, max_bits, count, e.log_codec, e.twist_cos,
+                     e.twist_sin, scale_before_tile);
+  ConvertArgs a{};
+  a.v = v;
+  a.max_bits = max_bits;
+  a.out = out;
+  a.count = count;
+  a.log_n = t.log_n;
+  a.out_limbs = out_limbs;
+  a.first = first;
+  a.moduli_id = moduli_id;
+  for (int l = 0; l < out_limbs - first; l++)
+    a.c[l] = r_mult[l];
+  hipLaunchKernelGGL(encode_convert_kernel,
+                     dim3(grid_for((int64_t(count) * out_limbs) << t.log_n)),
+                     dim3(kBlock), 0, stream, a, t);
+}
+
+size_t encode_work_bytes(const EncodeDevice &e, int count) {
+  const int64_t codec = int64_t(1) << e.log_codec;
+  return size_t(count) * codec * (16 + 8) + size_t(count) * 8 + 256;
+}
+
+} // namespace fhe

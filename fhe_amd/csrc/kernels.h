@@ -230,6 +230,25 @@ void launch_encrypt_finish(const DeviceTables &t, const int64_t *pt,
                            int count, int limbs, int moduli_id,
                            hipStream_t stream);
 
+// Encoding tables on the device (host-built, see CkksParams::encode_*).
+struct EncodeDevice {
+  const uint32_t *pre;      // slot -> codec index
+  const double *fft_cos, *fft_sin;
+  const double *twist_cos, *twist_sin;
+  int log_codec;
+};
+// Encode `count` messages x ([count][slots] doubles, or [count][slots][2]
+// for complex input) into out [count][out_limbs][N]: limbs [0, first) are
+// zero (the top level of the one-more-level chain), limbs from `first` use
+// primes moduli_id + (l - first) and the constants r_mult (host).  work:
+// encode_work_bytes(count) device bytes.
+size_t encode_work_bytes(const EncodeDevice &e, int count);
+void launch_encode(const DeviceTables &t, const EncodeDevice &e,
+                   const double *x, int complex_input, int count,
+                   double scale_before_tile, int out_limbs, int first,
+                   int moduli_id, const int64_t *r_mult, int64_t *out,
+                   void *work, hipStream_t stream);
+
 // Rescale of `polys` INTT-form polynomials (src/ckks/core_rescale.cpp:5-25).
 void launch_rescale(const DeviceTables &t, const int64_t *in, int64_t *out,
                     int polys, int limbs, int moduli_id, hipStream_t stream);

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <numbers>
 #include <stdexcept>
 
 #include "modarith.h"
@@ -517,6 +518,141 @@ CkksParams::level_down_multiplier_after_rescale(int source_level,
   const int dst_mid = moduli_id_of_level(destination_level);
   return signed_scalar(std::round(scale * diff), true, q.data() + dst_mid,
                        limbs_at(dst_mid));
+}
+
+// ---------------------------------------------------------------- encoding
+
+namespace {
+
+// The slot ordering of the canonical embedding: the circular shift of the
+// two halves conjugated by the powers-of-5 fold, inverted
+// (src/ckks/encode.cpp:24-150 with the steps of src/ckks/precomputed.cpp:
+// 50-58).  A permutation conjugation maps the k-th element of the cycle
+// listing of the first onto the k-th element of the second's listing; the
+// listing visits cycles from their smallest unvisited start, each from its
+// start's image.
+std::vector<uint64_t> cycle_listing(const std::vector<uint64_t> &perm) {
+  const size_t n = perm.size();
+  std::vector<char> seen(n, 0);
+  std::vector<uint64_t> out;
+  out.reserve(n);
+  size_t start = 0;
+  while (start < n) {
+    uint64_t cur = perm[start];
+    uint64_t next = perm[cur];
+    while (true) {
+      out.push_back(cur);
+      seen[cur] = 1;
+      cur = next;
+      if (seen[next])
+        break;
+      next = perm[cur];
+    }
+    start++;
+    while (start < n && seen[start])
+      start++;
+  }
+  return out;
+}
+
+} // namespace
+
+std::vector<uint32_t> CkksParams::encode_pre_permutation() const {
+  const int size = n; // codec coefficient count for full slots
+  const int half = size / 2;
+  std::vector<uint64_t> shift(size), fold(size);
+  shift[0] = half - 1;
+  for (int i = 1; i < half; i++)
+    shift[i] = i - 1;
+  for (int i = half; i < size - 1; i++)
+    shift[i] = i + 1;
+  shift[size - 1] = half;
+  // fold: (g (2 i + 1) mod 2 size - 1) / 2 with g = 5^(2 size - 1), the
+  // inverse of 5 modulo 2 size (src/ckks/encode.cpp:44-87), as a running sum
+  const int64_t two = 2 * int64_t(size);
+  int64_t jump = 1;
+  for (int64_t e = 0; e < two - 1; e++)
+    jump = (jump * 5) & (two - 1);
+  int64_t acc = jump;
+  for (int i = 0; i < size; i++) {
+    fold[i] = uint64_t((acc - 1) >> 1);
+    acc = (acc + jump * 2) % two;
+  }
+  const auto c1 = cycle_listing(fold), c2 = cycle_listing(shift);
+  std::vector<uint64_t> post(size);
+  for (int i = 0; i < size; i++)
+    post[c1[i]] = c2[i];
+  std::vector<uint32_t> pre(size);
+  for (int i = 0; i < size; i++)
+    pre[post[i]] = uint32_t(i);
+  return pre;
+}
+
+void CkksParams::encode_trig(std::vector<double> &fft_cos,
+                             std::vector<double> &fft_sin,
+                             std::vector<double> &twist_cos,
+                             std::vector<double> &twist_sin) const {
+  const int codec = n;
+  const double pi = std::numbers::pi;
+  fft_cos.resize(codec);
+  fft_sin.resize(codec);
+  twist_cos.resize(codec);
+  twist_sin.resize(codec);
+  const double unit_angle = -2 * pi / codec;
+  const double step = -pi / (codec * 2);
+  for (int i = 0; i < codec; i++) {
+    const double angle = i * unit_angle;
+    fft_cos[i] = std::cos(angle);
+    fft_sin[i] = std::sin(angle);
+    const int real_index = 2 * i;
+    twist_cos[i] = std::cos(real_index * step);
+    twist_sin[i] = std::sin(real_index * step);
+  }
+}
+
+CkksParams::EncodeLevel CkksParams::encode_level(int level) const {
+  EncodeLevel e;
+  int mid = moduli_id_of_level(level);
+  int chain = limbs_at(mid) + alpha;
+  const int log_scale = scale_bits;
+  const long double sc = scale;
+  e.ckks_scale = static_cast<long double>(deviation.at(level)) * sc * sc;
+  if (one_more_level && mid == 0)
+    e.ckks_scale = sc * static_cast<long double>(q[0]);
+  const uint64_t p0 = q[mid];
+  e.top_of_one_more_level =
+      one_more_level && mid == 0 && (e.ckks_scale >= p0 * scale);
+  if (e.top_of_one_more_level) {
+    e.scale_before_tile = double(e.ckks_scale / p0);
+    chain -= 1;
+    mid = moduli_id_of_level(level - 1);
+  } else {
+    int exponent;
+    const double mantissa = double(std::frexp(e.ckks_scale, &exponent));
+    if (exponent >= log_scale) {
+      e.scale_before_tile = mantissa * scale;
+      e.log_scale_after_tile = exponent - log_scale;
+    } else {
+      e.scale_before_tile = double(e.ckks_scale);
+      e.log_scale_after_tile = 0;
+    }
+  }
+  e.r_mult_scale_after_tile.resize(chain);
+  for (int i = 0; i < chain; i++) {
+    const int64_t qt = int64_t(q[mid + i]);
+    if (e.top_of_one_more_level) {
+      e.r_mult_scale_after_tile[i] = int64_t(
+          (__int128)kMontR * (__int128)int64_t(p0) % (__int128)qt);
+    } else {
+      int64_t v = kMontR % qt;
+      for (int64_t ls = e.log_scale_after_tile; ls > 0; ls -= 60) {
+        const int64_t part = int64_t(1) << std::min<int64_t>(60, ls);
+        v = int64_t((__int128)v * part % qt);
+      }
+      e.r_mult_scale_after_tile[i] = v;
+    }
+  }
+  return e;
 }
 
 } // namespace fhe

@@ -146,6 +146,24 @@ public:
   level_down_multiplier_after_rescale(int source_level,
                                       int destination_level) const;
 
+  // Encoding tables (src/ckks/precomputed.cpp:50-75, :138-215,
+  // src/ckks/encode.cpp:24-150): full slots, codec size 2 * slots = N.
+  struct EncodeLevel {
+    long double ckks_scale = 0;
+    double scale_before_tile = 0;
+    int64_t log_scale_after_tile = 0;
+    bool top_of_one_more_level = false; // limb 0 zero, values on limbs 1..
+    std::vector<int64_t> r_mult_scale_after_tile;
+  };
+  int slot_count() const { return n / 2; }
+  std::vector<uint32_t> encode_pre_permutation() const;
+  // cos / sin of index * (-2 pi / codec) (FFT twiddles) and of
+  // (2 index) * (-pi / (2 codec)) (the twister), index < codec
+  void encode_trig(std::vector<double> &fft_cos, std::vector<double> &fft_sin,
+                   std::vector<double> &twist_cos,
+                   std::vector<double> &twist_sin) const;
+  EncodeLevel encode_level(int level) const;
+
 private:
   void derive(bool twiddles);
   void derive_deviations();

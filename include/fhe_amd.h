@@ -281,6 +281,28 @@ int fhe_conjugate(fhe_context *ctx, int count, int moduli_id,
                   const int64_t *in, const int64_t *key, int key_moduli_id,
                   int64_t *out);
 
+/* Encoding of `count` slot vectors into plaintext limbs at `level`
+ * (CKKSEngine::encode_unit, src/ckks/engine_encode.cpp:10-36, through
+ * CKKSCore::encode_polynomial, src/ckks/core_encode.cpp:31-150): values is a
+ * device array [count][slots] of doubles (complex_input = 0) or
+ * [count][slots][2] (real, imaginary; complex_input = 1), slots = N / 2, the
+ * caller's short vectors zero-padded as CKKSCore::pad does; out is
+ * [count][limbs][N] INTT-form Montgomery limbs at
+ * moduli_id = full_level - level.  Double arithmetic matches the reference's
+ * operation for operation, so the limbs are bit-identical. */
+int fhe_encode(fhe_context *ctx, const double *values, int complex_input,
+               int count, int level, int64_t *out);
+/* The per-level encoding constants of CKKSPrecomputed
+ * (src/ckks/precomputed.cpp:138-215: get_ckks_scale, get_scale_before_tile,
+ * get_log_scale_after_tile, get_r_mult_scale_after_tile over the level's
+ * chain including the special primes) and whether the level is the top of a
+ * one-more-level chain (src/ckks/core_encode.cpp:64-78); host only. */
+int fhe_params_encode_constants(const fhe_params *params, int level,
+                                double *ckks_scale, double *scale_before_tile,
+                                int64_t *log_scale_after_tile,
+                                int *top_of_one_more_level, int64_t *r_mult,
+                                int capacity, int *count);
+
 /* Stage timing (no counterpart in the reference, which has no device
  * stages): while enabled, every pipeline stage that a call launches on the
  * context's stream is bracketed by HIP events; fhe_profile_count / _entry

@@ -27,6 +27,7 @@
  */
 #include "ckks_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -812,4 +813,181 @@ void oracle_decrypt(const oracle_ctx *c, int mid, const int64_t *ct,
       out[o] = guard(guard(ct[o] + out[o], q2), q);
     }
   }
+}
+
+/* ---- encoding ------------------------------------------------------------
+ * The double arithmetic below is written one IEEE operation at a time in the
+ * reference's order; this file is built as ISO C for x86-64, where nothing is
+ * contracted into fused multiply-adds, as in the reference's build. */
+
+static const double kPi = 3.14159265358979323846;
+
+/* Lists the cycles of a permutation, each from its smallest unvisited start
+ * (src/ckks/encode.cpp:89-124). */
+static void permutation_cycles(const uint64_t *perm, int size, uint64_t *out) {
+  char *seen = calloc((size_t)size, 1);
+  int w = 0, start = 0;
+  while (start < size) {
+    uint64_t cur = perm[start];
+    uint64_t nxt = perm[cur];
+    for (;;) {
+      out[w++] = cur;
+      seen[cur] = 1;
+      cur = nxt;
+      if (seen[nxt])
+        break;
+      nxt = perm[cur];
+    }
+    start++;
+    while (start < size && seen[start])
+      start++;
+  }
+  free(seen);
+}
+
+/* src/ckks/encode.cpp:24-150: post = conjugate(fold, shift), pre = post^-1,
+ * with shift the circular shift of the two halves and fold the map
+ * i -> (g (2 i + 1) mod 2 codec - 1) / 2, g = 5^(2 codec - 1) mod 2 codec. */
+void oracle_encode_pre_permutation(int codec, uint64_t *pre) {
+  if (codec < 2)
+    return;
+  const int half = codec / 2;
+  const int64_t two = 2 * (int64_t)codec;
+  uint64_t *shift = malloc(sizeof(uint64_t) * codec);
+  uint64_t *fold = malloc(sizeof(uint64_t) * codec);
+  uint64_t *c1 = malloc(sizeof(uint64_t) * codec);
+  uint64_t *c2 = malloc(sizeof(uint64_t) * codec);
+  uint64_t *post = malloc(sizeof(uint64_t) * codec);
+  shift[0] = (uint64_t)(half - 1);
+  for (int i = 1; i < half; i++)
+    shift[i] = (uint64_t)(i - 1);
+  for (int i = half; i < codec - 1; i++)
+    shift[i] = (uint64_t)(i + 1);
+  shift[codec - 1] = (uint64_t)half;
+  int64_t g = 1;
+  for (int64_t e = 0; e < two - 1; e++)
+    g = (g * 5) % two;
+  for (int i = 0; i < codec; i++)
+    fold[i] = (uint64_t)(((g * (2 * (int64_t)i + 1)) % two - 1) / 2);
+  permutation_cycles(fold, codec, c1);
+  permutation_cycles(shift, codec, c2);
+  for (int i = 0; i < codec; i++)
+    post[c1[i]] = c2[i];
+  for (int i = 0; i < codec; i++)
+    pre[post[i]] = (uint64_t)i;
+  free(shift);
+  free(fold);
+  free(c1);
+  free(c2);
+  free(post);
+}
+
+/* positive_modulo (src/arithmetic/arithmetic.cpp:271-273) */
+static int64_t positive_mod(int64_t x, int64_t q) { return (q + x % q) % q; }
+
+/* CKKSCore::encode_polynomial (src/ckks/core_encode.cpp:31-150) with
+ * encode_to_double_ (:11-29): pre_permute (src/ckks/encode.cpp:156-185),
+ * bit_reversal_permute and fft (src/operation/fft.cpp:57-102), normalize
+ * (:50-55), apply_twister (src/ckks/encode.cpp:264-275), then llround +
+ * tile_unsigned_reduce (src/processor/single.cpp:555-566, :315-326) or
+ * double_to_rns above 2^59 (src/arithmetic/arithmetic.cpp:242-269), and
+ * mont_enter with r_mult (src/montgomery/unit.cpp:12-17). */
+void oracle_encode(const oracle_ctx *c, const double *values,
+                   int complex_input, double scale_before_tile, int first,
+                   int enc_mid, int out_limbs, const int64_t *r_mult,
+                   int64_t *out) {
+  const int codec = c->n, slots = c->n / 2;
+  uint64_t *pre = malloc(sizeof(uint64_t) * codec);
+  double *re = malloc(sizeof(double) * codec);
+  double *im = malloc(sizeof(double) * codec);
+  double *v = malloc(sizeof(double) * codec);
+  oracle_encode_pre_permutation(codec, pre);
+  for (int i = 0; i < slots; i++) {
+    const int64_t p = (int64_t)pre[i];
+    const double a = complex_input ? values[2 * i] : values[i];
+    const double b = complex_input ? values[2 * i + 1] : 0.0;
+    re[p] = a;
+    im[p] = b;
+    re[codec - 1 - p] = a;
+    im[codec - 1 - p] = complex_input ? -b : 0.0;
+  }
+  /* bit reversal: index i <-> its log2(codec)-bit reverse */
+  int log_codec = 0;
+  while ((1 << log_codec) < codec)
+    log_codec++;
+  for (int i = 0; i < codec; i++) {
+    int r = 0;
+    for (int b = 0; b < log_codec; b++)
+      if (i & (1 << b))
+        r |= 1 << (log_codec - 1 - b);
+    if (i < r) {
+      double t = re[i];
+      re[i] = re[r];
+      re[r] = t;
+      t = im[i];
+      im[i] = im[r];
+      im[r] = t;
+    }
+  }
+  /* iterative radix-2 decimation in time with w_t = exp(-2 pi i t / codec) */
+  const double unit_angle = -2 * kPi / codec;
+  for (int step = 2, tstep = codec / 2; step <= codec; step <<= 1, tstep >>= 1) {
+    const int h = step >> 1;
+    for (int off = 0; off < codec; off += step)
+      for (int j = 0; j < h; j++) {
+        const double angle = (j * tstep) * unit_angle;
+        const double wr = cos(angle), wi = sin(angle);
+        const int x = off + j, y = off + j + h;
+        const double pr = wr * re[y] - wi * im[y];
+        const double pi_ = wr * im[y] + wi * re[y];
+        const double lr = re[x], li = im[x];
+        re[x] = lr + pr;
+        im[x] = li + pi_;
+        re[y] = lr - pr;
+        im[y] = li - pi_;
+      }
+  }
+  /* normalize, twist, scale */
+  const double twist_step = -kPi / (codec * 2);
+  double max_abs = 0.0;
+  for (int i = 0; i < codec; i++) {
+    const double a = re[i] / codec, b = im[i] / codec;
+    const double ang = (2 * i) * twist_step;
+    v[i] = (a * cos(ang) - b * sin(ang)) * scale_before_tile;
+    if (fabs(v[i]) > max_abs)
+      max_abs = fabs(v[i]);
+  }
+  const int big = max_abs > (double)(UINT64_C(1) << 59);
+  for (int l = 0; l < out_limbs; l++) {
+    int64_t *o = out + (size_t)l * codec;
+    if (l < first) {
+      memset(o, 0, sizeof(int64_t) * codec);
+      continue;
+    }
+    const int e = l - first, p = enc_mid + e;
+    const int64_t q = (int64_t)c->q[p];
+    for (int j = 0; j < codec; j++) {
+      int64_t y;
+      if (big) {
+        if (fabs(v[j]) < (double)(INT64_C(1) << 62)) {
+          y = positive_mod((int64_t)(v[j] + 0.5), q);
+        } else {
+          int exponent;
+          const double m = frexp(v[j] + 0.5, &exponent);
+          y = positive_mod((int64_t)(m * (double)(INT64_C(1) << 52)), q);
+          for (int le = 52; le < exponent; le++) {
+            y <<= 1;
+            y %= q;
+          }
+        }
+      } else {
+        y = unsigned_reduce(llround(v[j]), q);
+      }
+      o[j] = oracle_mont_mult(y, r_mult[e], c->q[p], c->k[p]);
+    }
+  }
+  free(pre);
+  free(re);
+  free(im);
+  free(v);
 }

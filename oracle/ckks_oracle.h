@@ -107,6 +107,19 @@ void oracle_encrypt_pk(const oracle_ctx *ctx, int moduli_id, const int64_t *pt,
 void oracle_decrypt(const oracle_ctx *ctx, int moduli_id, const int64_t *ct,
                     const int64_t *sk, int sk_moduli_id, int64_t *out);
 
+/* CKKS encoding (src/ckks/core_encode.cpp:11-150): the slot vector
+ * (n/2 doubles, or n/2 {re, im} pairs when complex_input) at a level whose
+ * constants are given: scale_before_tile, and r_mult (the level's
+ * r_mult_scale_after_tile, one per encoded limb).  out [out_limbs][n]:
+ * limbs [0, first) zero, limb first + e on prime enc_moduli_id + e. */
+void oracle_encode(const oracle_ctx *ctx, const double *values,
+                   int complex_input, double scale_before_tile, int first,
+                   int enc_moduli_id, int out_limbs, const int64_t *r_mult,
+                   int64_t *out);
+/* the slot -> codec index map of the encoder (src/ckks/encode.cpp:24-150,
+ * src/ckks/precomputed.cpp:50-58), codec = n */
+void oracle_encode_pre_permutation(int codec, uint64_t *pre);
+
 /* one limb of a per-coefficient primitive (codes of FHE_EW_*) */
 void oracle_elementwise_limb(int op, const int64_t *x1, const int64_t *x2,
                              int64_t *y, int n, uint64_t q, uint64_t k,

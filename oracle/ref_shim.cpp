@@ -10,6 +10,7 @@
 // (src/operation/ntt.h, src/ckks/precomputed.h).  Ciphertexts and keys are
 // exchanged as plain int64 arrays in the reference's limb order
 // [polynomial][limb][coefficient].
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -528,6 +529,70 @@ int ref_decrypt_limbs(void *handle, void *ct, void *sk, int64_t *shape,
       copy_out(pt, out);
     delete pt;
     return 0;
+  });
+}
+
+// CKKSEngine::encode_unit (src/ckks/engine_encode.cpp:10-36): values are
+// slot_count doubles, or slot_count {re, im} pairs when complex_input.
+// shape as ref_ct_shape; out [chain][N] when given.
+int ref_encode(void *handle, const double *values, int complex_input,
+               int level, int64_t *shape, int64_t *out) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    const int slots = ref->precomputed->slot_count_;
+    UnitPlaintext *pt = nullptr;
+    if (complex_input) {
+      std::vector<std::complex<double>> v(slots);
+      for (int i = 0; i < slots; i++)
+        v[i] = {values[2 * i], values[2 * i + 1]};
+      pt = ref->engine->encode_unit(v, level);
+    } else {
+      std::vector<double> v(values, values + slots);
+      pt = ref->engine->encode_unit(v, level);
+    }
+    data_shape(pt, shape);
+    if (out)
+      copy_out(pt, out);
+    delete pt;
+    return 0;
+  });
+}
+
+// CKKSPrecomputed's per-level encoding constants
+// (src/ckks/precomputed.cpp:138-215); r_mult gets the level's
+// r_mult_scale_after_tile, *count entries: the chain with the special primes,
+// one less at the top of a one-more-level chain (:161-170).
+int ref_encode_constants(void *handle, int level, double *ckks_scale,
+                         double *scale_before_tile,
+                         int64_t *log_scale_after_tile, int64_t *r_mult,
+                         int *count) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    const auto *pc = ref->precomputed;
+    const auto &params = pc->parameters_;
+    const int mid = params.get_moduli_id(level);
+    const long double cs = pc->get_ckks_scale(level);
+    const auto q = pc->get_q(mid);
+    const bool top = pc->one_more_level_ && mid == 0 && (cs >= q[0] * pc->scale_);
+    const int n = int(params.get_chain_count(mid, true)) - (top ? 1 : 0);
+    const int64_t *r = pc->get_r_mult_scale_after_tile(level);
+    for (int i = 0; i < n; i++)
+      r_mult[i] = r[i];
+    *count = n;
+    *ckks_scale = double(cs);
+    *scale_before_tile = pc->get_scale_before_tile(level);
+    *log_scale_after_tile = pc->get_log_scale_after_tile(level);
+    return top ? 1 : 0;
+  });
+}
+
+// CKKSPrecomputed::pre_permutation_ (src/ckks/precomputed.cpp:50-58)
+int ref_pre_permutation(void *handle, uint64_t *out) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    const auto &pre = ref->precomputed->pre_permutation_;
+    std::memcpy(out, pre.data(), pre.size() * sizeof(uint64_t));
+    return int(pre.size());
   });
 }
 

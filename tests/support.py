@@ -90,6 +90,9 @@ class Oracle:
             lib.oracle_rotate.argtypes = [VP, i32, VP, VP, i32, i32, VP]
             lib.oracle_conjugate.argtypes = [VP, i32, VP, VP, i32, VP]
             lib.oracle_switch_key.argtypes = [VP, i32, VP, VP, i32, VP]
+            lib.oracle_encode.argtypes = [VP, VP, i32, ctypes.c_double, i32,
+                                          i32, i32, VP, VP]
+            lib.oracle_encode_pre_permutation.argtypes = [i32, VP]
             cls._lib = lib
         return cls._lib
 
@@ -237,6 +240,23 @@ class Oracle:
                                   ptr(out))
         return out
 
+    def encode(self, values, scale_before_tile, first, enc_mid, out_limbs,
+               r_mult):
+        """values: n/2 doubles or complex numbers."""
+        values = np.asarray(values)
+        if np.iscomplexobj(values):
+            flat = np.stack([values.real, values.imag], -1).astype(np.float64)
+        else:
+            flat = values.astype(np.float64)
+        flat = np.ascontiguousarray(flat)
+        r = np.ascontiguousarray(r_mult, dtype=np.int64)
+        out = np.zeros((out_limbs, self.n), np.int64)
+        self.lib().oracle_encode(self.h, ptr(flat),
+                                 1 if np.iscomplexobj(values) else 0,
+                                 float(scale_before_tile), first, enc_mid,
+                                 out_limbs, ptr(r), ptr(out))
+        return out
+
     def partitions(self, mid, key_mid):
         s = np.zeros(128, np.int32)
         z = np.zeros(128, np.int32)
@@ -331,6 +351,9 @@ class Reference:
             "ref_add_ct": (VP, [VP, VP, VP]),
             "ref_subtract_ct": (VP, [VP, VP, VP]),
             "ref_decrypt_limbs": (i32, [VP, VP, VP, VP, VP]),
+            "ref_encode": (i32, [VP, VP, i32, i32, VP, VP]),
+            "ref_encode_constants": (i32, [VP, i32, VP, VP, VP, VP, VP]),
+            "ref_pre_permutation": (i32, [VP, VP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -390,6 +413,43 @@ class Reference:
         out = np.zeros((int(s[2]), int(s[3])), np.int64)
         self.lib.ref_decrypt_limbs(self.e, ct, sk, ptr(s), ptr(out))
         return int(s[0]), out
+
+    def encode(self, values, level):
+        """CKKSEngine::encode_unit -> (moduli_id, limbs [chain][N])."""
+        values = np.asarray(values)
+        is_complex = np.iscomplexobj(values)
+        if is_complex:
+            flat = np.stack([values.real, values.imag], -1).astype(np.float64)
+        else:
+            flat = values.astype(np.float64)
+        flat = np.ascontiguousarray(flat)
+        s = np.zeros(4, np.int64)
+        rc = self.lib.ref_encode(self.e, ptr(flat), int(is_complex), level,
+                                 ptr(s), None)
+        if rc != 0 or s[3] == 0:
+            raise RuntimeError(self.lib.ref_last_error().decode())
+        out = np.zeros((int(s[2]), int(s[3])), np.int64)
+        self.lib.ref_encode(self.e, ptr(flat), int(is_complex), level, ptr(s),
+                            ptr(out))
+        return int(s[0]), out
+
+    def encode_constants(self, level):
+        ckks, sbt = np.zeros(1), np.zeros(1)
+        log_after = np.zeros(1, np.int64)
+        r = np.zeros(512, np.int64)
+        count = np.zeros(1, np.int32)
+        top = self.lib.ref_encode_constants(self.e, level, ptr(ckks), ptr(sbt),
+                                            ptr(log_after), ptr(r),
+                                            ptr(count))
+        return dict(ckks_scale=float(ckks[0]), scale_before_tile=float(sbt[0]),
+                    log_scale_after_tile=int(log_after[0]),
+                    top_of_one_more_level=bool(top),
+                    r_mult_scale_after_tile=r[:int(count[0])])
+
+    def pre_permutation(self):
+        out = np.zeros(self.n, np.uint64)
+        self.lib.ref_pre_permutation(self.e, ptr(out))
+        return out
 
     def key(self, rlk):
         s = np.zeros(4, np.int64)
