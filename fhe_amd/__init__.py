@@ -31,7 +31,8 @@ EXPORTED_SYMBOLS = (
     "fhe_multiplier", "fhe_params_multiplier", "fhe_multiply_double",
     "fhe_multiply_int", "fhe_add_double", "fhe_level_down",
     "fhe_encrypt_pk", "fhe_decrypt", "fhe_rotate_batch", "fhe_switch_key",
-    "fhe_encode", "fhe_params_encode_constants",
+    "fhe_encode", "fhe_params_encode_constants", "fhe_decode",
+    "fhe_decoded_scale",
 )
 
 
@@ -125,8 +126,12 @@ def library():
             ctypes.POINTER(Params), i32, ctypes.c_double, ctypes.c_int64, i32,
             i32, vp, i32, ctypes.POINTER(i32)]
         lib.fhe_encode.argtypes = [vp, vp, i32, i32, i32, vp]
+        lib.fhe_decode.argtypes = [vp, vp, i32, i32, ctypes.c_longdouble,
+                                   i32, vp]
+        lib.fhe_decoded_scale.argtypes = [vp, i32, ctypes.c_longdouble,
+                                          ctypes.POINTER(i32), vp]
         lib.fhe_params_encode_constants.argtypes = [
-            ctypes.POINTER(Params), i32, ctypes.POINTER(ctypes.c_double),
+            ctypes.POINTER(Params), i32, vp,
             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
             ctypes.POINTER(i32), vp, i32, ctypes.POINTER(i32)]
         lib.fhe_profile_count.argtypes = [vp, ctypes.POINTER(i32)]
@@ -135,6 +140,13 @@ def library():
             ctypes.POINTER(ctypes.c_int64)]
         _lib = lib
     return _lib
+
+
+def _longdouble(x):
+    """An exact ctypes long double from a numpy long double (ctypes' own
+    constructor goes through a Python float)."""
+    buf = np.array([x], dtype=np.longdouble)
+    return ctypes.c_longdouble.from_buffer(buf)
 
 
 def _check(code):
@@ -190,13 +202,14 @@ def params_encode_constants(log_n, special_prime_count, level, scale_bits=40,
                     one_more_level, -1, primes)
     out = np.zeros(512, np.int64)
     count = ctypes.c_int(0)
-    ckks_scale, sbt = ctypes.c_double(), ctypes.c_double()
+    ckks_scale, sbt = np.zeros(1, np.longdouble), ctypes.c_double()
     log_after, top = ctypes.c_int64(), ctypes.c_int()
     _check(library().fhe_params_encode_constants(
-        ctypes.byref(p), level, ctypes.byref(ckks_scale), ctypes.byref(sbt),
+        ctypes.byref(p), level, ckks_scale.ctypes.data, ctypes.byref(sbt),
         ctypes.byref(log_after), ctypes.byref(top), out.ctypes.data, 512,
         ctypes.byref(count)))
-    return dict(ckks_scale=ckks_scale.value, scale_before_tile=sbt.value,
+    return dict(ckks_scale=ckks_scale[0],
+                scale_before_tile=sbt.value,
                 log_scale_after_tile=log_after.value,
                 top_of_one_more_level=bool(top.value),
                 r_mult_scale_after_tile=out[:count.value])
@@ -461,6 +474,40 @@ class Context:
         _check(library().fhe_encode(self.handle, values.ptr,
                                     1 if complex_input else 0, count, level,
                                     out.ptr))
+
+    def decode(self, pt, moduli_id, ckks_scale, complex_output=False):
+        """Decode device plaintexts [count][limbs][N] at moduli_id with their
+        (long double) ckks_scale; returns host values [count][slots]."""
+        count = int(np.prod(pt.shape[:-2]))
+        slots = self.n // 2
+        width = 2 if complex_output else 1
+        out = self.array((count, slots * width))
+        try:
+            self.decode_device(pt, count, moduli_id, ckks_scale,
+                               complex_output, out)
+            host = out.download().view(np.float64)
+        finally:
+            out.free()
+        if complex_output:
+            host = host.reshape(count, slots, 2)
+            return host[..., 0] + 1j * host[..., 1]
+        return host.reshape(count, slots)
+
+    def decode_device(self, pt, count, moduli_id, ckks_scale, complex_output,
+                      out):
+        """fhe_decode into a device array of doubles (viewed as int64)."""
+        _check(library().fhe_decode(self.handle, pt.ptr, count, moduli_id,
+                                    _longdouble(ckks_scale),
+                                    1 if complex_output else 0, out.ptr))
+
+    def decoded_scale(self, moduli_id, ckks_scale):
+        mid = ctypes.c_int()
+        scale = np.zeros(1, np.longdouble)
+        _check(library().fhe_decoded_scale(self.handle, moduli_id,
+                                           _longdouble(ckks_scale),
+                                           ctypes.byref(mid),
+                                           scale.ctypes.data))
+        return mid.value, scale[0]
 
     def profile(self, on):
         """Start (clearing totals) or stop stage timing."""

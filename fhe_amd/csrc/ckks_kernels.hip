@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
   }
   // rescale (src/ckks/core_rescale.cpp:5-25): drop limb 0 with rounding
   const int64_t y0 = relin(0);
-  const int64_t rounder = y0 > int64_t(t.qh[moduli_id]) ? 1 : 0;
+  const int64_t rounder = uint64_t(y0) > t.qh[moduli_id] ? 1 : 0;
   int64_t *o = out + ch * (limbs - 1) * n + coef;
 #pragma unroll 4
   for (int i = 1; i < limbs; i++) {
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void rescale_kernel(
   const int64_t *x = in + poly * limbs * n + coef;
   int64_t *o = out + poly * (limbs - 1) * n + coef;
   const int64_t x0 = x[0];
-  const int64_t rounder = x0 > int64_t(t.qh[moduli_id]) ? 1 : 0;
+  const int64_t rounder = uint64_t(x0) > t.qh[moduli_id] ? 1 : 0;
   for (int i = 1; i < limbs; i++) {
     const int p = moduli_id + i;
     const uint64_t q = t.q[p], k = t.k[p];
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void scalar_rescale_kernel(
                  int64_t(t.q[p]));
   };
   const int64_t y0 = entered(0);
-  const int64_t rounder = y0 > int64_t(t.qh[a.moduli_id]) ? 1 : 0;
+  const int64_t rounder = uint64_t(y0) > t.qh[a.moduli_id] ? 1 : 0;
   for (int i = 1 + a.skip; i < a.limbs; i++) {
     const int p = a.moduli_id + i;
     const uint64_t q = t.q[p], k = t.k[p];

@@ -110,6 +110,10 @@ struct fhe_context {
   // encoding tables, uploaded on the first encode
   fhe::EncodeDevice encode{};
   bool encode_ready = false;
+  // decoding tables, uploaded on the first decode; CRT constants per moduli_id
+  fhe::DecodeDevice decode{};
+  bool decode_ready = false;
+  std::map<int, fhe::DecodeCrtDevice> decode_crt_cache;
   // stage timing: HIP events recorded on `stream` around each stage while
   // enabled, folded into per-stage totals when read
   bool timing = false;
@@ -250,6 +254,38 @@ struct fhe_context {
       encode_ready = true;
     }
     return encode;
+  }
+
+  const fhe::DecodeDevice &decode_tables() {
+    if (!decode_ready) {
+      std::vector<double> sc, ss, ic, is;
+      params.decode_trig(sc, ss, ic, is);
+      decode.post = upload(params.decode_post_permutation(), owned);
+      decode.skew_cos = upload(sc, owned);
+      decode.skew_sin = upload(ss, owned);
+      decode.ifft_cos = upload(ic, owned);
+      decode.ifft_sin = upload(is, owned);
+      decode.log_codec = params.log_n;
+      decode_ready = true;
+    }
+    return decode;
+  }
+
+  const fhe::DecodeCrtDevice &decode_crt(int moduli_id) {
+    auto it = decode_crt_cache.find(moduli_id);
+    if (it != decode_crt_cache.end())
+      return it->second;
+    const auto h = params.decode_crt(moduli_id);
+    require(fhe::decode_crt_supported(h.words),
+            "Decode: the modulus product is too wide");
+    fhe::DecodeCrtDevice d{};
+    d.reverser = upload(h.reverser, owned);
+    d.q_product = upload(h.q_product, owned);
+    d.q_half = upload(h.q_half, owned);
+    d.upper = upload(h.upper, owned);
+    d.k = h.k;
+    d.words = h.words;
+    return decode_crt_cache[moduli_id] = d;
   }
 
   const int64_t *modup_constants(int moduli_id, int key_moduli_id) {
@@ -799,7 +835,8 @@ int fhe_params_multiplier(const fhe_params *p, int kind, double value,
 }
 
 int fhe_params_encode_constants(const fhe_params *p, int level,
-                                double *ckks_scale, double *scale_before_tile,
+                                long double *ckks_scale,
+                                double *scale_before_tile,
                                 int64_t *log_scale_after_tile,
                                 int *top_of_one_more_level, int64_t *r_mult,
                                 int capacity, int *count) {
@@ -813,7 +850,7 @@ int fhe_params_encode_constants(const fhe_params *p, int level,
     const auto e = P.encode_level(level);
     const auto &v = e.r_mult_scale_after_tile;
     require(capacity >= int(v.size()), "encode constants: capacity too small");
-    *ckks_scale = double(e.ckks_scale);
+    *ckks_scale = e.ckks_scale;
     *scale_before_tile = e.scale_before_tile;
     *log_scale_after_tile = e.log_scale_after_tile;
     *top_of_one_more_level = e.top_of_one_more_level ? 1 : 0;
@@ -849,6 +886,81 @@ int fhe_encode(fhe_context *ctx, const double *values, int complex_input,
                          ctx->stream);
     });
     ctx->check_launch("Encode");
+  });
+}
+
+int fhe_decode(fhe_context *ctx, const int64_t *pt, int count, int moduli_id,
+               long double ckks_scale, int complex_output, double *out) {
+  return guarded([&] {
+    require(ctx && pt && out, "null argument");
+    require(count > 0, "Decode: empty batch");
+    const auto &P = ctx->params;
+    require(moduli_id >= 0 && moduli_id < P.chain_count,
+            "Decode: moduli_id out of range.");
+    // CKKSCore::convert_plaintext_to_double (src/ckks/core_decode.cpp:10-107)
+    require(P.limbs_at(moduli_id) > 1,
+            "A plaintext needs at least two chains to be decoded.");
+    long double modulus = 1;
+    for (int i = 0; i < P.limbs_at(moduli_id); i++)
+      modulus *= static_cast<long double>(P.q[moduli_id + i]);
+    require(!(ckks_scale > modulus),
+            "Overflow due to large ckks_scale and small unit plaintext "
+            "modulus.");
+    // rescale while the rescaled scale stays above the base scale
+    int mid = moduli_id;
+    long double scale = ckks_scale;
+    const int64_t N = ctx->n();
+    const int64_t *src = pt;
+    int64_t *bufs = nullptr;
+    int which = 0;
+    while (mid <= P.full_level) {
+      const long double rescaled = scale / P.q[mid];
+      if (rescaled <= P.scale)
+        break;
+      if (!bufs)
+        bufs = ctx->scratch(size_t(2) * count * P.limbs_at(moduli_id) * N * 8);
+      int64_t *dst = bufs + int64_t(which) * count * P.limbs_at(moduli_id) * N;
+      ctx->stage("decode(rescale)", [&] {
+        fhe::launch_rescale(ctx->tables, src, dst, count, P.limbs_at(mid), mid,
+                            ctx->stream);
+      });
+      src = dst;
+      which ^= 1;
+      scale = rescaled;
+      mid++;
+    }
+    const double scale_inverse = static_cast<double>(1.0 / scale);
+    const auto &d = ctx->decode_tables();
+    const auto &crt = ctx->decode_crt(mid);
+    void *work = ctx->aux_scratch(fhe::decode_work_bytes(d, count));
+    ctx->stage("decode", [&] {
+      fhe::launch_decode(ctx->tables, d, crt, src, count, P.limbs_at(mid), mid,
+                         scale_inverse, complex_output ? 1 : 0, out, work,
+                         ctx->stream);
+    });
+    ctx->check_launch("Decode");
+  });
+}
+
+int fhe_decoded_scale(const fhe_context *ctx, int moduli_id,
+                      long double ckks_scale, int *decoded_moduli_id,
+                      long double *decoded_scale) {
+  return guarded([&] {
+    require(ctx && decoded_moduli_id && decoded_scale, "null argument");
+    const auto &P = ctx->params;
+    require(moduli_id >= 0 && moduli_id < P.chain_count,
+            "Decode: moduli_id out of range.");
+    int mid = moduli_id;
+    long double scale = ckks_scale;
+    while (mid <= P.full_level) {
+      const long double rescaled = scale / P.q[mid];
+      if (rescaled <= P.scale)
+        break;
+      scale = rescaled;
+      mid++;
+    }
+    *decoded_moduli_id = mid;
+    *decoded_scale = scale;
   });
 }
 

@@ -1,4 +1,6 @@
-// CKKS encoding on the device (CKKSCore::encode_polynomial,
+// CKKS encoding and decoding on the device.
+//
+// Encoding (CKKSCore::encode_polynomial,
 // src/ckks/core_encode.cpp:11-150): slot permutation, bit reversal, the
 // complex FFT over 2 * slots points with the host-built twiddles
 // (src/operation/fft.cpp:57-102), normalization and twisting
@@ -6,6 +8,14 @@
 // reference's double_to_rns when the largest coefficient exceeds 2^59,
 // src/arithmetic/arithmetic.cpp:242-269), the lift to every limb and the
 // Montgomery entry with the level's scale constants.
+//
+// Decoding (CKKSCore::convert_plaintext_to_double and decode_,
+// src/ckks/core_decode.cpp:10-124): make_unsigned_reduce, the CRT
+// reconstruction of rns_to_double (src/arithmetic/arithmetic.cpp:169-197) in
+// multiword integers with the reference's BigInt steps
+// (src/arithmetic/big_int.cpp:176-225: Montgomery reduction by 2^62, the
+// centered lift, and get_double's low-to-high sum), the skewer, the inverse
+// FFT and the slot gather with the scale inverse.
 //
 // Bit-exactness with the reference's double arithmetic: every product and
 // sum below is a separate IEEE operation in the reference's order (this file
@@ -205,6 +215,250 @@ void launch_encode(const DeviceTables &t, const EncodeDevice &e,
 size_t encode_work_bytes(const EncodeDevice &e, int count) {
   const int64_t codec = int64_t(1) << e.log_codec;
   return size_t(count) * codec * (16 + 8) + size_t(count) * 8 + 256;
+}
+
+namespace {
+
+// One coefficient of every plaintext: the signed integer the limbs encode
+// (CRT, centered), as a double.
+template <int MAXW>
+__global__ __launch_bounds__(kBlock) void decode_crt_kernel(
+    const int64_t *in, double *v, int count, int limbs, int mid,
+    DecodeCrtDevice crt, DeviceTables t) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= int64_t(count) * n)
+    return;
+  const int64_t c = idx >> t.log_n;
+  const int64_t coef = idx & (n - 1);
+  const int64_t *x = in + c * limbs * n + coef;
+  if (limbs == 1) {
+    // make_signed (src/montgomery/unit.cpp:36-42; unsigned compare) and the
+    // int64 -> double cast
+    const int64_t q = int64_t(t.q[mid]);
+    const int64_t xv = x[0];
+    const int64_t y = uint64_t(xv) <= t.qh[mid] ? xv : xv - q;
+    v[idx] = __ll2double_rn(y);
+    return;
+  }
+  const int words = crt.words;
+  uint64_t acc[MAXW + 2];
+#pragma unroll
+  for (int w = 0; w < MAXW + 2; w++)
+    acc[w] = 0;
+  for (int l = 0; l < limbs; l++) {
+    const int p = mid + l;
+    const int64_t q = int64_t(t.q[p]);
+    int64_t xl = x[int64_t(l) * n];
+    xl = xl < 0 ? xl + q : guard(xl, q);
+    // x 2^62 mod q, canonical: a Montgomery product with R^2 mod q
+    const uint64_t xr =
+        uint64_t(guard(mont_mult(xl, t.r2[p], t.q[p], t.k[p]), q));
+    const uint64_t *rv = crt.reverser + int64_t(l) * words;
+    uint64_t carry = 0;
+#pragma unroll
+    for (int w = 0; w < MAXW + 2; w++) {
+      if (w < words) {
+        const uint64_t r = rv[w];
+        const uint64_t lo = r * xr;
+        uint64_t hi = mulhi64(r, xr);
+        uint64_t s = lo + acc[w];
+        hi += s < lo;
+        const uint64_t s2 = s + carry;
+        hi += s2 < s;
+        acc[w] = s2;
+        carry = hi;
+      } else {
+        const uint64_t s = acc[w] + carry;
+        carry = s < carry;
+        acc[w] = s;
+      }
+    }
+  }
+  const uint64_t orig0 = acc[0];
+  // subtract U = Q 2^62 while acc >= U (acc < limbs U)
+  for (int it = 0; it <= limbs; it++) {
+    bool geq = true, decided = false;
+#pragma unroll
+    for (int w = MAXW + 1; w >= 0; w--) {
+      const uint64_t u = w <= words ? crt.upper[w] : 0;
+      if (!decided && acc[w] != u) {
+        geq = acc[w] > u;
+        decided = true;
+      }
+    }
+    if (!geq)
+      break;
+    uint64_t borrow = 0;
+#pragma unroll
+    for (int w = 0; w < MAXW + 2; w++) {
+      const uint64_t u = w <= words ? crt.upper[w] : 0;
+      const uint64_t d = acc[w] - u;
+      const uint64_t d2 = d - borrow;
+      borrow = (acc[w] < u) | (d < borrow);
+      acc[w] = d2;
+    }
+  }
+  // + Q m with m = (acc_0 k) mod 2^62, then the exact shift by 62
+  const uint64_t m = (orig0 * crt.k) & ((uint64_t(1) << 62) - 1);
+  {
+    uint64_t carry = 0;
+#pragma unroll
+    for (int w = 0; w < MAXW + 2; w++) {
+      if (w < words) {
+        const uint64_t qw = crt.q_product[w];
+        const uint64_t lo = qw * m;
+        uint64_t hi = mulhi64(qw, m);
+        const uint64_t s = lo + acc[w];
+        hi += s < lo;
+        const uint64_t s2 = s + carry;
+        hi += s2 < s;
+        acc[w] = s2;
+        carry = hi;
+      } else {
+        const uint64_t s = acc[w] + carry;
+        carry = s < carry;
+        acc[w] = s;
+      }
+    }
+  }
+  uint64_t quo[MAXW + 1];
+#pragma unroll
+  for (int w = 0; w < MAXW + 1; w++)
+    quo[w] = (acc[w] >> 62) | (acc[w + 1] << 2);
+  // quo < 2Q: one conditional subtraction of Q, then the centered lift
+  auto compare = [&](const uint64_t *b) {
+    // quo >= b (b has `words` words)
+    bool geq = true, decided = false;
+#pragma unroll
+    for (int w = MAXW; w >= 0; w--) {
+      const uint64_t bw = w < words ? b[w] : 0;
+      if (!decided && quo[w] != bw) {
+        geq = quo[w] > bw;
+        decided = true;
+      }
+    }
+    return geq;
+  };
+  if (compare(crt.q_product)) {
+    uint64_t borrow = 0;
+#pragma unroll
+    for (int w = 0; w < MAXW + 1; w++) {
+      const uint64_t bw = w < words ? crt.q_product[w] : 0;
+      const uint64_t d = quo[w] - bw;
+      const uint64_t d2 = d - borrow;
+      borrow = (quo[w] < bw) | (d < borrow);
+      quo[w] = d2;
+    }
+  }
+  const bool negative = compare(crt.q_half);
+  if (negative) {
+    // Q - quo
+    uint64_t borrow = 0;
+#pragma unroll
+    for (int w = 0; w < MAXW + 1; w++) {
+      const uint64_t bw = w < words ? crt.q_product[w] : 0;
+      const uint64_t d = bw - quo[w];
+      const uint64_t d2 = d - borrow;
+      borrow = (bw < quo[w]) | (d < borrow);
+      quo[w] = d2;
+    }
+  }
+  // get_double over the words up to the highest nonzero one:
+  // y += double(word_i) * 2^(64 i), low to high
+  int size = 1;
+#pragma unroll
+  for (int w = 1; w < MAXW + 1; w++)
+    if (quo[w] != 0)
+      size = w + 1;
+  double y = 0.0;
+#pragma unroll
+  for (int w = 0; w < MAXW + 1; w++) {
+    if (w < size) {
+      const double pw = 64 * w < 1024 ? ldexp(1.0, 64 * w) : __longlong_as_double(
+                                                               0x7ff0000000000000ll);
+      y = __dadd_rn(y, __dmul_rn(__ull2double_rn(quo[w]), pw));
+    }
+  }
+  v[idx] = negative ? -y : y;
+}
+
+// skewer (src/ckks/encode.cpp:289-297) into bit-reversed complex order
+__global__ __launch_bounds__(kBlock) void decode_skew_kernel(
+    const double *v, Cplx *y, int count, int log_codec, const double *cos_t,
+    const double *sin_t) {
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= (int64_t(count) << log_codec))
+    return;
+  const int64_t c = idx >> log_codec;
+  const int i = int(idx & ((int64_t(1) << log_codec) - 1));
+  const double x = v[idx];
+  y[(c << log_codec) + bit_reverse(uint32_t(i), log_codec)] = {
+      __dmul_rn(x, cos_t[i]), __dmul_rn(x, sin_t[i])};
+}
+
+// permute_after_ifft (src/ckks/encode.cpp:481-503)
+__global__ __launch_bounds__(kBlock) void decode_gather_kernel(
+    const Cplx *y, double *out, int count, int log_codec, const uint32_t *post,
+    double scale_inverse, int complex_output) {
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= (int64_t(count) << log_codec))
+    return;
+  const int64_t c = idx >> log_codec;
+  const int i = int(idx & ((int64_t(1) << log_codec) - 1));
+  const int slots = 1 << (log_codec - 1);
+  const uint32_t p = post[i];
+  if (p >= uint32_t(slots))
+    return;
+  const int64_t o = c * slots + p;
+  if (complex_output) {
+    out[2 * o] = __dmul_rn(y[idx].re, scale_inverse);
+    out[2 * o + 1] = __dmul_rn(y[idx].im, scale_inverse);
+  } else {
+    out[o] = __dmul_rn(y[idx].re, scale_inverse);
+  }
+}
+
+} // namespace
+
+size_t decode_work_bytes(const DecodeDevice &d, int count) {
+  const int64_t codec = int64_t(1) << d.log_codec;
+  return size_t(count) * codec * (16 + 8) + 256;
+}
+
+bool decode_crt_supported(int words) { return words <= kMaxDecodeWords; }
+
+void launch_decode(const DeviceTables &t, const DecodeDevice &d,
+                   const DecodeCrtDevice &crt, const int64_t *in, int count,
+                   int limbs, int moduli_id, double scale_inverse,
+                   int complex_output, double *out, void *work,
+                   hipStream_t stream) {
+  const int codec = 1 << d.log_codec;
+  double *v = static_cast<double *>(work);
+  Cplx *y = reinterpret_cast<Cplx *>(v + int64_t(count) * codec);
+  const dim3 grid(grid_for(int64_t(count) * codec));
+  if (crt.words <= 8)
+    hipLaunchKernelGGL(decode_crt_kernel<8>, grid, dim3(kBlock), 0, stream, in,
+                       v, count, limbs, moduli_id, crt, t);
+  else if (crt.words <= 16)
+    hipLaunchKernelGGL(decode_crt_kernel<16>, grid, dim3(kBlock), 0, stream,
+                       in, v, count, limbs, moduli_id, crt, t);
+  else if (crt.words <= 24)
+    hipLaunchKernelGGL(decode_crt_kernel<24>, grid, dim3(kBlock), 0, stream,
+                       in, v, count, limbs, moduli_id, crt, t);
+  else
+    hipLaunchKernelGGL(decode_crt_kernel<kMaxDecodeWords>, grid, dim3(kBlock),
+                       0, stream, in, v, count, limbs, moduli_id, crt, t);
+  hipLaunchKernelGGL(decode_skew_kernel, grid, dim3(kBlock), 0, stream, v, y,
+                     count, d.log_codec, d.skew_cos, d.skew_sin);
+  for (int s = 0; s < d.log_codec; s++)
+    hipLaunchKernelGGL(encode_fft_stage_kernel,
+                       dim3(grid_for(int64_t(count) * (codec / 2))),
+                       dim3(kBlock), 0, stream, y, count, d.log_codec, s,
+                       d.ifft_cos, d.ifft_sin);
+  hipLaunchKernelGGL(decode_gather_kernel, grid, dim3(kBlock), 0, stream, y,
+                     out, count, d.log_codec, d.post, scale_inverse,
+                     complex_output);
 }
 
 } // namespace fhe

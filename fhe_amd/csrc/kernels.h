@@ -249,6 +249,31 @@ void launch_encode(const DeviceTables &t, const EncodeDevice &e,
                    int moduli_id, const int64_t *r_mult, int64_t *out,
                    void *work, hipStream_t stream);
 
+// Decoding tables on the device (host-built, see CkksParams::decode_*).
+struct DecodeDevice {
+  const uint32_t *post;     // codec index -> slot
+  const double *skew_cos, *skew_sin;
+  const double *ifft_cos, *ifft_sin;
+  int log_codec;
+};
+// CRT constants of one moduli_id (CkksParams::DecodeCrt on the device).
+struct DecodeCrtDevice {
+  const uint64_t *reverser; // [limbs][words]
+  const uint64_t *q_product, *q_half, *upper;
+  uint64_t k;
+  int words;
+};
+constexpr int kMaxDecodeWords = 32;
+bool decode_crt_supported(int words);
+size_t decode_work_bytes(const DecodeDevice &d, int count);
+// Decode `count` INTT-form plaintexts in [count][limbs][N] at moduli_id into
+// out: [count][slots] real parts, or [count][slots][2] when complex_output.
+void launch_decode(const DeviceTables &t, const DecodeDevice &d,
+                   const DecodeCrtDevice &crt, const int64_t *in, int count,
+                   int limbs, int moduli_id, double scale_inverse,
+                   int complex_output, double *out, void *work,
+                   hipStream_t stream);
+
 // Rescale of `polys` INTT-form polynomials (src/ckks/core_rescale.cpp:5-25).
 void launch_rescale(const DeviceTables &t, const int64_t *in, int64_t *out,
                     int polys, int limbs, int moduli_id, hipStream_t stream);

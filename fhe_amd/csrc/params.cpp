@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <math.h>
 #include <numbers>
 #include <stdexcept>
 
@@ -588,6 +589,18 @@ std::vector<uint32_t> CkksParams::encode_pre_permutation() const {
   return pre;
 }
 
+namespace {
+
+// libm's sin and sincos, called through volatile pointers so that the
+// compiler neither folds nor pairs them: the reference's -O2 build computes
+// every same-argument (cos x, sin x) pair with glibc's sincos, whose results
+// differ from separate sin and cos calls in the last bit for some x, and
+// computes sin(-x) next to such a pair with a separate sin call.
+double (*volatile libm_sin)(double) = ::sin;
+void (*volatile libm_sincos)(double, double *, double *) = ::sincos;
+
+} // namespace
+
 void CkksParams::encode_trig(std::vector<double> &fft_cos,
                              std::vector<double> &fft_sin,
                              std::vector<double> &twist_cos,
@@ -602,11 +615,9 @@ void CkksParams::encode_trig(std::vector<double> &fft_cos,
   const double step = -pi / (codec * 2);
   for (int i = 0; i < codec; i++) {
     const double angle = i * unit_angle;
-    fft_cos[i] = std::cos(angle);
-    fft_sin[i] = std::sin(angle);
+    libm_sincos(angle, &fft_sin[i], &fft_cos[i]);
     const int real_index = 2 * i;
-    twist_cos[i] = std::cos(real_index * step);
-    twist_sin[i] = std::sin(real_index * step);
+    libm_sincos(real_index * step, &twist_sin[i], &twist_cos[i]);
   }
 }
 
@@ -653,6 +664,110 @@ CkksParams::EncodeLevel CkksParams::encode_level(int level) const {
     }
   }
   return e;
+}
+
+namespace {
+
+// Little-endian multiword helpers for the decoding constants.
+using Words = std::vector<uint64_t>;
+
+void trim(Words &a) {
+  while (a.size() > 1 && a.back() == 0)
+    a.pop_back();
+}
+
+Words mul_word(const Words &a, uint64_t b) {
+  Words out(a.size() + 1, 0);
+  u128 carry = 0;
+  for (size_t i = 0; i < a.size(); i++) {
+    const u128 p = u128(a[i]) * b + carry;
+    out[i] = uint64_t(p);
+    carry = p >> 64;
+  }
+  out[a.size()] = uint64_t(carry);
+  trim(out);
+  return out;
+}
+
+// exact quotient a / b and the remainder
+Words div_word(const Words &a, uint64_t b, uint64_t *rem) {
+  Words out(a.size(), 0);
+  u128 r = 0;
+  for (size_t i = a.size(); i-- > 0;) {
+    const u128 cur = (r << 64) | a[i];
+    out[i] = uint64_t(cur / b);
+    r = cur % b;
+  }
+  if (rem)
+    *rem = uint64_t(r);
+  trim(out);
+  return out;
+}
+
+} // namespace
+
+CkksParams::DecodeCrt CkksParams::decode_crt(int moduli_id) const {
+  DecodeCrt d;
+  d.limbs = limbs_at(moduli_id);
+  Words prod{1};
+  for (int i = 0; i < d.limbs; i++)
+    prod = mul_word(prod, q[moduli_id + i]);
+  d.words = int(prod.size());
+  auto pad = [&](Words w, int n) {
+    w.resize(size_t(n), 0);
+    return w;
+  };
+  d.q_product = pad(prod, d.words);
+  uint64_t rem = 0;
+  d.q_half = pad(div_word(prod, 2, &rem), d.words);
+  d.upper = pad(mul_word(prod, uint64_t(1) << 62), d.words + 1);
+  d.reverser.assign(size_t(d.limbs) * d.words, 0);
+  for (int i = 0; i < d.limbs; i++) {
+    const uint64_t qi = q[moduli_id + i];
+    const Words cofactor = div_word(prod, qi, nullptr);
+    uint64_t cof_mod = 0;
+    div_word(cofactor, qi, &cof_mod);
+    const int64_t inv = mod_inverse(int64_t(cof_mod), int64_t(qi));
+    const Words r = pad(mul_word(cofactor, uint64_t(inv)), d.words);
+    std::copy(r.begin(), r.end(), d.reverser.begin() + size_t(i) * d.words);
+  }
+  // -Q^-1 mod 2^62 from Q mod 2^64 (Newton iteration for the inverse)
+  const uint64_t q0 = prod[0];
+  uint64_t inv = q0; // correct to 3 bits for odd q0
+  for (int it = 0; it < 6; it++)
+    inv *= 2 - q0 * inv;
+  d.k = (uint64_t(0) - inv) & ((uint64_t(1) << 62) - 1);
+  return d;
+}
+
+std::vector<uint32_t> CkksParams::decode_post_permutation() const {
+  const auto pre = encode_pre_permutation();
+  std::vector<uint32_t> post(pre.size());
+  for (size_t i = 0; i < pre.size(); i++)
+    post[pre[i]] = uint32_t(i);
+  return post;
+}
+
+void CkksParams::decode_trig(std::vector<double> &skew_cos,
+                             std::vector<double> &skew_sin,
+                             std::vector<double> &ifft_cos,
+                             std::vector<double> &ifft_sin) const {
+  const int codec = n;
+  const double pi = std::numbers::pi;
+  skew_cos.resize(codec);
+  skew_sin.resize(codec);
+  ifft_cos.resize(codec);
+  ifft_sin.resize(codec);
+  const double step = pi / (codec * 2);
+  const double unit_angle = -2 * pi / codec;
+  for (int i = 0; i < codec; i++) {
+    const int real_index = 2 * i;
+    libm_sincos(real_index * step, &skew_sin[i], &skew_cos[i]);
+    const double angle = i * unit_angle;
+    double unused;
+    libm_sincos(angle, &unused, &ifft_cos[i]);
+    ifft_sin[i] = libm_sin(-angle);
+  }
 }
 
 } // namespace fhe

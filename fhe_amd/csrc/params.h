@@ -164,6 +164,27 @@ public:
                    std::vector<double> &twist_sin) const;
   EncodeLevel encode_level(int level) const;
 
+  // Decoding (src/ckks/core_decode.cpp:10-124): the CRT constants of
+  // DecodeParameters (src/parameter/decode.cpp:5-51) for the limbs from
+  // moduli_id on, as little-endian 64-bit words.
+  struct DecodeCrt {
+    int limbs = 0;
+    int words = 0;                   // words of Q = prod of the limbs' primes
+    std::vector<uint64_t> reverser;  // [limbs][words]: (Q/q_i)((Q/q_i)^-1 mod q_i)
+    std::vector<uint64_t> q_product; // Q, `words` words
+    std::vector<uint64_t> q_half;    // floor(Q / 2), `words` words
+    std::vector<uint64_t> upper;     // Q 2^62, words + 1 words
+    uint64_t k = 0;                  // -Q^-1 mod 2^62
+  };
+  DecodeCrt decode_crt(int moduli_id) const;
+  // codec index -> slot (the inverse of the encoding permutation)
+  std::vector<uint32_t> decode_post_permutation() const;
+  // cos / sin of (2 index) * (pi / (2 codec)) (the skewer) and the inverse
+  // FFT twiddles cos(a), sin(-a), a = index * (-2 pi / codec)
+  void decode_trig(std::vector<double> &skew_cos, std::vector<double> &skew_sin,
+                   std::vector<double> &ifft_cos,
+                   std::vector<double> &ifft_sin) const;
+
 private:
   void derive(bool twiddles);
   void derive_deviations();

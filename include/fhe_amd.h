@@ -298,10 +298,30 @@ int fhe_encode(fhe_context *ctx, const double *values, int complex_input,
  * chain including the special primes) and whether the level is the top of a
  * one-more-level chain (src/ckks/core_encode.cpp:64-78); host only. */
 int fhe_params_encode_constants(const fhe_params *params, int level,
-                                double *ckks_scale, double *scale_before_tile,
+                                long double *ckks_scale,
+                                double *scale_before_tile,
                                 int64_t *log_scale_after_tile,
                                 int *top_of_one_more_level, int64_t *r_mult,
                                 int capacity, int *count);
+
+/* Decoding of `count` INTT-form plaintexts pt [count][limbs][N] at
+ * moduli_id with the plaintexts' ckks_scale (CKKSEngine::decode_unit /
+ * decode_complex_unit_plaintext, src/ckks/engine_decode.cpp:6-18, through
+ * CKKSCore::convert_plaintext_to_double and decode_,
+ * src/ckks/core_decode.cpp:10-124): the plaintexts are rescaled while
+ * ckks_scale / q stays above the base scale, the limbs are lifted to centered
+ * integers by CRT, and the inverse canonical embedding gives
+ * out [count][slots] real parts (complex_output = 0) or [count][slots][2]
+ * (complex_output = 1), device memory.  The reference's errors: fewer than
+ * two limbs, or a scale above the limbs' modulus product.  pt is not
+ * modified. */
+int fhe_decode(fhe_context *ctx, const int64_t *pt, int count, int moduli_id,
+               long double ckks_scale, int complex_output, double *out);
+/* The moduli_id and scale the decoder reaches after its rescales (host
+ * only; the same loop as fhe_decode). */
+int fhe_decoded_scale(const fhe_context *ctx, int moduli_id,
+                      long double ckks_scale, int *decoded_moduli_id,
+                      long double *decoded_scale);
 
 /* Stage timing (no counterpart in the reference, which has no device
  * stages): while enabled, every pipeline stage that a call launches on the

@@ -25,6 +25,7 @@
  * static_cast<uint128_t>(int64_t) does, so negative intermediates in ModDown
  * and rescale reproduce bit for bit.
  */
+#define _GNU_SOURCE /* sincos */
 #include "ckks_oracle.h"
 
 #include <math.h>
@@ -561,7 +562,7 @@ void oracle_rescale(const oracle_ctx *c, int mid, int polys, const int64_t *x,
   const int n = c->n;
   const int chain = c->chain_count - mid;
   const int next = chain - 1;
-  const int64_t qh0 = (int64_t)c->qh[mid];
+  const uint64_t qh0 = c->qh[mid];
   for (int p = 0; p < polys; p++) {
     const int64_t *xp = x + (size_t)p * chain * n;
     int64_t *yp = y + (size_t)p * next * n;
@@ -573,7 +574,9 @@ void oracle_rescale(const oracle_ctx *c, int mid, int polys, const int64_t *x,
       const int64_t *xi = LIMB(xp, i + 1);
       int64_t *yi = LIMB(yp, i);
       for (int t = 0; t < n; t++) {
-        const int64_t rounder = x0[t] > qh0 ? 1 : 0;
+        /* get_rounder compares int64 against the unsigned q_half
+         * (src/montgomery/unit.cpp:67-71): negative inputs round up */
+        const int64_t rounder = (uint64_t)x0[t] > qh0 ? 1 : 0;
         int64_t v = xi[t] - x0[t];
         v = v >= 0 ? v : v + q2;
         v = guard(v, q);
@@ -822,6 +825,14 @@ void oracle_decrypt(const oracle_ctx *c, int mid, const int64_t *ct,
 
 static const double kPi = 3.14159265358979323846;
 
+/* glibc's sin and sincos through volatile pointers, unfoldable: the
+ * reference's -O2 build computes every same-argument (cos x, sin x) pair
+ * with sincos (its results differ from separate calls in the last bit for
+ * some x) and sin(-x) beside such a pair with a separate sin call
+ * (src/ckks/precomputed.cpp:67-75, src/ckks/encode.cpp:264-297). */
+static double (*volatile libm_sin)(double) = sin;
+static void (*volatile libm_sincos)(double, double *, double *) = sincos;
+
 /* Lists the cycles of a permutation, each from its smallest unvisited start
  * (src/ckks/encode.cpp:89-124). */
 static void permutation_cycles(const uint64_t *perm, int size, uint64_t *out) {
@@ -936,7 +947,8 @@ void oracle_encode(const oracle_ctx *c, const double *values,
     for (int off = 0; off < codec; off += step)
       for (int j = 0; j < h; j++) {
         const double angle = (j * tstep) * unit_angle;
-        const double wr = cos(angle), wi = sin(angle);
+        double wr, wi;
+        libm_sincos(angle, &wi, &wr);
         const int x = off + j, y = off + j + h;
         const double pr = wr * re[y] - wi * im[y];
         const double pi_ = wr * im[y] + wi * re[y];
@@ -953,7 +965,9 @@ void oracle_encode(const oracle_ctx *c, const double *values,
   for (int i = 0; i < codec; i++) {
     const double a = re[i] / codec, b = im[i] / codec;
     const double ang = (2 * i) * twist_step;
-    v[i] = (a * cos(ang) - b * sin(ang)) * scale_before_tile;
+    double sn, cs;
+    libm_sincos(ang, &sn, &cs);
+    v[i] = (a * cs - b * sn) * scale_before_tile;
     if (fabs(v[i]) > max_abs)
       max_abs = fabs(v[i]);
   }
@@ -990,4 +1004,291 @@ void oracle_encode(const oracle_ctx *c, const double *values,
   free(re);
   free(im);
   free(v);
+}
+
+/* ---- decoding ------------------------------------------------------------
+ * Multiword integers are little-endian uint64 arrays with an explicit length
+ * (BigInt, src/arithmetic/big_int.cpp, keeps its words trimmed of high
+ * zeros; comparisons here are numeric, which is what its trimmed >= is). */
+
+typedef struct {
+  int len;
+  uint64_t w[80];
+} mw;
+
+static void mw_trim(mw *a) {
+  while (a->len > 1 && a->w[a->len - 1] == 0)
+    a->len--;
+}
+
+static void mw_set(mw *a, uint64_t v) {
+  memset(a, 0, sizeof(*a));
+  a->len = 1;
+  a->w[0] = v;
+}
+
+static void mw_mul_word(const mw *a, uint64_t b, mw *out) {
+  mw r;
+  memset(&r, 0, sizeof(r));
+  u128 carry = 0;
+  for (int i = 0; i < a->len; i++) {
+    const u128 p = (u128)a->w[i] * b + carry;
+    r.w[i] = (uint64_t)p;
+    carry = p >> 64;
+  }
+  r.w[a->len] = (uint64_t)carry;
+  r.len = a->len + 1;
+  mw_trim(&r);
+  *out = r;
+}
+
+static void mw_add(const mw *a, const mw *b, mw *out) {
+  mw r;
+  memset(&r, 0, sizeof(r));
+  const int len = a->len > b->len ? a->len : b->len;
+  u128 carry = 0;
+  for (int i = 0; i < len; i++) {
+    const u128 s = (u128)(i < a->len ? a->w[i] : 0) +
+                   (i < b->len ? b->w[i] : 0) + carry;
+    r.w[i] = (uint64_t)s;
+    carry = s >> 64;
+  }
+  r.w[len] = (uint64_t)carry;
+  r.len = len + 1;
+  mw_trim(&r);
+  *out = r;
+}
+
+static int mw_geq(const mw *a, const mw *b) {
+  if (a->len != b->len)
+    return a->len > b->len;
+  for (int i = a->len - 1; i >= 0; i--)
+    if (a->w[i] != b->w[i])
+      return a->w[i] > b->w[i];
+  return 1;
+}
+
+/* a - b for a >= b */
+static void mw_sub(const mw *a, const mw *b, mw *out) {
+  mw r;
+  memset(&r, 0, sizeof(r));
+  uint64_t borrow = 0;
+  for (int i = 0; i < a->len; i++) {
+    const uint64_t bw = i < b->len ? b->w[i] : 0;
+    const uint64_t d = a->w[i] - bw;
+    r.w[i] = d - borrow;
+    borrow = (a->w[i] < bw) | (d < borrow);
+  }
+  r.len = a->len;
+  mw_trim(&r);
+  *out = r;
+}
+
+static uint64_t mw_div_word(const mw *a, uint64_t b, mw *out) {
+  mw r;
+  memset(&r, 0, sizeof(r));
+  u128 rem = 0;
+  for (int i = a->len - 1; i >= 0; i--) {
+    const u128 cur = (rem << 64) | a->w[i];
+    r.w[i] = (uint64_t)(cur / b);
+    rem = cur % b;
+  }
+  r.len = a->len;
+  mw_trim(&r);
+  if (out)
+    *out = r;
+  return (uint64_t)rem;
+}
+
+static int64_t inverse_mod(int64_t a, int64_t m) {
+  /* a^(m-2) mod m for prime m */
+  int64_t r = 1, b = a % m, e = m - 2;
+  while (e > 0) {
+    if (e & 1)
+      r = (int64_t)((i128)r * b % m);
+    b = (int64_t)((i128)b * b % m);
+    e >>= 1;
+  }
+  return r;
+}
+
+/* the CRT lift of one plaintext at moduli_id to doubles
+ * (src/ckks/core_decode.cpp:68-98, src/arithmetic/arithmetic.cpp:169-197) */
+void oracle_decode_crt(const oracle_ctx *c, const int64_t *cur, int mid,
+                       double *v) {
+  const int n = c->n;
+  const int limbs = c->chain_count - mid;
+  if (limbs == 1) {
+    const int64_t q = (int64_t)c->q[mid];
+    for (int j = 0; j < n; j++) {
+      const int64_t x = cur[j];
+      v[j] = (double)((uint64_t)x <= c->qh[mid] ? x : x - q);
+    }
+  } else {
+    /* DecodeParameters: Q, Q/2, reversers, -Q^-1 mod 2^62 */
+    mw Q, half, upper, *rev = malloc(sizeof(mw) * limbs);
+    mw_set(&Q, 1);
+    for (int i = 0; i < limbs; i++)
+      mw_mul_word(&Q, c->q[mid + i], &Q);
+    mw_div_word(&Q, 2, &half);
+    for (int i = 0; i < limbs; i++) {
+      const uint64_t qi = c->q[mid + i];
+      mw cof;
+      mw_div_word(&Q, qi, &cof);
+      uint64_t inv = 1;
+      for (int j = 0; j < limbs; j++)
+        if (j != i)
+          inv = (uint64_t)((u128)inv *
+                           (uint64_t)inverse_mod((int64_t)(c->q[mid + j] % qi),
+                                                 (int64_t)qi) % qi);
+      mw_mul_word(&cof, inv, &rev[i]);
+    }
+    const uint64_t r62 = UINT64_C(1) << 62, mask = r62 - 1;
+    uint64_t qinv = Q.w[0];
+    for (int it = 0; it < 6; it++)
+      qinv *= 2 - Q.w[0] * qinv;
+    const uint64_t kq = (0 - qinv) & mask;
+    mw_mul_word(&Q, r62, &upper);
+    for (int j = 0; j < n; j++) {
+      mw acc;
+      mw_set(&acc, 0);
+      for (int i = 0; i < limbs; i++) {
+        const int64_t q = (int64_t)c->q[mid + i];
+        int64_t x = cur[(size_t)i * n + j];
+        x = x < 0 ? x + q : guard(x, q);
+        const uint64_t xr = (uint64_t)((u128)(uint64_t)x * r62 % (uint64_t)q);
+        mw t;
+        mw_mul_word(&rev[i], xr, &t);
+        mw_add(&acc, &t, &acc);
+      }
+      const uint64_t w0 = acc.w[0];
+      while (mw_geq(&acc, &upper))
+        mw_sub(&acc, &upper, &acc);
+      mw prod, quo;
+      mw_mul_word(&Q, (w0 * kq) & mask, &prod);
+      mw_add(&acc, &prod, &acc);
+      /* exact division by 2^62 */
+      memset(&quo, 0, sizeof(quo));
+      for (int w = 0; w < acc.len; w++)
+        quo.w[w] = (acc.w[w] >> 62) | (w + 1 < acc.len ? acc.w[w + 1] << 2 : 0);
+      quo.len = acc.len;
+      mw_trim(&quo);
+      if (mw_geq(&quo, &Q))
+        mw_sub(&quo, &Q, &quo);
+      int negative = 0;
+      if (mw_geq(&quo, &half)) {
+        mw_sub(&Q, &quo, &quo);
+        negative = 1;
+      }
+      double y = 0;
+      for (int w = 0; w < quo.len; w++)
+        y += (double)quo.w[w] * pow(2.0, 64 * w);
+      v[j] = negative ? -y : y;
+    }
+    free(rev);
+  }
+}
+
+/* src/ckks/core_decode.cpp:10-124 with rns_to_double
+ * (src/arithmetic/arithmetic.cpp:169-197), DecodeParameters
+ * (src/parameter/decode.cpp:5-51), BigInt::montgomery_modulus and get_double
+ * (src/arithmetic/big_int.cpp:202-225), apply_skewer
+ * (src/ckks/encode.cpp:289-297), the FFT with the inverse twiddles and
+ * permute_after_ifft (src/ckks/encode.cpp:481-503). */
+int oracle_decode(const oracle_ctx *c, const int64_t *pt, int mid,
+                  long double ckks_scale, double base_scale,
+                  int complex_output, double *out) {
+  const int n = c->n, codec = c->n, slots = c->n / 2;
+  const int full_level = c->chain_count - 2;
+  int limbs = c->chain_count - mid;
+  if (limbs == 1)
+    return -1;
+  long double modulus = 1;
+  for (int i = 0; i < limbs; i++)
+    modulus *= (long double)c->q[mid + i];
+  if (ckks_scale > modulus)
+    return -1;
+  int64_t *cur = malloc(sizeof(int64_t) * (size_t)limbs * n);
+  memcpy(cur, pt, sizeof(int64_t) * (size_t)limbs * n);
+  long double scale = ckks_scale;
+  while (mid <= full_level) {
+    const long double rescaled = scale / c->q[mid];
+    if (rescaled <= base_scale)
+      break;
+    int64_t *next = malloc(sizeof(int64_t) * (size_t)(limbs - 1) * n);
+    oracle_rescale(c, mid, 1, cur, next);
+    free(cur);
+    cur = next;
+    scale = rescaled;
+    mid++;
+    limbs--;
+  }
+  double *v = malloc(sizeof(double) * codec);
+  oracle_decode_crt(c, cur, mid, v);
+  free(cur);
+  /* skewer, bit reversal, FFT with (cos a, sin -a), gather */
+  double *re = malloc(sizeof(double) * codec);
+  double *im = malloc(sizeof(double) * codec);
+  const double skew_step = kPi / (codec * 2);
+  for (int i = 0; i < codec; i++) {
+    const double a = (2 * i) * skew_step;
+    double sn, cs;
+    libm_sincos(a, &sn, &cs);
+    re[i] = v[i] * cs;
+    im[i] = v[i] * sn;
+  }
+  int log_codec = 0;
+  while ((1 << log_codec) < codec)
+    log_codec++;
+  for (int i = 0; i < codec; i++) {
+    int r = 0;
+    for (int b = 0; b < log_codec; b++)
+      if (i & (1 << b))
+        r |= 1 << (log_codec - 1 - b);
+    if (i < r) {
+      double t = re[i];
+      re[i] = re[r];
+      re[r] = t;
+      t = im[i];
+      im[i] = im[r];
+      im[r] = t;
+    }
+  }
+  const double unit_angle = -2 * kPi / codec;
+  for (int step = 2, tstep = codec / 2; step <= codec; step <<= 1, tstep >>= 1) {
+    const int h = step >> 1;
+    for (int off = 0; off < codec; off += step)
+      for (int j = 0; j < h; j++) {
+        const double angle = (j * tstep) * unit_angle;
+        double wr, unused;
+        libm_sincos(angle, &unused, &wr);
+        const double wi = libm_sin(-angle);
+        const int x = off + j, y = off + j + h;
+        const double pr = wr * re[y] - wi * im[y];
+        const double pi_ = wr * im[y] + wi * re[y];
+        const double lr = re[x], li = im[x];
+        re[x] = lr + pr;
+        im[x] = li + pi_;
+        re[y] = lr - pr;
+        im[y] = li - pi_;
+      }
+  }
+  uint64_t *pre = malloc(sizeof(uint64_t) * codec);
+  oracle_encode_pre_permutation(codec, pre);
+  const double scale_inverse = (double)(1.0 / scale);
+  for (int i = 0; i < slots; i++) {
+    /* post[index] = i  <=>  index = pre[i] */
+    const uint64_t index = pre[i];
+    if (complex_output) {
+      out[2 * i] = re[index] * scale_inverse;
+      out[2 * i + 1] = im[index] * scale_inverse;
+    } else {
+      out[i] = re[index] * scale_inverse;
+    }
+  }
+  free(pre);
+  free(re);
+  free(im);
+  free(v);
+  return 0;
 }

@@ -120,6 +120,19 @@ void oracle_encode(const oracle_ctx *ctx, const double *values,
  * src/ckks/precomputed.cpp:50-58), codec = n */
 void oracle_encode_pre_permutation(int codec, uint64_t *pre);
 
+/* CKKS decoding (src/ckks/core_decode.cpp:10-124) of one plaintext pt
+ * [chain_count - moduli_id][n] with its long double ckks_scale and the base
+ * scale 2^scale_bits: the rescale loop, the CRT lift, the inverse canonical
+ * embedding.  out: n/2 doubles, or n/2 {re, im} pairs when complex_output.
+ * Returns 0, or -1 for the reference's errors (one limb, scale overflow). */
+int oracle_decode(const oracle_ctx *ctx, const int64_t *pt, int moduli_id,
+                  long double ckks_scale, double base_scale, int complex_output,
+                  double *out);
+
+/* its CRT step alone: limbs at moduli_id -> n signed values as doubles */
+void oracle_decode_crt(const oracle_ctx *ctx, const int64_t *pt, int moduli_id,
+                       double *out);
+
 /* one limb of a per-coefficient primitive (codes of FHE_EW_*) */
 void oracle_elementwise_limb(int op, const int64_t *x1, const int64_t *x2,
                              int64_t *y, int n, uint64_t q, uint64_t k,

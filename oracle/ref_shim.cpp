@@ -17,7 +17,9 @@
 #include <string>
 #include <vector>
 
+#include "arithmetic/arithmetic.h"
 #include "ckks/config.h"
+#include "ckks/core.h"
 #include "ckks/encode.h"
 #include "ckks/engine.h"
 #include "ckks/precomputed.h"
@@ -27,6 +29,8 @@
 #include "operation/scalar.h"
 #include "parameter/multiplier.h"
 #include "operation/ntt.h"
+#include "operation/fft.h"
+#include "arithmetic/pi.h"
 
 using namespace desilo::fhe;
 
@@ -562,7 +566,7 @@ int ref_encode(void *handle, const double *values, int complex_input,
 // (src/ckks/precomputed.cpp:138-215); r_mult gets the level's
 // r_mult_scale_after_tile, *count entries: the chain with the special primes,
 // one less at the top of a one-more-level chain (:161-170).
-int ref_encode_constants(void *handle, int level, double *ckks_scale,
+int ref_encode_constants(void *handle, int level, long double *ckks_scale,
                          double *scale_before_tile,
                          int64_t *log_scale_after_tile, int64_t *r_mult,
                          int *count) {
@@ -579,10 +583,140 @@ int ref_encode_constants(void *handle, int level, double *ckks_scale,
     for (int i = 0; i < n; i++)
       r_mult[i] = r[i];
     *count = n;
-    *ckks_scale = double(cs);
+    *ckks_scale = cs;
     *scale_before_tile = pc->get_scale_before_tile(level);
     *log_scale_after_tile = pc->get_log_scale_after_tile(level);
     return top ? 1 : 0;
+  });
+}
+
+// CKKSEngine::decode_unit / decode_complex_unit_plaintext
+// (src/ckks/engine_decode.cpp:6-18) of a plaintext at `level` with the given
+// limbs [chain][N] and long double ckks_scale.  out: slot_count doubles or
+// {re, im} pairs.
+int ref_decode(void *handle, const int64_t *limbs, int level,
+               const long double *ckks_scale, int complex_output,
+               double *out) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    const int slots = ref->precomputed->slot_count_;
+    std::vector<double> zeros(slots, 0.0);
+    auto shape = ref->engine->encode_unit(zeros, level);
+    auto src = shape->get_polynomial(0);
+    auto poly = new Polynomial(src->flag_, src->moduli_id_, shape->chain_count_,
+                               shape->coeff_count_);
+    std::vector<Polynomial *> polys{poly};
+    auto pt = new UnitPlaintext(
+        polys, shape->flag_, shape->scheme_preset_, shape->log_slot_count_,
+        shape->log_coeff_count_, shape->special_prime_count_,
+        shape->moduli_id_, shape->level_, shape->chain_count_,
+        shape->coeff_count_, *ckks_scale);
+    delete shape;
+    copy_in(pt, limbs);
+    if (complex_output) {
+      auto v = ref->engine->decode_complex_unit_plaintext(pt);
+      for (int i = 0; i < slots; i++) {
+        out[2 * i] = v[i].real();
+        out[2 * i + 1] = v[i].imag();
+      }
+    } else {
+      auto v = ref->engine->decode_unit(pt);
+      std::memcpy(out, v.data(), sizeof(double) * slots);
+    }
+    delete pt;
+    return slots;
+  });
+}
+
+// The CRT step of the decoder alone: rns_to_double
+// (src/arithmetic/arithmetic.cpp:169-197) of limbs [chain][N] at moduli_id
+// after make_unsigned_reduce (src/ckks/core_decode.cpp:77-98).
+int ref_rns_to_double(void *handle, const int64_t *limbs, int moduli_id,
+                      double *out) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    const auto *pc = ref->precomputed;
+    const int chain = pc->parameters_.get_chain_count(moduli_id);
+    const int n = pc->coeff_count_;
+    const auto q = pc->get_q(moduli_id);
+    std::vector<std::vector<int64_t>> x(chain, std::vector<int64_t>(n));
+    std::vector<int64_t *> rows(chain);
+    for (int i = 0; i < chain; i++) {
+      for (int j = 0; j < n; j++) {
+        const int64_t v = limbs[(size_t)i * n + j];
+        x[i][j] = v < 0 ? v + int64_t(q[i]) : guard(v, int64_t(q[i]));
+      }
+      rows[i] = x[i].data();
+    }
+    auto dp = pc->make_decode_parameters(chain, moduli_id);
+    rns_to_double(rows.data(), out, q, dp->q_product_, dp->q_product_half_,
+                  dp->rns_reverser_.data(), dp->r_minus_k_product_, chain, n,
+                  0, n);
+    delete dp;
+    return 0;
+  });
+}
+
+// The decoder after its CRT step (CKKSCore::decode_,
+// src/ckks/core_decode.cpp:101-112): skewer, bit reversal, FFT with the
+// inverse twiddles, permute_after_ifft with scale_inverse.  v: N doubles.
+int ref_decode_tail(void *handle, const double *v, double scale_inverse,
+                    int complex_output, double *out) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    const auto *pc = ref->precomputed;
+    const int codec = pc->codec_coeff_count_, slots = pc->slot_count_;
+    std::vector<double> x(v, v + codec), y(2 * codec);
+    apply_skewer(x.data(), y.data(), codec, 0, pi / (codec * 2));
+    auto c = reinterpret_cast<std::complex<double> *>(y.data());
+    bit_reversal_permute(c, c, codec);
+    fft(c, c, pc->ifft_twiddle_factor_.data(), codec);
+    if (complex_output)
+      permute_after_ifft(y.data(), reinterpret_cast<std::complex<double> *>(out),
+                         pc->get_post_permutation(), slots, scale_inverse, 0,
+                         codec);
+    else
+      permute_after_ifft(y.data(), out, pc->get_post_permutation(), slots,
+                         scale_inverse, 0, codec);
+    return 0;
+  });
+}
+
+// CKKSCore::rescale of a plaintext (src/ckks/core_rescale.cpp:55-81): limbs
+// [chain][N] at `level` -> out [chain - 1][N].
+int ref_rescale_plaintext(void *handle, const int64_t *limbs, int level,
+                          int64_t *out) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    CKKSCore core(ref->config);
+    std::vector<double> zeros(ref->precomputed->slot_count_, 0.0);
+    auto pt = core.encode(zeros.data(), level, false);
+    copy_in(pt, limbs);
+    auto r = core.rescale(pt);
+    copy_out(r, out);
+    delete pt;
+    delete r;
+    return 0;
+  });
+}
+
+// The long double ckks_scale of a ciphertext.
+int ref_ct_scale_ld(void *ct, long double *out) {
+  *out = static_cast<UnitCiphertext *>(ct)->ckks_scale_;
+  return 0;
+}
+
+// CKKSPrecomputed's FFT twiddle tables (src/ckks/precomputed.cpp:67-75) as
+// {re, im} pairs, codec entries each.
+int ref_fft_twiddles(void *handle, double *fft, double *ifft) {
+  return guarded([&]() -> int {
+    auto ref = static_cast<RefEngine *>(handle);
+    const auto *pc = ref->precomputed;
+    const int codec = pc->codec_coeff_count_;
+    std::memcpy(fft, pc->fft_twiddle_factor_.data(), sizeof(double) * 2 * codec);
+    std::memcpy(ifft, pc->ifft_twiddle_factor_.data(),
+                sizeof(double) * 2 * codec);
+    return codec;
   });
 }
 

@@ -93,6 +93,9 @@ class Oracle:
             lib.oracle_encode.argtypes = [VP, VP, i32, ctypes.c_double, i32,
                                           i32, i32, VP, VP]
             lib.oracle_encode_pre_permutation.argtypes = [i32, VP]
+            lib.oracle_decode.restype = i32
+            lib.oracle_decode.argtypes = [VP, VP, i32, ctypes.c_longdouble,
+                                          ctypes.c_double, i32, VP]
             cls._lib = lib
         return cls._lib
 
@@ -257,6 +260,21 @@ class Oracle:
                                  out_limbs, ptr(r), ptr(out))
         return out
 
+    def decode(self, pt, mid, ckks_scale, base_scale, complex_output=False):
+        """Decode plaintext limbs at moduli_id with a long double scale."""
+        slots = self.n // 2
+        out = np.zeros(slots * (2 if complex_output else 1))
+        buf = np.array([ckks_scale], dtype=np.longdouble)
+        rc = self.lib().oracle_decode(
+            self.h, ptr(np.ascontiguousarray(pt, np.int64)), mid,
+            ctypes.c_longdouble.from_buffer(buf), float(base_scale),
+            int(complex_output), ptr(out))
+        if rc != 0:
+            raise ValueError("oracle_decode refused the plaintext")
+        if complex_output:
+            return out[0::2] + 1j * out[1::2]
+        return out
+
     def partitions(self, mid, key_mid):
         s = np.zeros(128, np.int32)
         z = np.zeros(128, np.int32)
@@ -354,6 +372,8 @@ class Reference:
             "ref_encode": (i32, [VP, VP, i32, i32, VP, VP]),
             "ref_encode_constants": (i32, [VP, i32, VP, VP, VP, VP, VP]),
             "ref_pre_permutation": (i32, [VP, VP]),
+            "ref_decode": (i32, [VP, VP, i32, VP, i32, VP]),
+            "ref_ct_scale_ld": (i32, [VP, VP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -434,17 +454,37 @@ class Reference:
         return int(s[0]), out
 
     def encode_constants(self, level):
-        ckks, sbt = np.zeros(1), np.zeros(1)
+        ckks, sbt = np.zeros(1, np.longdouble), np.zeros(1)
         log_after = np.zeros(1, np.int64)
         r = np.zeros(512, np.int64)
         count = np.zeros(1, np.int32)
         top = self.lib.ref_encode_constants(self.e, level, ptr(ckks), ptr(sbt),
                                             ptr(log_after), ptr(r),
                                             ptr(count))
-        return dict(ckks_scale=float(ckks[0]), scale_before_tile=float(sbt[0]),
+        return dict(ckks_scale=ckks[0], scale_before_tile=float(sbt[0]),
                     log_scale_after_tile=int(log_after[0]),
                     top_of_one_more_level=bool(top),
                     r_mult_scale_after_tile=r[:int(count[0])])
+
+    def decode(self, limbs, level, ckks_scale, complex_output=False):
+        """CKKSEngine::decode_unit of plaintext limbs [chain][N] at `level`
+        with a long double scale."""
+        scale = np.array([ckks_scale], dtype=np.longdouble)
+        out = np.zeros(self.slot_count * (2 if complex_output else 1))
+        rc = self.lib.ref_decode(self.e,
+                                 ptr(np.ascontiguousarray(limbs, np.int64)),
+                                 level, ptr(scale), int(complex_output),
+                                 ptr(out))
+        if rc != self.slot_count:
+            raise RuntimeError(self.lib.ref_last_error().decode())
+        if complex_output:
+            return out[0::2] + 1j * out[1::2]
+        return out
+
+    def ct_scale(self, ct):
+        out = np.zeros(1, np.longdouble)
+        self.lib.ref_ct_scale_ld(ct, ptr(out))
+        return out[0]
 
     def pre_permutation(self):
         out = np.zeros(self.n, np.uint64)
