@@ -11,7 +11,9 @@ Printed JSON line: metric/value (whole-job ciphertexts/s), plain (B = 1)
 throughput beside it, the roofline of the dominant kernel measured live with
 HIP events on the library's stream, and the CPU baseline (the reference
 engine built from its sources, CPU_PARALLEL on the host cores) on a bounded
-sample of the same workload.
+sample of the same workload.  "encode_decode" adds the device encoder and
+decoder on a batch of complex slot vectors at the same limb count, with the
+reference's single-threaded encode_unit / decode_unit beside them.
 """
 import argparse
 import ctypes
@@ -153,6 +155,72 @@ def cpu_baseline(budget_s=20.0):
             "kind": "port", "sample": f"{done} oracle HMult at Gold L=24"}
 
 
+def copy_probe(mib):
+    """A plain copy of known size on the device (torch's elementwise copy
+    kernel), for calibrating the HBM byte counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.int64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8,
+                      "copy_probe_bytes_written": n * 8, "launches": 5}))
+
+
+def codec_bench(ctx, ev, batch, steps, with_cpu):
+    """Encode and decode of `batch` complex slot vectors at the benchmark's
+    level (24) on the device, values and plaintexts resident in HBM; with
+    with_cpu, the reference engine's encode_unit / decode_unit on one host
+    thread (its encoder runs as a single task in every processor mode)."""
+    import fhe_amd
+    level = ctx.info["full_level"] - MODULI_ID
+    slots = ctx.n // 2
+    rng = np.random.Generator(np.random.PCG64(3))
+    vals = rng.uniform(-1, 1, (batch, slots, 2))
+    dv = ctx.to_device(vals.view(np.int64))
+    limbs = ctx.limbs_at(MODULI_ID)
+    pt = ctx.array((batch, limbs, ctx.n))
+    dec = ctx.array((batch, 2 * slots))
+    scale = fhe_amd.params_encode_constants(LOG_N, ALPHA, level)["ckks_scale"]
+    enc_ms = time_steps(ctx, ev, lambda: ctx.encode_device(dv, True, batch,
+                                                           level, pt),
+                        steps, 2)
+    dec_ms = time_steps(ctx, ev, lambda: ctx.decode_device(pt, batch,
+                                                           MODULI_ID, scale,
+                                                           True, dec),
+                        steps, 2)
+    for arr in (dv, pt, dec):
+        arr.free()
+    out = {"level": level, "batch": batch,
+           "encode_messages_per_s": round(batch / (enc_ms / 1e3), 1),
+           "decode_messages_per_s": round(batch / (dec_ms / 1e3), 1),
+           "encode_ms_per_batch": round(enc_ms, 4),
+           "decode_ms_per_batch": round(dec_ms, 4)}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        from support import Reference
+        if Reference.available():
+            ref = Reference(LOG_N, ALPHA)
+            msg = vals[0, :, 0] + 1j * vals[0, :, 1]
+            reps, t0 = 0, time.perf_counter()
+            while reps < 3 or time.perf_counter() - t0 < 2.0:
+                _, limbs_h = ref.encode(msg, level)
+                reps += 1
+            enc_s = (time.perf_counter() - t0) / reps
+            reps, t0 = 0, time.perf_counter()
+            while reps < 3 or time.perf_counter() - t0 < 2.0:
+                ref.decode(limbs_h, level, scale, True)
+                reps += 1
+            dec_s = (time.perf_counter() - t0) / reps
+            ref.close()
+            out["cpu_encode_messages_per_s"] = round(1 / enc_s, 2)
+            out["cpu_decode_messages_per_s"] = round(1 / dec_s, 2)
+            out["cpu_kind"] = "reference, 1 thread"
+    return out
+
+
 # Algorithmic HBM bytes of one launch of each single-kernel stage (read each
 # input limb once, write each output limb once; N * 8 bytes per limb) and
 # the modular products it performs (one per butterfly, one per pointwise
@@ -217,7 +285,16 @@ def main():
                     help="skip the one-pair (B = 1) comparison, e.g. for a "
                          "profile whose per-kernel averages cover batch B "
                          "launches only")
+    ap.add_argument("--no-codec", action="store_true",
+                    help="skip the encode / decode measurement")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times "
+                         "(the reference kernel for scaling the FETCH_SIZE "
+                         "and WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only (--gpus 1): batches shard "
               "across GPUs with replicated keys and no communication, and "
@@ -257,6 +334,10 @@ def main():
         parts = -(-limbs // ALPHA)
         for arr in (da, db, dk, out):
             arr.free()
+        codec = None
+        if not args.no_codec:
+            codec = codec_bench(ctx, ev, args.batch, max(5, args.steps // 2),
+                                not args.no_cpu)
 
     value = args.batch / (step_ms / 1e3)
     shape = dict(n=n, count=args.batch, limbs=limbs, ext=limbs + ALPHA,
@@ -285,6 +366,7 @@ def main():
                              if plain_ms else None),
         "roofline": roofline,
         "stages": breakdown,
+        "encode_decode": codec,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()

@@ -877,6 +877,7 @@ int fhe_encode(fhe_context *ctx, const double *values, int complex_input,
     const int enc_mid =
         e.top_of_one_more_level ? P.moduli_id_of_level(level - 1) : mid;
     require(limbs - first <= fhe::kMaxEwLimbs, "Encode: too many limbs");
+    require(P.n % fhe::kBlock == 0, "Encode: N below the kernels' block size");
     const auto &tables = ctx->encode_tables();
     void *work = ctx->aux_scratch(fhe::encode_work_bytes(tables, count));
     ctx->stage("encode", [&] {

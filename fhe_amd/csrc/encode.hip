@@ -96,10 +96,9 @@ __global__ __launch_bounds__(kBlock) void encode_fft_stage_kernel(
 __global__ __launch_bounds__(kBlock) void encode_twist_kernel(
     const Cplx *y, double *out, unsigned long long *max_bits, int count,
     int log_codec, const double *cos_t, const double *sin_t, double scale) {
+  // codec >= kBlock, so a block's elements belong to one message
   const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   const int codec = 1 << log_codec;
-  if (idx >= (int64_t(count) << log_codec))
-    return;
   const int i = int(idx & (codec - 1));
   const double size = double(codec);
   const double re = __ddiv_rn(y[idx].re, size);
@@ -107,9 +106,16 @@ __global__ __launch_bounds__(kBlock) void encode_twist_kernel(
   const double v = __dmul_rn(
       __dsub_rn(__dmul_rn(re, cos_t[i]), __dmul_rn(im, sin_t[i])), scale);
   out[idx] = v;
-  // |v| compares as its bit pattern
-  atomicMax(max_bits + (idx >> log_codec),
-            (unsigned long long)__double_as_longlong(fabs(v)));
+  // |v| compares as its bit pattern: wave maximum by butterfly shuffles,
+  // then one atomic per wave
+  unsigned long long m = (unsigned long long)__double_as_longlong(fabs(v));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0)
+    atomicMax(max_bits + (idx >> log_codec), m);
 }
 
 struct ConvertArgs {
@@ -192,8 +198,11 @@ void launch_encode(const DeviceTables &t, const EncodeDevice &e,
                        dim3(grid_for(int64_t(count) * (codec / 2))),
                        dim3(kBlock), 0, stream, y, count, e.log_codec, s,
                        e.fft_cos, e.fft_sin);
+  // the twist kernel's grid covers count * codec exactly (codec is a
+  // multiple of kBlock): every thread of a wave takes part in its shuffles
   hipLaunchKernelGGL(encode_twist_kernel,
-                     dim3(grid_for(int64_t(count) * codec)), dim3(kBlock), 0,
+                     dim3(unsigned((int64_t(count) * codec) / kBlock)),
+                     dim3(kBlock), 0,
                      stream, y, v, max_bits, count, e.log_codec, e.twist_cos,
                      e.twist_sin, scale_before_tile);
   ConvertArgs a{};
