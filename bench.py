@@ -248,6 +248,26 @@ def stage_model(name, s):
     return None, None
 
 
+# Kernel name of each single-kernel stage, for the recorded HBM traffic.
+STAGE_KERNEL = {"ks_inner_product_r16": "ks_inner_product_r16_kernel"}
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "hmult_gold_b32_traffic.md")
+
+
+def recorded_traffic(stage, batch):
+    """HBM bytes per launch of a stage's kernel from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of known
+    size), recorded for the default batch of 32; None when absent."""
+    kernel = STAGE_KERNEL.get(stage)
+    if kernel is None or batch != 32 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    key = f"traffic_bytes_per_launch {kernel}:"
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            if line.startswith(key):
+                return int(line[len(key):].strip())
+    return None
+
+
 def stage_report(stages, total_ms, shape):
     """Per-stage share of the step and the roofline of the dominant
     single-kernel stage (one kernel launch per stage call)."""
@@ -266,7 +286,8 @@ def stage_report(stages, total_ms, shape):
     gbs = nbytes / (avg / 1e3) / 1e9
     roofline = {"bound": "hbm", "kernel": name, "achieved": round(gbs, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "traffic": recorded_traffic(name, shape["count"]),
                 "avg_launch_ms": round(avg, 4),
                 "algorithmic_bytes": nbytes}
     if mults is not None:
