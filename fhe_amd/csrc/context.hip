@@ -326,12 +326,21 @@ struct fhe_context {
 
   // Key switching of the INTT-form polynomial src + c * src_stride of each
   // ciphertext c into acc ([count][2][ext][N], INTT form, reduced).
+  // own (optional): the NTT form of the same polynomials (limb i of
+  // ciphertext c at own + c * own_stride + i * N); the fused path then takes
+  // each partition's own extended limbs from it instead of transforming them.
   void key_switch(int count, int moduli_id, const int64_t *key,
                   int key_moduli_id, const int64_t *src, int64_t src_stride,
-                  int64_t *ext, int64_t *acc) {
+                  int64_t *ext, int64_t *acc, const int64_t *own = nullptr,
+                  int64_t own_stride = 0) {
     if (fhe::ks_fused_supported(tables)) {
-      ks_decompose(count, moduli_id, key_moduli_id, src, src_stride, ext);
-      ks_apply(count, moduli_id, key, key_moduli_id, ext, acc);
+      if (int(params.partitions(moduli_id, key_moduli_id).size()) >
+          fhe::kMaxFusedParts)
+        own = nullptr;
+      ks_decompose(count, moduli_id, key_moduli_id, src, src_stride, ext,
+                   own != nullptr);
+      ks_apply(count, moduli_id, key, key_moduli_id, ext, acc, own,
+               own_stride);
       return;
     }
     const int limbs = params.limbs_at(moduli_id);
@@ -371,7 +380,8 @@ struct fhe_context {
   // accumulators are only read by ModDown after the canonicalizing exit,
   // so with lazy_key_switch their arithmetic is lazy.
   void ks_decompose(int count, int moduli_id, int key_moduli_id,
-                    const int64_t *src, int64_t src_stride, int64_t *ext) {
+                    const int64_t *src, int64_t src_stride, int64_t *ext,
+                    bool skip_own = false) {
     const int limbs = params.limbs_at(moduli_id);
     const int ext_limbs = limbs + params.alpha;
     const int64_t N = n();
@@ -402,7 +412,7 @@ struct fhe_context {
                                  limbs, moduli_id, starts.data(), sizes.data(),
                                  is_signed, np, cst, stride,
                                  params.ks_ext_shoup_offset(moduli_id),
-                                 lazy_key_switch, stream);
+                                 lazy_key_switch, skip_own, stream);
       });
       return;
     }
@@ -435,7 +445,8 @@ struct fhe_context {
   // product over all partitions, both INTT passes (with exit and reduce)
   // into acc [count][2][ext][N].
   void ks_apply(int count, int moduli_id, const int64_t *key,
-                int key_moduli_id, const int64_t *ext, int64_t *acc) {
+                int key_moduli_id, const int64_t *ext, int64_t *acc,
+                const int64_t *own = nullptr, int64_t own_stride = 0) {
     const int limbs = params.limbs_at(moduli_id);
     const int ext_limbs = limbs + params.alpha;
     const int key_chain = params.limbs_at(key_moduli_id) + params.alpha;
@@ -444,11 +455,18 @@ struct fhe_context {
     const int np = int(parts.size());
     const fhe::LazySpec lazy = lazy_key_switch ? fhe::kAllLazy
                                                : fhe::LazySpec();
+    std::vector<int> starts(np), sizes(np);
+    for (int p = 0; p < np; p++) {
+      starts[p] = parts[p].start;
+      sizes[p] = parts[p].size;
+    }
+    if (np > fhe::kMaxFusedParts)
+      own = nullptr;
     stage("ks_inner_product_r16", [&] {
-      fhe::launch_ks_inner_product_fused(tables, ext, key, key_chain,
-                                         key_offset, parts[0].key_index, np,
-                                         acc, count, ext_limbs, moduli_id,
-                                         lazy_key_switch, stream);
+      fhe::launch_ks_inner_product_fused(
+          tables, ext, key, key_chain, key_offset, parts[0].key_index, np, acc,
+          count, ext_limbs, moduli_id, lazy_key_switch, own, own_stride,
+          starts.data(), sizes.data(), stream);
     });
     stage("intt_pass_high(acc)", [&] {
       fhe::ntt_inverse_second(tables, acc, acc, int64_t(count) * 2 * ext_limbs,
@@ -691,8 +709,10 @@ int fhe_relinearize(fhe_context *ctx, int count, int moduli_id,
       fhe::ntt_inverse(ctx->tables, trios, trio, trio_l, limbs, moduli_id,
                        fhe::kInttExit, ctx->stream, d01);
     });
+    // the caller's NTT-form d2 provides the partitions' own extended limbs
     ctx->key_switch(count, moduli_id, key, key_moduli_id, trio + 2 * limbs * N,
-                    3 * limbs * N, ext, acc);
+                    3 * limbs * N, ext, acc, trios + 2 * limbs * N,
+                    3 * limbs * N);
     ctx->stage("moddown_relin", [&] {
       fhe::launch_moddown_relin(ctx->tables, acc, trio, out, count, limbs,
                                 moduli_id, ctx->stream);
@@ -719,10 +739,13 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
     const int64_t ext_l =
         int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
     const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
-    int64_t *ws =
-        ctx->scratch(size_t(2 * duo_l + trio_l + ext_l + acc_l) * N * 8);
+    // trio_c: the coefficient form of the trio; trio keeps the NTT form,
+    // whose d2 limbs serve the partitions' own extended limbs
+    int64_t *ws = ctx->scratch(
+        size_t(2 * duo_l + 2 * trio_l + ext_l + acc_l) * N * 8);
     int64_t *an = ws, *bn = an + duo_l * N, *trio = bn + duo_l * N;
-    int64_t *ext = trio + trio_l * N, *acc = ext + ext_l * N;
+    int64_t *trio_c = trio + trio_l * N;
+    int64_t *ext = trio_c + trio_l * N, *acc = ext + ext_l * N;
     // prepare_for_multiply: to NTT form (src/ckks/engine_multiply.cpp:13-27)
     // Only a1 * b1 reaches the representative-sensitive ModUp (through the
     // exact INTT of d2), so the transforms of a0 and b0 may be lazy.
@@ -760,14 +783,19 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
                            ctx->stream);
       });
       ctx->stage("intt(trio)", [&] {
-        fhe::ntt_inverse(ctx->tables, trio, trio, trio_l, limbs, moduli_id,
+        fhe::ntt_inverse(ctx->tables, trio, trio_c, trio_l, limbs, moduli_id,
                          fhe::kInttExit, ctx->stream, d01);
       });
     }
-    ctx->key_switch(count, moduli_id, key, key_moduli_id, trio + 2 * limbs * N,
-                    3 * limbs * N, ext, acc);
+    const bool fused_trio = ctx->fuse_tensor &&
+                            fhe::tensor_fused_supported(ctx->tables);
+    const int64_t *coef = fused_trio ? trio : trio_c;
+    ctx->key_switch(count, moduli_id, key, key_moduli_id, coef + 2 * limbs * N,
+                    3 * limbs * N, ext, acc,
+                    fused_trio ? nullptr : trio + 2 * limbs * N,
+                    3 * limbs * N);
     ctx->stage("moddown_relin_rescale", [&] {
-      fhe::launch_moddown_relin_rescale(ctx->tables, acc, trio, out, count,
+      fhe::launch_moddown_relin_rescale(ctx->tables, acc, coef, out, count,
                                         limbs, moduli_id, ctx->stream);
     });
     ctx->check_launch("Multiply");

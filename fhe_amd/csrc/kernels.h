@@ -55,6 +55,8 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
                                    int key_offset, int key_partition0,
                                    int parts, int64_t *acc, int count,
                                    int ext_limbs, int moduli_id, bool lazy,
+                                   const int64_t *own, int64_t own_stride,
+                                   const int *starts, const int *sizes,
                                    hipStream_t stream);
 
 // out = mont_mult(a, b) elementwise (src/montgomery/unit.cpp:5-10).
@@ -111,7 +113,8 @@ void launch_staged_modup(const DeviceTables &t, const int64_t *x,
                          const int *starts, const int *sizes,
                          const bool *is_signed, int parts,
                          const int64_t *constants, int constants_stride,
-                         int ext_shoup_offset, bool lazy, hipStream_t stream);
+                         int ext_shoup_offset, bool lazy, bool skip_own,
+                         hipStream_t stream);
 
 // Second forward NTT pass of both input duos, tensor and first inverse NTT
 // pass of the trio in one kernel (N = 2^16).  a, b: [count][2][limbs][N]

@@ -31,6 +31,15 @@ struct KsArgs {
   int key_partition0;  // key partition index of the ciphertext's partition 0
   int swizzle;
   int lazy;            // lazy transforms (the ext pass before was lazy too)
+  // NTT form of the key-switched polynomial, when the caller has it: limb i
+  // of ciphertext c at own + c * own_stride + i * N.  The extended limbs of a
+  // partition's own primes are that polynomial's limbs, so their transforms
+  // are taken from here instead of the extended polynomial (any
+  // representative serves: the key products are canonicalized later).
+  const int64_t *own;
+  int64_t own_stride;
+  int start[kMaxFusedParts];
+  int size[kMaxFusedParts];
 };
 
 // One (extended limb, tile, ciphertext) work item with arithmetic A for the
@@ -64,15 +73,26 @@ __device__ __forceinline__ void ks_tile(const KsArgs &a, const DeviceTables &t,
 
   int64_t acc0[16], acc1[16], v[16];
   for (int pp = 0; pp < a.parts; pp++) {
-    const int64_t *src =
-        a.ext + ((int64_t(ct) * a.parts + pp) * a.ext_limbs + i) * n + off;
+    // block-uniform: every thread of the block has the same (i, pp)
+    const bool own =
+        a.own != nullptr && i >= a.start[pp] && i < a.start[pp] + a.size[pp];
+    if (own) {
+      const int64_t *src =
+          a.own + int64_t(ct) * a.own_stride + int64_t(i) * n + off + 16 * r;
 #pragma unroll
-    for (int m = 0; m < 16; m++)
-      v[m] = src[r + 16 * m];
-    r16_forward_high<A>(v, f);
-    r16_exchange(v, lds, c, r, true);
-    r16_forward_low<A>(v, f);
-    r16_forward_settle<A>(v, f);
+      for (int m = 0; m < 16; m++)
+        v[m] = src[m];
+    } else {
+      const int64_t *src =
+          a.ext + ((int64_t(ct) * a.parts + pp) * a.ext_limbs + i) * n + off;
+#pragma unroll
+      for (int m = 0; m < 16; m++)
+        v[m] = src[r + 16 * m];
+      r16_forward_high<A>(v, f);
+      r16_exchange(v, lds, c, r, true);
+      r16_forward_low<A>(v, f);
+      r16_forward_settle<A>(v, f);
+    }
     // v[m] is now the NTT value at coefficient off + 16 r + m
     const int kp = a.key_partition0 + pp;
     const int64_t *k0 =
@@ -344,6 +364,8 @@ struct ExtendArgs {
   int cst_stride, ext_shoup_offset;
   int count, parts, ext_limbs, moduli_id, maxz, swizzle;
   int size[kMaxFusedParts];
+  int start[kMaxFusedParts];
+  int skip_own;          // leave each partition's own limbs unwritten
   uint32_t signed_mask;
 };
 
@@ -413,6 +435,9 @@ __global__ __launch_bounds__(kBlock) void extend_ntt_r16_kernel(
   const int tile = int(rest % tiles);
   const int64_t cp = rest / tiles;
   const int part = int(cp % a.parts);
+  // the whole block leaves (before any barrier) for a partition's own limb
+  if (a.skip_own && i >= a.start[part] && i < a.start[part] + a.size[part])
+    return;
   const bool sgn = (a.signed_mask >> part) & 1;
   if (!LAZY) {
     if (sgn)
@@ -441,7 +466,8 @@ void launch_staged_modup(const DeviceTables &t, const int64_t *x,
                          const int *starts, const int *sizes,
                          const bool *is_signed, int parts,
                          const int64_t *constants, int constants_stride,
-                         int ext_shoup_offset, bool lazy, hipStream_t stream) {
+                         int ext_shoup_offset, bool lazy, bool skip_own,
+                         hipStream_t stream) {
   int maxz = 1;
   uint32_t mask = 0;
   bool any_signed = false;
@@ -474,10 +500,12 @@ void launch_staged_modup(const DeviceTables &t, const int64_t *x,
   e.moduli_id = moduli_id;
   e.maxz = maxz;
   e.signed_mask = mask;
+  e.skip_own = skip_own ? 1 : 0;
   for (int p = 0; p < parts; p++) {
     d.start[p] = starts[p];
     d.size[p] = sizes[p];
     e.size[p] = sizes[p];
+    e.start[p] = starts[p];
   }
   // the lazy extension needs canonical digits: only for non-negative ones
   const bool use_lazy = lazy && !any_signed;
@@ -564,13 +592,30 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
                                    int key_offset, int key_partition0,
                                    int parts, int64_t *acc, int count,
                                    int ext_limbs, int moduli_id, bool lazy,
+                                   const int64_t *own, int64_t own_stride,
+                                   const int *starts, const int *sizes,
                                    hipStream_t stream) {
   const int tiles = int((int64_t(1) << t.log_n) >> 12);
   const int64_t blocks = int64_t(ext_limbs) * tiles * count;
-  KsArgs a{ext,        key,       acc,        count,
-           parts,      ext_limbs, moduli_id,  key_chain,
-           key_offset, key_partition0, blocks % 8 == 0 ? 1 : 0,
-           lazy ? 1 : 0};
+  KsArgs a{};
+  a.ext = ext;
+  a.key = key;
+  a.acc = acc;
+  a.count = count;
+  a.parts = parts;
+  a.ext_limbs = ext_limbs;
+  a.moduli_id = moduli_id;
+  a.key_chain = key_chain;
+  a.key_offset = key_offset;
+  a.key_partition0 = key_partition0;
+  a.swizzle = blocks % 8 == 0 ? 1 : 0;
+  a.lazy = lazy ? 1 : 0;
+  a.own = own;
+  a.own_stride = own_stride;
+  for (int p = 0; p < parts && p < kMaxFusedParts; p++) {
+    a.start[p] = own ? starts[p] : 0;
+    a.size[p] = own ? sizes[p] : 0;
+  }
   hipLaunchKernelGGL(ks_inner_product_r16_kernel, dim3(unsigned(blocks)),
                      dim3(kBlock), 0, stream, a, t);
 }
