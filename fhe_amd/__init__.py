@@ -32,7 +32,7 @@ EXPORTED_SYMBOLS = (
     "fhe_multiply_int", "fhe_add_double", "fhe_level_down",
     "fhe_encrypt_pk", "fhe_decrypt", "fhe_rotate_batch", "fhe_switch_key",
     "fhe_encode", "fhe_params_encode_constants", "fhe_decode",
-    "fhe_decoded_scale",
+    "fhe_decoded_scale", "fhe_context_set_option",
 )
 
 
@@ -126,6 +126,7 @@ def library():
             ctypes.POINTER(Params), i32, ctypes.c_double, ctypes.c_int64, i32,
             i32, vp, i32, ctypes.POINTER(i32)]
         lib.fhe_encode.argtypes = [vp, vp, i32, i32, i32, vp]
+        lib.fhe_context_set_option.argtypes = [vp, ctypes.c_char_p, i32]
         lib.fhe_decode.argtypes = [vp, vp, i32, i32, ctypes.c_longdouble,
                                    i32, vp]
         lib.fhe_decoded_scale.argtypes = [vp, i32, ctypes.c_longdouble,
@@ -508,6 +509,11 @@ class Context:
                                            ctypes.byref(mid),
                                            scale.ctypes.data))
         return mid.value, scale[0]
+
+    def set_option(self, name, value):
+        """fhe_context_set_option: choose among equivalent pipelines."""
+        _check(library().fhe_context_set_option(self.handle, name.encode(),
+                                                int(value)))
 
     def profile(self, on):
         """Start (clearing totals) or stop stage timing."""

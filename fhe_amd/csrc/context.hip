@@ -104,6 +104,10 @@ struct fhe_context {
   // ModUp staged through the Garner digits (digits kernel + extension fused
   // with the first NTT pass)
   bool staged_modup = true;
+  // each partition's own extended limbs from the NTT-form d2 when at hand
+  bool own_limbs = true;
+  // small-prime key products accumulated without guards, reduced once
+  bool ks_lazy_accumulate = true;
   // second growable device buffer (Garner digits of the staged ModUp)
   void *aux = nullptr;
   size_t aux_bytes = 0;
@@ -145,6 +149,8 @@ struct fhe_context {
     t.p_inverse_shoup = upload(params.p_inverse_shoup, owned);
     t.q_top = upload(params.q_top, owned);
     t.rescale = upload(params.rescale_scale, owned);
+    t.host_q = params.q.data();
+    t.split_variants = 1;
     t.prime_count = params.prime_count;
     t.log_n = params.log_n;
     t.alpha = params.alpha;
@@ -334,8 +340,8 @@ struct fhe_context {
                   int64_t *ext, int64_t *acc, const int64_t *own = nullptr,
                   int64_t own_stride = 0) {
     if (fhe::ks_fused_supported(tables)) {
-      if (int(params.partitions(moduli_id, key_moduli_id).size()) >
-          fhe::kMaxFusedParts)
+      if (!own_limbs || int(params.partitions(moduli_id, key_moduli_id)
+                                .size()) > fhe::kMaxFusedParts)
         own = nullptr;
       ks_decompose(count, moduli_id, key_moduli_id, src, src_stride, ext,
                    own != nullptr);
@@ -466,7 +472,7 @@ struct fhe_context {
       fhe::launch_ks_inner_product_fused(
           tables, ext, key, key_chain, key_offset, parts[0].key_index, np, acc,
           count, ext_limbs, moduli_id, lazy_key_switch, own, own_stride,
-          starts.data(), sizes.data(), stream);
+          starts.data(), sizes.data(), ks_lazy_accumulate, stream);
     });
     stage("intt_pass_high(acc)", [&] {
       fhe::ntt_inverse_second(tables, acc, acc, int64_t(count) * 2 * ext_limbs,
@@ -585,6 +591,30 @@ int fhe_context_prime_constants(const fhe_context *ctx, int i,
 
 void *fhe_context_stream(fhe_context *ctx) {
   return ctx ? static_cast<void *>(ctx->stream) : nullptr;
+}
+
+int fhe_context_set_option(fhe_context *ctx, const char *name, int value) {
+  return guarded([&] {
+    require(ctx && name, "null argument");
+    const std::string n(name);
+    hip_check(hipStreamSynchronize(ctx->stream), "option sync");
+    if (n == "lazy_key_switch")
+      ctx->lazy_key_switch = value != 0;
+    else if (n == "staged_modup")
+      ctx->staged_modup = value != 0;
+    else if (n == "fuse_modup")
+      ctx->fuse_modup = value != 0;
+    else if (n == "fuse_tensor")
+      ctx->fuse_tensor = value != 0;
+    else if (n == "own_limbs")
+      ctx->own_limbs = value != 0;
+    else if (n == "ks_lazy_accumulate")
+      ctx->ks_lazy_accumulate = value != 0;
+    else if (n == "split_variants")
+      ctx->tables.split_variants = value != 0 ? 1 : 0;
+    else
+      require(false, "unknown option " + n);
+  });
 }
 
 int fhe_synchronize(fhe_context *ctx) {

@@ -28,6 +28,12 @@ struct DeviceTables {
   const uint64_t *p_inverse_shoup; // [alpha][prime_count][2] standard form
   const uint64_t *q_top;        // largest multiple of q below 2^63
   const int64_t *rescale;       // [prime_count][prime_count]
+  // HOST copy of the primes (for launch decisions on the host; kernels
+  // never read it)
+  const uint64_t *host_q;
+  // launch choice (host): one launch per arithmetic variant instead of a
+  // per-block choice inside one kernel
+  int split_variants;
   int prime_count;
   int log_n;
   int alpha;

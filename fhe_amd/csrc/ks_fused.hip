@@ -40,6 +40,7 @@ struct KsArgs {
   int64_t own_stride;
   int start[kMaxFusedParts];
   int size[kMaxFusedParts];
+  int lazy_acc;        // small primes: accumulate without guards
 };
 
 // One (extended limb, tile, ciphertext) work item with arithmetic A for the
@@ -105,6 +106,14 @@ __device__ __forceinline__ void ks_tile(const KsArgs &a, const DeviceTables &t,
         acc0[m] = mont_mult_u(v[m], k0[m], f.q, f.k);
         acc1[m] = mont_mult_u(v[m], k1[m], f.q, f.k);
       }
+    } else if (A == kLazySmall && a.lazy_acc) {
+      // q < 2^42: the sum of the parts' products (each below 2q) stays far
+      // below 2^63; it is brought back under 2q once, after the loop
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        acc0[m] = add64(mont_mult_u(v[m], k0[m], f.q, f.k), acc0[m]);
+        acc1[m] = add64(mont_mult_u(v[m], k1[m], f.q, f.k), acc1[m]);
+      }
     } else {
 #pragma unroll
       for (int m = 0; m < 16; m++) {
@@ -118,6 +127,19 @@ __device__ __forceinline__ void ks_tile(const KsArgs &a, const DeviceTables &t,
     }
   }
 
+  if (A == kLazySmall && a.lazy_acc && a.parts > 1) {
+    // acc < 2q * parts: halving guards from 2q * 2^ceil(log2(parts) - 1)
+    int64_t bound = f.q2;
+    while (bound < f.q2 * a.parts)
+      bound <<= 1;
+    for (bound >>= 1; bound >= f.q2; bound >>= 1) {
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        acc0[m] = guard_u(acc0[m], bound);
+        acc1[m] = guard_u(acc1[m], bound);
+      }
+    }
+  }
   R16Pass g = f;
   g.tw = t.root_inverse + int64_t(p) * n;
   g.ts = t.root_inverse_shoup + int64_t(p) * n * 2;
@@ -594,7 +616,7 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
                                    int ext_limbs, int moduli_id, bool lazy,
                                    const int64_t *own, int64_t own_stride,
                                    const int *starts, const int *sizes,
-                                   hipStream_t stream) {
+                                   bool lazy_accumulate, hipStream_t stream) {
   const int tiles = int((int64_t(1) << t.log_n) >> 12);
   const int64_t blocks = int64_t(ext_limbs) * tiles * count;
   KsArgs a{};
@@ -612,6 +634,7 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
   a.lazy = lazy ? 1 : 0;
   a.own = own;
   a.own_stride = own_stride;
+  a.lazy_acc = lazy_accumulate ? 1 : 0;
   for (int p = 0; p < parts && p < kMaxFusedParts; p++) {
     a.start[p] = own ? starts[p] : 0;
     a.size[p] = own ? sizes[p] : 0;

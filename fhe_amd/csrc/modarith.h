@@ -156,7 +156,10 @@ FHE_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
   const uint64_t ll = uint64_t(a0) * b0;
   const uint64_t lh = uint64_t(a0) * b1 + (ll >> 32);
   const uint64_t hl = uint64_t(a1) * b0 + uint32_t(lh);
-  return uint64_t(a1) * b1 + (lh >> 32) + (hl >> 32);
+  // the last sum through 32-bit carry adds (a plain 64-bit add becomes the
+  // slower v_lshl_add_u64)
+  return uint64_t(add64(int64_t(uint64_t(a1) * b1 + (lh >> 32)),
+                        int64_t(hl >> 32)));
 }
 
 // Shoup product (a representative of w y mod q in [0, 2q)) for any

@@ -15,6 +15,8 @@
 // A pass covering bits [b, b+L) sees sub-transform elements
 //   i = (hi << (b+L)) | (j << b) | lo,   j in [0, 2^L)
 // and in both directions the twiddle offset is i >> (b + u + 1) for local bit u.
+#include <stdexcept>
+
 #include "kernels.h"
 #include "ntt_r16.h"
 
@@ -223,16 +225,46 @@ __device__ __forceinline__ void r16_pass(const PassArgs &a,
     dst[int64_t(cols_store ? 16 * r + m : r + 16 * m) << b] = v[m];
 }
 
-// Register-resident pass kernel.  Polynomials whose index modulo
-// a.lazy_period is below a.lazy_count take the lazy arithmetic of their
-// prime's size class; the others the exact one.
-template <bool INV, bool COLS>
+// Register-resident pass kernel with one arithmetic A per launch (a single
+// code path per kernel keeps the instruction footprint and the register
+// allocation to that path).  The launch covers the polynomials
+// poly = (j / sel_count) * sel_period + sel_offset + j % sel_count,
+// j < sel_polys, and the limbs l0 .. l0 + limbs_in_launch - 1.
+struct PolySelect {
+  int polys;   // selected polynomials
+  int period, count, offset;
+  int l0;      // first limb of the launch
+};
+
+template <bool INV, bool COLS, int A>
 __global__ __launch_bounds__(kBlock) void ntt_radix16_kernel(
-    PassArgs a, DeviceTables t, int polys, int tiles_per_limb, int swizzle) {
+    PassArgs a, DeviceTables t, PolySelect sel, int tiles_per_limb,
+    int swizzle) {
   __shared__ int64_t lds[kR16Lds];
   // block -> (limb l, tile, polynomial), polynomial fastest so blocks sharing
   // a prime's twiddles run together; with swizzle each of the 8 XCDs (blocks
   // are dealt round-robin to XCDs) gets one contiguous range of that order.
+  int64_t w = blockIdx.x;
+  if (swizzle) {
+    const int64_t per = int64_t(gridDim.x) >> 3;
+    w = int64_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  const int64_t j = w % sel.polys;
+  const int64_t rest = w / sel.polys;
+  const int tile = int(rest % tiles_per_limb);
+  const int l = sel.l0 + int(rest / tiles_per_limb);
+  const int64_t poly =
+      (j / sel.count) * sel.period + sel.offset + (j % sel.count);
+  r16_pass<INV, COLS, A>(a, t, lds, poly, tile, l);
+}
+
+// The same pass with the arithmetic chosen per block: polynomials whose index
+// modulo a.lazy_period is below a.lazy_count take the lazy arithmetic of
+// their prime's size class, the others the exact one (one launch for all).
+template <bool INV, bool COLS>
+__global__ __launch_bounds__(kBlock) void ntt_radix16_mixed_kernel(
+    PassArgs a, DeviceTables t, int polys, int tiles_per_limb, int swizzle) {
+  __shared__ int64_t lds[kR16Lds];
   int64_t w = blockIdx.x;
   if (swizzle) {
     const int64_t per = int64_t(gridDim.x) >> 3;
@@ -268,24 +300,85 @@ void launch_pass(bool inverse, const PassArgs &a, const DeviceTables &t,
     const int64_t total_limbs = a.total_subs >> (t.log_n - 8);
     const int polys = int(total_limbs / a.limbs);
     const int tiles = int(int64_t(1) << (t.log_n - 12));
-    const int64_t blocks = total_limbs * tiles;
-    const int swizzle = (blocks % 8 == 0) ? 1 : 0;
     const bool cols = a.low_bit > 0;
-#define FHE_R16(I, C)                                                          \
-  hipLaunchKernelGGL((ntt_radix16_kernel<I, C>), dim3(unsigned(blocks)),      \
-                     dim3(kBlock), 0, stream, a, t, polys, tiles, swizzle)
-    if (inverse) {
-      if (cols)
-        FHE_R16(true, true);
-      else
-        FHE_R16(true, false);
-    } else {
-      if (cols)
-        FHE_R16(false, true);
-      else
-        FHE_R16(false, false);
-    }
+    auto launch = [&](int arith, PolySelect sel, int limbs_in_launch) {
+      const int64_t blocks = int64_t(sel.polys) * tiles * limbs_in_launch;
+      if (blocks == 0)
+        return;
+      const int swizzle = (blocks % 8 == 0) ? 1 : 0;
+#define FHE_R16(I, C, A)                                                       \
+  hipLaunchKernelGGL((ntt_radix16_kernel<I, C, A>), dim3(unsigned(blocks)),   \
+                     dim3(kBlock), 0, stream, a, t, sel, tiles, swizzle)
+#define FHE_R16_A(I, C)                                                        \
+  do {                                                                         \
+    if (arith == kExact)                                                       \
+      FHE_R16(I, C, kExact);                                                   \
+    else if (arith == kLazySmall)                                              \
+      FHE_R16(I, C, kLazySmall);                                               \
+    else                                                                       \
+      FHE_R16(I, C, kLazyLarge);                                               \
+  } while (0)
+      if (inverse) {
+        if (cols)
+          FHE_R16_A(true, true);
+        else
+          FHE_R16_A(true, false);
+      } else {
+        if (cols)
+          FHE_R16_A(false, true);
+        else
+          FHE_R16_A(false, false);
+      }
+#undef FHE_R16_A
 #undef FHE_R16
+    };
+    if (!t.split_variants) {
+      const int64_t blocks = total_limbs * tiles;
+      const int swizzle = (blocks % 8 == 0) ? 1 : 0;
+#define FHE_R16M(I, C)                                                         \
+  hipLaunchKernelGGL((ntt_radix16_mixed_kernel<I, C>), dim3(unsigned(blocks)),\
+                     dim3(kBlock), 0, stream, a, t, polys, tiles, swizzle)
+      if (inverse) {
+        if (cols)
+          FHE_R16M(true, true);
+        else
+          FHE_R16M(true, false);
+      } else {
+        if (cols)
+          FHE_R16M(false, true);
+        else
+          FHE_R16M(false, false);
+      }
+#undef FHE_R16M
+      return;
+    }
+    const int period = a.lazy_period > 0 ? a.lazy_period : 1;
+    const int lazy_count = a.lazy_period > 0 ? a.lazy_count : 0;
+    // polynomials in whole periods (every caller passes whole periods)
+    if (polys % period != 0)
+      throw std::invalid_argument(
+          "NTT: the polynomial count is not a whole number of lazy periods");
+    const int periods = polys / period;
+    // exact polynomials: all limbs in one launch
+    if (lazy_count < period)
+      launch(kExact,
+             PolySelect{periods * (period - lazy_count), period,
+                        period - lazy_count, lazy_count, 0},
+             a.limbs);
+    // lazy polynomials: one launch per run of limbs of one prime class
+    if (lazy_count > 0) {
+      int l = 0;
+      while (l < a.limbs) {
+        const bool small = t_host_q_small(t, a.moduli_id + l);
+        int e = l + 1;
+        while (e < a.limbs && t_host_q_small(t, a.moduli_id + e) == small)
+          e++;
+        launch(small ? kLazySmall : kLazyLarge,
+               PolySelect{periods * lazy_count, period, lazy_count, 0, l},
+               e - l);
+        l = e;
+      }
+    }
     return;
   }
   const int C = kTile >> a.bits;

@@ -37,6 +37,11 @@ enum NttArith : int { kExact = 0, kLazySmall = 1, kLazyLarge = 2 };
 // Primes below this bound take the kLazySmall forms.
 constexpr uint64_t kLazySmallBound = uint64_t(1) << 42;
 
+// Host side: whether prime p takes the kLazySmall forms.
+inline bool t_host_q_small(const DeviceTables &t, int p) {
+  return t.host_q[p] < kLazySmallBound;
+}
+
 // Operands are lazy residues in [0, 2q), so every guard sees a non-negative
 // value.  Cooley-Tukey: p = mont(w, y); x' = guard(x + p, 2q);
 // y' = guard(x + 2q - p, 2q)
@@ -82,12 +87,12 @@ __device__ __forceinline__ void fwd_bfly(int64_t &x, int64_t &y,
     return;
   }
   const uint64_t w = s.ts[2 * idx], wq = s.ts[2 * idx + 1];
-  uint64_t x0 = uint64_t(x);
-  if (A == kLazyLarge && x0 >= uint64_t(s.q2))
-    x0 -= uint64_t(s.q2);
-  const uint64_t t = shoup_mult(uint64_t(y), w, wq, s.q);
-  x = int64_t(x0 + t);
-  y = int64_t(x0 + uint64_t(s.q2) - t);
+  int64_t x0 = x;
+  if (A == kLazyLarge)
+    x0 = guard_u(x0, s.q2);
+  const int64_t t = int64_t(shoup_mult(uint64_t(y), w, wq, s.q));
+  x = add64(x0, t);
+  y = sub64(add64(x0, s.q2), t);
 }
 
 // One inverse butterfly at global stage t.
@@ -100,17 +105,15 @@ __device__ __forceinline__ void inv_bfly(int64_t &x, int64_t &y,
     return;
   }
   const uint64_t w = s.ts[2 * idx], wq = s.ts[2 * idx + 1];
-  const uint64_t x0 = uint64_t(x), y0 = uint64_t(y);
-  uint64_t sum = x0 + y0;
-  uint64_t off = uint64_t(s.q2);
-  if (A == kLazyLarge) {
-    if (sum >= uint64_t(s.q2))
-      sum -= uint64_t(s.q2);
-  } else {
-    off = s.q << (1 + t);
-  }
-  x = int64_t(sum);
-  y = int64_t(shoup_mult(x0 + off - y0, w, wq, s.q));
+  int64_t sum = add64(x, y);
+  int64_t off = s.q2;
+  if (A == kLazyLarge)
+    sum = guard_u(sum, s.q2);
+  else
+    off = int64_t(s.q << (1 + t));
+  const int64_t d = sub64(add64(x, off), y);
+  x = sum;
+  y = int64_t(shoup_mult(uint64_t(d), w, wq, s.q));
 }
 
 // forward local bits 7..4, ROWS ownership

@@ -323,6 +323,16 @@ int fhe_decoded_scale(const fhe_context *ctx, int moduli_id,
                       long double ckks_scale, int *decoded_moduli_id,
                       long double *decoded_scale);
 
+/* Pipeline choices of a context (no counterpart in the reference: they
+ * select among equivalent device implementations, all bit-exact), for tuning
+ * and A/B measurement.  Names and defaults: "lazy_key_switch" 1 (lazy
+ * transforms where results are canonicalized later), "staged_modup" 1,
+ * "fuse_modup" 0, "fuse_tensor" 0, "own_limbs" 1 (partitions' own extended
+ * limbs from the NTT-form input), "split_variants" 1 (one NTT launch per
+ * arithmetic variant), "ks_lazy_accumulate" 1 (small-prime key products
+ * summed without intermediate reductions).  Synchronizes the context's stream. */
+int fhe_context_set_option(fhe_context *ctx, const char *name, int value);
+
 /* Stage timing (no counterpart in the reference, which has no device
  * stages): while enabled, every pipeline stage that a call launches on the
  * context's stream is bracketed by HIP events; fhe_profile_count / _entry
