@@ -108,6 +108,10 @@ struct fhe_context {
   bool own_limbs = true;
   // small-prime key products accumulated without guards, reduced once
   bool ks_lazy_accumulate = true;
+  // key inner product loads the next partition's tile ahead
+  bool ks_prefetch = false;
+  // waves per SIMD targeted by the key inner product's register allocation
+  int ks_occupancy = 2;
   // second growable device buffer (Garner digits of the staged ModUp)
   void *aux = nullptr;
   size_t aux_bytes = 0;
@@ -472,7 +476,8 @@ struct fhe_context {
       fhe::launch_ks_inner_product_fused(
           tables, ext, key, key_chain, key_offset, parts[0].key_index, np, acc,
           count, ext_limbs, moduli_id, lazy_key_switch, own, own_stride,
-          starts.data(), sizes.data(), ks_lazy_accumulate, stream);
+          starts.data(), sizes.data(), ks_lazy_accumulate, ks_prefetch,
+          ks_occupancy, stream);
     });
     stage("intt_pass_high(acc)", [&] {
       fhe::ntt_inverse_second(tables, acc, acc, int64_t(count) * 2 * ext_limbs,
@@ -608,6 +613,10 @@ int fhe_context_set_option(fhe_context *ctx, const char *name, int value) {
       ctx->fuse_tensor = value != 0;
     else if (n == "own_limbs")
       ctx->own_limbs = value != 0;
+    else if (n == "ks_occupancy")
+      ctx->ks_occupancy = value;
+    else if (n == "ks_prefetch")
+      ctx->ks_prefetch = value != 0;
     else if (n == "ks_lazy_accumulate")
       ctx->ks_lazy_accumulate = value != 0;
     else if (n == "split_variants")

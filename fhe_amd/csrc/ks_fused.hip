@@ -47,7 +47,7 @@ struct KsArgs {
 // two transforms (the key products and the accumulation are the same in
 // every form: Montgomery products of operands whose product stays below
 // q * 2^62, accumulated with the reference's guards).
-template <int A>
+template <int A, bool PF>
 __device__ __forceinline__ void ks_tile(const KsArgs &a, const DeviceTables &t,
                                         int64_t *lds, int ct, int tile,
                                         int i) {
@@ -72,23 +72,48 @@ __device__ __forceinline__ void ks_tile(const KsArgs &a, const DeviceTables &t,
   f.q2 = int64_t(t.q2[p]);
   const int64_t qq = int64_t(f.q);
 
+  // block-uniform: every thread of the block has the same (i, pp)
+  auto is_own = [&](int pp) {
+    return a.own != nullptr && i >= a.start[pp] && i < a.start[pp] + a.size[pp];
+  };
+  auto ext_tile = [&](int pp) {
+    return a.ext + ((int64_t(ct) * a.parts + pp) * a.ext_limbs + i) * n + off;
+  };
   int64_t acc0[16], acc1[16], v[16];
+  // with PF the next partition's extended tile is loaded while the current
+  // one is transformed and multiplied
+  int64_t nx[PF ? 16 : 1];
+  if (PF && !is_own(0)) {
+    const int64_t *src = ext_tile(0);
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      nx[m] = src[r + 16 * m];
+  }
   for (int pp = 0; pp < a.parts; pp++) {
-    // block-uniform: every thread of the block has the same (i, pp)
-    const bool own =
-        a.own != nullptr && i >= a.start[pp] && i < a.start[pp] + a.size[pp];
+    const bool own = is_own(pp);
     if (own) {
       const int64_t *src =
           a.own + int64_t(ct) * a.own_stride + int64_t(i) * n + off + 16 * r;
 #pragma unroll
       for (int m = 0; m < 16; m++)
         v[m] = src[m];
+    } else if (PF) {
+#pragma unroll
+      for (int m = 0; m < 16; m++)
+        v[m] = nx[m];
     } else {
-      const int64_t *src =
-          a.ext + ((int64_t(ct) * a.parts + pp) * a.ext_limbs + i) * n + off;
+      const int64_t *src = ext_tile(pp);
 #pragma unroll
       for (int m = 0; m < 16; m++)
         v[m] = src[r + 16 * m];
+    }
+    if (PF && pp + 1 < a.parts && !is_own(pp + 1)) {
+      const int64_t *src = ext_tile(pp + 1);
+#pragma unroll
+      for (int m = 0; m < 16; m++)
+        nx[m] = src[r + 16 * m];
+    }
+    if (!own) {
       r16_forward_high<A>(v, f);
       r16_exchange(v, lds, c, r, true);
       r16_forward_low<A>(v, f);
@@ -159,8 +184,8 @@ __device__ __forceinline__ void ks_tile(const KsArgs &a, const DeviceTables &t,
     d1[r + 16 * m] = acc1[m];
 }
 
-__global__ __launch_bounds__(kBlock) void ks_inner_product_r16_kernel(
-    KsArgs a, DeviceTables t) {
+template <bool PF>
+__device__ __forceinline__ void ks_block(const KsArgs &a, const DeviceTables &t) {
   __shared__ int64_t lds[kR16Lds];
   const int64_t n = int64_t(1) << t.log_n;
   const int tiles = int(n >> 12);
@@ -176,11 +201,11 @@ __global__ __launch_bounds__(kBlock) void ks_inner_product_r16_kernel(
   const int i = int(rest / tiles);
   const uint64_t q = t.q[a.moduli_id + i];
   if (!a.lazy)
-    ks_tile<kExact>(a, t, lds, ct, tile, i);
+    ks_tile<kExact, PF>(a, t, lds, ct, tile, i);
   else if (q < kLazySmallBound)
-    ks_tile<kLazySmall>(a, t, lds, ct, tile, i);
+    ks_tile<kLazySmall, PF>(a, t, lds, ct, tile, i);
   else
-    ks_tile<kLazyLarge>(a, t, lds, ct, tile, i);
+    ks_tile<kLazyLarge, PF>(a, t, lds, ct, tile, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -473,6 +498,19 @@ __global__ __launch_bounds__(kBlock) void extend_ntt_r16_kernel(
   }
 }
 
+template <bool PF>
+__global__ __launch_bounds__(kBlock) void ks_inner_product_r16_kernel(
+    KsArgs a, DeviceTables t) {
+  ks_block<PF>(a, t);
+}
+
+// The same with the register allocation held to three waves per SIMD (part
+// of the loop's registers spill to scratch in exchange for occupancy).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+void ks_inner_product_r16_occ3_kernel(KsArgs a, DeviceTables t) {
+  ks_block<false>(a, t);
+}
+
 } // namespace
 
 bool ks_fused_supported(const DeviceTables &t) { return t.log_n == 16; }
@@ -616,7 +654,8 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
                                    int ext_limbs, int moduli_id, bool lazy,
                                    const int64_t *own, int64_t own_stride,
                                    const int *starts, const int *sizes,
-                                   bool lazy_accumulate, hipStream_t stream) {
+                                   bool lazy_accumulate, bool prefetch,
+                                   int occupancy, hipStream_t stream) {
   const int tiles = int((int64_t(1) << t.log_n) >> 12);
   const int64_t blocks = int64_t(ext_limbs) * tiles * count;
   KsArgs a{};
@@ -639,8 +678,15 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
     a.start[p] = own ? starts[p] : 0;
     a.size[p] = own ? sizes[p] : 0;
   }
-  hipLaunchKernelGGL(ks_inner_product_r16_kernel, dim3(unsigned(blocks)),
-                     dim3(kBlock), 0, stream, a, t);
+  if (prefetch)
+    hipLaunchKernelGGL(ks_inner_product_r16_kernel<true>,
+                       dim3(unsigned(blocks)), dim3(kBlock), 0, stream, a, t);
+  else if (occupancy >= 3)
+    hipLaunchKernelGGL(ks_inner_product_r16_occ3_kernel,
+                       dim3(unsigned(blocks)), dim3(kBlock), 0, stream, a, t);
+  else
+    hipLaunchKernelGGL(ks_inner_product_r16_kernel<false>,
+                       dim3(unsigned(blocks)), dim3(kBlock), 0, stream, a, t);
 }
 
 } // namespace fhe
