@@ -32,7 +32,8 @@ EXPORTED_SYMBOLS = (
     "fhe_multiply_int", "fhe_add_double", "fhe_level_down",
     "fhe_encrypt_pk", "fhe_decrypt", "fhe_rotate_batch", "fhe_switch_key",
     "fhe_encode", "fhe_params_encode_constants", "fhe_decode",
-    "fhe_decoded_scale", "fhe_context_set_option",
+    "fhe_decoded_scale", "fhe_context_set_option", "fhe_multiply_imaginary",
+    "fhe_mod_raise",
 )
 
 
@@ -127,6 +128,9 @@ def library():
             i32, vp, i32, ctypes.POINTER(i32)]
         lib.fhe_encode.argtypes = [vp, vp, i32, i32, i32, vp]
         lib.fhe_context_set_option.argtypes = [vp, ctypes.c_char_p, i32]
+        lib.fhe_multiply_imaginary.argtypes = [vp, i32, i32, vp,
+                                               ctypes.c_int64, vp]
+        lib.fhe_mod_raise.argtypes = [vp, i32, vp, vp, i32, vp, i32, vp]
         lib.fhe_decode.argtypes = [vp, vp, i32, i32, ctypes.c_longdouble,
                                    i32, vp]
         lib.fhe_decoded_scale.argtypes = [vp, i32, ctypes.c_longdouble,
@@ -509,6 +513,19 @@ class Context:
                                            ctypes.byref(mid),
                                            scale.ctypes.data))
         return mid.value, scale[0]
+
+    def multiply_imaginary(self, cts, moduli_id, value, out):
+        count = int(np.prod(cts.shape[:-3]))
+        _check(library().fhe_multiply_imaginary(self.handle, count, moduli_id,
+                                                cts.ptr, int(value), out.ptr))
+
+    def mod_raise(self, cts, to_sparse, sparse_moduli_id, to_dense,
+                  dense_moduli_id, out):
+        count = int(np.prod(cts.shape[:-3]))
+        _check(library().fhe_mod_raise(self.handle, count, cts.ptr,
+                                       to_sparse.ptr, sparse_moduli_id,
+                                       to_dense.ptr, dense_moduli_id,
+                                       out.ptr))
 
     def set_option(self, name, value):
         """fhe_context_set_option: choose among equivalent pipelines."""

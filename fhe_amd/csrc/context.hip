@@ -115,6 +115,10 @@ struct fhe_context {
   // second growable device buffer (Garner digits of the staged ModUp)
   void *aux = nullptr;
   size_t aux_bytes = 0;
+  // third growable buffer: intermediates of composite operations whose steps
+  // use the other two
+  void *stash_buf = nullptr;
+  size_t stash_bytes = 0;
   // encoding tables, uploaded on the first encode
   fhe::EncodeDevice encode{};
   bool encode_ready = false;
@@ -175,6 +179,8 @@ struct fhe_context {
       hipFree(workspace);
     if (aux)
       hipFree(aux);
+    if (stash_buf)
+      hipFree(stash_buf);
     if (stream)
       hipStreamDestroy(stream);
   }
@@ -236,6 +242,40 @@ struct fhe_context {
       aux_bytes = bytes;
     }
     return static_cast<int64_t *>(aux);
+  }
+
+  int64_t *stash(size_t bytes) {
+    if (bytes > stash_bytes) {
+      hip_check(hipStreamSynchronize(stream), "workspace sync");
+      if (stash_buf)
+        hip_check(hipFree(stash_buf), "workspace free");
+      stash_buf = nullptr;
+      stash_bytes = 0;
+      hip_check(hipMalloc(&stash_buf, bytes), "workspace allocation");
+      stash_bytes = bytes;
+    }
+    return static_cast<int64_t *>(stash_buf);
+  }
+
+  // CKKSCore::switch_key (src/ckks/core.cpp:1058-1083) of `count` INTT-form
+  // duos at moduli_id: key switching of c1, then c0 added to the first half
+  // and the second half copied.  Uses the scratch workspace.
+  void switch_key_into(int count, int moduli_id, const int64_t *in,
+                       const int64_t *key, int key_moduli_id, int64_t *out,
+                       bool signed_input = false) {
+    const int limbs = params.limbs_at(moduli_id);
+    const int ext_limbs = limbs + params.alpha;
+    const int64_t N = n();
+    const int64_t ext_l = int64_t(count) * ext_slots(moduli_id) * ext_limbs;
+    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
+    int64_t *ws = scratch(size_t(ext_l + acc_l) * N * 8);
+    int64_t *ext = ws, *acc = ext + ext_l * N;
+    key_switch(count, moduli_id, key, key_moduli_id, in + limbs * N,
+               2 * limbs * N, ext, acc, nullptr, 0, signed_input);
+    stage("moddown_add", [&] {
+      fhe::launch_moddown_add(tables, acc, in, 2 * limbs * N, 1, out, count,
+                              limbs, moduli_id, stream);
+    });
   }
 
   int64_t *scratch(size_t bytes) {
@@ -339,16 +379,19 @@ struct fhe_context {
   // own (optional): the NTT form of the same polynomials (limb i of
   // ciphertext c at own + c * own_stride + i * N); the fused path then takes
   // each partition's own extended limbs from it instead of transforming them.
+  // signed_input: the source limbs may be any int64 (the signed tile of
+  // modulus raising), not residues in [0, 2q): every partition then takes
+  // the signed Garner digits and the exact extension.
   void key_switch(int count, int moduli_id, const int64_t *key,
                   int key_moduli_id, const int64_t *src, int64_t src_stride,
                   int64_t *ext, int64_t *acc, const int64_t *own = nullptr,
-                  int64_t own_stride = 0) {
+                  int64_t own_stride = 0, bool signed_input = false) {
     if (fhe::ks_fused_supported(tables)) {
       if (!own_limbs || int(params.partitions(moduli_id, key_moduli_id)
                                 .size()) > fhe::kMaxFusedParts)
         own = nullptr;
       ks_decompose(count, moduli_id, key_moduli_id, src, src_stride, ext,
-                   own != nullptr);
+                   own != nullptr, signed_input);
       ks_apply(count, moduli_id, key, key_moduli_id, ext, acc, own,
                own_stride);
       return;
@@ -366,8 +409,9 @@ struct fhe_context {
       fhe::launch_modup(tables, src, src_stride, ext, ext_limbs * N, count,
                         limbs, moduli_id, part.start, part.size,
                         cst + p * stride, params.ks_ext_shoup_offset(moduli_id),
-                        params.modup_non_negative(moduli_id, part), false,
-                        stream);
+                        !signed_input &&
+                            params.modup_non_negative(moduli_id, part),
+                        false, stream);
       fhe::ntt_forward(tables, ext, ext, int64_t(count) * ext_limbs,
                        ext_limbs, moduli_id, nullptr, stream);
       const int64_t *key0 =
@@ -391,7 +435,7 @@ struct fhe_context {
   // so with lazy_key_switch their arithmetic is lazy.
   void ks_decompose(int count, int moduli_id, int key_moduli_id,
                     const int64_t *src, int64_t src_stride, int64_t *ext,
-                    bool skip_own = false) {
+                    bool skip_own = false, bool signed_input = false) {
     const int limbs = params.limbs_at(moduli_id);
     const int ext_limbs = limbs + params.alpha;
     const int64_t N = n();
@@ -408,7 +452,8 @@ struct fhe_context {
       starts[p] = parts[p].start;
       sizes[p] = parts[p].size;
       if (p < fhe::kMaxFusedParts)
-        is_signed[p] = !params.modup_non_negative(moduli_id, parts[p]);
+        is_signed[p] = signed_input ||
+                       !params.modup_non_negative(moduli_id, parts[p]);
       max_size = std::max(max_size, sizes[p]);
     }
     if (staged_modup && fhe::staged_modup_supported(tables, max_size, np)) {
@@ -442,8 +487,9 @@ struct fhe_context {
                           int64_t(np) * ext_limbs * N, count, limbs, moduli_id,
                           parts[p].start, parts[p].size, cst + p * stride,
                           params.ks_ext_shoup_offset(moduli_id),
-                          params.modup_non_negative(moduli_id, parts[p]),
-                          lazy_key_switch, stream);
+                          !signed_input &&
+                              params.modup_non_negative(moduli_id, parts[p]),
+                          lazy_key_switch && !signed_input, stream);
       });
     stage("ntt_pass_high(ext)", [&] {
       fhe::ntt_forward_first(tables, ext, ext, int64_t(count) * np * ext_limbs,
@@ -1395,24 +1441,103 @@ int fhe_switch_key(fhe_context *ctx, int count, int moduli_id,
     require(count > 0, "Switch key: empty batch");
     ctx->check_moduli(moduli_id, 1, "Switch key");
     ctx->check_key(moduli_id, key_moduli_id, "Switch key");
+    ctx->switch_key_into(count, moduli_id, in, key, key_moduli_id, out);
+    ctx->check_launch("Switch key");
+  });
+}
+
+int fhe_multiply_imaginary(fhe_context *ctx, int count, int moduli_id,
+                           const int64_t *in, int64_t value, int64_t *out) {
+  return guarded([&] {
+    require(ctx && in && out, "null argument");
+    require(count > 0, "Multiply imaginary: empty batch");
+    ctx->check_moduli(moduli_id, 1, "Multiply imaginary");
     const auto &P = ctx->params;
     const int limbs = P.limbs_at(moduli_id);
-    const int ext_limbs = limbs + P.alpha;
     const int64_t N = ctx->n();
-    const int64_t ext_l =
-        int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
-    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
-    int64_t *ws = ctx->scratch(size_t(ext_l + acc_l) * N * 8);
-    int64_t *ext = ws, *acc = ext + ext_l * N;
-    // CKKSCore::switch_key (src/ckks/core.cpp:1058-1083): key switching of
-    // c1, then c0 added to the first half and the second half copied
-    ctx->key_switch(count, moduli_id, key, key_moduli_id, in + limbs * N,
-                    2 * limbs * N, ext, acc);
-    ctx->stage("moddown_add", [&] {
-      fhe::launch_moddown_add(ctx->tables, acc, in, 2 * limbs * N, 1, out,
-                              count, limbs, moduli_id, ctx->stream);
+    const int64_t poly = int64_t(limbs) * N;
+    // the plaintext value * X^(N/2) (CKKSCore::encode_imaginary,
+    // src/ckks/core_encode.cpp:190-210), tiled, entered and transformed
+    int64_t *sample = ctx->stash(size_t(N + poly) * 8);
+    int64_t *pt = sample + N;
+    std::vector<int64_t> host(size_t(N), 0);
+    host[size_t(N / 2)] = value;
+    hip_check(hipMemcpyAsync(sample, host.data(), size_t(N) * 8,
+                             hipMemcpyHostToDevice, ctx->stream),
+              "plaintext upload");
+    ctx->stage("multiply_imaginary", [&] {
+      fhe::launch_tile_unsigned(ctx->tables, sample, pt, 1, limbs, moduli_id,
+                                ctx->stream);
+      fhe::ntt_forward(ctx->tables, pt, pt, limbs, limbs, moduli_id,
+                       ctx->tables.r2, ctx->stream);
+      // ciphertext to NTT form, product, back to INTT form
+      // (src/ckks/engine_multiply.cpp:468-491)
+      fhe::ntt_forward(ctx->tables, in, out, int64_t(count) * 2 * limbs, limbs,
+                       moduli_id, ctx->tables.r2, ctx->stream);
+      for (int c = 0; c < 2 * count; c++)
+        fhe::launch_mont_mult(ctx->tables, out + c * poly, pt, out + c * poly,
+                              limbs, limbs, moduli_id, ctx->stream);
+      fhe::ntt_inverse(ctx->tables, out, out, int64_t(count) * 2 * limbs,
+                       limbs, moduli_id, fhe::kInttExit, ctx->stream);
     });
-    ctx->check_launch("Switch key");
+    // the host vector must outlive the asynchronous copy
+    hip_check(hipStreamSynchronize(ctx->stream), "plaintext upload");
+    ctx->check_launch("Multiply imaginary");
+  });
+}
+
+int fhe_mod_raise(fhe_context *ctx, int count, const int64_t *in,
+                  const int64_t *to_sparse, int sparse_moduli_id,
+                  const int64_t *to_dense, int dense_moduli_id,
+                  int64_t *out) {
+  return guarded([&] {
+    require(ctx && in && to_sparse && to_dense && out, "null argument");
+    require(count > 0, "Mod raise: empty batch");
+    const auto &P = ctx->params;
+    // input: level 0, not rescaled (moduli_id full_level, two limbs), the
+    // form CKKSEngine::mod_raise hands over after level_down(ct, 0); the
+    // rescale leaves the base prime alone
+    const int low = P.full_level + 1;
+    const int high = P.moduli_id_of_level(P.max_key_switching_level());
+    require(low < P.chain_count, "Mod raise: no level below level 0");
+    ctx->check_key(low, sparse_moduli_id, "Mod raise");
+    ctx->check_key(high, dense_moduli_id, "Mod raise");
+    const int64_t N = ctx->n();
+    const int high_limbs = P.limbs_at(high);
+    // stash: the rescaled duo, the switched duo, its signed limbs (one limb
+    // each) and the tiled duo
+    int64_t *r1 = ctx->stash(size_t(count) * (2 + 2 + 2 + 2 * high_limbs) *
+                             N * 8);
+    int64_t *sw = r1 + 2 * int64_t(count) * N;
+    int64_t *sg = sw + 2 * int64_t(count) * N;
+    int64_t *tiled = sg + 2 * int64_t(count) * N;
+    // CKKSEngine::mod_raise_ (src/ckks/engine_mod_raise.cpp:5-34)
+    ctx->stage("rescale", [&] {
+      fhe::launch_rescale(ctx->tables, in, r1, 2 * count, 2, P.full_level,
+                          ctx->stream);
+    });
+    ctx->switch_key_into(count, low, r1, to_sparse, sparse_moduli_id, sw);
+    // CKKSCore::make_signed_tile (src/ckks/core.cpp:1085-1110)
+    ctx->stage("elementwise", [&] {
+      fhe::launch_elementwise(ctx->tables, fhe::kEwMakeSigned, sw, nullptr, sg,
+                              2 * int64_t(count), 1, low, nullptr,
+                              ctx->stream);
+      fhe::launch_tile_unsigned(ctx->tables, sg, tiled, 2 * count, high_limbs,
+                                high, ctx->stream);
+    });
+    // the tiled limbs are signed values, not residues
+    ctx->switch_key_into(count, high, tiled, to_dense, dense_moduli_id, out,
+                         true);
+    // multiply by the integer ratio (CKKSCore::get_mod_raise_ratio,
+    // src/ckks/core.cpp:1112-1119)
+    const auto c = P.mult_multiplier_int(P.mod_raise_ratio(), high);
+    require(int(c.size()) <= fhe::kMaxEwLimbs, "Mod raise: too many limbs");
+    ctx->stage("elementwise", [&] {
+      fhe::launch_elementwise(ctx->tables, fhe::kEwEnterReduce, out, nullptr,
+                              out, 2 * int64_t(count) * high_limbs, high_limbs,
+                              high, c.data(), ctx->stream);
+    });
+    ctx->check_launch("Mod raise");
   });
 }
 

@@ -66,6 +66,12 @@ __device__ __forceinline__ int64_t extend_limb(const int64_t (&d)[MAXZ],
     if (j < size)
       y = guard_fast(
           add64(modup_mult<SIGNED>(d[j], ec[j * ext_limbs + i], q, k), y), q2);
+  // a negative digit can leave y in (-q, 0): the extended limbs only reach
+  // the (unsigned) transforms and the key products, whose results are
+  // canonicalized before use, so the non-negative representative y + 2q
+  // serves in its place
+  if (SIGNED && y < 0)
+    y += q2;
   return y;
 }
 

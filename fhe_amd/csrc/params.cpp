@@ -245,6 +245,14 @@ CkksParams CkksParams::from_primes(int log_n, const std::vector<uint64_t> &prime
   return p;
 }
 
+int64_t CkksParams::mod_raise_ratio() const {
+  const double s = scale;
+  const double partial_sum_ratio = 1; // full slots
+  // the base prime: the last prime of the ciphertext chain
+  const double base = static_cast<double>(q[chain_count - 1]);
+  return static_cast<int64_t>(std::round(s * s / (partial_sum_ratio * base)));
+}
+
 int CkksParams::max_key_switching_level() const {
   if (one_more_level && max_level == full_level)
     return full_level - 1;

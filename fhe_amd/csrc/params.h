@@ -95,6 +95,8 @@ public:
     return rescaled ? full_level - moduli_id + 1 : full_level - moduli_id;
   }
   int max_key_switching_level() const;
+  // CKKSCore::get_mod_raise_ratio (src/ckks/core.cpp:1112-1119), full slots
+  int64_t mod_raise_ratio() const;
   int partition_count(int moduli_id) const {
     return (limbs_at(moduli_id) + alpha - 1) / alpha;
   }

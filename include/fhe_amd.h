@@ -323,6 +323,32 @@ int fhe_decoded_scale(const fhe_context *ctx, int moduli_id,
                       long double ckks_scale, int *decoded_moduli_id,
                       long double *decoded_scale);
 
+/* Multiplication of `count` INTT-form duos at moduli_id by value * i, the
+ * imaginary unit times an integer (CKKSEngine::multiply_imaginary,
+ * src/ckks/engine_multiply.cpp:468-491): the ciphertext and the plaintext
+ * value * X^(N/2) (CKKSCore::encode_imaginary, src/ckks/core_encode.cpp:
+ * 190-210) are taken to NTT form, multiplied and brought back.  The scale is
+ * unchanged.  out may alias in. */
+int fhe_multiply_imaginary(fhe_context *ctx, int count, int moduli_id,
+                           const int64_t *in, int64_t value, int64_t *out);
+
+/* Modulus raising of `count` INTT-form duos at level 0 for bootstrapping
+ * (CKKSEngine::mod_raise_, src/ckks/engine_mod_raise.cpp:5-34), on the
+ * not-rescaled level-0 form that CKKSEngine::mod_raise passes after
+ * level_down(ct, 0) (moduli_id full_level, two limbs): rescale to the base
+ * prime, switch to the sparse secret with to_sparse (a key at
+ * sparse_moduli_id), lift the
+ * signed residues onto every prime of the maximum key-switching level
+ * (CKKSCore::make_signed_tile, src/ckks/core.cpp:1085-1110), switch back
+ * with to_dense (at dense_moduli_id) and multiply by the integer ratio
+ * (CKKSCore::get_mod_raise_ratio, src/ckks/core.cpp:1112-1119).  out:
+ * [count][2][limbs][N] at the moduli_id of the maximum key-switching level;
+ * its scale is scale^2 * ratio. */
+int fhe_mod_raise(fhe_context *ctx, int count, const int64_t *in,
+                  const int64_t *to_sparse, int sparse_moduli_id,
+                  const int64_t *to_dense, int dense_moduli_id,
+                  int64_t *out);
+
 /* Pipeline choices of a context (no counterpart in the reference: they
  * select among equivalent device implementations, all bit-exact), for tuning
  * and A/B measurement.  Names and defaults: "lazy_key_switch" 1 (lazy

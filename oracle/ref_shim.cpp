@@ -339,6 +339,10 @@ static Data *ksk_of(void *key, int kind) {
     return static_cast<ConjugationKey *>(key)->key_switching_key_;
   if (kind == 3)
     return static_cast<SecretKey *>(key);
+  if (kind == 5)
+    return static_cast<SmallBootstrapKey *>(key)->to_sparse_;
+  if (kind == 6)
+    return static_cast<SmallBootstrapKey *>(key)->to_dense_;
   return static_cast<PublicKey *>(key);
 }
 // kind: 0 relinearization, 1 fixed rotation, 2 conjugation, 3 secret key,
@@ -698,6 +702,62 @@ int ref_rescale_plaintext(void *handle, const int64_t *limbs, int level,
     delete r;
     return 0;
   });
+}
+
+// CKKSEngine::create_small_bootstrap_key (src/ckks/engine.cpp:140-160):
+// key-switching keys to the sparse secret (at level 0) and back (at the
+// maximum key-switching level); kinds 5 and 6 of ref_key_*.
+void *ref_small_bootstrap_key(void *handle, void *sk) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    return ref->engine->create_small_bootstrap_key(static_cast<SecretKey *>(sk));
+  });
+}
+void ref_free_small_bootstrap_key(void *p) {
+  delete static_cast<SmallBootstrapKey *>(p);
+}
+
+// CKKSEngine::mod_raise_ (src/ckks/engine_mod_raise.cpp:5-34) of a
+// ciphertext already at level 0.
+void *ref_mod_raise(void *handle, void *ct, void *sbk) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    return ref->engine->mod_raise_(static_cast<UnitCiphertext *>(ct),
+                                   static_cast<SmallBootstrapKey *>(sbk));
+  });
+}
+
+// The steps of CKKSEngine::mod_raise_ one at a time through CKKSCore
+// (step 0: rescale, 1: switch_key to the sparse secret, 2: make_signed_tile,
+// 3: switch_key back to the dense secret); returns the step's output.
+void *ref_mod_raise_step(void *handle, void *ct, void *sbk, int step) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    CKKSCore core(ref->config);
+    auto c = static_cast<UnitCiphertext *>(ct);
+    auto k = static_cast<SmallBootstrapKey *>(sbk);
+    if (step == 0)
+      return core.rescale(c);
+    if (step == 1)
+      return core.switch_key(c, k->to_sparse_);
+    if (step == 2)
+      return core.make_signed_tile(c);
+    return core.switch_key(c, k->to_dense_);
+  });
+}
+
+// CKKSEngine::multiply_imaginary (src/ckks/engine_multiply.cpp:468-491).
+void *ref_multiply_imaginary(void *handle, void *ct, int64_t value) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    return ref->engine->multiply_imaginary(static_cast<UnitCiphertext *>(ct),
+                                           value);
+  });
+}
+
+// The flag of a ciphertext (DataFlag bits).
+int ref_ct_flag(void *ct) {
+  return int(static_cast<UnitCiphertext *>(ct)->flag_);
 }
 
 // The long double ckks_scale of a ciphertext.

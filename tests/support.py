@@ -373,6 +373,11 @@ class Reference:
             "ref_encode_constants": (i32, [VP, i32, VP, VP, VP, VP, VP]),
             "ref_pre_permutation": (i32, [VP, VP]),
             "ref_decode": (i32, [VP, VP, i32, VP, i32, VP]),
+            "ref_small_bootstrap_key": (VP, [VP, VP]),
+            "ref_free_small_bootstrap_key": (None, [VP]),
+            "ref_mod_raise": (VP, [VP, VP, VP]),
+            "ref_multiply_imaginary": (VP, [VP, VP, ctypes.c_int64]),
+            "ref_ct_flag": (i32, [VP]),
             "ref_ct_scale_ld": (i32, [VP, VP]),
         }
         for name, (res, args) in sigs.items():
