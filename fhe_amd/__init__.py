@@ -33,8 +33,12 @@ EXPORTED_SYMBOLS = (
     "fhe_encrypt_pk", "fhe_decrypt", "fhe_rotate_batch", "fhe_switch_key",
     "fhe_encode", "fhe_params_encode_constants", "fhe_decode",
     "fhe_decoded_scale", "fhe_context_set_option", "fhe_multiply_imaginary",
-    "fhe_mod_raise",
+    "fhe_mod_raise", "fhe_multiply", "fhe_multiply_plain",
 )
+
+# data flags of a text (FHE_FLAG_* in include/fhe_amd.h)
+FLAG_NTT_FORM = 2
+FLAG_RESCALED = 4
 
 
 # fhe_elementwise operation codes (FHE_EW_* in include/fhe_amd.h)
@@ -131,6 +135,10 @@ def library():
         lib.fhe_multiply_imaginary.argtypes = [vp, i32, i32, vp,
                                                ctypes.c_int64, vp]
         lib.fhe_mod_raise.argtypes = [vp, i32, vp, vp, i32, vp, i32, vp]
+        lib.fhe_multiply.argtypes = [vp, i32, vp, i32, i32, vp, i32, i32, vp,
+                                     i32, vp]
+        lib.fhe_multiply_plain.argtypes = [vp, i32, vp, i32, i32, i32, vp,
+                                           i32, i32, vp]
         lib.fhe_decode.argtypes = [vp, vp, i32, i32, ctypes.c_longdouble,
                                    i32, vp]
         lib.fhe_decoded_scale.argtypes = [vp, i32, ctypes.c_longdouble,
@@ -352,6 +360,27 @@ class Context:
         _check(library().fhe_hmult_relin_rescale(self.handle, count, moduli_id,
                                                  a.ptr, b.ptr, key.ptr,
                                                  key_moduli_id, out.ptr))
+
+    def multiply(self, a, a_level, a_flags, b, b_level, b_flags, out,
+                 key=None, key_moduli_id=0):
+        """fhe_multiply: duos a, b in any form (level, FLAG_* bits); out is
+        the NTT-form trio, or with a relinearization key the INTT-form duo,
+        at level min(a_level, b_level) - 1."""
+        count = int(np.prod(a.shape[:-3]))
+        _check(library().fhe_multiply(self.handle, count, a.ptr, a_level,
+                                      a_flags, b.ptr, b_level, b_flags,
+                                      key.ptr if key is not None else None,
+                                      key_moduli_id, out.ptr))
+
+    def multiply_plain(self, ct, ct_level, ct_flags, pt, pt_level, pt_flags,
+                       out):
+        """fhe_multiply_plain: duos or trios ct [count][polys][limbs][N]
+        times plaintexts pt [count][limbs][N]; out in NTT form."""
+        count = int(np.prod(ct.shape[:-3]))
+        polys = int(ct.shape[-3])
+        _check(library().fhe_multiply_plain(self.handle, count, ct.ptr, polys,
+                                            ct_level, ct_flags, pt.ptr,
+                                            pt_level, pt_flags, out.ptr))
 
     def automorphism(self, src, dst, moduli_id, delta, ntt_form=False):
         polys, limbs = _polys_limbs(src, self.n)

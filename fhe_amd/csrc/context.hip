@@ -2,6 +2,7 @@
 // declared in include/fhe_amd.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -336,6 +337,145 @@ struct fhe_context {
     d.k = h.k;
     d.words = h.words;
     return decode_crt_cache[moduli_id] = d;
+  }
+
+  // Level down of `polys` INTT-form polynomials at `level` (rescaled or not)
+  // to dst_level < level; out is not rescaled at dst_level.
+  void level_down_into(const int64_t *in, int64_t *out, int polys, int level,
+                       bool rescaled, int dst_level) {
+    const auto &P = params;
+    const int dst_mid = P.moduli_id_of_level(dst_level);
+    if (rescaled) {
+      // CKKSCore::level_down_after_rescale (src/ckks/core.cpp:743-779): the
+      // rescaled input sits at moduli_id_of_level(level) + 1
+      const int src_mid = P.moduli_id_of_level(level) + 1;
+      const auto c = P.level_down_multiplier_after_rescale(level, dst_level);
+      const int limbs = int(c.size());
+      require(limbs <= fhe::kMaxEwLimbs, "Level down: too many limbs");
+      const int64_t src_poly = int64_t(P.limbs_at(src_mid)) << P.log_n;
+      const int64_t dst_poly = int64_t(limbs) << P.log_n;
+      const int64_t offset = int64_t(dst_mid - src_mid) << P.log_n;
+      stage("elementwise", [&] {
+        for (int p = 0; p < polys; p++)
+          fhe::launch_elementwise(tables, fhe::kEwEnterReduce,
+                                  in + p * src_poly + offset, nullptr,
+                                  out + p * dst_poly, limbs, limbs, dst_mid,
+                                  c.data(), stream);
+      });
+    } else {
+      // CKKSCore::level_down (src/ckks/core.cpp:792-838)
+      const int src_mid = P.moduli_id_of_level(level);
+      const int next_mid = P.moduli_id_of_level(level - 1);
+      const auto c = P.level_down_multiplier(level, dst_level);
+      require(int(c.size()) <= fhe::kMaxEwLimbs, "Level down: too many limbs");
+      stage("scalar_rescale", [&] {
+        fhe::launch_scalar_rescale(tables, in, out, polys, int(c.size()),
+                                   src_mid, c.data(), dst_mid - next_mid,
+                                   stream);
+      });
+    }
+  }
+
+  // Chain position of a text at `level`: a rescaled text sits one modulus
+  // above the not-rescaled text of its level (the not_rescaled_levels_ /
+  // rescaled_levels_ maps, src/ckks/parameters.cpp:60-90).
+  int moduli_id_of(int level, bool rescaled) const {
+    return params.moduli_id_of_level(level) + (rescaled ? 1 : 0);
+  }
+
+  // CKKSEngine::prepare_for_multiply (src/ckks/engine_multiply.cpp:13-27)
+  // of `polys` polynomials at (level, flags) into NTT-form, rescaled
+  // polynomials at dst_level <= level: intt_ when the level changes,
+  // level_down_ (src/ckks/engine_level_down.cpp:5-37), rescale_
+  // (src/ckks/engine_rescale.cpp:5-33), ntt_ (src/ckks/engine_ntt.cpp:5-21).
+  // t0 and t1 hold polys x limbs(level, flags) x N each; out may not alias
+  // them.
+  void prepare_for_multiply_into(const int64_t *in, int64_t polys, int level,
+                                 int flags, int dst_level, int64_t *out,
+                                 int64_t *t0, int64_t *t1) {
+    const int64_t N = n();
+    bool rescaled = (flags & FHE_FLAG_RESCALED) != 0;
+    bool ntt = (flags & FHE_FLAG_NTT_FORM) != 0;
+    const int64_t *cur = in;
+    int64_t *bufs[2] = {t0, t1};
+    int next = 0;
+    auto target = [&] {
+      int64_t *b = bufs[next];
+      next ^= 1;
+      return b;
+    };
+    auto to_intt = [&] {
+      const int mid = moduli_id_of(level, rescaled);
+      const int limbs = params.limbs_at(mid);
+      int64_t *y = target();
+      stage("intt", [&] {
+        fhe::ntt_inverse(tables, cur, y, polys * limbs, limbs, mid,
+                         fhe::kInttExit, stream);
+      });
+      cur = y;
+      ntt = false;
+    };
+    if (level > dst_level) {
+      if (ntt)
+        to_intt();
+      int64_t *y = target();
+      level_down_into(cur, y, int(polys), level, rescaled, dst_level);
+      cur = y;
+      rescaled = false;
+      level = dst_level;
+    }
+    if (!rescaled) {
+      if (ntt)
+        to_intt();
+      const int mid = moduli_id_of(level, false);
+      int64_t *y = target();
+      stage("rescale", [&] {
+        fhe::launch_rescale(tables, cur, y, int(polys), params.limbs_at(mid),
+                            mid, stream);
+      });
+      cur = y;
+      rescaled = true;
+    }
+    const int mid = moduli_id_of(dst_level, true);
+    const int limbs = params.limbs_at(mid);
+    if (!ntt) {
+      stage("ntt", [&] {
+        fhe::ntt_forward(tables, cur, out, polys * limbs, limbs, mid,
+                         tables.r2, stream);
+      });
+    } else {
+      hip_check(hipMemcpyAsync(out, cur, size_t(polys * limbs) * N * 8,
+                               hipMemcpyDeviceToDevice, stream),
+                "copy");
+    }
+  }
+
+  // Relinearization of `count` NTT-form trios at moduli_id into INTT-form
+  // duos (CKKSCore::relinearize, src/ckks/core.cpp:713-741).  Uses the
+  // scratch workspace; the trios are not modified.
+  void relinearize_into(int count, int moduli_id, const int64_t *trios,
+                        const int64_t *key, int key_moduli_id, int64_t *out) {
+    const int limbs = params.limbs_at(moduli_id);
+    const int ext_limbs = limbs + params.alpha;
+    const int64_t N = n();
+    const int64_t trio_l = int64_t(count) * 3 * limbs;
+    const int64_t ext_l = int64_t(count) * ext_slots(moduli_id) * ext_limbs;
+    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
+    int64_t *ws = scratch(size_t(trio_l + ext_l + acc_l) * N * 8);
+    int64_t *trio = ws, *ext = trio + trio_l * N, *acc = ext + ext_l * N;
+    const fhe::LazySpec d01 =
+        lazy_key_switch ? fhe::LazySpec{3, 2} : fhe::LazySpec();
+    stage("intt(trio)", [&] {
+      fhe::ntt_inverse(tables, trios, trio, trio_l, limbs, moduli_id,
+                       fhe::kInttExit, stream, d01);
+    });
+    // the caller's NTT-form d2 provides the partitions' own extended limbs
+    key_switch(count, moduli_id, key, key_moduli_id, trio + 2 * limbs * N,
+               3 * limbs * N, ext, acc, trios + 2 * limbs * N, 3 * limbs * N);
+    stage("moddown_relin", [&] {
+      fhe::launch_moddown_relin(tables, acc, trio, out, count, limbs,
+                                moduli_id, stream);
+    });
   }
 
   const int64_t *modup_constants(int moduli_id, int key_moduli_id) {
@@ -778,30 +918,7 @@ int fhe_relinearize(fhe_context *ctx, int count, int moduli_id,
     require(count > 0, "Relinearize: empty batch");
     ctx->check_moduli(moduli_id, 1, "Relinearize");
     ctx->check_key(moduli_id, key_moduli_id, "Relinearize");
-    const auto &P = ctx->params;
-    const int limbs = P.limbs_at(moduli_id);
-    const int ext_limbs = limbs + P.alpha;
-    const int64_t N = ctx->n();
-    const int64_t trio_l = int64_t(count) * 3 * limbs;
-    const int64_t ext_l =
-        int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
-    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
-    int64_t *ws = ctx->scratch(size_t(trio_l + ext_l + acc_l) * N * 8);
-    int64_t *trio = ws, *ext = trio + trio_l * N, *acc = ext + ext_l * N;
-    const fhe::LazySpec d01 =
-        ctx->lazy_key_switch ? fhe::LazySpec{3, 2} : fhe::LazySpec();
-    ctx->stage("intt(trio)", [&] {
-      fhe::ntt_inverse(ctx->tables, trios, trio, trio_l, limbs, moduli_id,
-                       fhe::kInttExit, ctx->stream, d01);
-    });
-    // the caller's NTT-form d2 provides the partitions' own extended limbs
-    ctx->key_switch(count, moduli_id, key, key_moduli_id, trio + 2 * limbs * N,
-                    3 * limbs * N, ext, acc, trios + 2 * limbs * N,
-                    3 * limbs * N);
-    ctx->stage("moddown_relin", [&] {
-      fhe::launch_moddown_relin(ctx->tables, acc, trio, out, count, limbs,
-                                moduli_id, ctx->stream);
-    });
+    ctx->relinearize_into(count, moduli_id, trios, key, key_moduli_id, out);
     ctx->check_launch("Relinearize");
   });
 }
@@ -883,6 +1000,102 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
       fhe::launch_moddown_relin_rescale(ctx->tables, acc, coef, out, count,
                                         limbs, moduli_id, ctx->stream);
     });
+    ctx->check_launch("Multiply");
+  });
+}
+
+static void require_text_form(const fhe::CkksParams &P, int level, int flags,
+                              const char *op) {
+  require((flags & ~(FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)) == 0,
+          std::string(op) + ": unknown data flags");
+  // Requirement::POSITIVE_LEVEL (src/ckks/engine_require.cpp:495-500)
+  require(level >= 1 && level <= P.full_level,
+          std::string(op) + ": the text should have a positive level.");
+}
+
+// Shared body of fhe_multiply and fhe_multiply_plain: both inputs prepared
+// at the lower level, then the tensor (b with two polynomials) or the
+// plaintext product (b with one), then the optional relinearization.
+static void multiply_texts(fhe_context *ctx, int count, const int64_t *a,
+                           int a_polys, int a_level, int a_flags,
+                           const int64_t *b, int b_polys, int b_level,
+                           int b_flags, const int64_t *key, int key_moduli_id,
+                           int64_t *out, const char *op) {
+  const auto &P = ctx->params;
+  require_text_form(P, a_level, a_flags, op);
+  require_text_form(P, b_level, b_flags, op);
+  const int level = std::min(a_level, b_level);
+  const int mid = ctx->moduli_id_of(level, true);
+  const int limbs = P.limbs_at(mid);
+  const int64_t N = ctx->n();
+  const int64_t poly = int64_t(limbs) * N;
+  const int prod_polys = b_polys == 2 ? 3 : a_polys;
+  if (key)
+    ctx->check_key(mid, key_moduli_id, op);
+  // prepared inputs and the product in the stash; the preparation's
+  // temporaries in the scratch workspace, which relinearization reuses
+  const int64_t an_l = int64_t(count) * a_polys * poly;
+  const int64_t bn_l = int64_t(count) * b_polys * poly;
+  const int64_t prod_l = int64_t(count) * prod_polys * poly;
+  int64_t *an = ctx->stash(size_t(an_l + bn_l + (key ? prod_l : 0)) * 8);
+  int64_t *bn = an + an_l;
+  int64_t *prod = key ? bn + bn_l : out;
+  auto limbs_of = [&](int l, int f) {
+    return int64_t(P.limbs_at(ctx->moduli_id_of(l, f & FHE_FLAG_RESCALED)));
+  };
+  const int64_t t_l =
+      std::max(int64_t(count) * a_polys * limbs_of(a_level, a_flags),
+               int64_t(count) * b_polys * limbs_of(b_level, b_flags)) *
+      N;
+  int64_t *t0 = ctx->scratch(size_t(2 * t_l) * 8), *t1 = t0 + t_l;
+  ctx->prepare_for_multiply_into(a, int64_t(count) * a_polys, a_level,
+                                 a_flags, level, an, t0, t1);
+  ctx->prepare_for_multiply_into(b, int64_t(count) * b_polys, b_level,
+                                 b_flags, level, bn, t0, t1);
+  if (b_polys == 2) {
+    // CKKSCore::multiply / square (src/ckks/core.cpp:284-343)
+    ctx->stage("tensor", [&] {
+      fhe::launch_tensor(ctx->tables, an, bn, prod, count, limbs, mid,
+                         ctx->stream);
+    });
+  } else {
+    // CKKSCore::multiply(ct, pt) (src/ckks/core.cpp:257-282)
+    ctx->stage("mont_mult", [&] {
+      for (int c = 0; c < count; c++)
+        for (int p = 0; p < a_polys; p++)
+          fhe::launch_mont_mult(ctx->tables, an + (c * a_polys + p) * poly,
+                                bn + c * poly,
+                                prod + (c * prod_polys + p) * poly, limbs,
+                                limbs, mid, ctx->stream);
+    });
+  }
+  if (key)
+    ctx->relinearize_into(count, mid, prod, key, key_moduli_id, out);
+}
+
+int fhe_multiply(fhe_context *ctx, int count, const int64_t *a, int a_level,
+                 int a_flags, const int64_t *b, int b_level, int b_flags,
+                 const int64_t *key, int key_moduli_id, int64_t *out) {
+  return guarded([&] {
+    require(ctx && a && b && out, "null argument");
+    require(count > 0, "Multiply: empty batch");
+    multiply_texts(ctx, count, a, 2, a_level, a_flags, b, 2, b_level, b_flags,
+                   key, key_moduli_id, out, "Multiply");
+    ctx->check_launch("Multiply");
+  });
+}
+
+int fhe_multiply_plain(fhe_context *ctx, int count, const int64_t *ct,
+                       int ct_polys, int ct_level, int ct_flags,
+                       const int64_t *pt, int pt_level, int pt_flags,
+                       int64_t *out) {
+  return guarded([&] {
+    require(ctx && ct && pt && out, "null argument");
+    require(count > 0, "Multiply: empty batch");
+    require(ct_polys == 2 || ct_polys == 3,
+            "Multiply: the ciphertext should be a duo or a trio.");
+    multiply_texts(ctx, count, ct, ct_polys, ct_level, ct_flags, pt, 1,
+                   pt_level, pt_flags, nullptr, 0, out, "Multiply");
     ctx->check_launch("Multiply");
   });
 }
@@ -1180,36 +1393,7 @@ int fhe_level_down(fhe_context *ctx, const int64_t *in, int64_t *out,
     require(level >= 0 && level <= P.full_level && dst_level >= 0 &&
                 dst_level < level,
             "Level down: the destination level must be below the level.");
-    const int dst_mid = P.moduli_id_of_level(dst_level);
-    if (rescaled) {
-      // CKKSCore::level_down_after_rescale (src/ckks/core.cpp:743-779): the
-      // rescaled input sits at moduli_id_of_level(level) + 1
-      const int src_mid = P.moduli_id_of_level(level) + 1;
-      const auto c = P.level_down_multiplier_after_rescale(level, dst_level);
-      const int limbs = int(c.size());
-      require(limbs <= fhe::kMaxEwLimbs, "Level down: too many limbs");
-      const int64_t src_poly = int64_t(P.limbs_at(src_mid)) << P.log_n;
-      const int64_t dst_poly = int64_t(limbs) << P.log_n;
-      const int64_t offset = int64_t(dst_mid - src_mid) << P.log_n;
-      ctx->stage("elementwise", [&] {
-        for (int p = 0; p < polys; p++)
-          fhe::launch_elementwise(ctx->tables, fhe::kEwEnterReduce,
-                                  in + p * src_poly + offset, nullptr,
-                                  out + p * dst_poly, limbs, limbs, dst_mid,
-                                  c.data(), ctx->stream);
-      });
-    } else {
-      // CKKSCore::level_down (src/ckks/core.cpp:792-838)
-      const int src_mid = P.moduli_id_of_level(level);
-      const int next_mid = P.moduli_id_of_level(level - 1);
-      const auto c = P.level_down_multiplier(level, dst_level);
-      require(int(c.size()) <= fhe::kMaxEwLimbs, "Level down: too many limbs");
-      ctx->stage("scalar_rescale", [&] {
-        fhe::launch_scalar_rescale(ctx->tables, in, out, polys, int(c.size()),
-                                   src_mid, c.data(), dst_mid - next_mid,
-                                   ctx->stream);
-      });
-    }
+    ctx->level_down_into(in, out, polys, level, rescaled != 0, dst_level);
     ctx->check_launch("Level down");
   });
 }

@@ -149,6 +149,28 @@ FHE_HD int64_t guard_u(int64_t x, int64_t q) {
   return uint64_t(x) < uint64_t(q) ? x : x - q;
 }
 
+// Montgomery product of arbitrary int64 operands: the 128-bit product of the
+// sign-extended values, i.e. the unsigned product corrected in its high word.
+FHE_HD int64_t mont_mult(int64_t a, int64_t b, uint64_t q, uint64_t k) {
+  U128 x = mul_wide(uint64_t(a), uint64_t(b));
+  if (a < 0)
+    x.hi -= uint64_t(b);
+  if (b < 0)
+    x.hi -= uint64_t(a);
+  return int64_t(mont_reduce(x, q, k));
+}
+
+// mont_mult(c, b) for a constant c in [0, 2^62) and a SIGNED operand b in
+// [-2^62, 2^62) (the values the transforms and ModUp see: residues, lazy
+// residues, and inputs in (-q, q) with q < 2^61), through the fast unsigned
+// product: b' = b + 2^62 is non-negative and below 2^63, and
+// mont(c, b') = mont(c, b) + c exactly, because the added c * 2^62 is a
+// multiple of the radix (it leaves (x * k) mod 2^62 unchanged and adds
+// exactly c to the shifted sum).
+FHE_HD int64_t mont_mult_cs(int64_t c, int64_t b, uint64_t q, uint64_t k) {
+  return sub64(mont_mult_u(c, add64(b, kMontR), q, k), c);
+}
+
 // High 64 bits of a 64 x 64-bit product from 32-bit multiply-adds.
 FHE_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
   const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32);
@@ -168,16 +190,6 @@ FHE_HD uint64_t shoup_mult(uint64_t y, uint64_t w, uint64_t wq, uint64_t q) {
   return w * y - mulhi64(wq, y) * q;
 }
 
-// Montgomery product of arbitrary int64 operands: the 128-bit product of the
-// sign-extended values, i.e. the unsigned product corrected in its high word.
-FHE_HD int64_t mont_mult(int64_t a, int64_t b, uint64_t q, uint64_t k) {
-  U128 x = mul_wide(uint64_t(a), uint64_t(b));
-  if (a < 0)
-    x.hi -= uint64_t(b);
-  if (b < 0)
-    x.hi -= uint64_t(a);
-  return int64_t(mont_reduce(x, q, k));
-}
 
 // Montgomery exit x * R^-1 (src/montgomery/unit.h:25-31): the single-word
 // operand is sign-extended to 128 bits.

@@ -10,12 +10,14 @@
 
 namespace fhe {
 
-// Montgomery product for ModUp: the signed form where a digit can be
-// negative, the fast unsigned form where the host has shown it cannot.
+// Montgomery product of a value a with a table constant b for ModUp: the
+// general signed form for the signed partitions (whose values may exceed
+// 2^62), else the fast form for values in [-2^62, 2^62) (inputs rescaled by
+// a larger prime can be negative).
 template <bool SIGNED>
 __device__ __forceinline__ int64_t modup_mult(int64_t a, int64_t b,
                                               uint64_t q, uint64_t k) {
-  return SIGNED ? mont_mult(a, b, q, k) : mont_mult_u(a, b, q, k);
+  return SIGNED ? mont_mult(a, b, q, k) : mont_mult_cs(b, a, q, k);
 }
 
 // d[j] = the partition's Garner digits from its limb values x[j] (limb
@@ -70,7 +72,7 @@ __device__ __forceinline__ int64_t extend_limb(const int64_t (&d)[MAXZ],
   // the (unsigned) transforms and the key products, whose results are
   // canonicalized before use, so the non-negative representative y + 2q
   // serves in its place
-  if (SIGNED && y < 0)
+  if (y < 0)
     y += q2;
   return y;
 }
@@ -87,13 +89,18 @@ __device__ __forceinline__ int64_t extend_limb_lazy(const int64_t (&d)[MAXZ],
                                                     int ext_limbs,
                                                     const uint64_t *es,
                                                     uint64_t q, bool small) {
+  // a negative digit (the first one is the input value itself, negative
+  // for limbs that a larger prime rescaled) enters as d + big, big a
+  // multiple of q in [2^61, 2^62) above every |d|: the same residue mod q
+  const uint64_t big = q << (__builtin_clzll(q) - 2);
+  auto lift = [&](int64_t v) { return uint64_t(v) + (v < 0 ? big : 0); };
   const uint64_t *c0 = es + 2 * i;
-  uint64_t y = shoup_mult(uint64_t(d[0]), c0[0], c0[1], q);
+  uint64_t y = shoup_mult(lift(d[0]), c0[0], c0[1], q);
 #pragma unroll
   for (int j = 1; j < MAXZ; j++)
     if (j < size) {
       const uint64_t *c = es + 2 * (j * ext_limbs + i);
-      y += shoup_mult(uint64_t(d[j]), c[0], c[1], q);
+      y += shoup_mult(lift(d[j]), c[0], c[1], q);
       if (!small && y >= 2 * q)
         y -= 2 * q;
     }

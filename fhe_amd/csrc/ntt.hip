@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void ntt_pass_kernel(PassArgs a,
       v = a.src[f * n + i];
       if (a.pre) {
         const int p = a.moduli_id + int(f % a.limbs);
-        v = mont_mult_u(v, a.pre[p], t.q[p], t.k[p]);
+        v = mont_mult_cs(a.pre[p], v, t.q[p], t.k[p]);
       }
     }
     lds[j * C + c] = v;
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void ntt_pass_kernel(PassArgs a,
     if (INV && a.post != kInttRaw) {
       const int p = a.moduli_id + int(f % a.limbs);
       const uint64_t q = t.q[p], k = t.k[p];
-      v = mont_exit(mont_mult_u(v, t.ninv[p], q, k), q, k);
+      v = mont_exit(mont_mult_cs(t.ninv[p], v, q, k), q, k);
       if (a.post == kInttExitReduce)
         v = guard(v, int64_t(q));
     }
@@ -192,7 +192,7 @@ __device__ __forceinline__ void r16_pass(const PassArgs &a,
     const int64_t cst = a.pre[p];
 #pragma unroll
     for (int m = 0; m < 16; m++)
-      v[m] = mont_mult_u(v[m], cst, s.q, s.k);
+      v[m] = load_product<A>(v[m], cst, s.q, s.k);
   }
   if (INV && !COLS)
     r16_exchange(v, lds, c, r, true);
@@ -213,7 +213,11 @@ __device__ __forceinline__ void r16_pass(const PassArgs &a,
       const int64_t ninv = t.ninv[p];
 #pragma unroll
       for (int m = 0; m < 16; m++) {
-        const int64_t y = mont_exit(mont_mult_u(v[m], ninv, s.q, s.k), s.q, s.k);
+        // only the exact arithmetic carries negative values
+        const int64_t y = mont_exit(
+            A == kExact ? mont_mult_cs(ninv, v[m], s.q, s.k)
+                        : mont_mult_u(v[m], ninv, s.q, s.k),
+            s.q, s.k);
         v[m] = a.post == kInttExitReduce ? guard(y, int64_t(s.q)) : y;
       }
     }

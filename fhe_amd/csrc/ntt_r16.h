@@ -42,16 +42,20 @@ inline bool t_host_q_small(const DeviceTables &t, int p) {
   return t.host_q[p] < kLazySmallBound;
 }
 
-// Operands are lazy residues in [0, 2q), so every guard sees a non-negative
-// value.  Cooley-Tukey: p = mont(w, y); x' = guard(x + p, 2q);
+// Exact butterflies with the reference's signed arithmetic.  Operands are
+// the reference's representatives: lazy residues in [0, 2q), and at the
+// input of a transform possibly negative values (limbs that a larger prime
+// rescaled lie in (-q, q)).  The signed product and compares reproduce the
+// reference for both; on non-negative operands they equal the unsigned
+// forms.  Cooley-Tukey: p = mont(w, y); x' = guard(x + p, 2q);
 // y' = guard(x + 2q - p, 2q)
 __device__ __forceinline__ void ct_butterfly(int64_t &x, int64_t &y, int64_t w,
                                              int64_t q2, uint64_t q,
                                              uint64_t k) {
-  const int64_t p = mont_mult_u(w, y, q, k);
+  const int64_t p = mont_mult_cs(w, y, q, k);
   const int64_t x0 = x;
-  x = guard_u(add64(x0, p), q2);
-  y = guard_u(sub64(add64(x0, q2), p), q2);
+  x = guard(add64(x0, p), q2);
+  y = guard(sub64(add64(x0, q2), p), q2);
 }
 
 // Gentleman-Sande: x' = guard(x + y, 2q); y' = mont(w, guard(x + 2q - y, 2q))
@@ -59,8 +63,22 @@ __device__ __forceinline__ void gs_butterfly(int64_t &x, int64_t &y, int64_t w,
                                              int64_t q2, uint64_t q,
                                              uint64_t k) {
   const int64_t x0 = x, y0 = y;
-  x = guard_u(add64(x0, y0), q2);
-  y = mont_mult_u(w, guard_u(sub64(add64(x0, q2), y0), q2), q, k);
+  x = guard(add64(x0, y0), q2);
+  y = mont_mult_cs(w, guard(sub64(add64(x0, q2), y0), q2), q, k);
+}
+
+// Load-time product with a per-prime constant (mont_enter by R^2 for the
+// reference's to_ntt) of a possibly negative input value.  The lazy
+// arithmetic takes non-negative operands only: the signed product lies in
+// (-q, q), and a negative one is replaced by the residue's representative
+// in [0, q).
+template <int A>
+__device__ __forceinline__ int64_t load_product(int64_t v, int64_t c,
+                                                uint64_t q, uint64_t k) {
+  int64_t y = mont_mult_cs(c, v, q, k);
+  if (A != kExact && y < 0)
+    y += int64_t(q);
+  return y;
 }
 
 // Everything a thread needs to address twiddles in one 8-bit pass: the pass

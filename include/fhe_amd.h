@@ -134,6 +134,36 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
                             const int64_t *key, int key_moduli_id,
                             int64_t *out);
 
+/* Data flags of a text, the reference's DataFlag values (src/data/flag.h:7-14).
+ * A text at level L sits at moduli_id full_level - L, or one higher when
+ * rescaled. */
+enum { FHE_FLAG_NTT_FORM = 2, FHE_FLAG_RESCALED = 4 };
+
+/* CKKSEngine::multiply(ct, ct) and, with a key, multiply(ct, ct, rlk)
+ * (src/ckks/engine_multiply.cpp:39-57, 134-147) of `count` duos in any form:
+ * a [count][2][limbs][N] at (a_level, a_flags), b likewise.  Both go through
+ * prepare_for_multiply (src/ckks/engine_multiply.cpp:13-27) at the lower
+ * level L: INTT when the level changes, level down (plain or after rescale),
+ * rescale, NTT.  key = NULL: out is the NTT-form trio [count][3][limbs][N];
+ * with a relinearization key at key_moduli_id: the INTT-form duo
+ * [count][2][limbs][N].  Either is not rescaled at level L - 1.  a == b with
+ * equal forms gives square(ct) (src/ckks/engine_multiply.cpp:86-99).
+ * Levels must be positive. */
+int fhe_multiply(fhe_context *ctx, int count, const int64_t *a, int a_level,
+                 int a_flags, const int64_t *b, int b_level, int b_flags,
+                 const int64_t *key, int key_moduli_id, int64_t *out);
+
+/* CKKSEngine::multiply(ct, pt) (src/ckks/engine_multiply.cpp:210-227) of
+ * `count` duos or trios ct [count][ct_polys][limbs][N] with one plaintext
+ * each, pt [count][limbs][N], both in any form: prepared as in
+ * fhe_multiply, then every polynomial times the plaintext
+ * (CKKSCore::multiply, src/ckks/core.cpp:257-282).  out: NTT form
+ * [count][ct_polys][limbs][N], not rescaled at level min - 1. */
+int fhe_multiply_plain(fhe_context *ctx, int count, const int64_t *ct,
+                       int ct_polys, int ct_level, int ct_flags,
+                       const int64_t *pt, int pt_level, int pt_flags,
+                       int64_t *out);
+
 /* Per-coefficient primitives over polys x limbs device limbs at moduli_id:
  * y = op(x1, x2, c[limb]) for the reference's limb operations
  * (src/montgomery/unit.cpp:5-86, src/montgomery/compound.cpp:6-109,

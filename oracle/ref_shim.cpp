@@ -805,4 +805,77 @@ void *ref_hmult_relin_rescale(void *handle, void *a, void *b, void *rlk) {
   });
 }
 
+// UnitPlaintext handles: encode_unit of real values at a level
+// (src/ckks/engine_encode.cpp:10-36), their shape (as ref_ct_shape) and limbs.
+void *ref_encode_pt(void *handle, const double *values, int level) {
+  return guarded([&]() -> void * {
+    auto ref = static_cast<RefEngine *>(handle);
+    std::vector<double> v(values, values + ref->precomputed->slot_count_);
+    return ref->engine->encode_unit(v, level);
+  });
+}
+void ref_free_pt(void *p) { delete static_cast<UnitPlaintext *>(p); }
+int ref_pt_shape(void *pt, int64_t *shape) {
+  auto p = static_cast<UnitPlaintext *>(pt);
+  data_shape(p, shape);
+  shape[4] = static_cast<int64_t>(p->flag_);
+  shape[5] = p->level_;
+  return 0;
+}
+int ref_pt_get(void *pt, int64_t *out) {
+  copy_out(static_cast<UnitPlaintext *>(pt), out);
+  return 0;
+}
+
+// Forms of texts: CKKSEngine::ntt / rescale of a ciphertext (plain = 0) or
+// a plaintext (plain = 1) (src/ckks/engine_ntt.cpp:26-42,
+// src/ckks/engine_rescale.cpp:38-60).
+void *ref_ntt_text(void *handle, void *p, int plain) {
+  return guarded([&]() -> void * {
+    auto e = static_cast<RefEngine *>(handle)->engine;
+    if (plain)
+      return e->ntt(static_cast<UnitPlaintext *>(p));
+    return e->ntt(static_cast<UnitCiphertext *>(p));
+  });
+}
+void *ref_rescale_text(void *handle, void *p, int plain) {
+  return guarded([&]() -> void * {
+    auto e = static_cast<RefEngine *>(handle)->engine;
+    if (plain)
+      return e->rescale(static_cast<UnitPlaintext *>(p));
+    return e->rescale(static_cast<UnitCiphertext *>(p));
+  });
+}
+void *ref_level_down_pt(void *handle, void *p, int level) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->level_down(
+        static_cast<UnitPlaintext *>(p), level);
+  });
+}
+
+// CKKSEngine::multiply(ct, ct), square(ct) with or without a relinearization
+// key, multiply(ct, pt) (src/ckks/engine_multiply.cpp:39-57, 86-99, 175-186,
+// 210-227).
+void *ref_multiply_ct(void *handle, void *a, void *b) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->multiply(
+        static_cast<UnitCiphertext *>(a), static_cast<UnitCiphertext *>(b));
+  });
+}
+void *ref_square(void *handle, void *a, void *rlk) {
+  return guarded([&]() -> void * {
+    auto e = static_cast<RefEngine *>(handle)->engine;
+    auto ct = static_cast<UnitCiphertext *>(a);
+    if (rlk)
+      return e->square(ct, static_cast<RelinearizationKey *>(rlk));
+    return e->square(ct);
+  });
+}
+void *ref_multiply_pt(void *handle, void *ct, void *pt) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->multiply(
+        static_cast<UnitCiphertext *>(ct), static_cast<UnitPlaintext *>(pt));
+  });
+}
+
 } // extern "C"

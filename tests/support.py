@@ -379,6 +379,16 @@ class Reference:
             "ref_multiply_imaginary": (VP, [VP, VP, ctypes.c_int64]),
             "ref_ct_flag": (i32, [VP]),
             "ref_ct_scale_ld": (i32, [VP, VP]),
+            "ref_encode_pt": (VP, [VP, VP, i32]),
+            "ref_free_pt": (None, [VP]),
+            "ref_pt_shape": (i32, [VP, VP]),
+            "ref_pt_get": (i32, [VP, VP]),
+            "ref_ntt_text": (VP, [VP, VP, i32]),
+            "ref_rescale_text": (VP, [VP, VP, i32]),
+            "ref_level_down_pt": (VP, [VP, VP, i32]),
+            "ref_multiply_ct": (VP, [VP, VP, VP]),
+            "ref_square": (VP, [VP, VP, VP]),
+            "ref_multiply_pt": (VP, [VP, VP, VP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -417,6 +427,17 @@ class Reference:
         mid, polys, chain, n, _, _ = self.shape(ct)
         out = np.zeros((polys, chain, n), np.int64)
         self.lib.ref_ct_get(ct, ptr(out))
+        return out
+
+    def pt_shape(self, pt):
+        s = np.zeros(6, np.int64)
+        self.lib.ref_pt_shape(pt, ptr(s))
+        return [int(v) for v in s]
+
+    def pt_get(self, pt):
+        mid, polys, chain, n, _, _ = self.pt_shape(pt)
+        out = np.zeros((polys, chain, n), np.int64)
+        self.lib.ref_pt_get(pt, ptr(out))
         return out
 
     def set(self, ct, data):
