@@ -33,7 +33,7 @@ EXPORTED_SYMBOLS = (
     "fhe_encrypt_pk", "fhe_decrypt", "fhe_rotate_batch", "fhe_switch_key",
     "fhe_encode", "fhe_params_encode_constants", "fhe_decode",
     "fhe_decoded_scale", "fhe_context_set_option", "fhe_multiply_imaginary",
-    "fhe_mod_raise", "fhe_multiply", "fhe_multiply_plain",
+    "fhe_mod_raise", "fhe_multiply", "fhe_multiply_plain", "fhe_add_texts",
 )
 
 # data flags of a text (FHE_FLAG_* in include/fhe_amd.h)
@@ -139,6 +139,8 @@ def library():
                                      i32, vp]
         lib.fhe_multiply_plain.argtypes = [vp, i32, vp, i32, i32, i32, vp,
                                            i32, i32, vp]
+        lib.fhe_add_texts.argtypes = [vp, i32, vp, i32, i32, i32, vp, i32, i32,
+                                      i32, i32, vp]
         lib.fhe_decode.argtypes = [vp, vp, i32, i32, ctypes.c_longdouble,
                                    i32, vp]
         lib.fhe_decoded_scale.argtypes = [vp, i32, ctypes.c_longdouble,
@@ -381,6 +383,16 @@ class Context:
         _check(library().fhe_multiply_plain(self.handle, count, ct.ptr, polys,
                                             ct_level, ct_flags, pt.ptr,
                                             pt_level, pt_flags, out.ptr))
+
+    def add(self, a, a_level, a_flags, b, b_level, b_flags, out,
+            subtract=False):
+        """fhe_add_texts: a + b (or a - b) of duos or trios in any form."""
+        count = int(np.prod(a.shape[:-3]))
+        _check(library().fhe_add_texts(self.handle, count, a.ptr,
+                                       int(a.shape[-3]), a_level, a_flags,
+                                       b.ptr, int(b.shape[-3]), b_level,
+                                       b_flags, 1 if subtract else 0,
+                                       out.ptr))
 
     def automorphism(self, src, dst, moduli_id, delta, ntt_form=False):
         polys, limbs = _polys_limbs(src, self.n)

@@ -383,19 +383,21 @@ struct fhe_context {
     return params.moduli_id_of_level(level) + (rescaled ? 1 : 0);
   }
 
-  // CKKSEngine::prepare_for_multiply (src/ckks/engine_multiply.cpp:13-27)
-  // of `polys` polynomials at (level, flags) into NTT-form, rescaled
-  // polynomials at dst_level <= level: intt_ when the level changes,
-  // level_down_ (src/ckks/engine_level_down.cpp:5-37), rescale_
-  // (src/ckks/engine_rescale.cpp:5-33), ntt_ (src/ckks/engine_ntt.cpp:5-21).
-  // t0 and t1 hold polys x limbs(level, flags) x N each; out may not alias
-  // them.
-  void prepare_for_multiply_into(const int64_t *in, int64_t polys, int level,
-                                 int flags, int dst_level, int64_t *out,
-                                 int64_t *t0, int64_t *t1) {
+  // The engine's form changes of `polys` polynomials at (level, flags), in
+  // its order: intt_ first when asked (CKKSEngine::intt_), level_down_ to
+  // dst_level <= level (src/ckks/engine_level_down.cpp:5-37: after rescale
+  // in the current form, else in the coefficient domain), rescale_ when
+  // `rescale` and not yet rescaled (src/ckks/engine_rescale.cpp:5-33), ntt_
+  // when `ntt` (src/ckks/engine_ntt.cpp:5-21).  Returns `in` itself when
+  // nothing changes, else `out`; t0 and t1 hold polys x limbs(level, flags)
+  // x N each, and out may not alias them.
+  const int64_t *transform_into(const int64_t *in, int64_t polys, int level,
+                                int flags, bool intt_first, int dst_level,
+                                bool rescale, bool ntt, int64_t *out,
+                                int64_t *t0, int64_t *t1) {
     const int64_t N = n();
     bool rescaled = (flags & FHE_FLAG_RESCALED) != 0;
-    bool ntt = (flags & FHE_FLAG_NTT_FORM) != 0;
+    bool is_ntt = (flags & FHE_FLAG_NTT_FORM) != 0;
     const int64_t *cur = in;
     int64_t *bufs[2] = {t0, t1};
     int next = 0;
@@ -413,19 +415,39 @@ struct fhe_context {
                          fhe::kInttExit, stream);
       });
       cur = y;
-      ntt = false;
+      is_ntt = false;
     };
+    auto to_ntt = [&] {
+      const int mid = moduli_id_of(level, rescaled);
+      const int limbs = params.limbs_at(mid);
+      int64_t *y = target();
+      stage("ntt", [&] {
+        fhe::ntt_forward(tables, cur, y, polys * limbs, limbs, mid, tables.r2,
+                         stream);
+      });
+      cur = y;
+      is_ntt = true;
+    };
+    if (intt_first && is_ntt)
+      to_intt();
     if (level > dst_level) {
-      if (ntt)
+      // a not-rescaled NTT-form text goes down in the coefficient domain
+      // and comes back
+      const bool back = !rescaled && is_ntt;
+      if (back)
         to_intt();
       int64_t *y = target();
       level_down_into(cur, y, int(polys), level, rescaled, dst_level);
       cur = y;
       rescaled = false;
       level = dst_level;
+      if (back)
+        to_ntt();
     }
-    if (!rescaled) {
-      if (ntt)
+    if (rescale && !rescaled) {
+      require(level >= 1, "Rescale: the text should have a positive level.");
+      const bool back = is_ntt;
+      if (back)
         to_intt();
       const int mid = moduli_id_of(level, false);
       int64_t *y = target();
@@ -435,19 +457,28 @@ struct fhe_context {
       });
       cur = y;
       rescaled = true;
+      if (back)
+        to_ntt();
     }
-    const int mid = moduli_id_of(dst_level, true);
-    const int limbs = params.limbs_at(mid);
-    if (!ntt) {
-      stage("ntt", [&] {
-        fhe::ntt_forward(tables, cur, out, polys * limbs, limbs, mid,
-                         tables.r2, stream);
-      });
-    } else {
-      hip_check(hipMemcpyAsync(out, cur, size_t(polys * limbs) * N * 8,
-                               hipMemcpyDeviceToDevice, stream),
-                "copy");
-    }
+    if (ntt && !is_ntt)
+      to_ntt();
+    if (cur == in)
+      return in;
+    const int limbs = params.limbs_at(moduli_id_of(level, rescaled));
+    hip_check(hipMemcpyAsync(out, cur, size_t(polys * limbs) * N * 8,
+                             hipMemcpyDeviceToDevice, stream),
+              "copy");
+    return out;
+  }
+
+  // CKKSEngine::prepare_for_multiply (src/ckks/engine_multiply.cpp:13-27):
+  // intt_ when the level changes, level_down_, rescale_, ntt_.
+  const int64_t *prepare_for_multiply_into(const int64_t *in, int64_t polys,
+                                           int level, int flags,
+                                           int dst_level, int64_t *out,
+                                           int64_t *t0, int64_t *t1) {
+    return transform_into(in, polys, level, flags, level > dst_level,
+                          dst_level, true, true, out, t0, t1);
   }
 
   // Relinearization of `count` NTT-form trios at moduli_id into INTT-form
@@ -1048,14 +1079,14 @@ static void multiply_texts(fhe_context *ctx, int count, const int64_t *a,
                int64_t(count) * b_polys * limbs_of(b_level, b_flags)) *
       N;
   int64_t *t0 = ctx->scratch(size_t(2 * t_l) * 8), *t1 = t0 + t_l;
-  ctx->prepare_for_multiply_into(a, int64_t(count) * a_polys, a_level,
-                                 a_flags, level, an, t0, t1);
-  ctx->prepare_for_multiply_into(b, int64_t(count) * b_polys, b_level,
-                                 b_flags, level, bn, t0, t1);
+  const int64_t *ap = ctx->prepare_for_multiply_into(
+      a, int64_t(count) * a_polys, a_level, a_flags, level, an, t0, t1);
+  const int64_t *bp = ctx->prepare_for_multiply_into(
+      b, int64_t(count) * b_polys, b_level, b_flags, level, bn, t0, t1);
   if (b_polys == 2) {
     // CKKSCore::multiply / square (src/ckks/core.cpp:284-343)
     ctx->stage("tensor", [&] {
-      fhe::launch_tensor(ctx->tables, an, bn, prod, count, limbs, mid,
+      fhe::launch_tensor(ctx->tables, ap, bp, prod, count, limbs, mid,
                          ctx->stream);
     });
   } else {
@@ -1063,8 +1094,8 @@ static void multiply_texts(fhe_context *ctx, int count, const int64_t *a,
     ctx->stage("mont_mult", [&] {
       for (int c = 0; c < count; c++)
         for (int p = 0; p < a_polys; p++)
-          fhe::launch_mont_mult(ctx->tables, an + (c * a_polys + p) * poly,
-                                bn + c * poly,
+          fhe::launch_mont_mult(ctx->tables, ap + (c * a_polys + p) * poly,
+                                bp + c * poly,
                                 prod + (c * prod_polys + p) * poly, limbs,
                                 limbs, mid, ctx->stream);
     });
@@ -1097,6 +1128,94 @@ int fhe_multiply_plain(fhe_context *ctx, int count, const int64_t *ct,
     multiply_texts(ctx, count, ct, ct_polys, ct_level, ct_flags, pt, 1,
                    pt_level, pt_flags, nullptr, 0, out, "Multiply");
     ctx->check_launch("Multiply");
+  });
+}
+
+int fhe_add_texts(fhe_context *ctx, int count, const int64_t *a,
+                  int a_polys, int a_level, int a_flags, const int64_t *b,
+                  int b_polys, int b_level, int b_flags, int subtract,
+                  int64_t *out) {
+  return guarded([&] {
+    require(ctx && a && b && out, "null argument");
+    require(count > 0, "Add: empty batch");
+    require((a_polys == 2 || a_polys == 3) && (b_polys == 2 || b_polys == 3),
+            "Add: the ciphertexts should be duos or trios.");
+    const auto &P = ctx->params;
+    for (int f : {a_flags, b_flags})
+      require((f & ~(FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)) == 0,
+              "Add: unknown data flags");
+    require(a_level >= 0 && a_level <= P.full_level && b_level >= 0 &&
+                b_level <= P.full_level,
+            "Add: level out of range");
+    // CKKSEngine::match_form (src/ckks/engine_add.cpp:37-74): the first
+    // text's form decides the domain; the higher text leaves the NTT domain
+    // before it goes down; a rescale where only one of them is rescaled
+    const bool ntt_form = (a_flags & FHE_FLAG_NTT_FORM) != 0;
+    const int level = std::min(a_level, b_level);
+    const bool ra = (a_flags & FHE_FLAG_RESCALED) && a_level == level;
+    const bool rb = (b_flags & FHE_FLAG_RESCALED) && b_level == level;
+    const bool rescaled = ra || rb;
+    const int mid = ctx->moduli_id_of(level, rescaled);
+    const int limbs = P.limbs_at(mid);
+    const int64_t N = ctx->n();
+    const int64_t poly = int64_t(limbs) * N;
+    auto limbs_of = [&](int l, int f) {
+      return int64_t(P.limbs_at(ctx->moduli_id_of(l, f & FHE_FLAG_RESCALED)));
+    };
+    const int64_t an_l = int64_t(count) * a_polys * poly;
+    const int64_t bn_l = int64_t(count) * b_polys * poly;
+    int64_t *an = ctx->stash(size_t(an_l + bn_l) * 8), *bn = an + an_l;
+    const int64_t t_l =
+        std::max(int64_t(count) * a_polys * limbs_of(a_level, a_flags),
+                 int64_t(count) * b_polys * limbs_of(b_level, b_flags)) *
+        N;
+    int64_t *t0 = ctx->scratch(size_t(2 * t_l) * 8), *t1 = t0 + t_l;
+    const int64_t *ap = ctx->transform_into(
+        a, int64_t(count) * a_polys, a_level, a_flags,
+        !ntt_form || a_level > b_level, level, rescaled, ntt_form, an, t0, t1);
+    const int64_t *bp = ctx->transform_into(
+        b, int64_t(count) * b_polys, b_level, b_flags,
+        !ntt_form || b_level > a_level, level, rescaled, ntt_form, bn, t0, t1);
+    const int op = subtract ? fhe::kEwSubReduce : fhe::kEwAddReduce;
+    ctx->stage("add", [&] {
+      if (a_polys == b_polys) {
+        // CKKSCore::add / subtract (src/ckks/core.cpp:415-435, 598-614)
+        fhe::launch_elementwise(ctx->tables, op, ap, bp, out,
+                                int64_t(count) * a_polys * limbs, limbs, mid,
+                                nullptr, ctx->stream);
+        return;
+      }
+      // add_trio_and_duo, subtract_trio_by_duo, subtract_duo_by_trio
+      // (src/ckks/core.cpp:437-460, 615-660): the first two polynomials
+      // combined, the trio's third copied, or reduced and negated when the
+      // trio is subtracted
+      const bool a_trio = a_polys == 3;
+      for (int c = 0; c < count; c++) {
+        const int64_t *x1 = ap + int64_t(c) * a_polys * poly;
+        const int64_t *x2 = bp + int64_t(c) * b_polys * poly;
+        int64_t *y = out + int64_t(c) * 3 * poly;
+        if (!subtract && !a_trio)
+          std::swap(x1, x2);
+        fhe::launch_elementwise(ctx->tables, op, x1, x2, y, 2 * limbs, limbs,
+                                mid, nullptr, ctx->stream);
+        const int64_t *third = (a_trio ? ap + int64_t(c) * 3 * poly
+                                       : bp + int64_t(c) * 3 * poly) +
+                               2 * poly;
+        if (subtract && !a_trio) {
+          fhe::launch_elementwise(ctx->tables, fhe::kEwReduce, third, nullptr,
+                                  y + 2 * poly, limbs, limbs, mid, nullptr,
+                                  ctx->stream);
+          fhe::launch_elementwise(ctx->tables, fhe::kEwNegateMakeUnsigned,
+                                  y + 2 * poly, nullptr, y + 2 * poly, limbs,
+                                  limbs, mid, nullptr, ctx->stream);
+        } else {
+          hip_check(hipMemcpyAsync(y + 2 * poly, third, size_t(poly) * 8,
+                                   hipMemcpyDeviceToDevice, ctx->stream),
+                    "copy");
+        }
+      }
+    });
+    ctx->check_launch("Add");
   });
 }
 

@@ -164,6 +164,20 @@ int fhe_multiply_plain(fhe_context *ctx, int count, const int64_t *ct,
                        const int64_t *pt, int pt_level, int pt_flags,
                        int64_t *out);
 
+/* CKKSEngine::add(ct, ct) / subtract(ct, ct) (src/ckks/engine_add.cpp:82-110,
+ * src/ckks/engine_subtract.cpp:5-35) of `count` pairs of duos or trios in
+ * any form, a [count][a_polys][limbs][N] at (a_level, a_flags), b likewise;
+ * subtract = 1 gives a - b.  Both go through match_form
+ * (src/ckks/engine_add.cpp:37-74): the first text's NTT flag decides the
+ * domain, the higher text leaves the NTT domain and goes down to the lower
+ * level, a rescale where only one of them is rescaled.  out
+ * [count][max(a_polys, b_polys)][limbs][N] at the lower level, in a's
+ * domain, rescaled if either matched input is. */
+int fhe_add_texts(fhe_context *ctx, int count, const int64_t *a,
+                  int a_polys, int a_level, int a_flags, const int64_t *b,
+                  int b_polys, int b_level, int b_flags, int subtract,
+                  int64_t *out);
+
 /* Per-coefficient primitives over polys x limbs device limbs at moduli_id:
  * y = op(x1, x2, c[limb]) for the reference's limb operations
  * (src/montgomery/unit.cpp:5-86, src/montgomery/compound.cpp:6-109,
