@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "fhe_encode", "fhe_params_encode_constants", "fhe_decode",
     "fhe_decoded_scale", "fhe_context_set_option", "fhe_multiply_imaginary",
     "fhe_mod_raise", "fhe_multiply", "fhe_multiply_plain", "fhe_add_texts",
+    "fhe_round",
 )
 
 # data flags of a text (FHE_FLAG_* in include/fhe_amd.h)
@@ -141,6 +142,8 @@ def library():
                                            i32, i32, vp]
         lib.fhe_add_texts.argtypes = [vp, i32, vp, i32, i32, i32, vp, i32, i32,
                                       i32, i32, vp]
+        lib.fhe_round.argtypes = [vp, vp, i32, vp, i32, vp,
+                                  ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fhe_decode.argtypes = [vp, vp, i32, i32, ctypes.c_longdouble,
                                    i32, vp]
         lib.fhe_decoded_scale.argtypes = [vp, i32, ctypes.c_longdouble,
@@ -393,6 +396,15 @@ class Context:
                                        b.ptr, int(b.shape[-3]), b_level,
                                        b_flags, 1 if subtract else 0,
                                        out.ptr))
+
+    def round(self, ct, level, key, key_moduli_id, out):
+        """fhe_round (EvalMod) of one INTT-form, not rescaled duo; returns
+        the result's (level, flags)."""
+        lv, fl = ctypes.c_int(), ctypes.c_int()
+        _check(library().fhe_round(self.handle, ct.ptr, level, key.ptr,
+                                   key_moduli_id, out.ptr, ctypes.byref(lv),
+                                   ctypes.byref(fl)))
+        return lv.value, fl.value
 
     def automorphism(self, src, dst, moduli_id, delta, ntt_form=False):
         polys, limbs = _polys_limbs(src, self.n)

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -1876,6 +1877,347 @@ int fhe_conjugate(fhe_context *ctx, int count, int moduli_id,
                               count, limbs, moduli_id, ctx->stream);
     });
     ctx->check_launch("Conjugate");
+  });
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Engine-level compositions of the operations above on single device texts.
+
+namespace {
+
+// A ciphertext on the device with the metadata of the reference's
+// UnitCiphertext that the limbs depend on (the scale does not enter them).
+struct Text {
+  int64_t *data = nullptr;
+  int polys = 2;
+  int level = 0;
+  int flags = 0;
+};
+
+void ok(int rc) {
+  if (rc != FHE_OK)
+    throw FheError(rc, fhe_last_error());
+}
+
+// CKKSEngine's operations on device texts, each with the engine's form
+// handling (src/ckks/engine_*.cpp); results are new device buffers, freed
+// with the engine.
+class TextEngine {
+public:
+  TextEngine(fhe_context *c, const int64_t *rlk, int rlk_mid)
+      : ctx(c), key(rlk), key_mid(rlk_mid) {}
+  ~TextEngine() {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : owned)
+      (void)hipFree(p);
+  }
+
+  int limbs(int level, int flags) const {
+    return ctx->params.limbs_at(
+        ctx->moduli_id_of(level, (flags & FHE_FLAG_RESCALED) != 0));
+  }
+  int64_t words(const Text &t) const {
+    return int64_t(t.polys) * limbs(t.level, t.flags) * ctx->n();
+  }
+  Text alloc(int polys, int level, int flags) {
+    Text t;
+    t.polys = polys;
+    t.level = level;
+    t.flags = flags;
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, size_t(words(t)) * 8), "text allocation");
+    owned.push_back(p);
+    t.data = static_cast<int64_t *>(p);
+    return t;
+  }
+  Text wrap(const int64_t *data, int polys, int level, int flags) {
+    Text t;
+    t.data = const_cast<int64_t *>(data);
+    t.polys = polys;
+    t.level = level;
+    t.flags = flags;
+    return t;
+  }
+  Text clone(const Text &a) {
+    Text t = alloc(a.polys, a.level, a.flags);
+    hip_check(hipMemcpyAsync(t.data, a.data, size_t(words(a)) * 8,
+                             hipMemcpyDeviceToDevice, ctx->stream),
+              "copy");
+    return t;
+  }
+  // intt_ (src/ckks/engine_intt.cpp): a copy in the coefficient domain
+  Text intt(const Text &a) {
+    if (!(a.flags & FHE_FLAG_NTT_FORM))
+      return a;
+    Text t = clone(a);
+    t.flags &= ~FHE_FLAG_NTT_FORM;
+    const int l = limbs(a.level, a.flags);
+    ok(fhe_intt(ctx, t.data, a.polys, l,
+                ctx->moduli_id_of(a.level, a.flags & FHE_FLAG_RESCALED),
+                1));
+    return t;
+  }
+  // multiply(ct, ct[, rlk]), square (src/ckks/engine_multiply.cpp:39-199)
+  Text multiply(const Text &a, const Text &b, bool relin) {
+    require(a.polys == 2 && b.polys == 2, "Multiply: duos expected");
+    const int level = std::min(a.level, b.level);
+    Text out = alloc(relin ? 2 : 3, level - 1, relin ? 0 : FHE_FLAG_NTT_FORM);
+    ok(fhe_multiply(ctx, 1, a.data, a.level, a.flags, b.data, b.level,
+                    b.flags, relin ? key : nullptr, key_mid, out.data));
+    return out;
+  }
+  // add / subtract (src/ckks/engine_add.cpp:82-110,
+  // src/ckks/engine_subtract.cpp:5-35), forms as match_form leaves them
+  Text add(const Text &a, const Text &b, bool subtract) {
+    const int level = std::min(a.level, b.level);
+    const bool ra = (a.flags & FHE_FLAG_RESCALED) && a.level == level;
+    const bool rb = (b.flags & FHE_FLAG_RESCALED) && b.level == level;
+    const int flags = (a.flags & FHE_FLAG_NTT_FORM) |
+                      (ra || rb ? FHE_FLAG_RESCALED : 0);
+    Text out = alloc(std::max(a.polys, b.polys), level, flags);
+    ok(fhe_add_texts(ctx, 1, a.data, a.polys, a.level, a.flags, b.data,
+                     b.polys, b.level, b.flags, subtract ? 1 : 0, out.data));
+    return out;
+  }
+  // add_inplace of one form (CKKSCore::add_inplace,
+  // src/ckks/core.cpp:498-512)
+  void add_inplace(Text &x, const Text &y) {
+    require(x.polys == y.polys && x.level == y.level && x.flags == y.flags,
+            "Add inplace: the forms differ");
+    const int l = limbs(x.level, x.flags);
+    fhe::launch_elementwise(ctx->tables, fhe::kEwAddReduce, x.data, y.data,
+                            x.data, int64_t(x.polys) * l, l,
+                            ctx->moduli_id_of(x.level,
+                                              x.flags & FHE_FLAG_RESCALED),
+                            nullptr, ctx->stream);
+  }
+  // add(ct, double) of a coefficient-domain, not rescaled text
+  // (src/ckks/engine_add.cpp:518-536 with CKKSCore::add(ct, double))
+  Text add_double(const Text &a, double value) {
+    require(!(a.flags & (FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)),
+            "Add: a scalar joins only an INTT-form, not rescaled text here");
+    Text out = alloc(a.polys, a.level, a.flags);
+    ok(fhe_add_double(ctx, a.data, out.data, 1, a.polys, a.level, value));
+    return out;
+  }
+  // multiply(ct, double) (src/ckks/engine_multiply.cpp:418-432): intt_,
+  // then the scalar product and rescale to level - 1, not rescaled
+  Text multiply_double(const Text &a, double value) {
+    require(!(a.flags & FHE_FLAG_RESCALED) && a.level >= 1,
+            "Multiply: the text should not be rescaled and have a positive "
+            "level.");
+    const Text in = intt(a);
+    Text out = alloc(a.polys, a.level - 1, in.flags);
+    ok(fhe_multiply_double(ctx, in.data, out.data, a.polys, a.level, value));
+    return out;
+  }
+  // multiply(ct, int64_t) (src/ckks/engine_multiply.cpp:443-457)
+  Text multiply_int(const Text &a, int64_t value) {
+    require(!(a.flags & FHE_FLAG_RESCALED),
+            "Multiply: the text should not be rescaled.");
+    const Text in = intt(a);
+    Text out = alloc(a.polys, a.level, in.flags);
+    ok(fhe_multiply_int(ctx, in.data, out.data, a.polys,
+                        ctx->moduli_id_of(a.level, false), value));
+    return out;
+  }
+  // level_down (src/ckks/engine_level_down.cpp:5-37)
+  Text level_down(const Text &a, int level) {
+    if (a.level == level)
+      return clone(a);
+    Text out = alloc(a.polys, level, a.flags & ~FHE_FLAG_RESCALED);
+    const int64_t t_l = words(a);
+    int64_t *t0 = ctx->scratch(size_t(2 * t_l) * 8);
+    ctx->transform_into(a.data, a.polys, a.level, a.flags, false, level,
+                        false, (a.flags & FHE_FLAG_NTT_FORM) != 0, out.data,
+                        t0, t0 + t_l);
+    return out;
+  }
+  // relinearize (src/ckks/engine_relinearize.cpp:13-28): ntt_, then
+  // CKKSCore::relinearize
+  Text relinearize(const Text &trio) {
+    require(trio.polys == 3, "Relinearize: a trio expected");
+    const bool rescaled = (trio.flags & FHE_FLAG_RESCALED) != 0;
+    const int mid = ctx->moduli_id_of(trio.level, rescaled);
+    Text in = trio;
+    if (!(trio.flags & FHE_FLAG_NTT_FORM)) {
+      in = clone(trio);
+      in.flags |= FHE_FLAG_NTT_FORM;
+      ok(fhe_ntt(ctx, in.data, 3, limbs(trio.level, trio.flags), mid, 1));
+    }
+    ctx->check_key(mid, key_mid, "Relinearize");
+    Text out = alloc(2, trio.level, trio.flags & ~FHE_FLAG_NTT_FORM);
+    ctx->relinearize_into(1, mid, in.data, key, key_mid, out.data);
+    return out;
+  }
+  // weighted_sum (src/ckks/engine_advanced.cpp:228-269)
+  Text weighted_sum(const std::vector<Text> &basis,
+                    const std::vector<double> &weights) {
+    const int degree = int(weights.size()) - 1;
+    Text result;
+    bool have = false;
+    for (int index = degree; index > 0; index--) {
+      const double weight = weights[index];
+      if (weight == 0)
+        continue;
+      Text product = multiply_double(basis[index - 1], weight);
+      if (!have) {
+        result = product;
+        have = true;
+        continue;
+      }
+      result = add(result, product, false);
+    }
+    require(have, "Weighted sum: every weight is zero");
+    if (weights[0] != 0)
+      result = add_double(result, weights[0]);
+    return result;
+  }
+
+private:
+  fhe_context *ctx;
+  const int64_t *key;
+  int key_mid;
+  std::vector<void *> owned;
+};
+
+// The Chebyshev coefficients of the reference's sine approximation
+// (src/ckks/engine_bootstrap.cpp:500-524): part of the algorithm's
+// definition, so the same values.
+const std::vector<double> kSineAuxiliary = {
+    -0.4674215360340989,   -0.24807246365084598, -0.03360736343811264,
+    0.10485552200765141,   0.012171628025753911, -0.031034212401993305,
+    -0.003985828764410408, 0.0};
+const std::vector<std::vector<double>> kSineQuotient = {
+    {-0.7500012154934053, 4.148627473785995e-06, 2.707983311367457e-07,
+     -4.324538856989888e-07, -2.648274421485692e-08, 3.9515438249932484e-08,
+     2.4720260690299275e-09, 0.0},
+    {-6.511205244110998e-09, -7.076428529570776e-10, 1.0183620860162692e-09,
+     0.0, 0.0, 0.0, 0.0, 4.0},
+    {0.007868944223231958, 0.0014193060148561195, -0.003081157124604844,
+     -0.00025275745577540515, 0.0005030348400875631, 3.734766164626263e-05,
+     -7.780382338014487e-05, 1.0}};
+const std::vector<std::vector<double>> kSineRemainder = {
+    {-1.1324455480972702, -0.8116709016985189, -0.20294197180913795,
+     0.5290215620326779, -0.32084902738708476, 0.43458317383580536,
+     -0.06303880005650665, 0.0},
+    {0.23316927298939902, 0.12938749185636858, -0.3042902283756468,
+     -0.008967445477730096, -0.6955791340472846, -0.07562064123442451,
+     1.2510854562849574, 2.0},
+    {0.1429148264191405, 0.4544397978460022, -0.8401926425765848,
+     -0.4292803758141986, -1.495563076527614, 0.3580084476476745,
+     1.7971594278092713, 1.0}};
+constexpr int kBabyDegree = 7;
+constexpr int kGiantDegree = 3;
+constexpr int kDoubleAngleIterations = 3;
+constexpr int64_t kRoundBound = 28;
+
+// chebyshev_paterson_stockmeyer (src/ckks/engine_bootstrap.cpp:459-498)
+Text chebyshev_ps(TextEngine &e, const std::vector<Text> &baby,
+                  const Text &giant,
+                  const std::vector<std::vector<double>> &c, bool relin) {
+  Text baby0 = e.weighted_sum(baby, c[0]);
+  Text added = e.add(giant, baby0, false);
+  Text baby1 = e.weighted_sum(baby, c[1]);
+  Text multiplied = e.multiply(added, baby1, relin);
+  Text baby2 = e.weighted_sum(baby, c[2]);
+  return e.add(multiplied, baby2, false);
+}
+
+// sine (src/ckks/engine_bootstrap.cpp:526-638): Chebyshev basis T1..T7,
+// giant steps T7, T14, T28 and the Paterson-Stockmeyer evaluation
+Text sine(TextEngine &e, const Text &x) {
+  std::vector<Text> baby(kBabyDegree);
+  baby[0] = e.clone(x);
+  for (int bi = 2; bi <= kBabyDegree; bi++) {
+    const int index = bi - 1, half = bi / 2 - 1;
+    if ((bi & 1) == 0) {
+      Text m = e.multiply(baby[half], baby[half], true);
+      e.add_inplace(m, m);
+      baby[index] = e.add_double(m, -1);
+    } else {
+      Text m = e.multiply(baby[half], baby[half + 1], true);
+      e.add_inplace(m, m);
+      baby[index] = e.add(m, baby[0], true);
+    }
+  }
+  const int destination = baby.back().level;
+  for (int i = 0; i < kBabyDegree - 2; i++)
+    if (baby[i].level > destination)
+      baby[i] = e.level_down(baby[i], destination);
+
+  std::vector<Text> giant(kGiantDegree);
+  giant[0] = e.clone(baby.back());
+  for (int gi = 1; gi < kGiantDegree; gi++) {
+    Text m = e.multiply(giant[gi - 1], giant[gi - 1], true);
+    e.add_inplace(m, m);
+    giant[gi] = e.add_double(m, -1);
+  }
+  Text ps = e.clone(giant[0]);
+  for (int gi = 1; gi < kGiantDegree - 1; gi++) {
+    Text m = e.multiply(ps, giant[gi], true);
+    e.add_inplace(m, m);
+    ps = e.add(m, giant[0], true);
+  }
+  {
+    Text m = e.multiply(ps, giant[kGiantDegree - 1], false);
+    e.add_inplace(m, m);
+    ps = e.add(m, giant[0], true);
+  }
+  Text aux = e.weighted_sum(baby, kSineAuxiliary);
+  Text added = e.add(aux, giant.back(), false);
+  Text quotient =
+      chebyshev_ps(e, baby, giant[kGiantDegree - 2], kSineQuotient, true);
+  Text multiplied = e.multiply(added, quotient, false);
+  Text remainder =
+      chebyshev_ps(e, baby, giant[kGiantDegree - 2], kSineRemainder, false);
+  added = e.add(multiplied, remainder, false);
+  Text subtracted = e.add(added, ps, true);
+  return e.relinearize(subtracted);
+}
+
+// apply_double_angle_iterations (src/ckks/engine_bootstrap.cpp:640-659)
+Text double_angle(TextEngine &e, const Text &x) {
+  Text current = e.clone(x);
+  // 2 pi from the double nearest pi (std::numbers::pi in the reference)
+  const double two_pi = 2 * 3.141592653589793238462643383279502884;
+  for (int it = 1; it <= kDoubleAngleIterations; it++) {
+    Text m = e.multiply(current, current, true);
+    e.add_inplace(m, m);
+    const double scalar =
+        -std::pow(two_pi, -std::pow(2, it - kDoubleAngleIterations));
+    current = e.add_double(m, scalar);
+  }
+  return current;
+}
+
+} // namespace
+
+extern "C" {
+
+int fhe_round(fhe_context *ctx, const int64_t *in, int level,
+              const int64_t *key, int key_moduli_id, int64_t *out,
+              int *out_level, int *out_flags) {
+  return guarded([&] {
+    require(ctx && in && key && out && out_level && out_flags,
+            "null argument");
+    require(level >= 1 && level <= ctx->params.full_level,
+            "Round: the level must be positive.");
+    TextEngine e(ctx, key, key_moduli_id);
+    const Text x = e.wrap(in, 2, level, 0);
+    // round = x * ROUND_BOUND - modulo(x), modulo = double angle of sine
+    // (src/ckks/engine_bootstrap.cpp:661-691)
+    const Text decimal = double_angle(e, sine(e, x));
+    const Text multiplied = e.multiply_int(x, kRoundBound);
+    const Text result = e.add(multiplied, decimal, true);
+    hip_check(hipMemcpyAsync(out, result.data, size_t(e.words(result)) * 8,
+                             hipMemcpyDeviceToDevice, ctx->stream),
+              "copy");
+    *out_level = result.level;
+    *out_flags = result.flags;
+    ctx->check_launch("Round");
   });
 }
 

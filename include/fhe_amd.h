@@ -178,6 +178,18 @@ int fhe_add_texts(fhe_context *ctx, int count, const int64_t *a,
                   int b_polys, int b_level, int b_flags, int subtract,
                   int64_t *out);
 
+/* CKKSEngine::round (src/ckks/engine_bootstrap.cpp:672-691), the bootstrap's
+ * EvalMod: x * 28 - modulo(x), modulo = three double-angle steps
+ * (:640-659) after the Chebyshev / Paterson-Stockmeyer sine (:459-638),
+ * composed of the engine operations above with the reference's operation
+ * order.  in: one INTT-form, not rescaled duo [2][limbs][N] at `level`; the
+ * relinearization key at key_moduli_id.  out: room for [2][limbs][N]; the
+ * result's level and data flags come back in *out_level, *out_flags (its
+ * limbs are those of that form). */
+int fhe_round(fhe_context *ctx, const int64_t *in, int level,
+              const int64_t *key, int key_moduli_id, int64_t *out,
+              int *out_level, int *out_flags);
+
 /* Per-coefficient primitives over polys x limbs device limbs at moduli_id:
  * y = op(x1, x2, c[limb]) for the reference's limb operations
  * (src/montgomery/unit.cpp:5-86, src/montgomery/compound.cpp:6-109,

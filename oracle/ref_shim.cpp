@@ -878,4 +878,22 @@ void *ref_multiply_pt(void *handle, void *ct, void *pt) {
   });
 }
 
+// CKKSEngine::relinearize (src/ckks/engine_relinearize.cpp:13-28).
+void *ref_relinearize(void *handle, void *ct, void *rlk) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->relinearize(
+        static_cast<UnitCiphertext *>(ct),
+        static_cast<RelinearizationKey *>(rlk));
+  });
+}
+
+// CKKSEngine::round (src/ckks/engine_bootstrap.cpp:672-691).
+void *ref_round(void *handle, void *ct, void *rlk) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->round(
+        static_cast<UnitCiphertext *>(ct),
+        static_cast<RelinearizationKey *>(rlk));
+  });
+}
+
 } // extern "C"

@@ -389,6 +389,8 @@ class Reference:
             "ref_multiply_ct": (VP, [VP, VP, VP]),
             "ref_square": (VP, [VP, VP, VP]),
             "ref_multiply_pt": (VP, [VP, VP, VP]),
+            "ref_round": (VP, [VP, VP, VP]),
+            "ref_relinearize": (VP, [VP, VP, VP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
