@@ -34,7 +34,8 @@ EXPORTED_SYMBOLS = (
     "fhe_encode", "fhe_params_encode_constants", "fhe_decode",
     "fhe_decoded_scale", "fhe_context_set_option", "fhe_multiply_imaginary",
     "fhe_mod_raise", "fhe_multiply", "fhe_multiply_plain", "fhe_add_texts",
-    "fhe_round",
+    "fhe_round", "fhe_coeff_to_slot", "fhe_slot_to_coeff", "fhe_bootstrap",
+    "fhe_bootstrap_twiddles",
 )
 
 # data flags of a text (FHE_FLAG_* in include/fhe_amd.h)
@@ -73,6 +74,45 @@ class Params(ctypes.Structure):
 
 
 _lib = None
+
+
+class BootstrapKeys(ctypes.Structure):
+    """fhe_bootstrap_keys: device keys of a bootstrap with a RotationKey."""
+    _fields_ = [("relin", ctypes.c_void_p), ("relin_moduli_id", ctypes.c_int),
+                ("conjugation", ctypes.c_void_p),
+                ("conjugation_moduli_id", ctypes.c_int),
+                ("rotation_count", ctypes.c_int),
+                ("rotation", ctypes.POINTER(ctypes.c_void_p)),
+                ("rotation_moduli_ids", ctypes.POINTER(ctypes.c_int)),
+                ("rotation_deltas", ctypes.POINTER(ctypes.c_int)),
+                ("to_sparse", ctypes.c_void_p),
+                ("to_sparse_moduli_id", ctypes.c_int),
+                ("to_dense", ctypes.c_void_p),
+                ("to_dense_moduli_id", ctypes.c_int)]
+
+    @classmethod
+    def make(cls, relin, relin_mid, conj, conj_mid, rotations, to_sparse,
+             sparse_mid, to_dense, dense_mid):
+        """rotations: [(device array, moduli_id, delta)] in the key's
+        order; the arrays stay owned by the caller."""
+        m = len(rotations)
+        k = cls()
+        k._keep = ((ctypes.c_void_p * m)(*[r[0].ptr.value for r in rotations]),
+                   (ctypes.c_int * m)(*[r[1] for r in rotations]),
+                   (ctypes.c_int * m)(*[r[2] for r in rotations]))
+        k.relin, k.relin_moduli_id = relin.ptr.value, relin_mid
+        k.conjugation, k.conjugation_moduli_id = conj.ptr.value, conj_mid
+        k.rotation_count = m
+        k.rotation = ctypes.cast(k._keep[0], ctypes.POINTER(ctypes.c_void_p))
+        k.rotation_moduli_ids = ctypes.cast(k._keep[1],
+                                            ctypes.POINTER(ctypes.c_int))
+        k.rotation_deltas = ctypes.cast(k._keep[2],
+                                        ctypes.POINTER(ctypes.c_int))
+        k.to_sparse = to_sparse.ptr.value if to_sparse is not None else None
+        k.to_sparse_moduli_id = sparse_mid
+        k.to_dense = to_dense.ptr.value if to_dense is not None else None
+        k.to_dense_moduli_id = dense_mid
+        return k
 
 
 def library():
@@ -144,6 +184,15 @@ def library():
                                       i32, i32, vp]
         lib.fhe_round.argtypes = [vp, vp, i32, vp, i32, vp,
                                   ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        bk = ctypes.POINTER(BootstrapKeys)
+        lib.fhe_coeff_to_slot.argtypes = [vp, vp, i32, i32, bk, i32, vp, vp,
+                                          ctypes.POINTER(i32),
+                                          ctypes.POINTER(i32)]
+        lib.fhe_slot_to_coeff.argtypes = [vp, vp, vp, i32, i32, bk, i32, vp,
+                                          ctypes.POINTER(i32),
+                                          ctypes.POINTER(i32)]
+        lib.fhe_bootstrap.argtypes = [vp, vp, i32, i32, bk, i32, vp,
+                                      ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fhe_decode.argtypes = [vp, vp, i32, i32, ctypes.c_longdouble,
                                    i32, vp]
         lib.fhe_decoded_scale.argtypes = [vp, i32, ctypes.c_longdouble,
@@ -404,6 +453,31 @@ class Context:
         _check(library().fhe_round(self.handle, ct.ptr, level, key.ptr,
                                    key_moduli_id, out.ptr, ctypes.byref(lv),
                                    ctypes.byref(fl)))
+        return lv.value, fl.value
+
+    def coeff_to_slot(self, ct, level, flags, keys, stage_count, real, imag):
+        """fhe_coeff_to_slot of one duo; returns the parts' (level, flags)."""
+        lv, fl = ctypes.c_int(), ctypes.c_int()
+        _check(library().fhe_coeff_to_slot(
+            self.handle, ct.ptr, level, flags, ctypes.byref(keys),
+            stage_count, real.ptr, imag.ptr, ctypes.byref(lv),
+            ctypes.byref(fl)))
+        return lv.value, fl.value
+
+    def slot_to_coeff(self, real, imag, level, flags, keys, stage_count,
+                      out):
+        lv, fl = ctypes.c_int(), ctypes.c_int()
+        _check(library().fhe_slot_to_coeff(
+            self.handle, real.ptr, imag.ptr, level, flags, ctypes.byref(keys),
+            stage_count, out.ptr, ctypes.byref(lv), ctypes.byref(fl)))
+        return lv.value, fl.value
+
+    def bootstrap(self, ct, level, flags, keys, stage_count, out):
+        """fhe_bootstrap of one INTT-form duo; returns (level, flags)."""
+        lv, fl = ctypes.c_int(), ctypes.c_int()
+        _check(library().fhe_bootstrap(
+            self.handle, ct.ptr, level, flags, ctypes.byref(keys),
+            stage_count, out.ptr, ctypes.byref(lv), ctypes.byref(fl)))
         return lv.value, fl.value
 
     def automorphism(self, src, dst, moduli_id, delta, ntt_form=False):

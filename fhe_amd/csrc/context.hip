@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -2076,6 +2078,123 @@ public:
     return result;
   }
 
+  // --- operations with the bootstrap keys ---------------------------------
+  const fhe_bootstrap_keys *keys = nullptr;
+
+  bool at_top(const Text &a) const {
+    return a.level == ctx->params.full_level &&
+           !(a.flags & FHE_FLAG_RESCALED);
+  }
+  // the one-more-level chain's top text goes one level down before a key
+  // switch (src/ckks/engine_rotate.cpp:63-67), then leaves the NTT domain
+  Text key_switch_input(const Text &a) {
+    return intt(at_top(a) ? level_down(a, a.level - 1) : a);
+  }
+  // core rotate of a coefficient-domain text by one fixed rotation key
+  Text rotate_core(const Text &a, int index) {
+    const int mid = ctx->moduli_id_of(a.level, a.flags & FHE_FLAG_RESCALED);
+    Text out = alloc(a.polys, a.level, a.flags);
+    ok(fhe_rotate(ctx, 1, mid, a.data, keys->rotation[index],
+                  keys->rotation_moduli_ids[index], keys->rotation_deltas[index],
+                  out.data));
+    return out;
+  }
+  int normalized(int delta, bool by_slots) const {
+    const int m = by_slots ? int(ctx->params.slot_count())
+                           : int(ctx->n() / 2);
+    return ((delta % m) + m) % m;
+  }
+  // rotate(ct, RotationKey, delta) (src/ckks/engine_rotate.cpp:111-139):
+  // one fixed rotation per set bit, in the key's order
+  Text rotate(const Text &a, int delta, bool by_slots = true) {
+    const int d = normalized(delta, by_slots);
+    if (d == 0)
+      return a.flags & FHE_FLAG_NTT_FORM ? intt(a) : clone(a);
+    Text current = a;
+    bool first = true;
+    for (int k = 0; k < keys->rotation_count; k++) {
+      if ((d & keys->rotation_deltas[k]) == 0)
+        continue;
+      current = rotate_core(first ? key_switch_input(current) : current, k);
+      first = false;
+    }
+    return current;
+  }
+  // rotate_batch(ct, RotationKey, deltas) (src/ckks/engine_rotate.cpp:
+  // 305-393): chains of fixed rotations from the prepared input, shared
+  // prefixes computed once
+  std::vector<Text> rotate_batch(const Text &a, const std::vector<int> &deltas) {
+    const Text base = key_switch_input(a);
+    std::map<int, Text> done{{0, base}};
+    std::vector<Text> out;
+    for (int delta : deltas) {
+      const int d = normalized(delta, true);
+      for (int k = 0; k < keys->rotation_count && !done.count(d); k++) {
+        const int kd = keys->rotation_deltas[k];
+        if ((d & kd) == 0)
+          continue;
+        const int prev = d & (kd - 1), cur = prev + kd;
+        if (!done.count(cur))
+          done[cur] = rotate_core(done.at(prev), k);
+      }
+      out.push_back(done.at(d));
+    }
+    return out;
+  }
+  // multiply(ct, complex message) (src/ckks/engine_multiply.cpp:292-305):
+  // the message encoded at the text's level, then multiply(ct, pt)
+  Text multiply_message(const Text &a,
+                        const std::vector<std::complex<double>> &m) {
+    const int slots = int(ctx->params.slot_count());
+    require(int(m.size()) == slots, "Multiply: message size");
+    void *dv = nullptr;
+    hip_check(hipMalloc(&dv, size_t(slots) * 16), "message allocation");
+    owned.push_back(dv);
+    hip_check(hipMemcpyAsync(dv, m.data(), size_t(slots) * 16,
+                             hipMemcpyHostToDevice, ctx->stream),
+              "message copy");
+    Text pt = alloc(1, a.level, 0);
+    ok(fhe_encode(ctx, static_cast<const double *>(dv), 1, 1, a.level,
+                  pt.data));
+    // the host vector may go once the copy has been consumed
+    hip_check(hipStreamSynchronize(ctx->stream), "message sync");
+    Text out = alloc(a.polys, std::min(a.level, pt.level) - 1,
+                     FHE_FLAG_NTT_FORM);
+    ok(fhe_multiply_plain(ctx, 1, a.data, a.polys, a.level, a.flags, pt.data,
+                          pt.level, 0, out.data));
+    return out;
+  }
+  // conjugate (src/ckks/engine_conjugate.cpp)
+  Text conjugate(const Text &a) {
+    const Text in = key_switch_input(a);
+    const int mid = ctx->moduli_id_of(in.level, in.flags & FHE_FLAG_RESCALED);
+    Text out = alloc(in.polys, in.level, in.flags);
+    ok(fhe_conjugate(ctx, 1, mid, in.data, keys->conjugation,
+                     keys->conjugation_moduli_id, out.data));
+    return out;
+  }
+  // multiply_imaginary of a coefficient-domain, not rescaled text
+  // (src/ckks/engine_multiply.cpp:468-491)
+  Text multiply_imaginary(const Text &a, int64_t value) {
+    require(!(a.flags & (FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)),
+            "Multiply imaginary: an INTT-form, not rescaled text here");
+    Text out = alloc(a.polys, a.level, a.flags);
+    ok(fhe_multiply_imaginary(ctx, 1, ctx->moduli_id_of(a.level, false),
+                              a.data, value, out.data));
+    return out;
+  }
+  // mod_raise_ (src/ckks/engine_mod_raise.cpp:5-34) of a level-0 text
+  Text mod_raise(const Text &a) {
+    require(a.level == 0 && !(a.flags & ~0), "Mod raise: a level-0 text");
+    const auto &P = ctx->params;
+    const int level = P.max_key_switching_level();
+    Text out = alloc(2, level, 0);
+    ok(fhe_mod_raise(ctx, 1, a.data, keys->to_sparse,
+                     keys->to_sparse_moduli_id, keys->to_dense,
+                     keys->to_dense_moduli_id, out.data));
+    return out;
+  }
+
 private:
   fhe_context *ctx;
   const int64_t *key;
@@ -2113,6 +2232,8 @@ constexpr int kBabyDegree = 7;
 constexpr int kGiantDegree = 3;
 constexpr int kDoubleAngleIterations = 3;
 constexpr int64_t kRoundBound = 28;
+// the double nearest pi (std::numbers::pi in the reference)
+constexpr double kPi = 3.141592653589793238462643383279502884;
 
 // chebyshev_paterson_stockmeyer (src/ckks/engine_bootstrap.cpp:459-498)
 Text chebyshev_ps(TextEngine &e, const std::vector<Text> &baby,
@@ -2181,8 +2302,7 @@ Text sine(TextEngine &e, const Text &x) {
 // apply_double_angle_iterations (src/ckks/engine_bootstrap.cpp:640-659)
 Text double_angle(TextEngine &e, const Text &x) {
   Text current = e.clone(x);
-  // 2 pi from the double nearest pi (std::numbers::pi in the reference)
-  const double two_pi = 2 * 3.141592653589793238462643383279502884;
+  const double two_pi = 2 * kPi;
   for (int it = 1; it <= kDoubleAngleIterations; it++) {
     Text m = e.multiply(current, current, true);
     e.add_inplace(m, m);
@@ -2193,9 +2313,376 @@ Text double_angle(TextEngine &e, const Text &x) {
   return current;
 }
 
+using Complex = std::complex<double>;
+using Vectors = std::vector<std::vector<Complex>>;
+
+int positive_modulo(int x, int m) { return ((x % m) + m) % m; }
+
+// glibc's sincos through a volatile pointer (neither folded nor split into
+// sin and cos, whose results differ from it in the last bit for some x)
+void (*volatile libm_sincos_fn)(double, double *, double *) = ::sincos;
+
+// get_phi / get_inverse_phi (src/operation/bootstrap.cpp:9-33): the powers
+// of exp(-+ 2 pi i / n).  The reference's complex exponential of (0, y) is
+// glibc's cexp: (cos y, sin y) from one sincos call (exp(0) = 1 leaves them
+// unchanged), y itself as the sine when |y| is not above DBL_MIN.
+std::vector<Complex> phi_table(int n, bool inverse) {
+  std::vector<Complex> out(n);
+  const double pi_over = kPi / n;
+  for (int i = 0; i < n; i++) {
+    const double y = inverse ? 2 * pi_over * i : -2 * pi_over * i;
+    double sn = y, cs = 1;
+    if (std::fabs(y) > DBL_MIN)
+      libm_sincos_fn(y, &sn, &cs);
+    out[i] = Complex(cs, sn);
+  }
+  return out;
+}
+
+struct Twiddles {
+  Vectors center, left, right;
+};
+
+// get_twiddle_factor / get_inverse_twiddle_factor
+// (src/operation/bootstrap.cpp:35-123): the per-level butterfly diagonals
+// of the slot transform in the rotation-by-5 ordering
+Twiddles twiddle_factors(int slots, int log_slots, bool inverse) {
+  Twiddles t;
+  const auto phi = phi_table(slots << 2, inverse);
+  int jump_size = inverse ? 1 : slots;
+  for (int step = 0; step < log_slots; step++) {
+    const int log_index = inverse ? log_slots - 1 - step : step;
+    const int jump_count = 1 << log_index;
+    if (!inverse)
+      jump_size >>= 1;
+    std::vector<int> phi_index;
+    int power_of_five = 1;
+    for (int i = 0; i < jump_size; i++) {
+      phi_index.push_back(power_of_five * jump_count);
+      power_of_five *= 5;
+      power_of_five &= (jump_size << 3) - 1;
+    }
+    std::vector<Complex> center(slots, 0), left(slots, 0), right(slots, 0);
+    for (int j = 0; j < jump_count; j++) {
+      int index = 2 * j * jump_size;
+      for (int i = 0; i < jump_size; i++) {
+        const Complex value = phi[phi_index[i]];
+        center[index] = 1;
+        center[jump_size + index] = -value;
+        if (inverse) {
+          left[index] = value;
+          right[jump_size + index] = 1;
+        } else {
+          left[index] = 1;
+          right[jump_size + index] = value;
+        }
+        index++;
+      }
+    }
+    t.center.push_back(center);
+    t.left.push_back(left);
+    t.right.push_back(right);
+    if (inverse)
+      jump_size <<= 1;
+  }
+  return t;
+}
+
+// get_{coeff_to_slot,slot_to_coeff}_log_radix_count
+// (src/operation/bootstrap.h:23-39)
+int cts_log_radix(int stage, int stages, int log_slots) {
+  return stage >= stages - log_slots % stages ? log_slots / stages + 1
+                                              : log_slots / stages;
+}
+int stc_log_radix(int stage, int stages, int log_slots) {
+  return stage < log_slots % stages ? log_slots / stages + 1
+                                    : log_slots / stages;
+}
+
+// get_coeff_to_slot_twiddles / get_slot_to_coeff_twiddles
+// (src/operation/bootstrap.cpp:125-263): the merged diagonals of each
+// stage's radix-2^r butterflies, the last CtS stage normalized by
+// 2 * slots * 28
+std::vector<Vectors> stage_twiddles(int slots, int log_slots, int stages,
+                                    bool to_slot) {
+  const Twiddles f = twiddle_factors(slots, log_slots, !to_slot);
+  std::vector<Vectors> out;
+  int radix_jump = to_slot ? slots : 1;
+  int offset = 0;
+  for (int stage = 0; stage < stages; stage++) {
+    const int log_radix = to_slot ? cts_log_radix(stage, stages, log_slots)
+                                  : stc_log_radix(stage, stages, log_slots);
+    const int radix = 1 << log_radix;
+    const int total = 2 * radix - 1;
+    Vectors st(total, std::vector<Complex>(slots, 1));
+    for (int lr = 0; lr < log_radix; lr++) {
+      if (to_slot)
+        radix_jump >>= 1;
+      const int radix_index = 1 << lr;
+      const int jump = to_slot ? radix / (2 * radix_index) : radix_index;
+      const int ti = offset + lr;
+      Vectors prev(total, std::vector<Complex>(slots, 0));
+      st.swap(prev);
+      const auto &center = f.center[ti];
+      const auto &left = f.left[ti];
+      const auto &right = f.right[ti];
+      for (int index = -radix_index + 1; index < radix_index; index++) {
+        const int unit = to_slot ? index * radix / radix_index : index;
+        const int ci = positive_modulo(unit, total);
+        const int li = positive_modulo(unit - jump, total);
+        const int ri = positive_modulo(unit + jump, total);
+        for (int sl = 0; sl < slots; sl++) {
+          st[ci][sl] += center[sl] * prev[ci][sl];
+          st[li][sl] +=
+              left[sl] * prev[ci][positive_modulo(sl + radix_jump, slots)];
+          st[ri][sl] +=
+              right[sl] * prev[ci][positive_modulo(sl - radix_jump, slots)];
+        }
+      }
+      if (!to_slot)
+        radix_jump <<= 1;
+    }
+    if (to_slot && stage == stages - 1) {
+      const int normalizing = 2 * slots * int(kRoundBound);
+      for (auto &v : st)
+        for (auto &c : v)
+          c /= normalizing;
+    }
+    out.push_back(std::move(st));
+    offset += log_radix;
+  }
+  return out;
+}
+
+// get_stage_loop_counts (src/operation/bootstrap.cpp:265-278)
+std::pair<int, int> stage_loop_counts(int twiddle_count) {
+  int bits = 0;
+  for (int t = twiddle_count; t > 0; t >>= 1)
+    bits++;
+  const int inner = 1 << ((bits + 1) / 2);
+  return {inner, (twiddle_count + 1) / inner};
+}
+
+// rotate_vector (src/ckks/engine_bootstrap.cpp:856-870)
+std::vector<Complex> rotate_vector(const std::vector<Complex> &in,
+                                   int delta) {
+  const int count = int(in.size());
+  const int d = (delta + 2 * count) % count;
+  std::vector<Complex> out(count);
+  for (int i = 0; i < count; i++)
+    out[(i + d) % count] = in[i];
+  return out;
+}
+
+// apply_butterfly with a RotationKey (src/ckks/engine_bootstrap.cpp:
+// 155-215): baby-step rotations, diagonal products, giant-step rotations
+Text apply_butterfly(TextEngine &e, const Text &x, const Vectors &tw,
+                     int rotate_count) {
+  const int count = int(tw.size());
+  const auto [inner, outer] = stage_loop_counts(count);
+  std::vector<int> deltas;
+  for (int i = 0; i < inner; i++)
+    deltas.push_back(rotate_count * i);
+  const std::vector<Text> rotated = e.rotate_batch(x, deltas);
+  Text outer_sum;
+  bool have_outer = false;
+  for (int o = 0; o < outer; o++) {
+    const int nidx = o - outer / 2;
+    const int tf_delta = inner * nidx * rotate_count;
+    Text inner_sum;
+    bool have_inner = false;
+    for (int i = (o == 0 ? 1 : 0); i < inner; i++) {
+      const int ti = inner * nidx + i;
+      const auto factor =
+          rotate_vector(tw[positive_modulo(ti, count)], -tf_delta);
+      Text m = e.multiply_message(rotated[i], factor);
+      if (!have_inner) {
+        inner_sum = m;
+        have_inner = true;
+      } else {
+        e.add_inplace(inner_sum, m);
+      }
+    }
+    require(have_inner, "Bootstrap: an empty butterfly row");
+    Text r = e.rotate(inner_sum, tf_delta);
+    if (!have_outer) {
+      outer_sum = r;
+      have_outer = true;
+    } else {
+      e.add_inplace(outer_sum, r);
+    }
+  }
+  return outer_sum;
+}
+
+int log2_int(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v)
+    l++;
+  return l;
+}
+
+// coeff_to_slot with a RotationKey (src/ckks/engine_bootstrap.cpp:330-366)
+std::pair<Text, Text> coeff_to_slot(TextEngine &e, const Text &x, int slots,
+                                    int stages) {
+  const int log_slots = log2_int(slots);
+  const auto tw = stage_twiddles(slots, log_slots, stages, true);
+  int rotate_count = slots;
+  Text current = x;
+  for (int st = 0; st < stages; st++) {
+    rotate_count >>= cts_log_radix(st, stages, log_slots);
+    current = apply_butterfly(e, current, tw[st], rotate_count);
+  }
+  const Text conjugated = e.conjugate(current);
+  Text real = e.add(current, conjugated, false);
+  const Text subtracted = e.add(current, conjugated, true);
+  Text imag = e.multiply_imaginary(subtracted, -1);
+  return {real, imag};
+}
+
+// slot_to_coeff with a RotationKey (src/ckks/engine_bootstrap.cpp:405-434)
+Text slot_to_coeff(TextEngine &e, const Text &real, const Text &imag,
+                   int slots, int stages) {
+  const int log_slots = log2_int(slots);
+  const Text multiplied = e.multiply_imaginary(imag, 1);
+  Text current = e.add(real, multiplied, false);
+  const auto tw = stage_twiddles(slots, log_slots, stages, false);
+  int rotate_count = 1;
+  for (int st = 0; st < stages; st++) {
+    current = apply_butterfly(e, current, tw[st], rotate_count);
+    rotate_count <<= stc_log_radix(st, stages, log_slots);
+  }
+  return current;
+}
+
+Text round_text(TextEngine &e, const Text &x) {
+  // round = x * ROUND_BOUND - modulo(x), modulo = double angle of sine
+  // (src/ckks/engine_bootstrap.cpp:661-691)
+  const Text decimal = double_angle(e, sine(e, x));
+  const Text multiplied = e.multiply_int(x, kRoundBound);
+  return e.add(multiplied, decimal, true);
+}
+
+void copy_out(fhe_context *ctx, TextEngine &e, const Text &t, int64_t *out,
+              int *out_level, int *out_flags) {
+  hip_check(hipMemcpyAsync(out, t.data, size_t(e.words(t)) * 8,
+                           hipMemcpyDeviceToDevice, ctx->stream),
+            "copy");
+  *out_level = t.level;
+  *out_flags = t.flags;
+}
+
+void require_bootstrap_keys(const fhe_bootstrap_keys *k) {
+  require(k && k->relin && k->conjugation && k->rotation_count > 0 &&
+              k->rotation && k->rotation_moduli_ids && k->rotation_deltas,
+          "Bootstrap: keys missing");
+  for (int i = 0; i < k->rotation_count; i++)
+    require(k->rotation[i] != nullptr, "Bootstrap: rotation key missing");
+}
+
 } // namespace
 
 extern "C" {
+
+int fhe_bootstrap_twiddles(int log_slots, int stage_count, int to_slot,
+                           int stage, double *out, int64_t capacity,
+                           int64_t *count) {
+  return guarded([&] {
+    require(out && count && log_slots >= 1 && log_slots <= 16 &&
+                stage_count >= 1 && stage >= 0 && stage < stage_count,
+            "Bootstrap twiddles: arguments out of range");
+    const int slots = 1 << log_slots;
+    const auto tw = stage_twiddles(slots, log_slots, stage_count, to_slot != 0);
+    const auto &st = tw[stage];
+    *count = int64_t(st.size());
+    require(capacity >= int64_t(st.size()) * slots * 2,
+            "Bootstrap twiddles: capacity too small");
+    for (size_t r = 0; r < st.size(); r++)
+      std::memcpy(out + r * slots * 2, st[r].data(), size_t(slots) * 16);
+  });
+}
+
+int fhe_coeff_to_slot(fhe_context *ctx, const int64_t *in, int level,
+                      int flags, const fhe_bootstrap_keys *keys,
+                      int stage_count, int64_t *real, int64_t *imag,
+                      int *out_level, int *out_flags) {
+  return guarded([&] {
+    require(ctx && in && real && imag && out_level && out_flags,
+            "null argument");
+    require_bootstrap_keys(keys);
+    require(stage_count >= 1, "Coeff to slot: stage count");
+    TextEngine e(ctx, keys->relin, keys->relin_moduli_id);
+    e.keys = keys;
+    const Text x = e.wrap(in, 2, level, flags);
+    const auto [r, i] =
+        coeff_to_slot(e, x, int(ctx->params.slot_count()), stage_count);
+    int l2, f2;
+    copy_out(ctx, e, r, real, out_level, out_flags);
+    copy_out(ctx, e, i, imag, &l2, &f2);
+    require(l2 == *out_level && f2 == *out_flags, "Coeff to slot: forms");
+    ctx->check_launch("Coeff to slot");
+  });
+}
+
+int fhe_slot_to_coeff(fhe_context *ctx, const int64_t *real,
+                      const int64_t *imag, int level, int flags,
+                      const fhe_bootstrap_keys *keys, int stage_count,
+                      int64_t *out, int *out_level, int *out_flags) {
+  return guarded([&] {
+    require(ctx && real && imag && out && out_level && out_flags,
+            "null argument");
+    require_bootstrap_keys(keys);
+    require(stage_count >= 1, "Slot to coeff: stage count");
+    TextEngine e(ctx, keys->relin, keys->relin_moduli_id);
+    e.keys = keys;
+    const Text r = e.wrap(real, 2, level, flags);
+    const Text i = e.wrap(imag, 2, level, flags);
+    const Text c =
+        slot_to_coeff(e, r, i, int(ctx->params.slot_count()), stage_count);
+    copy_out(ctx, e, c, out, out_level, out_flags);
+    ctx->check_launch("Slot to coeff");
+  });
+}
+
+int fhe_bootstrap(fhe_context *ctx, const int64_t *in, int level, int flags,
+                  const fhe_bootstrap_keys *keys, int stage_count,
+                  int64_t *out, int *out_level, int *out_flags) {
+  return guarded([&] {
+    require(ctx && in && out && out_level && out_flags, "null argument");
+    require_bootstrap_keys(keys);
+    require(keys->to_sparse && keys->to_dense, "Bootstrap: keys missing");
+    require(!(flags & FHE_FLAG_NTT_FORM) && stage_count >= 1,
+            "Bootstrap: an INTT-form duo and a positive stage count");
+    const auto &P = ctx->params;
+    TextEngine e(ctx, keys->relin, keys->relin_moduli_id);
+    e.keys = keys;
+    const int slots = int(P.slot_count());
+    // CKKSEngine::bootstrap with a RotationKey
+    // (src/ckks/engine_bootstrap.cpp:712-754)
+    const Text x = e.wrap(in, 2, level, flags);
+    const Text leveled = e.level_down(x, 0);
+    Text raised = e.mod_raise(leveled);
+    if (slots != ctx->n() / 2) {
+      // partial_sum_inplace (src/ckks/engine_bootstrap.cpp:315-329)
+      for (int delta = slots; delta < ctx->n() / 2; delta <<= 1) {
+        const Text r = e.rotate(raised, delta, false);
+        e.add_inplace(raised, r);
+      }
+    }
+    const auto [real, imag] = coeff_to_slot(e, raised, slots, stage_count);
+    const Text rounded_real = round_text(e, real);
+    const Text rounded_imag = round_text(e, imag);
+    const Text coeff =
+        slot_to_coeff(e, rounded_real, rounded_imag, slots, stage_count);
+    const Text bootstrapped = e.add(raised, coeff, true);
+    // CKKSCore::get_normalize_ratio (src/ckks/core.cpp:1121-1125)
+    const int64_t ratio = static_cast<int64_t>(
+        std::round(double(P.q[P.chain_count - 1]) / P.scale));
+    const Text result = e.multiply_int(bootstrapped, ratio);
+    copy_out(ctx, e, result, out, out_level, out_flags);
+    ctx->check_launch("Bootstrap");
+  });
+}
 
 int fhe_round(fhe_context *ctx, const int64_t *in, int level,
               const int64_t *key, int key_moduli_id, int64_t *out,
@@ -2207,16 +2694,7 @@ int fhe_round(fhe_context *ctx, const int64_t *in, int level,
             "Round: the level must be positive.");
     TextEngine e(ctx, key, key_moduli_id);
     const Text x = e.wrap(in, 2, level, 0);
-    // round = x * ROUND_BOUND - modulo(x), modulo = double angle of sine
-    // (src/ckks/engine_bootstrap.cpp:661-691)
-    const Text decimal = double_angle(e, sine(e, x));
-    const Text multiplied = e.multiply_int(x, kRoundBound);
-    const Text result = e.add(multiplied, decimal, true);
-    hip_check(hipMemcpyAsync(out, result.data, size_t(e.words(result)) * 8,
-                             hipMemcpyDeviceToDevice, ctx->stream),
-              "copy");
-    *out_level = result.level;
-    *out_flags = result.flags;
+    copy_out(ctx, e, round_text(e, x), out, out_level, out_flags);
     ctx->check_launch("Round");
   });
 }

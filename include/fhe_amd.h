@@ -190,6 +190,64 @@ int fhe_round(fhe_context *ctx, const int64_t *in, int level,
               const int64_t *key, int key_moduli_id, int64_t *out,
               int *out_level, int *out_flags);
 
+/* The keys of a bootstrap with a RotationKey, all on the device: the
+ * relinearization and conjugation keys, the RotationKey's fixed rotation
+ * keys in its own order (src/data/data.h:163-174) with their deltas, and the
+ * SmallBootstrapKey's two key-switching keys (src/data/bootstrap.cpp). */
+typedef struct fhe_bootstrap_keys {
+  const int64_t *relin;
+  int relin_moduli_id;
+  const int64_t *conjugation;
+  int conjugation_moduli_id;
+  int rotation_count;
+  const int64_t *const *rotation;
+  const int *rotation_moduli_ids;
+  const int *rotation_deltas;
+  const int64_t *to_sparse;
+  int to_sparse_moduli_id;
+  const int64_t *to_dense;
+  int to_dense_moduli_id;
+} fhe_bootstrap_keys;
+
+/* Host only: the merged butterfly diagonals of one stage
+ * (get_coeff_to_slot_twiddles / get_slot_to_coeff_twiddles,
+ * src/operation/bootstrap.cpp:125-263) as {re, im} doubles, *count vectors
+ * of 2^log_slots entries each. */
+int fhe_bootstrap_twiddles(int log_slots, int stage_count, int to_slot,
+                           int stage, double *out, int64_t capacity,
+                           int64_t *count);
+
+/* CKKSEngine::coeff_to_slot with a RotationKey
+ * (src/ckks/engine_bootstrap.cpp:330-366): stage_count butterfly stages
+ * (baby-step rotations, products with the merged diagonals of
+ * src/operation/bootstrap.cpp:125-218, giant-step rotations), then the real
+ * and imaginary parts by conjugation.  in: one duo at (level, flags); real
+ * and imag receive one duo each (room for the input's size), both of the
+ * form returned in *out_level, *out_flags. */
+int fhe_coeff_to_slot(fhe_context *ctx, const int64_t *in, int level,
+                      int flags, const fhe_bootstrap_keys *keys,
+                      int stage_count, int64_t *real, int64_t *imag,
+                      int *out_level, int *out_flags);
+
+/* CKKSEngine::slot_to_coeff with a RotationKey
+ * (src/ckks/engine_bootstrap.cpp:405-434): real + i * imag, then the
+ * inverse butterfly stages (src/operation/bootstrap.cpp:220-263).  real and
+ * imag share the form (level, flags). */
+int fhe_slot_to_coeff(fhe_context *ctx, const int64_t *real,
+                      const int64_t *imag, int level, int flags,
+                      const fhe_bootstrap_keys *keys, int stage_count,
+                      int64_t *out, int *out_level, int *out_flags);
+
+/* CKKSEngine::bootstrap with a RotationKey and a SmallBootstrapKey
+ * (src/ckks/engine_bootstrap.cpp:712-754): level down to 0, mod_raise,
+ * (partial sums below full slots), coeff_to_slot, round of both parts,
+ * slot_to_coeff, subtraction from the raised text and the normalizing
+ * integer product.  in: one INTT-form duo at (level, flags); out: room for
+ * the raised size; its form comes back. */
+int fhe_bootstrap(fhe_context *ctx, const int64_t *in, int level, int flags,
+                  const fhe_bootstrap_keys *keys, int stage_count,
+                  int64_t *out, int *out_level, int *out_flags);
+
 /* Per-coefficient primitives over polys x limbs device limbs at moduli_id:
  * y = op(x1, x2, c[limb]) for the reference's limb operations
  * (src/montgomery/unit.cpp:5-86, src/montgomery/compound.cpp:6-109,

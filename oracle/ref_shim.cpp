@@ -30,6 +30,7 @@
 #include "parameter/multiplier.h"
 #include "operation/ntt.h"
 #include "operation/fft.h"
+#include "operation/bootstrap.h"
 #include "arithmetic/pi.h"
 
 using namespace desilo::fhe;
@@ -884,6 +885,73 @@ void *ref_relinearize(void *handle, void *ct, void *rlk) {
     return static_cast<RefEngine *>(handle)->engine->relinearize(
         static_cast<UnitCiphertext *>(ct),
         static_cast<RelinearizationKey *>(rlk));
+  });
+}
+
+// RotationKey (src/ckks/engine.cpp:125-138): its fixed rotation keys, in
+// the key's own order, through the FixedRotationKey accessors above.
+void *ref_rotation_key(void *handle, void *sk) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->create_rotation_key(
+        static_cast<SecretKey *>(sk));
+  });
+}
+void ref_free_rotation_key(void *p) { delete static_cast<RotationKey *>(p); }
+int ref_rotation_key_count(void *rk) {
+  return int(static_cast<RotationKey *>(rk)->fixed_rotation_keys_.size());
+}
+void *ref_rotation_key_fixed(void *rk, int index) {
+  return static_cast<RotationKey *>(rk)->fixed_rotation_keys_.at(index);
+}
+
+// CKKSEngine::coeff_to_slot / slot_to_coeff / bootstrap with a RotationKey
+// (src/ckks/engine_bootstrap.cpp:330-454, 712-754).
+int ref_coeff_to_slot(void *handle, void *ct, void *conj, void *rk,
+                      int stage_count, void **real, void **imag) {
+  return guarded([&]() -> int {
+    auto [r, i] = static_cast<RefEngine *>(handle)->engine->coeff_to_slot(
+        static_cast<UnitCiphertext *>(ct), static_cast<ConjugationKey *>(conj),
+        static_cast<RotationKey *>(rk), stage_count);
+    *real = r;
+    *imag = i;
+    return 0;
+  });
+}
+void *ref_slot_to_coeff(void *handle, void *real, void *imag, void *conj,
+                        void *rk, int stage_count) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->slot_to_coeff(
+        static_cast<UnitCiphertext *>(real),
+        static_cast<UnitCiphertext *>(imag),
+        static_cast<ConjugationKey *>(conj), static_cast<RotationKey *>(rk),
+        stage_count);
+  });
+}
+void *ref_bootstrap(void *handle, void *ct, void *rlk, void *conj, void *rk,
+                    void *sbk, int stage_count) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->bootstrap(
+        static_cast<UnitCiphertext *>(ct),
+        static_cast<RelinearizationKey *>(rlk),
+        static_cast<ConjugationKey *>(conj), static_cast<RotationKey *>(rk),
+        static_cast<SmallBootstrapKey *>(sbk), stage_count);
+  });
+}
+
+// get_coeff_to_slot_twiddles / get_slot_to_coeff_twiddles
+// (src/operation/bootstrap.cpp:125-263), one stage as {re, im} pairs.
+int ref_bootstrap_twiddles(int log_slots, int stage_count, int to_slot,
+                           int stage, double *out) {
+  return guarded([&]() -> int {
+    const int slots = 1 << log_slots;
+    const auto tw =
+        to_slot ? get_coeff_to_slot_twiddles(slots, log_slots, stage_count)
+                : get_slot_to_coeff_twiddles(slots, log_slots, stage_count);
+    const auto &st = tw.at(stage);
+    if (out)
+      for (size_t r = 0; r < st.size(); r++)
+        std::memcpy(out + r * slots * 2, st[r].data(), size_t(slots) * 16);
+    return int(st.size());
   });
 }
 

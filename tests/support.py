@@ -391,6 +391,14 @@ class Reference:
             "ref_multiply_pt": (VP, [VP, VP, VP]),
             "ref_round": (VP, [VP, VP, VP]),
             "ref_relinearize": (VP, [VP, VP, VP]),
+            "ref_rotation_key": (VP, [VP, VP]),
+            "ref_free_rotation_key": (None, [VP]),
+            "ref_rotation_key_count": (i32, [VP]),
+            "ref_rotation_key_fixed": (VP, [VP, i32]),
+            "ref_coeff_to_slot": (i32, [VP, VP, VP, VP, i32, VP, VP]),
+            "ref_slot_to_coeff": (VP, [VP, VP, VP, VP, VP, i32]),
+            "ref_bootstrap": (VP, [VP, VP, VP, VP, VP, VP, i32]),
+            "ref_fixed_rotation_key_delta": (i32, [VP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
