@@ -221,6 +221,53 @@ def codec_bench(ctx, ev, batch, steps, with_cpu):
     return out
 
 
+def bootstrap_bench(ev, steps):
+    """One bootstrap with a RotationKey (fhe_bootstrap: level down,
+    mod_raise, coeff_to_slot, EvalMod of both parts, slot_to_coeff,
+    normalization) on the N = 2^16 bootstrap preset, wall-clock per
+    bootstrap including its host work.  Synthetic keys of the real shapes
+    (one random key stands for every key of the key-switching level: the
+    work does not depend on key values) and a random level-2 input."""
+    import fhe_amd
+    rng = np.random.Generator(np.random.PCG64(11))
+    with fhe_amd.Context(LOG_N, ALPHA, scale_bits=54,
+                         small_prime_bits=54) as ctx:
+        full = ctx.info["full_level"]
+        key_mid = ctx.key_moduli_id()
+        kp, kc, n = ctx.key_shape(key_mid)
+        key = np.stack([uniform_limbs(rng, ctx.primes, key_mid, kc, n)
+                        for _ in range(kp)])
+        sparse_mid = full
+        sp, sc, _ = ctx.key_shape(sparse_mid)
+        sparse = np.stack([uniform_limbs(rng, ctx.primes, sparse_mid, sc, n)
+                           for _ in range(sp)])
+        dk, dsp = ctx.to_device(key), ctx.to_device(sparse)
+        del key, sparse
+        rotations = [(dk, key_mid, 1 << i) for i in range(LOG_N - 1)]
+        keys = fhe_amd.BootstrapKeys.make(dk, key_mid, dk, key_mid, rotations,
+                                          dsp, sparse_mid, dk, key_mid)
+        level = 2
+        mid = full - level
+        ct = np.stack([uniform_limbs(rng, ctx.primes, mid, ctx.limbs_at(mid),
+                                     n) for _ in range(2)])
+        dct = ctx.to_device(ct)
+        out = ctx.array((2, ctx.chain_count - 1, n))
+        ctx.bootstrap(dct, level, 0, keys, 3, out)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.bootstrap(dct, level, 0, keys, 3, out)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        for arr in (dk, dsp, dct, out):
+            arr.free()
+    return {"workload": "bootstrap with a RotationKey (15 power-of-two "
+                        "keys), N=2^16 bootstrap preset, 2^15 slots, 3 "
+                        "CtS / StC stages, one ciphertext from level 2",
+            "ms_per_bootstrap": round(ms, 1), "steps": steps,
+            "timing": "wall clock, host work included"}
+
+
 # Algorithmic HBM bytes of one launch of each single-kernel stage (read each
 # input limb once, write each output limb once; N * 8 bytes per limb) and
 # the modular products it performs (one per butterfly, one per pointwise
@@ -308,6 +355,8 @@ def main():
                          "launches only")
     ap.add_argument("--no-codec", action="store_true",
                     help="skip the encode / decode measurement")
+    ap.add_argument("--no-bootstrap", action="store_true",
+                    help="skip the bootstrap measurement")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times "
                          "(the reference kernel for scaling the FETCH_SIZE "
@@ -360,6 +409,7 @@ def main():
             codec = codec_bench(ctx, ev, args.batch, max(5, args.steps // 2),
                                 not args.no_cpu)
 
+    boot = None if args.no_bootstrap else bootstrap_bench(ev, 3)
     value = args.batch / (step_ms / 1e3)
     shape = dict(n=n, count=args.batch, limbs=limbs, ext=limbs + ALPHA,
                  parts=parts)
@@ -388,6 +438,7 @@ def main():
         "roofline": roofline,
         "stages": breakdown,
         "encode_decode": codec,
+        "bootstrap": boot,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()

@@ -6,6 +6,7 @@
 #include <cfloat>
 #include <cmath>
 #include <complex>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -130,6 +131,25 @@ struct fhe_context {
   fhe::DecodeDevice decode{};
   bool decode_ready = false;
   std::map<int, fhe::DecodeCrtDevice> decode_crt_cache;
+  // device buffers of the engine-level compositions, kept by size for
+  // reuse by later calls
+  std::multimap<size_t, void *> text_pool;
+  // bootstrap slot-transform diagonals per (log slots, stages, direction)
+  std::map<std::tuple<int, int, bool>,
+           std::vector<std::vector<std::vector<std::complex<double>>>>>
+      boot_twiddles;
+
+  void *pool_take(size_t bytes) {
+    auto it = text_pool.find(bytes);
+    if (it != text_pool.end()) {
+      void *p = it->second;
+      text_pool.erase(it);
+      return p;
+    }
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, bytes), "text allocation");
+    return p;
+  }
   // stage timing: HIP events recorded on `stream` around each stage while
   // enabled, folded into per-stage totals when read
   bool timing = false;
@@ -179,6 +199,8 @@ struct fhe_context {
       hipFree(kv.second);
     for (void *p : owned)
       hipFree(p);
+    for (auto &kv : text_pool)
+      hipFree(kv.second);
     if (workspace)
       hipFree(workspace);
     if (aux)
@@ -1911,11 +1933,13 @@ public:
   TextEngine(fhe_context *c, const int64_t *rlk, int rlk_mid)
       : ctx(c), key(rlk), key_mid(rlk_mid) {}
   ~TextEngine() {
+    // the buffers go back to the context's pool once the stream is idle
     (void)hipStreamSynchronize(ctx->stream);
-    for (void *p : owned)
-      (void)hipFree(p);
+    for (auto &o : owned)
+      ctx->text_pool.emplace(o.second, o.first);
   }
 
+  fhe_context *context() const { return ctx; }
   int limbs(int level, int flags) const {
     return ctx->params.limbs_at(
         ctx->moduli_id_of(level, (flags & FHE_FLAG_RESCALED) != 0));
@@ -1928,10 +1952,7 @@ public:
     t.polys = polys;
     t.level = level;
     t.flags = flags;
-    void *p = nullptr;
-    hip_check(hipMalloc(&p, size_t(words(t)) * 8), "text allocation");
-    owned.push_back(p);
-    t.data = static_cast<int64_t *>(p);
+    t.data = static_cast<int64_t *>(take(size_t(words(t)) * 8));
     return t;
   }
   Text wrap(const int64_t *data, int polys, int level, int flags) {
@@ -2141,27 +2162,31 @@ public:
     }
     return out;
   }
-  // multiply(ct, complex message) (src/ckks/engine_multiply.cpp:292-305):
-  // the message encoded at the text's level, then multiply(ct, pt)
-  Text multiply_message(const Text &a,
-                        const std::vector<std::complex<double>> &m) {
-    const int slots = int(ctx->params.slot_count());
-    require(int(m.size()) == slots, "Multiply: message size");
-    void *dv = nullptr;
-    hip_check(hipMalloc(&dv, size_t(slots) * 16), "message allocation");
-    owned.push_back(dv);
-    hip_check(hipMemcpyAsync(dv, m.data(), size_t(slots) * 16,
-                             hipMemcpyHostToDevice, ctx->stream),
+  // encode_unit of `count` complex messages [count][slots] at `level`
+  // (src/ckks/engine_encode.cpp:10-36), one batched device encode
+  Text encode_messages(const std::vector<std::complex<double>> &m, int count,
+                       int level) {
+    const size_t bytes = m.size() * sizeof(std::complex<double>);
+    void *dv = take(bytes);
+    hip_check(hipMemcpyAsync(dv, m.data(), bytes, hipMemcpyHostToDevice,
+                             ctx->stream),
               "message copy");
-    Text pt = alloc(1, a.level, 0);
-    ok(fhe_encode(ctx, static_cast<const double *>(dv), 1, 1, a.level,
+    Text pt = alloc(count, level, 0);
+    ok(fhe_encode(ctx, static_cast<const double *>(dv), 1, count, level,
                   pt.data));
-    // the host vector may go once the copy has been consumed
+    // the host messages may go once the stream has consumed them
     hip_check(hipStreamSynchronize(ctx->stream), "message sync");
-    Text out = alloc(a.polys, std::min(a.level, pt.level) - 1,
-                     FHE_FLAG_NTT_FORM);
-    ok(fhe_multiply_plain(ctx, 1, a.data, a.polys, a.level, a.flags, pt.data,
-                          pt.level, 0, out.data));
+    return pt;
+  }
+  // multiply(ct, message) (src/ckks/engine_multiply.cpp:292-305) with the
+  // message's plaintext at the text's level, polynomial `index` of pts
+  Text multiply_encoded(const Text &a, const Text &pts, int index) {
+    require(pts.level == a.level, "Multiply: plaintext level");
+    const int64_t pt_words = int64_t(limbs(pts.level, 0)) * ctx->n();
+    Text out = alloc(a.polys, a.level - 1, FHE_FLAG_NTT_FORM);
+    ok(fhe_multiply_plain(ctx, 1, a.data, a.polys, a.level, a.flags,
+                          pts.data + index * pt_words, pts.level, 0,
+                          out.data));
     return out;
   }
   // conjugate (src/ckks/engine_conjugate.cpp)
@@ -2199,7 +2224,14 @@ private:
   fhe_context *ctx;
   const int64_t *key;
   int key_mid;
-  std::vector<void *> owned;
+  std::vector<std::pair<void *, size_t>> owned;
+
+public:
+  void *take(size_t bytes) {
+    void *p = ctx->pool_take(bytes);
+    owned.emplace_back(p, bytes);
+    return p;
+  }
 };
 
 // The Chebyshev coefficients of the reference's sine approximation
@@ -2484,6 +2516,22 @@ Text apply_butterfly(TextEngine &e, const Text &x, const Vectors &tw,
   for (int i = 0; i < inner; i++)
     deltas.push_back(rotate_count * i);
   const std::vector<Text> rotated = e.rotate_batch(x, deltas);
+  // every (outer, inner) diagonal, rotated as the reference rotates it,
+  // encoded in one batch at the rotated texts' level
+  const int slots = int(tw[0].size());
+  std::vector<Complex> factors;
+  for (int o = 0; o < outer; o++) {
+    const int nidx = o - outer / 2;
+    const int tf_delta = inner * nidx * rotate_count;
+    for (int i = (o == 0 ? 1 : 0); i < inner; i++) {
+      const int ti = inner * nidx + i;
+      const auto f = rotate_vector(tw[positive_modulo(ti, count)], -tf_delta);
+      factors.insert(factors.end(), f.begin(), f.end());
+    }
+  }
+  const Text pts = e.encode_messages(factors, int(factors.size() / slots),
+                                     rotated[0].level);
+  int next_pt = 0;
   Text outer_sum;
   bool have_outer = false;
   for (int o = 0; o < outer; o++) {
@@ -2492,10 +2540,7 @@ Text apply_butterfly(TextEngine &e, const Text &x, const Vectors &tw,
     Text inner_sum;
     bool have_inner = false;
     for (int i = (o == 0 ? 1 : 0); i < inner; i++) {
-      const int ti = inner * nidx + i;
-      const auto factor =
-          rotate_vector(tw[positive_modulo(ti, count)], -tf_delta);
-      Text m = e.multiply_message(rotated[i], factor);
+      Text m = e.multiply_encoded(rotated[i], pts, next_pt++);
       if (!have_inner) {
         inner_sum = m;
         have_inner = true;
@@ -2523,10 +2568,22 @@ int log2_int(int64_t v) {
 }
 
 // coeff_to_slot with a RotationKey (src/ckks/engine_bootstrap.cpp:330-366)
+const std::vector<Vectors> &cached_twiddles(fhe_context *ctx, int slots,
+                                            int stages, bool to_slot) {
+  const int log_slots = log2_int(slots);
+  const auto key = std::make_tuple(log_slots, stages, to_slot);
+  auto it = ctx->boot_twiddles.find(key);
+  if (it == ctx->boot_twiddles.end())
+    it = ctx->boot_twiddles
+             .emplace(key, stage_twiddles(slots, log_slots, stages, to_slot))
+             .first;
+  return it->second;
+}
+
 std::pair<Text, Text> coeff_to_slot(TextEngine &e, const Text &x, int slots,
                                     int stages) {
   const int log_slots = log2_int(slots);
-  const auto tw = stage_twiddles(slots, log_slots, stages, true);
+  const auto &tw = cached_twiddles(e.context(), slots, stages, true);
   int rotate_count = slots;
   Text current = x;
   for (int st = 0; st < stages; st++) {
@@ -2546,7 +2603,7 @@ Text slot_to_coeff(TextEngine &e, const Text &real, const Text &imag,
   const int log_slots = log2_int(slots);
   const Text multiplied = e.multiply_imaginary(imag, 1);
   Text current = e.add(real, multiplied, false);
-  const auto tw = stage_twiddles(slots, log_slots, stages, false);
+  const auto &tw = cached_twiddles(e.context(), slots, stages, false);
   int rotate_count = 1;
   for (int st = 0; st < stages; st++) {
     current = apply_butterfly(e, current, tw[st], rotate_count);
