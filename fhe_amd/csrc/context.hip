@@ -2178,15 +2178,40 @@ public:
     hip_check(hipStreamSynchronize(ctx->stream), "message sync");
     return pt;
   }
-  // multiply(ct, message) (src/ckks/engine_multiply.cpp:292-305) with the
-  // message's plaintext at the text's level, polynomial `index` of pts
-  Text multiply_encoded(const Text &a, const Text &pts, int index) {
-    require(pts.level == a.level, "Multiply: plaintext level");
-    const int64_t pt_words = int64_t(limbs(pts.level, 0)) * ctx->n();
-    Text out = alloc(a.polys, a.level - 1, FHE_FLAG_NTT_FORM);
-    ok(fhe_multiply_plain(ctx, 1, a.data, a.polys, a.level, a.flags,
-                          pts.data + index * pt_words, pts.level, 0,
-                          out.data));
+  // prepare_for_multiply (src/ckks/engine_multiply.cpp:13-27) of a text at
+  // its own level: rescale_ and ntt_; every polynomial of a batch of
+  // plaintexts in one pass
+  Text prepare(const Text &a) {
+    const bool rescaled = (a.flags & FHE_FLAG_RESCALED) != 0;
+    const bool ntt = (a.flags & FHE_FLAG_NTT_FORM) != 0;
+    if (rescaled && ntt)
+      return a;
+    Text out = alloc(a.polys, a.level, FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED);
+    const int64_t t_l = words(a);
+    int64_t *t0 = ctx->scratch(size_t(2 * t_l) * 8);
+    const int64_t *r = ctx->prepare_for_multiply_into(
+        a.data, a.polys, a.level, a.flags, a.level, out.data, t0, t0 + t_l);
+    require(r == out.data, "Prepare: unexpected form");
+    return out;
+  }
+  // CKKSCore::multiply(ct, pt) (src/ckks/core.cpp:257-282) of a prepared
+  // duo and plaintext `index` of a prepared batch: the product of
+  // multiply(ct, message) (src/ckks/engine_multiply.cpp:292-305), NTT form,
+  // not rescaled, one level down
+  Text multiply_prepared(const Text &ct, const Text &pts, int index) {
+    require(ct.level == pts.level && ct.flags == pts.flags &&
+                ct.flags == (FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED),
+            "Multiply: prepared operands expected");
+    const int mid = ctx->moduli_id_of(ct.level, true);
+    const int l = ctx->params.limbs_at(mid);
+    const int64_t poly = int64_t(l) * ctx->n();
+    Text out = alloc(ct.polys, ct.level - 1, FHE_FLAG_NTT_FORM);
+    ctx->stage("mont_mult", [&] {
+      for (int p = 0; p < ct.polys; p++)
+        fhe::launch_mont_mult(ctx->tables, ct.data + p * poly,
+                              pts.data + index * poly, out.data + p * poly, l,
+                              l, mid, ctx->stream);
+    });
     return out;
   }
   // conjugate (src/ckks/engine_conjugate.cpp)
@@ -2529,8 +2554,12 @@ Text apply_butterfly(TextEngine &e, const Text &x, const Vectors &tw,
       factors.insert(factors.end(), f.begin(), f.end());
     }
   }
-  const Text pts = e.encode_messages(factors, int(factors.size() / slots),
-                                     rotated[0].level);
+  const Text pts = e.prepare(e.encode_messages(
+      factors, int(factors.size() / slots), rotated[0].level));
+  // each baby-step text prepared once for all of its products
+  std::vector<Text> prepared;
+  for (const Text &r : rotated)
+    prepared.push_back(e.prepare(r));
   int next_pt = 0;
   Text outer_sum;
   bool have_outer = false;
@@ -2540,7 +2569,7 @@ Text apply_butterfly(TextEngine &e, const Text &x, const Vectors &tw,
     Text inner_sum;
     bool have_inner = false;
     for (int i = (o == 0 ? 1 : 0); i < inner; i++) {
-      Text m = e.multiply_encoded(rotated[i], pts, next_pt++);
+      Text m = e.multiply_prepared(prepared[i], pts, next_pt++);
       if (!have_inner) {
         inner_sum = m;
         have_inner = true;
