@@ -35,8 +35,20 @@ EXPORTED_SYMBOLS = (
     "fhe_decoded_scale", "fhe_context_set_option", "fhe_multiply_imaginary",
     "fhe_mod_raise", "fhe_multiply", "fhe_multiply_plain", "fhe_add_texts",
     "fhe_round", "fhe_coeff_to_slot", "fhe_slot_to_coeff", "fhe_bootstrap",
-    "fhe_bootstrap_twiddles",
+    "fhe_bootstrap_twiddles", "fhe_wire_size", "fhe_wire_write",
+    "fhe_wire_read_header", "fhe_wire_read",
 )
+
+# fhe_wire_* record kinds (FHE_WIRE_* in include/fhe_amd.h)
+WIRE = dict(PLAINTEXT=0, CIPHERTEXT=1, KEY=2)
+
+
+class WireHeader(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("polys", ctypes.c_int),
+                ("moduli_id", ctypes.c_int), ("limbs", ctypes.c_int),
+                ("level", ctypes.c_int), ("flags", ctypes.c_int),
+                ("scale", ctypes.c_longdouble),
+                ("payload_bytes", ctypes.c_size_t)]
 
 # data flags of a text (FHE_FLAG_* in include/fhe_amd.h)
 FLAG_NTT_FORM = 2
@@ -184,6 +196,13 @@ def library():
                                       i32, i32, vp]
         lib.fhe_round.argtypes = [vp, vp, i32, vp, i32, vp,
                                   ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        sz_p = ctypes.POINTER(ctypes.c_size_t)
+        lib.fhe_wire_size.argtypes = [vp, i32, i32, sz_p]
+        lib.fhe_wire_write.argtypes = [vp, i32, vp, i32, i32, i32, i32, i32,
+                                       ctypes.c_longdouble, vp, sz, sz_p]
+        lib.fhe_wire_read_header.argtypes = [vp, vp, sz,
+                                             ctypes.POINTER(WireHeader)]
+        lib.fhe_wire_read.argtypes = [vp, vp, sz, vp, sz]
         bk = ctypes.POINTER(BootstrapKeys)
         lib.fhe_coeff_to_slot.argtypes = [vp, vp, i32, i32, bk, i32, vp, vp,
                                           ctypes.POINTER(i32),
@@ -479,6 +498,36 @@ class Context:
             self.handle, ct.ptr, level, flags, ctypes.byref(keys),
             stage_count, out.ptr, ctypes.byref(lv), ctypes.byref(fl)))
         return lv.value, fl.value
+
+    def wire_write(self, kind, arr, moduli_id, level, flags, scale):
+        """Serialize a device array [polys][limbs][N] into bytes."""
+        polys, limbs = int(np.prod(arr.shape[:-2])), int(arr.shape[-2])
+        size = ctypes.c_size_t()
+        _check(library().fhe_wire_size(self.handle, polys, limbs,
+                                       ctypes.byref(size)))
+        buf = (ctypes.c_uint8 * size.value)()
+        written = ctypes.c_size_t()
+        _check(library().fhe_wire_write(
+            self.handle, kind, arr.ptr, polys, moduli_id, limbs, level, flags,
+            _longdouble(scale), buf, size.value, ctypes.byref(written)))
+        return bytes(buf)
+
+    def wire_header(self, data):
+        """The record's header as a dict; the scale as np.longdouble (a
+        ctypes c_longdouble field reads back as a double)."""
+        h = WireHeader()
+        _check(library().fhe_wire_read_header(self.handle, data, len(data),
+                                              ctypes.byref(h)))
+        raw = (ctypes.c_char * 16).from_address(ctypes.addressof(h) +
+                                                WireHeader.scale.offset)
+        scale = np.frombuffer(bytes(raw), dtype=np.longdouble, count=1)[0]
+        return dict(kind=h.kind, polys=h.polys, moduli_id=h.moduli_id,
+                    limbs=h.limbs, level=h.level, flags=h.flags, scale=scale,
+                    payload_bytes=h.payload_bytes)
+
+    def wire_read(self, data, out):
+        _check(library().fhe_wire_read(self.handle, data, len(data), out.ptr,
+                                       out.nbytes))
 
     def automorphism(self, src, dst, moduli_id, delta, ntt_form=False):
         polys, limbs = _polys_limbs(src, self.n)

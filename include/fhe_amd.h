@@ -483,6 +483,32 @@ int fhe_profile_count(fhe_context *ctx, int *count);
 int fhe_profile_entry(fhe_context *ctx, int index, char *name,
                       int name_capacity, double *total_ms, int64_t *launches);
 
+/* Wire format of device data for storage and exchange.  The reference has
+ * none (Data::write, src/data/data.h:66, is declared and never defined);
+ * this one records the prime chain (a fingerprint of the primes and log2 N),
+ * the shape and form (polys, moduli_id, limbs, level, data flags), the
+ * long double scale and a payload checksum ahead of the limbs in the
+ * reference's [poly][limb][N] order.  fhe_wire_write copies the device
+ * limbs into a host buffer of fhe_wire_size bytes; fhe_wire_read checks the
+ * header and the checksum against this context and copies the limbs to a
+ * device buffer. */
+enum { FHE_WIRE_PLAINTEXT = 0, FHE_WIRE_CIPHERTEXT = 1, FHE_WIRE_KEY = 2 };
+typedef struct fhe_wire_header {
+  int kind, polys, moduli_id, limbs, level, flags;
+  long double scale;
+  size_t payload_bytes;
+} fhe_wire_header;
+int fhe_wire_size(const fhe_context *ctx, int polys, int limbs,
+                  size_t *bytes);
+int fhe_wire_write(fhe_context *ctx, int kind, const int64_t *data, int polys,
+                   int moduli_id, int limbs, int level, int flags,
+                   long double scale, void *buffer, size_t capacity,
+                   size_t *written);
+int fhe_wire_read_header(const fhe_context *ctx, const void *buffer,
+                         size_t size, fhe_wire_header *header);
+int fhe_wire_read(fhe_context *ctx, const void *buffer, size_t size,
+                  int64_t *data, size_t capacity);
+
 #ifdef __cplusplus
 }
 #endif
