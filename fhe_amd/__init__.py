@@ -194,7 +194,7 @@ def library():
                                            i32, i32, vp]
         lib.fhe_add_texts.argtypes = [vp, i32, vp, i32, i32, i32, vp, i32, i32,
                                       i32, i32, vp]
-        lib.fhe_round.argtypes = [vp, vp, i32, vp, i32, vp,
+        lib.fhe_round.argtypes = [vp, i32, vp, i32, vp, i32, vp,
                                   ctypes.POINTER(i32), ctypes.POINTER(i32)]
         sz_p = ctypes.POINTER(ctypes.c_size_t)
         lib.fhe_wire_size.argtypes = [vp, i32, i32, sz_p]
@@ -204,13 +204,15 @@ def library():
                                              ctypes.POINTER(WireHeader)]
         lib.fhe_wire_read.argtypes = [vp, vp, sz, vp, sz]
         bk = ctypes.POINTER(BootstrapKeys)
-        lib.fhe_coeff_to_slot.argtypes = [vp, vp, i32, i32, bk, i32, vp, vp,
+        lib.fhe_coeff_to_slot.argtypes = [vp, i32, vp, i32, i32, bk, i32, vp,
+                                          vp,
                                           ctypes.POINTER(i32),
                                           ctypes.POINTER(i32)]
-        lib.fhe_slot_to_coeff.argtypes = [vp, vp, vp, i32, i32, bk, i32, vp,
+        lib.fhe_slot_to_coeff.argtypes = [vp, i32, vp, vp, i32, i32, bk, i32,
+                                          vp,
                                           ctypes.POINTER(i32),
                                           ctypes.POINTER(i32)]
-        lib.fhe_bootstrap.argtypes = [vp, vp, i32, i32, bk, i32, vp,
+        lib.fhe_bootstrap.argtypes = [vp, i32, vp, i32, i32, bk, i32, vp,
                                       ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fhe_decode.argtypes = [vp, vp, i32, i32, ctypes.c_longdouble,
                                    i32, vp]
@@ -469,7 +471,8 @@ class Context:
         """fhe_round (EvalMod) of one INTT-form, not rescaled duo; returns
         the result's (level, flags)."""
         lv, fl = ctypes.c_int(), ctypes.c_int()
-        _check(library().fhe_round(self.handle, ct.ptr, level, key.ptr,
+        _check(library().fhe_round(self.handle, _count(ct), ct.ptr, level,
+                                   key.ptr,
                                    key_moduli_id, out.ptr, ctypes.byref(lv),
                                    ctypes.byref(fl)))
         return lv.value, fl.value
@@ -478,7 +481,7 @@ class Context:
         """fhe_coeff_to_slot of one duo; returns the parts' (level, flags)."""
         lv, fl = ctypes.c_int(), ctypes.c_int()
         _check(library().fhe_coeff_to_slot(
-            self.handle, ct.ptr, level, flags, ctypes.byref(keys),
+            self.handle, _count(ct), ct.ptr, level, flags, ctypes.byref(keys),
             stage_count, real.ptr, imag.ptr, ctypes.byref(lv),
             ctypes.byref(fl)))
         return lv.value, fl.value
@@ -487,7 +490,7 @@ class Context:
                       out):
         lv, fl = ctypes.c_int(), ctypes.c_int()
         _check(library().fhe_slot_to_coeff(
-            self.handle, real.ptr, imag.ptr, level, flags, ctypes.byref(keys),
+            self.handle, _count(real), real.ptr, imag.ptr, level, flags, ctypes.byref(keys),
             stage_count, out.ptr, ctypes.byref(lv), ctypes.byref(fl)))
         return lv.value, fl.value
 
@@ -495,7 +498,7 @@ class Context:
         """fhe_bootstrap of one INTT-form duo; returns (level, flags)."""
         lv, fl = ctypes.c_int(), ctypes.c_int()
         _check(library().fhe_bootstrap(
-            self.handle, ct.ptr, level, flags, ctypes.byref(keys),
+            self.handle, _count(ct), ct.ptr, level, flags, ctypes.byref(keys),
             stage_count, out.ptr, ctypes.byref(lv), ctypes.byref(fl)))
         return lv.value, fl.value
 
@@ -726,6 +729,11 @@ class Context:
                                          ctypes.byref(launches)))
             out[name.value.decode()] = (ms.value, launches.value)
         return out
+
+
+def _count(arr):
+    """Ciphertexts in a device array [count][2][limbs][N] or [2][limbs][N]."""
+    return int(np.prod(arr.shape[:-3])) if len(arr.shape) > 3 else 1
 
 
 def _polys_limbs(arr, n):

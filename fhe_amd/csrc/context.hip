@@ -1918,6 +1918,8 @@ struct Text {
   int polys = 2;
   int level = 0;
   int flags = 0;
+  int count = 1;              // ciphertexts (or plaintexts) side by side
+  std::shared_ptr<void> hold; // returns the buffer to the context's pool
 };
 
 void ok(int rc) {
@@ -1926,18 +1928,17 @@ void ok(int rc) {
 }
 
 // CKKSEngine's operations on device texts, each with the engine's form
-// handling (src/ckks/engine_*.cpp); results are new device buffers, freed
-// with the engine.
+// handling (src/ckks/engine_*.cpp), on `count` ciphertexts at once (the
+// reference runs the same operations one ciphertext at a time).  Results
+// are new device buffers that go back to the context's pool when their last
+// Text is gone: later work on the one stream reuses them in order.
 class TextEngine {
 public:
   TextEngine(fhe_context *c, const int64_t *rlk, int rlk_mid)
       : ctx(c), key(rlk), key_mid(rlk_mid) {}
-  ~TextEngine() {
-    // the buffers go back to the context's pool once the stream is idle
-    (void)hipStreamSynchronize(ctx->stream);
-    for (auto &o : owned)
-      ctx->text_pool.emplace(o.second, o.first);
-  }
+  ~TextEngine() { (void)hipStreamSynchronize(ctx->stream); }
+
+  int count = 1;
 
   fhe_context *context() const { return ctx; }
   int limbs(int level, int flags) const {
@@ -1945,14 +1946,22 @@ public:
         ctx->moduli_id_of(level, (flags & FHE_FLAG_RESCALED) != 0));
   }
   int64_t words(const Text &t) const {
-    return int64_t(t.polys) * limbs(t.level, t.flags) * ctx->n();
+    return int64_t(t.count) * t.polys * limbs(t.level, t.flags) * ctx->n();
   }
-  Text alloc(int polys, int level, int flags) {
+  std::shared_ptr<void> take(size_t bytes) {
+    fhe_context *c = ctx;
+    return std::shared_ptr<void>(c->pool_take(bytes), [c, bytes](void *p) {
+      c->text_pool.emplace(bytes, p);
+    });
+  }
+  Text alloc(int polys, int level, int flags, int n_texts = 0) {
     Text t;
     t.polys = polys;
     t.level = level;
     t.flags = flags;
-    t.data = static_cast<int64_t *>(take(size_t(words(t)) * 8));
+    t.count = n_texts > 0 ? n_texts : count;
+    t.hold = take(size_t(words(t)) * 8);
+    t.data = static_cast<int64_t *>(t.hold.get());
     return t;
   }
   Text wrap(const int64_t *data, int polys, int level, int flags) {
@@ -1961,6 +1970,7 @@ public:
     t.polys = polys;
     t.level = level;
     t.flags = flags;
+    t.count = count;
     return t;
   }
   Text clone(const Text &a) {
@@ -1977,7 +1987,7 @@ public:
     Text t = clone(a);
     t.flags &= ~FHE_FLAG_NTT_FORM;
     const int l = limbs(a.level, a.flags);
-    ok(fhe_intt(ctx, t.data, a.polys, l,
+    ok(fhe_intt(ctx, t.data, a.count * a.polys, l,
                 ctx->moduli_id_of(a.level, a.flags & FHE_FLAG_RESCALED),
                 1));
     return t;
@@ -1987,7 +1997,7 @@ public:
     require(a.polys == 2 && b.polys == 2, "Multiply: duos expected");
     const int level = std::min(a.level, b.level);
     Text out = alloc(relin ? 2 : 3, level - 1, relin ? 0 : FHE_FLAG_NTT_FORM);
-    ok(fhe_multiply(ctx, 1, a.data, a.level, a.flags, b.data, b.level,
+    ok(fhe_multiply(ctx, a.count, a.data, a.level, a.flags, b.data, b.level,
                     b.flags, relin ? key : nullptr, key_mid, out.data));
     return out;
   }
@@ -2000,7 +2010,7 @@ public:
     const int flags = (a.flags & FHE_FLAG_NTT_FORM) |
                       (ra || rb ? FHE_FLAG_RESCALED : 0);
     Text out = alloc(std::max(a.polys, b.polys), level, flags);
-    ok(fhe_add_texts(ctx, 1, a.data, a.polys, a.level, a.flags, b.data,
+    ok(fhe_add_texts(ctx, a.count, a.data, a.polys, a.level, a.flags, b.data,
                      b.polys, b.level, b.flags, subtract ? 1 : 0, out.data));
     return out;
   }
@@ -2011,7 +2021,7 @@ public:
             "Add inplace: the forms differ");
     const int l = limbs(x.level, x.flags);
     fhe::launch_elementwise(ctx->tables, fhe::kEwAddReduce, x.data, y.data,
-                            x.data, int64_t(x.polys) * l, l,
+                            x.data, int64_t(x.count) * x.polys * l, l,
                             ctx->moduli_id_of(x.level,
                                               x.flags & FHE_FLAG_RESCALED),
                             nullptr, ctx->stream);
@@ -2022,7 +2032,8 @@ public:
     require(!(a.flags & (FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)),
             "Add: a scalar joins only an INTT-form, not rescaled text here");
     Text out = alloc(a.polys, a.level, a.flags);
-    ok(fhe_add_double(ctx, a.data, out.data, 1, a.polys, a.level, value));
+    ok(fhe_add_double(ctx, a.data, out.data, a.count, a.polys, a.level,
+                      value));
     return out;
   }
   // multiply(ct, double) (src/ckks/engine_multiply.cpp:418-432): intt_,
@@ -2033,7 +2044,8 @@ public:
             "level.");
     const Text in = intt(a);
     Text out = alloc(a.polys, a.level - 1, in.flags);
-    ok(fhe_multiply_double(ctx, in.data, out.data, a.polys, a.level, value));
+    ok(fhe_multiply_double(ctx, in.data, out.data, a.count * a.polys, a.level,
+                           value));
     return out;
   }
   // multiply(ct, int64_t) (src/ckks/engine_multiply.cpp:443-457)
@@ -2042,7 +2054,7 @@ public:
             "Multiply: the text should not be rescaled.");
     const Text in = intt(a);
     Text out = alloc(a.polys, a.level, in.flags);
-    ok(fhe_multiply_int(ctx, in.data, out.data, a.polys,
+    ok(fhe_multiply_int(ctx, in.data, out.data, a.count * a.polys,
                         ctx->moduli_id_of(a.level, false), value));
     return out;
   }
@@ -2053,7 +2065,8 @@ public:
     Text out = alloc(a.polys, level, a.flags & ~FHE_FLAG_RESCALED);
     const int64_t t_l = words(a);
     int64_t *t0 = ctx->scratch(size_t(2 * t_l) * 8);
-    ctx->transform_into(a.data, a.polys, a.level, a.flags, false, level,
+    ctx->transform_into(a.data, int64_t(a.count) * a.polys, a.level, a.flags,
+                        false, level,
                         false, (a.flags & FHE_FLAG_NTT_FORM) != 0, out.data,
                         t0, t0 + t_l);
     return out;
@@ -2068,11 +2081,12 @@ public:
     if (!(trio.flags & FHE_FLAG_NTT_FORM)) {
       in = clone(trio);
       in.flags |= FHE_FLAG_NTT_FORM;
-      ok(fhe_ntt(ctx, in.data, 3, limbs(trio.level, trio.flags), mid, 1));
+      ok(fhe_ntt(ctx, in.data, 3 * trio.count, limbs(trio.level, trio.flags),
+                 mid, 1));
     }
     ctx->check_key(mid, key_mid, "Relinearize");
     Text out = alloc(2, trio.level, trio.flags & ~FHE_FLAG_NTT_FORM);
-    ctx->relinearize_into(1, mid, in.data, key, key_mid, out.data);
+    ctx->relinearize_into(trio.count, mid, in.data, key, key_mid, out.data);
     return out;
   }
   // weighted_sum (src/ckks/engine_advanced.cpp:228-269)
@@ -2115,7 +2129,7 @@ public:
   Text rotate_core(const Text &a, int index) {
     const int mid = ctx->moduli_id_of(a.level, a.flags & FHE_FLAG_RESCALED);
     Text out = alloc(a.polys, a.level, a.flags);
-    ok(fhe_rotate(ctx, 1, mid, a.data, keys->rotation[index],
+    ok(fhe_rotate(ctx, a.count, mid, a.data, keys->rotation[index],
                   keys->rotation_moduli_ids[index], keys->rotation_deltas[index],
                   out.data));
     return out;
@@ -2164,16 +2178,17 @@ public:
   }
   // encode_unit of `count` complex messages [count][slots] at `level`
   // (src/ckks/engine_encode.cpp:10-36), one batched device encode
-  Text encode_messages(const std::vector<std::complex<double>> &m, int count,
-                       int level) {
+  Text encode_messages(const std::vector<std::complex<double>> &m,
+                       int messages, int level) {
     const size_t bytes = m.size() * sizeof(std::complex<double>);
-    void *dv = take(bytes);
-    hip_check(hipMemcpyAsync(dv, m.data(), bytes, hipMemcpyHostToDevice,
+    const auto dv = take(bytes);
+    hip_check(hipMemcpyAsync(dv.get(), m.data(), bytes, hipMemcpyHostToDevice,
                              ctx->stream),
               "message copy");
-    Text pt = alloc(count, level, 0);
-    ok(fhe_encode(ctx, static_cast<const double *>(dv), 1, count, level,
-                  pt.data));
+    // one plaintext per message, shared by every ciphertext
+    Text pt = alloc(messages, level, 0, 1);
+    ok(fhe_encode(ctx, static_cast<const double *>(dv.get()), 1, messages,
+                  level, pt.data));
     // the host messages may go once the stream has consumed them
     hip_check(hipStreamSynchronize(ctx->stream), "message sync");
     return pt;
@@ -2186,11 +2201,13 @@ public:
     const bool ntt = (a.flags & FHE_FLAG_NTT_FORM) != 0;
     if (rescaled && ntt)
       return a;
-    Text out = alloc(a.polys, a.level, FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED);
+    Text out = alloc(a.polys, a.level, FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED,
+                     a.count);
     const int64_t t_l = words(a);
     int64_t *t0 = ctx->scratch(size_t(2 * t_l) * 8);
     const int64_t *r = ctx->prepare_for_multiply_into(
-        a.data, a.polys, a.level, a.flags, a.level, out.data, t0, t0 + t_l);
+        a.data, int64_t(a.count) * a.polys, a.level, a.flags, a.level,
+        out.data, t0, t0 + t_l);
     require(r == out.data, "Prepare: unexpected form");
     return out;
   }
@@ -2207,7 +2224,7 @@ public:
     const int64_t poly = int64_t(l) * ctx->n();
     Text out = alloc(ct.polys, ct.level - 1, FHE_FLAG_NTT_FORM);
     ctx->stage("mont_mult", [&] {
-      for (int p = 0; p < ct.polys; p++)
+      for (int64_t p = 0; p < int64_t(ct.count) * ct.polys; p++)
         fhe::launch_mont_mult(ctx->tables, ct.data + p * poly,
                               pts.data + index * poly, out.data + p * poly, l,
                               l, mid, ctx->stream);
@@ -2219,7 +2236,7 @@ public:
     const Text in = key_switch_input(a);
     const int mid = ctx->moduli_id_of(in.level, in.flags & FHE_FLAG_RESCALED);
     Text out = alloc(in.polys, in.level, in.flags);
-    ok(fhe_conjugate(ctx, 1, mid, in.data, keys->conjugation,
+    ok(fhe_conjugate(ctx, in.count, mid, in.data, keys->conjugation,
                      keys->conjugation_moduli_id, out.data));
     return out;
   }
@@ -2229,7 +2246,7 @@ public:
     require(!(a.flags & (FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)),
             "Multiply imaginary: an INTT-form, not rescaled text here");
     Text out = alloc(a.polys, a.level, a.flags);
-    ok(fhe_multiply_imaginary(ctx, 1, ctx->moduli_id_of(a.level, false),
+    ok(fhe_multiply_imaginary(ctx, a.count, ctx->moduli_id_of(a.level, false),
                               a.data, value, out.data));
     return out;
   }
@@ -2239,7 +2256,7 @@ public:
     const auto &P = ctx->params;
     const int level = P.max_key_switching_level();
     Text out = alloc(2, level, 0);
-    ok(fhe_mod_raise(ctx, 1, a.data, keys->to_sparse,
+    ok(fhe_mod_raise(ctx, a.count, a.data, keys->to_sparse,
                      keys->to_sparse_moduli_id, keys->to_dense,
                      keys->to_dense_moduli_id, out.data));
     return out;
@@ -2249,14 +2266,6 @@ private:
   fhe_context *ctx;
   const int64_t *key;
   int key_mid;
-  std::vector<std::pair<void *, size_t>> owned;
-
-public:
-  void *take(size_t bytes) {
-    void *p = ctx->pool_take(bytes);
-    owned.emplace_back(p, bytes);
-    return p;
-  }
 };
 
 // The Chebyshev coefficients of the reference's sine approximation
@@ -2688,8 +2697,8 @@ int fhe_bootstrap_twiddles(int log_slots, int stage_count, int to_slot,
   });
 }
 
-int fhe_coeff_to_slot(fhe_context *ctx, const int64_t *in, int level,
-                      int flags, const fhe_bootstrap_keys *keys,
+int fhe_coeff_to_slot(fhe_context *ctx, int count, const int64_t *in,
+                      int level, int flags, const fhe_bootstrap_keys *keys,
                       int stage_count, int64_t *real, int64_t *imag,
                       int *out_level, int *out_flags) {
   return guarded([&] {
@@ -2697,8 +2706,10 @@ int fhe_coeff_to_slot(fhe_context *ctx, const int64_t *in, int level,
             "null argument");
     require_bootstrap_keys(keys);
     require(stage_count >= 1, "Coeff to slot: stage count");
+    require(count > 0, "Bootstrap: empty batch");
     TextEngine e(ctx, keys->relin, keys->relin_moduli_id);
     e.keys = keys;
+    e.count = count;
     const Text x = e.wrap(in, 2, level, flags);
     const auto [r, i] =
         coeff_to_slot(e, x, int(ctx->params.slot_count()), stage_count);
@@ -2710,7 +2721,7 @@ int fhe_coeff_to_slot(fhe_context *ctx, const int64_t *in, int level,
   });
 }
 
-int fhe_slot_to_coeff(fhe_context *ctx, const int64_t *real,
+int fhe_slot_to_coeff(fhe_context *ctx, int count, const int64_t *real,
                       const int64_t *imag, int level, int flags,
                       const fhe_bootstrap_keys *keys, int stage_count,
                       int64_t *out, int *out_level, int *out_flags) {
@@ -2719,8 +2730,10 @@ int fhe_slot_to_coeff(fhe_context *ctx, const int64_t *real,
             "null argument");
     require_bootstrap_keys(keys);
     require(stage_count >= 1, "Slot to coeff: stage count");
+    require(count > 0, "Bootstrap: empty batch");
     TextEngine e(ctx, keys->relin, keys->relin_moduli_id);
     e.keys = keys;
+    e.count = count;
     const Text r = e.wrap(real, 2, level, flags);
     const Text i = e.wrap(imag, 2, level, flags);
     const Text c =
@@ -2730,7 +2743,8 @@ int fhe_slot_to_coeff(fhe_context *ctx, const int64_t *real,
   });
 }
 
-int fhe_bootstrap(fhe_context *ctx, const int64_t *in, int level, int flags,
+int fhe_bootstrap(fhe_context *ctx, int count, const int64_t *in, int level,
+                  int flags,
                   const fhe_bootstrap_keys *keys, int stage_count,
                   int64_t *out, int *out_level, int *out_flags) {
   return guarded([&] {
@@ -2740,8 +2754,10 @@ int fhe_bootstrap(fhe_context *ctx, const int64_t *in, int level, int flags,
     require(!(flags & FHE_FLAG_NTT_FORM) && stage_count >= 1,
             "Bootstrap: an INTT-form duo and a positive stage count");
     const auto &P = ctx->params;
+    require(count > 0, "Bootstrap: empty batch");
     TextEngine e(ctx, keys->relin, keys->relin_moduli_id);
     e.keys = keys;
+    e.count = count;
     const int slots = int(P.slot_count());
     // CKKSEngine::bootstrap with a RotationKey
     // (src/ckks/engine_bootstrap.cpp:712-754)
@@ -2770,7 +2786,7 @@ int fhe_bootstrap(fhe_context *ctx, const int64_t *in, int level, int flags,
   });
 }
 
-int fhe_round(fhe_context *ctx, const int64_t *in, int level,
+int fhe_round(fhe_context *ctx, int count, const int64_t *in, int level,
               const int64_t *key, int key_moduli_id, int64_t *out,
               int *out_level, int *out_flags) {
   return guarded([&] {
@@ -2778,7 +2794,9 @@ int fhe_round(fhe_context *ctx, const int64_t *in, int level,
             "null argument");
     require(level >= 1 && level <= ctx->params.full_level,
             "Round: the level must be positive.");
+    require(count > 0, "Round: empty batch");
     TextEngine e(ctx, key, key_moduli_id);
+    e.count = count;
     const Text x = e.wrap(in, 2, level, 0);
     copy_out(ctx, e, round_text(e, x), out, out_level, out_flags);
     ctx->check_launch("Round");

@@ -182,11 +182,12 @@ int fhe_add_texts(fhe_context *ctx, int count, const int64_t *a,
  * EvalMod: x * 28 - modulo(x), modulo = three double-angle steps
  * (:640-659) after the Chebyshev / Paterson-Stockmeyer sine (:459-638),
  * composed of the engine operations above with the reference's operation
- * order.  in: one INTT-form, not rescaled duo [2][limbs][N] at `level`; the
- * relinearization key at key_moduli_id.  out: room for [2][limbs][N]; the
+ * order.  in: `count` INTT-form, not rescaled duos [count][2][limbs][N] at
+ * `level`; the relinearization key at key_moduli_id.  out: room for the
+ * input's size, the results packed as [count][2][limbs'][N]; the
  * result's level and data flags come back in *out_level, *out_flags (its
  * limbs are those of that form). */
-int fhe_round(fhe_context *ctx, const int64_t *in, int level,
+int fhe_round(fhe_context *ctx, int count, const int64_t *in, int level,
               const int64_t *key, int key_moduli_id, int64_t *out,
               int *out_level, int *out_flags);
 
@@ -221,11 +222,12 @@ int fhe_bootstrap_twiddles(int log_slots, int stage_count, int to_slot,
  * (src/ckks/engine_bootstrap.cpp:330-366): stage_count butterfly stages
  * (baby-step rotations, products with the merged diagonals of
  * src/operation/bootstrap.cpp:125-218, giant-step rotations), then the real
- * and imaginary parts by conjugation.  in: one duo at (level, flags); real
- * and imag receive one duo each (room for the input's size), both of the
- * form returned in *out_level, *out_flags. */
-int fhe_coeff_to_slot(fhe_context *ctx, const int64_t *in, int level,
-                      int flags, const fhe_bootstrap_keys *keys,
+ * and imaginary parts by conjugation.  in: `count` duos [count][2][limbs][N]
+ * at (level, flags); real and imag receive `count` duos each, packed (room
+ * for the input's size), both of the form returned in *out_level,
+ * *out_flags.  The ciphertexts share every key read and every diagonal. */
+int fhe_coeff_to_slot(fhe_context *ctx, int count, const int64_t *in,
+                      int level, int flags, const fhe_bootstrap_keys *keys,
                       int stage_count, int64_t *real, int64_t *imag,
                       int *out_level, int *out_flags);
 
@@ -233,7 +235,7 @@ int fhe_coeff_to_slot(fhe_context *ctx, const int64_t *in, int level,
  * (src/ckks/engine_bootstrap.cpp:405-434): real + i * imag, then the
  * inverse butterfly stages (src/operation/bootstrap.cpp:220-263).  real and
  * imag share the form (level, flags). */
-int fhe_slot_to_coeff(fhe_context *ctx, const int64_t *real,
+int fhe_slot_to_coeff(fhe_context *ctx, int count, const int64_t *real,
                       const int64_t *imag, int level, int flags,
                       const fhe_bootstrap_keys *keys, int stage_count,
                       int64_t *out, int *out_level, int *out_flags);
@@ -242,9 +244,12 @@ int fhe_slot_to_coeff(fhe_context *ctx, const int64_t *real,
  * (src/ckks/engine_bootstrap.cpp:712-754): level down to 0, mod_raise,
  * (partial sums below full slots), coeff_to_slot, round of both parts,
  * slot_to_coeff, subtraction from the raised text and the normalizing
- * integer product.  in: one INTT-form duo at (level, flags); out: room for
- * the raised size; its form comes back. */
-int fhe_bootstrap(fhe_context *ctx, const int64_t *in, int level, int flags,
+ * integer product.  in: `count` INTT-form duos [count][2][limbs][N] at
+ * (level, flags), bootstrapped together (each as the reference bootstraps
+ * it alone); out: room for [count][2][raised limbs][N], the results packed;
+ * their form comes back. */
+int fhe_bootstrap(fhe_context *ctx, int count, const int64_t *in, int level,
+                  int flags,
                   const fhe_bootstrap_keys *keys, int stage_count,
                   int64_t *out, int *out_level, int *out_flags);
 

@@ -5,8 +5,9 @@ preset, bit-exact against the reference engine (oracle/_ref):
     chains, products with the merged diagonals, giant-step rotations,
     conjugation into real and imaginary parts);
   * the whole bootstrap (level down, mod_raise, coeff_to_slot, EvalMod of
-    both parts, slot_to_coeff, subtraction, normalization), whose output also
-    decrypts to the input message.
+    both parts, slot_to_coeff, subtraction, normalization) of two ciphertexts
+    in one batched call, each equal to the reference's and decrypting to its
+    message.
 The reference's own run takes tens of seconds on the host cores.
 """
 import ctypes
@@ -111,26 +112,33 @@ def test_coeff_to_slot_matches_reference(boot):
 
 @pytest.mark.timeout(900)
 def test_bootstrap_matches_reference(boot):
+    """Two ciphertexts bootstrapped in one batched call, each equal to the
+    reference's bootstrap of it alone."""
     ref, ctx, keys, k = boot
     lib, e = ref.lib, ref.e
-    x = np.random.default_rng(6).uniform(-1, 1, ref.slot_count)
-    ct = lib.ref_encrypt(e, ptr(x), ref.slot_count, k["pk"], 2)
-    want_h = lib.ref_bootstrap(e, ct, k["rlk"], k["conj"], k["rk"], k["sbk"],
-                               STAGES)
-    assert want_h, lib.ref_last_error()
-    want = ref.get(want_h)
-    level, flags = form(ref, ct)
-    d = ctx.to_device(ref.get(ct))
-    raised_limbs = ref.chain_count - 1
-    out = ctx.array((2, raised_limbs, ctx.n))
+    rng = np.random.default_rng(6)
+    xs = [rng.uniform(-1, 1, ref.slot_count) for _ in range(2)]
+    cts = [lib.ref_encrypt(e, ptr(x), ref.slot_count, k["pk"], 2)
+           for x in xs]
+    wants, handles = [], []
+    for ct in cts:
+        h = lib.ref_bootstrap(e, ct, k["rlk"], k["conj"], k["rk"], k["sbk"],
+                              STAGES)
+        assert h, lib.ref_last_error()
+        handles.append(h)
+        wants.append(ref.get(h))
+    level, flags = form(ref, cts[0])
+    d = ctx.to_device(np.stack([ref.get(ct) for ct in cts]))
+    out = ctx.array((2, 2, ref.chain_count - 1, ctx.n))
     lv, fl = ctx.bootstrap(d, level, flags, keys, STAGES, out)
-    assert (lv, fl) == form(ref, want_h)
-    got = packed(out, want.shape)
-    assert np.array_equal(got, want)
-    dec = np.zeros(ref.slot_count)
-    lib.ref_decrypt(e, want_h, k["sk"], ptr(dec))
-    assert np.abs(dec - x).max() < 1e-2
-    lib.ref_free_ct(want_h)
-    lib.ref_free_ct(ct)
+    assert (lv, fl) == form(ref, handles[0])
+    got = packed(out, (2,) + wants[0].shape)
+    for i in range(2):
+        assert np.array_equal(got[i], wants[i]), i
+        dec = np.zeros(ref.slot_count)
+        lib.ref_decrypt(e, handles[i], k["sk"], ptr(dec))
+        assert np.abs(dec - xs[i]).max() < 1e-2
+    for h in handles + cts:
+        lib.ref_free_ct(h)
     d.free()
     out.free()

@@ -49,6 +49,23 @@ def test_round_matches_reference(gpu_lib):
                 lib.ref_free_ct(h)
             d.free()
             out.free()
+        # a batch of two in one call, each equal to the reference's round
+        cts = [lib.ref_encrypt(e, ptr(rng.uniform(-0.2, 0.2,
+                                                  ref.slot_count)),
+                               ref.slot_count, pk, top - 1) for _ in range(2)]
+        ws = [lib.ref_round(e, ct, rlk) for ct in cts]
+        wants = [ref.get(w) for w in ws]
+        d = ctx.to_device(np.stack([ref.get(ct) for ct in cts]))
+        out = ctx.array(d.shape)
+        ctx.round(d, top - 1, dkey, int(s[0]), out)
+        got = out.download().reshape(-1)[:2 * wants[0].size].reshape(
+            (2,) + wants[0].shape)
+        for i in range(2):
+            assert np.array_equal(got[i], wants[i]), i
+        for h in cts + ws:
+            lib.ref_free_ct(h)
+        d.free()
+        out.free()
         dkey.free()
     finally:
         lib.ref_free_relin_key(rlk)
