@@ -134,6 +134,13 @@ struct fhe_context {
   // device buffers of the engine-level compositions, kept by size for
   // reuse by later calls
   std::multimap<size_t, void *> text_pool;
+  // prepared (rescaled, NTT-form) plaintexts of every diagonal of one
+  // butterfly stage, resident per (direction, stages, stage, level)
+  struct StagePlaintexts {
+    void *data = nullptr;
+    int count = 0, level = 0;
+  };
+  std::map<std::tuple<bool, int, int, int>, StagePlaintexts> boot_diagonals;
   // bootstrap slot-transform diagonals per (log slots, stages, direction)
   std::map<std::tuple<int, int, bool>,
            std::vector<std::vector<std::vector<std::complex<double>>>>>
@@ -201,6 +208,8 @@ struct fhe_context {
       hipFree(p);
     for (auto &kv : text_pool)
       hipFree(kv.second);
+    for (auto &kv : boot_diagonals)
+      hipFree(kv.second.data);
     if (workspace)
       hipFree(workspace);
     if (aux)
@@ -2543,7 +2552,7 @@ std::vector<Complex> rotate_vector(const std::vector<Complex> &in,
 // apply_butterfly with a RotationKey (src/ckks/engine_bootstrap.cpp:
 // 155-215): baby-step rotations, diagonal products, giant-step rotations
 Text apply_butterfly(TextEngine &e, const Text &x, const Vectors &tw,
-                     int rotate_count) {
+                     int rotate_count, bool to_slot, int stages, int stage) {
   const int count = int(tw.size());
   const auto [inner, outer] = stage_loop_counts(count);
   std::vector<int> deltas;
@@ -2551,20 +2560,42 @@ Text apply_butterfly(TextEngine &e, const Text &x, const Vectors &tw,
     deltas.push_back(rotate_count * i);
   const std::vector<Text> rotated = e.rotate_batch(x, deltas);
   // every (outer, inner) diagonal, rotated as the reference rotates it,
-  // encoded in one batch at the rotated texts' level
-  const int slots = int(tw[0].size());
-  std::vector<Complex> factors;
-  for (int o = 0; o < outer; o++) {
-    const int nidx = o - outer / 2;
-    const int tf_delta = inner * nidx * rotate_count;
-    for (int i = (o == 0 ? 1 : 0); i < inner; i++) {
-      const int ti = inner * nidx + i;
-      const auto f = rotate_vector(tw[positive_modulo(ti, count)], -tf_delta);
-      factors.insert(factors.end(), f.begin(), f.end());
+  // encoded in one batch at the rotated texts' level and prepared; kept
+  // resident in the context for later bootstraps
+  const int level = rotated[0].level;
+  fhe_context *ctx = e.context();
+  const auto key = std::make_tuple(to_slot, stages, stage, level);
+  auto it = ctx->boot_diagonals.find(key);
+  if (it == ctx->boot_diagonals.end()) {
+    const int slots = int(tw[0].size());
+    std::vector<Complex> factors;
+    for (int o = 0; o < outer; o++) {
+      const int nidx = o - outer / 2;
+      const int tf_delta = inner * nidx * rotate_count;
+      for (int i = (o == 0 ? 1 : 0); i < inner; i++) {
+        const int ti = inner * nidx + i;
+        const auto f =
+            rotate_vector(tw[positive_modulo(ti, count)], -tf_delta);
+        factors.insert(factors.end(), f.begin(), f.end());
+      }
     }
+    const int messages = int(factors.size() / slots);
+    const Text prepared =
+        e.prepare(e.encode_messages(factors, messages, level));
+    fhe_context::StagePlaintexts sp;
+    sp.count = messages;
+    sp.level = level;
+    const size_t bytes = size_t(e.words(prepared)) * 8;
+    hip_check(hipMalloc(&sp.data, bytes), "diagonal allocation");
+    hip_check(hipMemcpyAsync(sp.data, prepared.data, bytes,
+                             hipMemcpyDeviceToDevice, ctx->stream),
+              "copy");
+    it = ctx->boot_diagonals.emplace(key, sp).first;
   }
-  const Text pts = e.prepare(e.encode_messages(
-      factors, int(factors.size() / slots), rotated[0].level));
+  Text pts = e.wrap(static_cast<const int64_t *>(it->second.data),
+                    it->second.count, level,
+                    FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED);
+  pts.count = 1;
   // each baby-step text prepared once for all of its products
   std::vector<Text> prepared;
   for (const Text &r : rotated)
@@ -2626,7 +2657,8 @@ std::pair<Text, Text> coeff_to_slot(TextEngine &e, const Text &x, int slots,
   Text current = x;
   for (int st = 0; st < stages; st++) {
     rotate_count >>= cts_log_radix(st, stages, log_slots);
-    current = apply_butterfly(e, current, tw[st], rotate_count);
+    current = apply_butterfly(e, current, tw[st], rotate_count, true,
+                              stages, st);
   }
   const Text conjugated = e.conjugate(current);
   Text real = e.add(current, conjugated, false);
@@ -2644,7 +2676,8 @@ Text slot_to_coeff(TextEngine &e, const Text &real, const Text &imag,
   const auto &tw = cached_twiddles(e.context(), slots, stages, false);
   int rotate_count = 1;
   for (int st = 0; st < stages; st++) {
-    current = apply_butterfly(e, current, tw[st], rotate_count);
+    current = apply_butterfly(e, current, tw[st], rotate_count, false,
+                              stages, st);
     rotate_count <<= stc_log_radix(st, stages, log_slots);
   }
   return current;
