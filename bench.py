@@ -366,6 +366,9 @@ def main():
                     help="skip the encode / decode measurement")
     ap.add_argument("--no-bootstrap", action="store_true",
                     help="skip the bootstrap measurement")
+    ap.add_argument("--only-bootstrap", action="store_true",
+                    help="only run the bootstrap measurement (e.g. under a "
+                         "profiler) and print its JSON object")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times "
                          "(the reference kernel for scaling the FETCH_SIZE "
@@ -382,6 +385,9 @@ def main():
 
     import fhe_amd
     ev = HipEvents()
+    if args.only_bootstrap:
+        print(json.dumps(bootstrap_bench(ev, 3)))
+        return
     with fhe_amd.Context(LOG_N, ALPHA) as ctx:
         limbs = ctx.limbs_at(MODULI_ID)
         n = ctx.n
