@@ -26,6 +26,21 @@ __global__ __launch_bounds__(kBlock) void mont_mult_kernel(
   out[idx] = mont_mult(a[idx], b[idx], t.q[p], t.k[p]);
 }
 
+// out = mont(a, b) where b is one polynomial of `limbs` limbs applied to
+// every polynomial of a (CKKSCore::multiply(ct, pt), src/ckks/core.cpp:
+// 257-282, over a batch of ciphertexts and their polynomials)
+__global__ __launch_bounds__(kBlock) void mont_mult_broadcast_kernel(
+    const int64_t *a, const int64_t *b, int64_t *out, int64_t total,
+    int limbs, int moduli_id, DeviceTables t) {
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= total)
+    return;
+  const int64_t period = int64_t(limbs) << t.log_n;
+  const int64_t r = idx % period;
+  const int p = moduli_id + int(r >> t.log_n);
+  out[idx] = mont_mult(a[idx], b[r], t.q[p], t.k[p]);
+}
+
 __global__ __launch_bounds__(kBlock) void tensor_kernel(
     const int64_t *a, const int64_t *b, int64_t *trio, int64_t total,
     int limbs, int moduli_id, DeviceTables t) {
@@ -335,6 +350,16 @@ void launch_mont_mult(const DeviceTables &t, const int64_t *a,
   const int64_t total = total_limbs << t.log_n;
   hipLaunchKernelGGL(mont_mult_kernel, dim3(grid_for(total)), dim3(kBlock), 0,
                      stream, a, b, out, total, limbs, moduli_id, t);
+}
+
+void launch_mont_mult_broadcast(const DeviceTables &t, const int64_t *a,
+                                const int64_t *b, int64_t *out,
+                                int64_t total_limbs, int limbs, int moduli_id,
+                                hipStream_t stream) {
+  const int64_t total = total_limbs << t.log_n;
+  hipLaunchKernelGGL(mont_mult_broadcast_kernel, dim3(grid_for(total)),
+                     dim3(kBlock), 0, stream, a, b, out, total, limbs,
+                     moduli_id, t);
 }
 
 void launch_tensor(const DeviceTables &t, const int64_t *a, const int64_t *b,

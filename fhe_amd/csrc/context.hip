@@ -2233,10 +2233,9 @@ public:
     const int64_t poly = int64_t(l) * ctx->n();
     Text out = alloc(ct.polys, ct.level - 1, FHE_FLAG_NTT_FORM);
     ctx->stage("mont_mult", [&] {
-      for (int64_t p = 0; p < int64_t(ct.count) * ct.polys; p++)
-        fhe::launch_mont_mult(ctx->tables, ct.data + p * poly,
-                              pts.data + index * poly, out.data + p * poly, l,
-                              l, mid, ctx->stream);
+      fhe::launch_mont_mult_broadcast(
+          ctx->tables, ct.data, pts.data + index * poly, out.data,
+          int64_t(ct.count) * ct.polys * l, l, mid, ctx->stream);
     });
     return out;
   }

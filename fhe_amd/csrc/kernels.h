@@ -65,6 +65,13 @@ void launch_mont_mult(const DeviceTables &t, const int64_t *a,
                       const int64_t *b, int64_t *out, int64_t total_limbs,
                       int limbs, int moduli_id, hipStream_t stream);
 
+// out = mont(a, b) with b one polynomial [limbs][N] for every polynomial of
+// a [total_limbs / limbs][limbs][N] (plaintext products of a batch).
+void launch_mont_mult_broadcast(const DeviceTables &t, const int64_t *a,
+                                const int64_t *b, int64_t *out,
+                                int64_t total_limbs, int limbs, int moduli_id,
+                                hipStream_t stream);
+
 // NTT-form tensor of `count` duo pairs -> trios (src/ckks/core.cpp:284-314).
 void launch_tensor(const DeviceTables &t, const int64_t *a, const int64_t *b,
                    int64_t *trio, int count, int limbs, int moduli_id,
