@@ -173,7 +173,8 @@ def test_invalid_calls_are_rejected(gpu_lib):
 @pytest.mark.reference
 @pytest.mark.skipif(not Reference.available(),
                     reason="oracle/_ref (compiled reference) not built")
-@pytest.mark.parametrize("log_n,alpha,level", [(14, 1, 5), (16, 4, 24)])
+@pytest.mark.parametrize("log_n,alpha,level", [(14, 1, 5), (16, 4, 24),
+                                               (16, 8, 20)])
 def test_real_keys_match_reference_and_decrypt(log_n, alpha, level, gpu_lib):
     ref = Reference(log_n, alpha, threads=8)
     lib, e = ref.lib, ref.e
