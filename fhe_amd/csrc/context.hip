@@ -1983,7 +1983,7 @@ public:
     return t;
   }
   Text clone(const Text &a) {
-    Text t = alloc(a.polys, a.level, a.flags);
+    Text t = alloc(a.polys, a.level, a.flags, a.count);
     hip_check(hipMemcpyAsync(t.data, a.data, size_t(words(a)) * 8,
                              hipMemcpyDeviceToDevice, ctx->stream),
               "copy");
@@ -2005,7 +2005,8 @@ public:
   Text multiply(const Text &a, const Text &b, bool relin) {
     require(a.polys == 2 && b.polys == 2, "Multiply: duos expected");
     const int level = std::min(a.level, b.level);
-    Text out = alloc(relin ? 2 : 3, level - 1, relin ? 0 : FHE_FLAG_NTT_FORM);
+    Text out = alloc(relin ? 2 : 3, level - 1, relin ? 0 : FHE_FLAG_NTT_FORM,
+                     a.count);
     ok(fhe_multiply(ctx, a.count, a.data, a.level, a.flags, b.data, b.level,
                     b.flags, relin ? key : nullptr, key_mid, out.data));
     return out;
@@ -2018,7 +2019,7 @@ public:
     const bool rb = (b.flags & FHE_FLAG_RESCALED) && b.level == level;
     const int flags = (a.flags & FHE_FLAG_NTT_FORM) |
                       (ra || rb ? FHE_FLAG_RESCALED : 0);
-    Text out = alloc(std::max(a.polys, b.polys), level, flags);
+    Text out = alloc(std::max(a.polys, b.polys), level, flags, a.count);
     ok(fhe_add_texts(ctx, a.count, a.data, a.polys, a.level, a.flags, b.data,
                      b.polys, b.level, b.flags, subtract ? 1 : 0, out.data));
     return out;
@@ -2040,7 +2041,7 @@ public:
   Text add_double(const Text &a, double value) {
     require(!(a.flags & (FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)),
             "Add: a scalar joins only an INTT-form, not rescaled text here");
-    Text out = alloc(a.polys, a.level, a.flags);
+    Text out = alloc(a.polys, a.level, a.flags, a.count);
     ok(fhe_add_double(ctx, a.data, out.data, a.count, a.polys, a.level,
                       value));
     return out;
@@ -2052,7 +2053,7 @@ public:
             "Multiply: the text should not be rescaled and have a positive "
             "level.");
     const Text in = intt(a);
-    Text out = alloc(a.polys, a.level - 1, in.flags);
+    Text out = alloc(a.polys, a.level - 1, in.flags, a.count);
     ok(fhe_multiply_double(ctx, in.data, out.data, a.count * a.polys, a.level,
                            value));
     return out;
@@ -2062,7 +2063,7 @@ public:
     require(!(a.flags & FHE_FLAG_RESCALED),
             "Multiply: the text should not be rescaled.");
     const Text in = intt(a);
-    Text out = alloc(a.polys, a.level, in.flags);
+    Text out = alloc(a.polys, a.level, in.flags, a.count);
     ok(fhe_multiply_int(ctx, in.data, out.data, a.count * a.polys,
                         ctx->moduli_id_of(a.level, false), value));
     return out;
@@ -2071,7 +2072,7 @@ public:
   Text level_down(const Text &a, int level) {
     if (a.level == level)
       return clone(a);
-    Text out = alloc(a.polys, level, a.flags & ~FHE_FLAG_RESCALED);
+    Text out = alloc(a.polys, level, a.flags & ~FHE_FLAG_RESCALED, a.count);
     const int64_t t_l = words(a);
     int64_t *t0 = ctx->scratch(size_t(2 * t_l) * 8);
     ctx->transform_into(a.data, int64_t(a.count) * a.polys, a.level, a.flags,
@@ -2094,7 +2095,8 @@ public:
                  mid, 1));
     }
     ctx->check_key(mid, key_mid, "Relinearize");
-    Text out = alloc(2, trio.level, trio.flags & ~FHE_FLAG_NTT_FORM);
+    Text out = alloc(2, trio.level, trio.flags & ~FHE_FLAG_NTT_FORM,
+                     trio.count);
     ctx->relinearize_into(trio.count, mid, in.data, key, key_mid, out.data);
     return out;
   }
@@ -2137,7 +2139,7 @@ public:
   // core rotate of a coefficient-domain text by one fixed rotation key
   Text rotate_core(const Text &a, int index) {
     const int mid = ctx->moduli_id_of(a.level, a.flags & FHE_FLAG_RESCALED);
-    Text out = alloc(a.polys, a.level, a.flags);
+    Text out = alloc(a.polys, a.level, a.flags, a.count);
     ok(fhe_rotate(ctx, a.count, mid, a.data, keys->rotation[index],
                   keys->rotation_moduli_ids[index], keys->rotation_deltas[index],
                   out.data));
@@ -2164,25 +2166,81 @@ public:
     }
     return current;
   }
+  // Advances every row's chain of fixed rotations (the set bits of its
+  // normalized delta, in the key's order) key by key, the rows that use a
+  // key batched into one call; rows are coefficient-domain texts of one
+  // form, replaced by their rotations.
+  void rotate_rows(std::vector<Text> &rows, const std::vector<int> &deltas) {
+    const int64_t row_words = words(rows[0]);
+    for (int k = 0; k < keys->rotation_count; k++) {
+      std::vector<int> members;
+      for (size_t r = 0; r < rows.size(); r++)
+        if (deltas[r] & keys->rotation_deltas[k])
+          members.push_back(int(r));
+      if (members.empty())
+        continue;
+      const Text &first = rows[members[0]];
+      Text batch = alloc(first.polys, first.level, first.flags,
+                         first.count * int(members.size()));
+      for (size_t m = 0; m < members.size(); m++)
+        hip_check(hipMemcpyAsync(batch.data + m * row_words,
+                                 rows[members[m]].data, size_t(row_words) * 8,
+                                 hipMemcpyDeviceToDevice, ctx->stream),
+                  "copy");
+      const Text rotated = rotate_core(batch, k);
+      for (size_t m = 0; m < members.size(); m++) {
+        Text view = rotated;
+        view.count = first.count;
+        view.data = rotated.data + m * row_words;
+        rows[members[m]] = view;
+      }
+    }
+  }
   // rotate_batch(ct, RotationKey, deltas) (src/ckks/engine_rotate.cpp:
   // 305-393): chains of fixed rotations from the prepared input, shared
   // prefixes computed once
   std::vector<Text> rotate_batch(const Text &a, const std::vector<int> &deltas) {
     const Text base = key_switch_input(a);
     std::map<int, Text> done{{0, base}};
-    std::vector<Text> out;
-    for (int delta : deltas) {
-      const int d = normalized(delta, true);
-      for (int k = 0; k < keys->rotation_count && !done.count(d); k++) {
-        const int kd = keys->rotation_deltas[k];
-        if ((d & kd) == 0)
+    std::vector<int> targets;
+    for (int delta : deltas)
+      targets.push_back(normalized(delta, true));
+    const int64_t row_words = words(base);
+    // keys in the key's order: every rotation by key k extends a prefix
+    // of lower bits, all of them done already; the rotations by one key
+    // run as one batched call
+    for (int k = 0; k < keys->rotation_count; k++) {
+      const int kd = keys->rotation_deltas[k];
+      std::vector<int> prevs;
+      for (int d : targets) {
+        if (!(d & kd))
           continue;
-        const int prev = d & (kd - 1), cur = prev + kd;
-        if (!done.count(cur))
-          done[cur] = rotate_core(done.at(prev), k);
+        const int prev = d & (kd - 1);
+        if (!done.count(prev + kd) &&
+            std::find(prevs.begin(), prevs.end(), prev) == prevs.end())
+          prevs.push_back(prev);
       }
-      out.push_back(done.at(d));
+      if (prevs.empty())
+        continue;
+      Text batch = alloc(base.polys, base.level, base.flags,
+                         base.count * int(prevs.size()));
+      for (size_t m = 0; m < prevs.size(); m++)
+        hip_check(hipMemcpyAsync(batch.data + m * row_words,
+                                 done.at(prevs[m]).data,
+                                 size_t(row_words) * 8,
+                                 hipMemcpyDeviceToDevice, ctx->stream),
+                  "copy");
+      const Text rotated = rotate_core(batch, k);
+      for (size_t m = 0; m < prevs.size(); m++) {
+        Text view = rotated;
+        view.count = base.count;
+        view.data = rotated.data + m * row_words;
+        done[prevs[m] + kd] = view;
+      }
     }
+    std::vector<Text> out;
+    for (int d : targets)
+      out.push_back(done.at(d));
     return out;
   }
   // encode_unit of `count` complex messages [count][slots] at `level`
@@ -2231,7 +2289,7 @@ public:
     const int mid = ctx->moduli_id_of(ct.level, true);
     const int l = ctx->params.limbs_at(mid);
     const int64_t poly = int64_t(l) * ctx->n();
-    Text out = alloc(ct.polys, ct.level - 1, FHE_FLAG_NTT_FORM);
+    Text out = alloc(ct.polys, ct.level - 1, FHE_FLAG_NTT_FORM, ct.count);
     ctx->stage("mont_mult", [&] {
       fhe::launch_mont_mult_broadcast(
           ctx->tables, ct.data, pts.data + index * poly, out.data,
@@ -2243,7 +2301,7 @@ public:
   Text conjugate(const Text &a) {
     const Text in = key_switch_input(a);
     const int mid = ctx->moduli_id_of(in.level, in.flags & FHE_FLAG_RESCALED);
-    Text out = alloc(in.polys, in.level, in.flags);
+    Text out = alloc(in.polys, in.level, in.flags, in.count);
     ok(fhe_conjugate(ctx, in.count, mid, in.data, keys->conjugation,
                      keys->conjugation_moduli_id, out.data));
     return out;
@@ -2253,7 +2311,7 @@ public:
   Text multiply_imaginary(const Text &a, int64_t value) {
     require(!(a.flags & (FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)),
             "Multiply imaginary: an INTT-form, not rescaled text here");
-    Text out = alloc(a.polys, a.level, a.flags);
+    Text out = alloc(a.polys, a.level, a.flags, a.count);
     ok(fhe_multiply_imaginary(ctx, a.count, ctx->moduli_id_of(a.level, false),
                               a.data, value, out.data));
     return out;
@@ -2263,7 +2321,7 @@ public:
     require(a.level == 0 && !(a.flags & ~0), "Mod raise: a level-0 text");
     const auto &P = ctx->params;
     const int level = P.max_key_switching_level();
-    Text out = alloc(2, level, 0);
+    Text out = alloc(2, level, 0, a.count);
     ok(fhe_mod_raise(ctx, a.count, a.data, keys->to_sparse,
                      keys->to_sparse_moduli_id, keys->to_dense,
                      keys->to_dense_moduli_id, out.data));
@@ -2600,31 +2658,34 @@ Text apply_butterfly(TextEngine &e, const Text &x, const Vectors &tw,
   for (const Text &r : rotated)
     prepared.push_back(e.prepare(r));
   int next_pt = 0;
-  Text outer_sum;
-  bool have_outer = false;
+  std::vector<Text> sums(outer);
+  std::vector<int> giant(outer);
   for (int o = 0; o < outer; o++) {
     const int nidx = o - outer / 2;
-    const int tf_delta = inner * nidx * rotate_count;
-    Text inner_sum;
+    giant[o] = e.normalized(inner * nidx * rotate_count, true);
     bool have_inner = false;
     for (int i = (o == 0 ? 1 : 0); i < inner; i++) {
       Text m = e.multiply_prepared(prepared[i], pts, next_pt++);
       if (!have_inner) {
-        inner_sum = m;
+        sums[o] = m;
         have_inner = true;
       } else {
-        e.add_inplace(inner_sum, m);
+        e.add_inplace(sums[o], m);
       }
     }
     require(have_inner, "Bootstrap: an empty butterfly row");
-    Text r = e.rotate(inner_sum, tf_delta);
-    if (!have_outer) {
-      outer_sum = r;
-      have_outer = true;
-    } else {
-      e.add_inplace(outer_sum, r);
-    }
   }
+  // giant steps: rotate(inner_sum, RotationKey, delta) of every row, each
+  // the chain of fixed rotations of its delta's bits in the key's order
+  // (src/ckks/engine_rotate.cpp:111-139).  The chains advance together key
+  // by key, the rows that need a key rotated in one batched call: each
+  // row's sequence of operations is unchanged.
+  for (int o = 0; o < outer; o++)
+    sums[o] = giant[o] == 0 ? e.intt(sums[o]) : e.key_switch_input(sums[o]);
+  e.rotate_rows(sums, giant);
+  Text outer_sum = sums[0];
+  for (int o = 1; o < outer; o++)
+    e.add_inplace(outer_sum, sums[o]);
   return outer_sum;
 }
 
