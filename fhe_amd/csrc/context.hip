@@ -2865,8 +2865,20 @@ int fhe_bootstrap(fhe_context *ctx, int count, const int64_t *in, int level,
       }
     }
     const auto [real, imag] = coeff_to_slot(e, raised, slots, stage_count);
-    const Text rounded_real = round_text(e, real);
-    const Text rounded_imag = round_text(e, imag);
+    // the two parts through one batched EvalMod: every ciphertext of the
+    // batch follows its own sequence of operations
+    Text both = e.alloc(real.polys, real.level, real.flags, 2 * real.count);
+    const int64_t part = e.words(real);
+    hip_check(hipMemcpyAsync(both.data, real.data, size_t(part) * 8,
+                             hipMemcpyDeviceToDevice, ctx->stream),
+              "copy");
+    hip_check(hipMemcpyAsync(both.data + part, imag.data, size_t(part) * 8,
+                             hipMemcpyDeviceToDevice, ctx->stream),
+              "copy");
+    const Text rounded = round_text(e, both);
+    Text rounded_real = rounded, rounded_imag = rounded;
+    rounded_real.count = rounded_imag.count = real.count;
+    rounded_imag.data = rounded.data + e.words(rounded_real);
     const Text coeff =
         slot_to_coeff(e, rounded_real, rounded_imag, slots, stage_count);
     const Text bootstrapped = e.add(raised, coeff, true);
