@@ -195,29 +195,30 @@ struct fhe_context {
     t.alpha = params.alpha;
   }
 
+  // teardown: errors are not actionable here, so they are ignored
   ~fhe_context() {
     if (stream)
-      hipStreamSynchronize(stream);
+      (void)hipStreamSynchronize(stream);
     for (auto &st : pending) {
-      hipEventDestroy(st.start);
-      hipEventDestroy(st.stop);
+      (void)hipEventDestroy(st.start);
+      (void)hipEventDestroy(st.stop);
     }
     for (auto &kv : modup_cache)
-      hipFree(kv.second);
+      (void)hipFree(kv.second);
     for (void *p : owned)
-      hipFree(p);
+      (void)hipFree(p);
     for (auto &kv : text_pool)
-      hipFree(kv.second);
+      (void)hipFree(kv.second);
     for (auto &kv : boot_diagonals)
-      hipFree(kv.second.data);
+      (void)hipFree(kv.second.data);
     if (workspace)
-      hipFree(workspace);
+      (void)hipFree(workspace);
     if (aux)
-      hipFree(aux);
+      (void)hipFree(aux);
     if (stash_buf)
-      hipFree(stash_buf);
+      (void)hipFree(stash_buf);
     if (stream)
-      hipStreamDestroy(stream);
+      (void)hipStreamDestroy(stream);
   }
 
   int64_t n() const { return params.n; }
@@ -253,8 +254,8 @@ struct fhe_context {
       }
       stage_ms[i] += ms;
       stage_launches[i]++;
-      hipEventDestroy(st.start);
-      hipEventDestroy(st.stop);
+      (void)hipEventDestroy(st.start);
+      (void)hipEventDestroy(st.stop);
     }
     pending.clear();
   }
@@ -2318,7 +2319,8 @@ public:
   }
   // mod_raise_ (src/ckks/engine_mod_raise.cpp:5-34) of a level-0 text
   Text mod_raise(const Text &a) {
-    require(a.level == 0 && !(a.flags & ~0), "Mod raise: a level-0 text");
+    require(a.level == 0 && a.flags == 0,
+            "Mod raise: an INTT-form, not rescaled level-0 text");
     const auto &P = ctx->params;
     const int level = P.max_key_switching_level();
     Text out = alloc(2, level, 0, a.count);
