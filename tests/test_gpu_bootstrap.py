@@ -3,7 +3,8 @@ preset, bit-exact against the reference engine (oracle/_ref):
 
   * coeff_to_slot of a raised ciphertext (3 stages: baby-step rotation
     chains, products with the merged diagonals, giant-step rotations,
-    conjugation into real and imaginary parts);
+    conjugation into real and imaginary parts), and slot_to_coeff of the
+    two parts;
   * the whole bootstrap (level down, mod_raise, coeff_to_slot, EvalMod of
     both parts, slot_to_coeff, subtraction, normalization) of two ciphertexts
     in one batched call, each equal to the reference's and decrypting to its
@@ -104,9 +105,20 @@ def test_coeff_to_slot_matches_reference(boot):
     assert (lv, fl) == form(ref, real.value)
     assert np.array_equal(packed(out_r, want_r.shape), want_r)
     assert np.array_equal(packed(out_i, want_i.shape), want_i)
-    for h in (ct, raised, real.value, imag.value):
+    # slot_to_coeff of the two parts, against the reference's
+    stc = lib.ref_slot_to_coeff(e, real.value, imag.value, k["conj"],
+                                k["rk"], STAGES)
+    assert stc, lib.ref_last_error()
+    want_c = ref.get(stc)
+    dr = ctx.to_device(want_r)
+    di = ctx.to_device(want_i)
+    out_c = ctx.array(d.shape)
+    assert ctx.slot_to_coeff(dr, di, lv, fl, keys, STAGES, out_c) == \
+        form(ref, stc)
+    assert np.array_equal(packed(out_c, want_c.shape), want_c)
+    for h in (ct, raised, real.value, imag.value, stc):
         lib.ref_free_ct(h)
-    for a in (d, out_r, out_i):
+    for a in (d, out_r, out_i, dr, di, out_c):
         a.free()
 
 
