@@ -111,6 +111,12 @@ struct fhe_context {
   bool staged_modup = true;
   // each partition's own extended limbs from the NTT-form d2 when at hand
   bool own_limbs = true;
+  // the coefficient forms of d0 and d1 on a second stream, concurrent with
+  // the key switching of d2 (a second stream and two events, default
+  // priority)
+  bool overlap_trio = true;
+  hipStream_t side = nullptr;
+  hipEvent_t fork_event = nullptr, join_event = nullptr;
   // small-prime key products accumulated without guards, reduced once
   bool ks_lazy_accumulate = true;
   // key inner product loads the next partition's tile ahead
@@ -173,6 +179,12 @@ struct fhe_context {
     hip_check(hipGetDevice(&device), "hipGetDevice");
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking),
               "stream creation");
+    hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking),
+              "stream creation");
+    hip_check(hipEventCreateWithFlags(&fork_event, hipEventDisableTiming),
+              "event creation");
+    hip_check(hipEventCreateWithFlags(&join_event, hipEventDisableTiming),
+              "event creation");
     auto &t = tables;
     t.q = upload(params.q, owned);
     t.k = upload(params.k, owned);
@@ -217,6 +229,14 @@ struct fhe_context {
       (void)hipFree(aux);
     if (stash_buf)
       (void)hipFree(stash_buf);
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+    if (fork_event)
+      (void)hipEventDestroy(fork_event);
+    if (join_event)
+      (void)hipEventDestroy(join_event);
     if (stream)
       (void)hipStreamDestroy(stream);
   }
@@ -865,6 +885,8 @@ int fhe_context_set_option(fhe_context *ctx, const char *name, int value) {
       ctx->fuse_tensor = value != 0;
     else if (n == "own_limbs")
       ctx->own_limbs = value != 0;
+    else if (n == "overlap_trio")
+      ctx->overlap_trio = value != 0;
     else if (n == "ks_occupancy")
       ctx->ks_occupancy = value;
     else if (n == "ks_prefetch")
@@ -1023,6 +1045,10 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
     // are canonicalized by the relinearization add, d2 feeds ModUp exactly
     const fhe::LazySpec d01 =
         ctx->lazy_key_switch ? fhe::LazySpec{3, 2} : fhe::LazySpec();
+    const bool overlap =
+        ctx->overlap_trio && ctx->lazy_key_switch &&
+        fhe::ntt_parts_supported(ctx->tables) &&
+        !(ctx->fuse_tensor && fhe::tensor_fused_supported(ctx->tables));
     if (ctx->fuse_tensor && fhe::tensor_fused_supported(ctx->tables)) {
       ctx->stage("ntt_pass_high(inputs)", [&] {
         fhe::ntt_forward_first(ctx->tables, a, an, duo_l, limbs, moduli_id,
@@ -1050,10 +1076,28 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
         fhe::launch_tensor(ctx->tables, an, bn, trio, count, limbs, moduli_id,
                            ctx->stream);
       });
-      ctx->stage("intt(trio)", [&] {
+      if (overlap) {
+        // d0, d1 (lazy) on the side stream once the trio is written; d2
+        // (exact, feeds ModUp) here
+        fhe::LazySpec lazy_part = d01, exact_part = d01;
+        lazy_part.part = 2;
+        exact_part.part = 1;
+        hip_check(hipEventRecord(ctx->fork_event, ctx->stream), "event");
+        hip_check(hipStreamWaitEvent(ctx->side, ctx->fork_event, 0), "event");
         fhe::ntt_inverse(ctx->tables, trio, trio_c, trio_l, limbs, moduli_id,
-                         fhe::kInttExit, ctx->stream, d01);
-      });
+                         fhe::kInttExit, ctx->side, lazy_part);
+        hip_check(hipEventRecord(ctx->join_event, ctx->side), "event");
+        ctx->stage("intt(trio)", [&] {
+          fhe::ntt_inverse(ctx->tables, trio, trio_c, trio_l, limbs,
+                           moduli_id, fhe::kInttExit, ctx->stream,
+                           exact_part);
+        });
+      } else {
+        ctx->stage("intt(trio)", [&] {
+          fhe::ntt_inverse(ctx->tables, trio, trio_c, trio_l, limbs,
+                           moduli_id, fhe::kInttExit, ctx->stream, d01);
+        });
+      }
     }
     const bool fused_trio = ctx->fuse_tensor &&
                             fhe::tensor_fused_supported(ctx->tables);
@@ -1062,6 +1106,8 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
                     3 * limbs * N, ext, acc,
                     fused_trio ? nullptr : trio + 2 * limbs * N,
                     3 * limbs * N);
+    if (overlap)
+      hip_check(hipStreamWaitEvent(ctx->stream, ctx->join_event, 0), "event");
     ctx->stage("moddown_relin_rescale", [&] {
       fhe::launch_moddown_relin_rescale(ctx->tables, acc, coef, out, count,
                                         limbs, moduli_id, ctx->stream);

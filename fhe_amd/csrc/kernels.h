@@ -15,8 +15,15 @@ namespace fhe {
 struct LazySpec {
   int period = 0;
   int count = 0;
+  // 0: every polynomial; 1: only the exact ones; 2: only the lazy ones
+  // (split per-variant launches of the radix-16 passes only)
+  int part = 0;
 };
 constexpr LazySpec kAllLazy{1, 1};
+
+// Whether LazySpec::part may select a subset of the polynomials: the
+// radix-16 passes with per-variant launches.
+bool ntt_parts_supported(const DeviceTables &t);
 
 // In-place capable forward NTT of total_limbs limbs.  pre (device, indexed by
 // absolute prime) multiplies each input value in Montgomery form first, e.g.

@@ -39,6 +39,7 @@ struct PassArgs {
   int last;           // the pass completes the transform
   int lazy_period;    // lazy arithmetic for polynomial index % period <
   int lazy_count;     //   count (radix-16 passes only; 0: all exact)
+  int lazy_part = 0;  // LazySpec::part
 };
 
 template <bool INV>
@@ -364,13 +365,13 @@ void launch_pass(bool inverse, const PassArgs &a, const DeviceTables &t,
           "NTT: the polynomial count is not a whole number of lazy periods");
     const int periods = polys / period;
     // exact polynomials: all limbs in one launch
-    if (lazy_count < period)
+    if (lazy_count < period && a.lazy_part != 2)
       launch(kExact,
              PolySelect{periods * (period - lazy_count), period,
                         period - lazy_count, lazy_count, 0},
              a.limbs);
     // lazy polynomials: one launch per run of limbs of one prime class
-    if (lazy_count > 0) {
+    if (lazy_count > 0 && a.lazy_part != 1) {
       int l = 0;
       while (l < a.limbs) {
         const bool small = t_host_q_small(t, a.moduli_id + l);
@@ -397,6 +398,12 @@ void launch_pass(bool inverse, const PassArgs &a, const DeviceTables &t,
 
 } // namespace
 
+bool ntt_parts_supported(const DeviceTables &t) {
+  int l1, l2;
+  split_bits(t.log_n, l1, l2);
+  return t.split_variants && l1 == 8 && l2 == 8;
+}
+
 void ntt_forward(const DeviceTables &t, const int64_t *src, int64_t *dst,
                  int64_t total_limbs, int limbs, int moduli_id,
                  const int64_t *pre, hipStream_t stream, LazySpec lazy) {
@@ -406,13 +413,13 @@ void ntt_forward(const DeviceTables &t, const int64_t *src, int64_t *dst,
   PassArgs a{src,     dst,      total_limbs << (t.log_n - l1),
              limbs,   moduli_id, t.log_n - l1,
              l1,      pre,       kInttRaw,
-             l2 == 0, lazy.period, lazy.count};
+             l2 == 0, lazy.period, lazy.count, lazy.part};
   launch_pass(false, a, t, stream);
   if (l2 > 0) {
     PassArgs b2{dst,   dst,      total_limbs << (t.log_n - l2),
                 limbs, moduli_id, 0,
                 l2,    nullptr,   kInttRaw,
-                1,     lazy.period, lazy.count};
+                1,     lazy.period, lazy.count, lazy.part};
     launch_pass(false, b2, t, stream);
   }
 }
@@ -426,13 +433,13 @@ void ntt_inverse(const DeviceTables &t, const int64_t *src, int64_t *dst,
   PassArgs a{src,     dst,      total_limbs << (t.log_n - l1),
              limbs,   moduli_id, 0,
              l1,      nullptr,   l2 > 0 ? int(kInttRaw) : post,
-             l2 == 0, lazy.period, lazy.count};
+             l2 == 0, lazy.period, lazy.count, lazy.part};
   launch_pass(true, a, t, stream);
   if (l2 > 0) {
     PassArgs b2{dst,   dst,      total_limbs << (t.log_n - l2),
                 limbs, moduli_id, l1,
                 l2,    nullptr,   post,
-                1,     lazy.period, lazy.count};
+                1,     lazy.period, lazy.count, lazy.part};
     launch_pass(true, b2, t, stream);
   }
 }
@@ -446,7 +453,7 @@ void ntt_forward_first(const DeviceTables &t, const int64_t *src, int64_t *dst,
   PassArgs a{src,     dst,      total_limbs << (t.log_n - l1),
              limbs,   moduli_id, t.log_n - l1,
              l1,      pre,       kInttRaw,
-             l2 == 0, lazy.period, lazy.count};
+             l2 == 0, lazy.period, lazy.count, lazy.part};
   launch_pass(false, a, t, stream);
 }
 
@@ -459,7 +466,7 @@ void ntt_inverse_second(const DeviceTables &t, const int64_t *src,
   PassArgs a{src,   dst,      total_limbs << (t.log_n - l2),
              limbs, moduli_id, l1,
              l2,    nullptr,   post,
-             1,     lazy.period, lazy.count};
+             1,     lazy.period, lazy.count, lazy.part};
   launch_pass(true, a, t, stream);
 }
 

@@ -19,10 +19,12 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not Reference.available(),
                                  reason="oracle/_ref not built")]
 
-OPTIONS = [{}, {"split_variants": 0}, {"own_limbs": 0}, {"staged_modup": 0},
+OPTIONS = [{}, {"split_variants": 0}, {"own_limbs": 0}, {"overlap_trio": 0},
+           {"staged_modup": 0},
            {"ks_lazy_accumulate": 0}, {"lazy_key_switch": 0},
            {"fuse_tensor": 1}, {"staged_modup": 0, "fuse_modup": 1}]
-DEFAULTS = {"split_variants": 1, "own_limbs": 1, "staged_modup": 1,
+DEFAULTS = {"split_variants": 1, "own_limbs": 1, "overlap_trio": 1,
+            "staged_modup": 1,
             "ks_lazy_accumulate": 1, "lazy_key_switch": 1, "fuse_tensor": 0,
             "fuse_modup": 0}
 
