@@ -96,41 +96,71 @@ def time_steps(ctx, ev, fn, steps, warmup):
     return ms / steps
 
 
-def cpu_baseline(budget_s=20.0):
+def host_cpu_share():
+    """Host threads this process may use: every core it sees, capped by the
+    thread count the machine's launcher grants it (OMP_NUM_THREADS, set to
+    the per-GPU CPU share on a shared GPU host)."""
+    nproc = os.cpu_count() or 1
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", nproc))
+    except ValueError:
+        share = nproc
+    return nproc, max(1, min(nproc, share))
+
+
+def _reference_rate(ref, budget_s, cap):
+    """HMult+relin+rescale ops/s of the reference engine `ref` on Gold L=24
+    fresh encryptions (rescaled inputs, as the device benchmark's)."""
+    lib, e = ref.lib, ref.e
+    sk = lib.ref_secret_key(e)
+    pk = lib.ref_public_key(e, sk)
+    rlk = lib.ref_relin_key(e, sk)
+    slots = ref.slot_count
+    zeros = np.zeros(slots)
+    level = ref.chain_count - MODULI_ID - 1
+    a0 = lib.ref_encrypt(e, zeros.ctypes.data, slots, pk, level)
+    b0 = lib.ref_encrypt(e, zeros.ctypes.data, slots, pk, level)
+    a, b = lib.ref_rescale(e, a0), lib.ref_rescale(e, b0)
+    done, t0 = 0, time.perf_counter()
+    while True:
+        c = lib.ref_hmult_relin_rescale(e, a, b, rlk)
+        lib.ref_free_ct(c)
+        done += 1
+        if time.perf_counter() - t0 > budget_s or done >= cap:
+            break
+    dt = time.perf_counter() - t0
+    for h in (a0, b0, a, b):
+        lib.ref_free_ct(h)
+    lib.ref_free_relin_key(rlk)
+    lib.ref_free_public_key(pk)
+    lib.ref_free_secret_key(sk)
+    return done, dt
+
+
+def cpu_baseline(budget_s=12.0):
     """The reference engine (oracle/_ref, compiled from its sources) on the
-    host cores; falls back to the oracle port (1 thread) when absent."""
+    host's CPU share with CPU_PARALLEL, and on one thread (CPU); falls back
+    to the oracle port (1 thread) when absent."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from support import Oracle, Reference
     from seeded_inputs import uniform_limbs
-    threads = min(os.cpu_count() or 1, 16)
+    nproc, threads = host_cpu_share()
     if Reference.available():
         ref = Reference(LOG_N, ALPHA, threads=threads)
-        lib, e = ref.lib, ref.e
-        sk = lib.ref_secret_key(e)
-        pk = lib.ref_public_key(e, sk)
-        rlk = lib.ref_relin_key(e, sk)
-        slots = ref.slot_count
-        zeros = np.zeros(slots)
-        level = ref.chain_count - MODULI_ID - 1
-        a0 = lib.ref_encrypt(e, zeros.ctypes.data, slots, pk, level)
-        b0 = lib.ref_encrypt(e, zeros.ctypes.data, slots, pk, level)
-        a, b = lib.ref_rescale(e, a0), lib.ref_rescale(e, b0)
-        done, t0 = 0, time.perf_counter()
-        while True:
-            c = lib.ref_hmult_relin_rescale(e, a, b, rlk)
-            lib.ref_free_ct(c)
-            done += 1
-            if time.perf_counter() - t0 > budget_s or done >= 200:
-                break
-        dt = time.perf_counter() - t0
-        for h in (a0, b0, a, b):
-            lib.ref_free_ct(h)
+        done, dt = _reference_rate(ref, budget_s, 200)
         ref.close()
+        ref1 = Reference(LOG_N, ALPHA, threads=1)
+        done1, dt1 = _reference_rate(ref1, budget_s, 40)
+        ref1.close()
         return {"value": done / dt, "unit": "ciphertexts/s", "cores": threads,
-                "kind": "reference",
+                "nproc": nproc, "kind": "reference",
+                "one_thread": {"value": done1 / dt1, "cores": 1,
+                               "sample": f"{done1} ops, EngineProcessor::CPU, "
+                                         f"{dt1:.1f} s"},
                 "sample": f"{done} HMult+relin+rescale at Gold L=24 through "
                           f"CKKSEngine::multiply+rescale, CPU_PARALLEL "
-                          f"thread_count={threads}, {dt:.1f} s"}
+                          f"thread_count={threads} of nproc={nproc} (the "
+                          f"host's per-GPU CPU share), {dt:.1f} s"}
     import fhe_amd
     primes = fhe_amd.resolve_primes(LOG_N, ALPHA)
     orc = Oracle(LOG_N, primes, ALPHA)
@@ -235,17 +265,27 @@ def bootstrap_bench(ev, steps):
         full = ctx.info["full_level"]
         key_mid = ctx.key_moduli_id()
         kp, kc, n = ctx.key_shape(key_mid)
-        key = np.stack([uniform_limbs(rng, ctx.primes, key_mid, kc, n)
-                        for _ in range(kp)])
+
+        def device_key(mid, polys, chain):
+            # one polynomial at a time keeps the host copy small
+            d = ctx.array((polys, chain, n))
+            for p in range(polys):
+                d.upload_at(uniform_limbs(rng, ctx.primes, mid, chain, n),
+                            p * chain * n)
+            return d
+
+        # every key in its own device buffer, as a real key set is: the
+        # relinearization, conjugation and to_dense keys and the 15
+        # power-of-two rotation keys (18 x 217 MiB), and to_sparse
+        relin, conj, dense = (device_key(key_mid, kp, kc) for _ in range(3))
+        rot = [device_key(key_mid, kp, kc) for _ in range(LOG_N - 1)]
         sparse_mid = full
         sp, sc, _ = ctx.key_shape(sparse_mid)
-        sparse = np.stack([uniform_limbs(rng, ctx.primes, sparse_mid, sc, n)
-                           for _ in range(sp)])
-        dk, dsp = ctx.to_device(key), ctx.to_device(sparse)
-        del key, sparse
-        rotations = [(dk, key_mid, 1 << i) for i in range(LOG_N - 1)]
-        keys = fhe_amd.BootstrapKeys.make(dk, key_mid, dk, key_mid, rotations,
-                                          dsp, sparse_mid, dk, key_mid)
+        dsp = device_key(sparse_mid, sp, sc)
+        rotations = [(rot[i], key_mid, 1 << i) for i in range(LOG_N - 1)]
+        keys = fhe_amd.BootstrapKeys.make(relin, key_mid, conj, key_mid,
+                                          rotations, dsp, sparse_mid, dense,
+                                          key_mid)
         level = 2
         mid = full - level
         ms = {}
@@ -265,48 +305,62 @@ def bootstrap_bench(ev, steps):
             ms[batch] = (time.perf_counter() - t0) * 1e3 / steps
             for arr in (dct, out):
                 arr.free()
-        for arr in (dk, dsp):
+        for arr in [relin, conj, dense, dsp] + rot:
             arr.free()
     return {"workload": "bootstrap with a RotationKey (15 power-of-two "
                         "keys), N=2^16 bootstrap preset, 2^15 slots, 3 "
                         "CtS / StC stages, ciphertexts from level 2, one "
-                        "per call and 8 per batched call",
+                        "per call and 8 per batched call; every key in its "
+                        "own device buffer (synthetic values of the real "
+                        "shapes)",
+            "precision": "bit-identical to the reference engine's bootstrap "
+                         "(tests/test_gpu_bootstrap.py), whose decrypted "
+                         "precision that test reports",
             "ms_per_bootstrap": round(ms[1], 1),
             "ms_per_batch_of_8": round(ms[8], 1),
             "ms_per_bootstrap_batched": round(ms[8] / 8, 1), "steps": steps,
             "timing": "wall clock, host work included"}
 
 
-# Algorithmic HBM bytes of one launch of each single-kernel stage (read each
-# input limb once, write each output limb once; N * 8 bytes per limb) and
-# the modular products it performs (one per butterfly, one per pointwise
-# product).  Shape keys: n, count, limbs, ext (= limbs + alpha), parts.
-def stage_model(name, s):
-    limb = 8 * s["n"]
-    c, l, e, p, n = s["count"], s["limbs"], s["ext"], s["parts"], s["n"]
+# The unit of work's algorithmic cost (Gold, N = 2^16, l = 25 -> 24,
+# alpha = 4): HBM bytes of one HMult+relin+rescale when B ciphertexts share
+# one key stream, 8N [4l + 2(l - 1) + 2 ceil(l / alpha)(l + alpha) / B] (read
+# both rescaled inputs, write the output, read the key once per batch), and
+# its modular products: 436 NTT-equivalents (100 input NTTs, 75 trio INTTs,
+# 203 ModUp NTTs, 58 ModDown INTTs) of (N / 2) log2 N butterflies each, one
+# product per butterfly, plus about 1.37e8 elementwise products (tensor,
+# Garner digits and basis extension, key products, ModDown, rescale).
+def unit_bytes(n, limbs, alpha, batch):
+    parts = -(-limbs // alpha)
+    return 8 * n * (4 * limbs + 2 * (limbs - 1) +
+                    2 * parts * (limbs + alpha) / batch)
+
+
+def unit_modmuls(n, limbs, alpha):
+    parts = -(-limbs // alpha)
+    ntts = 4 * limbs + 3 * limbs + parts * (limbs + alpha) + 2 * (limbs +
+                                                                   alpha)
+    log_n = n.bit_length() - 1
+    return ntts * (n // 2) * log_n + 1.37e8
+
+
+# Modular products of one launch of each single-kernel stage (one per
+# butterfly, one per pointwise product); shape keys: n, count, limbs, ext
+# (= limbs + alpha), parts.
+def stage_modmuls(name, s):
+    c, e, p, n = s["count"], s["ext"], s["parts"], s["n"]
     half_pass = 4 * n  # butterflies of one 8-bit pass over one limb (N/2 * 8)
     if name == "ks_inner_product_r16":
-        # ext (all partitions) and the key in, both accumulators out; the low
-        # forward pass and two key products per ext limb, the low inverse
-        # pass per accumulator limb
-        return (limb * (c * p * e + 2 * p * e + 2 * c * e),
-                c * p * e * (half_pass + 2 * n) + 2 * c * e * half_pass)
-    if name == "modup":
-        return limb * c * (ALPHA + e), None
-    if name == "ntt_pass_high(ext)":
-        return 2 * limb * c * p * e, c * p * e * half_pass
-    if name == "intt_pass_high(acc)":
-        return 2 * limb * 2 * c * e, 2 * c * e * (half_pass + n)
-    if name == "moddown_relin_rescale":
-        return limb * c * (2 * e + 2 * l + 2 * (l - 1)), None
-    if name == "tensor":
-        return limb * c * 7 * l, 4 * c * l * n
-    return None, None
+        # the low forward pass and two key products per extended limb of
+        # every partition, the low inverse pass of both accumulators
+        return c * p * e * (half_pass + 2 * n) + 2 * c * e * half_pass
+    return None
 
 
 # Kernel name of each single-kernel stage, for the recorded HBM traffic.
 STAGE_KERNEL = {"ks_inner_product_r16": "ks_inner_product_r16_kernel"}
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "hmult_gold_b32_traffic.md")
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles",
+                               "ks_inner_product_gold_b32_traffic.md")
 
 
 def recorded_traffic(stage, batch):
@@ -324,30 +378,53 @@ def recorded_traffic(stage, batch):
     return None
 
 
-def stage_report(stages, total_ms, shape):
-    """Per-stage share of the step and the roofline of the dominant
-    single-kernel stage (one kernel launch per stage call)."""
+def stage_report(stages, step_ms, prof_steps, shape, peaks):
+    """Per-stage share of the step and the roofline of the dominant stage
+    (the one with the largest total time; one kernel launch per stage
+    call), against both bounds: HBM with the unit's algorithmic bytes, and
+    the device's measured modular-product rate."""
+    total_ms = step_ms * prof_steps
     breakdown = {}
-    best = None
     for name, (ms, launches) in sorted(stages.items(), key=lambda kv: -kv[1][0]):
-        avg = ms / launches
-        breakdown[name] = {"avg_launch_ms": round(avg, 4),
+        breakdown[name] = {"avg_launch_ms": round(ms / launches, 4),
                            "launches": launches,
                            "share": round(ms / total_ms, 4)}
-        nbytes, _ = stage_model(name, shape)
-        if nbytes is not None and (best is None or ms > best[1]):
-            best = (name, ms, avg)
-    name, _, avg = best
-    nbytes, mults = stage_model(name, shape)
+    name = max(stages, key=lambda k: stages[k][0])
+    avg = stages[name][0] / stages[name][1]
+    n, limbs, batch = shape["n"], shape["limbs"], shape["count"]
+    nbytes = unit_bytes(n, limbs, ALPHA, batch) * batch
     gbs = nbytes / (avg / 1e3) / 1e9
-    roofline = {"bound": "hbm", "kernel": name, "achieved": round(gbs, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4),
-                "traffic": recorded_traffic(name, shape["count"]),
-                "avg_launch_ms": round(avg, 4),
-                "algorithmic_bytes": nbytes}
-    if mults is not None:
-        roofline["modmul_per_s"] = round(mults / (avg / 1e3), 0)
+    step_gbs = nbytes / (step_ms / 1e3) / 1e9
+    kernel_mults = stage_modmuls(name, shape)
+    step_mults = unit_modmuls(n, limbs, ALPHA) * batch / (step_ms / 1e3)
+    valu = {"unit": "modular products/s",
+            "peak_shoup": round(peaks[0], 0),
+            "peak_montgomery": round(peaks[1], 0),
+            "step_achieved": round(step_mults, 0),
+            "step_frac": round(step_mults / peaks[0], 4)}
+    if kernel_mults is not None:
+        k = kernel_mults / (avg / 1e3)
+        valu["kernel_achieved"] = round(k, 0)
+        valu["kernel_frac"] = round(k / peaks[0], 4)
+    roofline = {
+        "bound": "valu",
+        "kernel": name,
+        "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(gbs / HBM_PEAK_GBS, 4),
+        "traffic": recorded_traffic(name, batch),
+        "avg_launch_ms": round(avg, 4),
+        "algorithmic_bytes": int(nbytes),
+        "algorithmic_bytes_basis": "8N[4l+2(l-1)+2ceil(l/a)(l+a)/B] per "
+                                   f"ciphertext x B={batch} (l={limbs}, "
+                                   f"a={ALPHA})",
+        "step_achieved": round(step_gbs, 1),
+        "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "valu": valu,
+        "binding": "VALU: 62-bit modular products from 32-bit multiply-adds; "
+                   "the HBM fractions are of the algorithmic bytes, the "
+                   "product fractions of the Shoup-product rate measured "
+                   "on this device (fhe_modmul_peak)",
+    }
     return breakdown, roofline
 
 
@@ -416,6 +493,7 @@ def main():
             ctx.hmult_relin_rescale(da, db, dk, key_mid, out, MODULI_ID)
         stages = ctx.profile_entries()
         ctx.profile(False)
+        peaks = (ctx.modmul_peak(0), ctx.modmul_peak(1))
         parts = -(-limbs // ALPHA)
         for arr in (da, db, dk, out):
             arr.free()
@@ -428,7 +506,8 @@ def main():
     value = args.batch / (step_ms / 1e3)
     shape = dict(n=n, count=args.batch, limbs=limbs, ext=limbs + ALPHA,
                  parts=parts)
-    breakdown, roofline = stage_report(stages, step_ms * prof_steps, shape)
+    breakdown, roofline = stage_report(stages, step_ms, prof_steps, shape,
+                                       peaks)
     line = {
         "metric": "HMult+relin+rescale ciphertexts/s at N=2^16,L=24 (batched)",
         "value": round(value, 2),

@@ -36,7 +36,7 @@ EXPORTED_SYMBOLS = (
     "fhe_mod_raise", "fhe_multiply", "fhe_multiply_plain", "fhe_add_texts",
     "fhe_round", "fhe_coeff_to_slot", "fhe_slot_to_coeff", "fhe_bootstrap",
     "fhe_bootstrap_twiddles", "fhe_wire_size", "fhe_wire_write",
-    "fhe_wire_read_header", "fhe_wire_read",
+    "fhe_wire_read_header", "fhe_wire_read", "fhe_modmul_peak",
 )
 
 # fhe_wire_* record kinds (FHE_WIRE_* in include/fhe_amd.h)
@@ -222,6 +222,8 @@ def library():
             ctypes.POINTER(Params), i32, vp,
             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
             ctypes.POINTER(i32), vp, i32, ctypes.POINTER(i32)]
+        lib.fhe_modmul_peak.argtypes = [vp, i32,
+                                        ctypes.POINTER(ctypes.c_double)]
         lib.fhe_profile_count.argtypes = [vp, ctypes.POINTER(i32)]
         lib.fhe_profile_entry.argtypes = [
             vp, i32, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
@@ -320,6 +322,15 @@ class DeviceArray:
         assert host.nbytes == self.nbytes, (host.shape, self.shape)
         _check(library().fhe_copy_to_device(self.ctx.handle, self.ptr,
                                             host.ctypes.data, self.nbytes))
+        return self
+
+    def upload_at(self, host, offset):
+        """Copy host int64 words into the buffer from word `offset` on."""
+        host = np.ascontiguousarray(host, dtype=np.int64)
+        assert 0 <= offset and offset * 8 + host.nbytes <= self.nbytes
+        _check(library().fhe_copy_to_device(
+            self.ctx.handle, ctypes.c_void_p(self.ptr.value + offset * 8),
+            host.ctypes.data, host.nbytes))
         return self
 
     def download(self):
@@ -501,6 +512,13 @@ class Context:
             self.handle, _count(ct), ct.ptr, level, flags, ctypes.byref(keys),
             stage_count, out.ptr, ctypes.byref(lv), ctypes.byref(fl)))
         return lv.value, fl.value
+
+    def modmul_peak(self, form=0):
+        """fhe_modmul_peak: modular products per second of the device (form
+        0 Shoup, 1 Montgomery)."""
+        v = ctypes.c_double()
+        _check(library().fhe_modmul_peak(self.handle, form, ctypes.byref(v)))
+        return v.value
 
     def wire_write(self, kind, arr, moduli_id, level, flags, scale):
         """Serialize a device array [polys][limbs][N] into bytes."""

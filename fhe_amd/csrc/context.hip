@@ -211,6 +211,8 @@ struct fhe_context {
   ~fhe_context() {
     if (stream)
       (void)hipStreamSynchronize(stream);
+    if (side)
+      (void)hipStreamSynchronize(side);
     for (auto &st : pending) {
       (void)hipEventDestroy(st.start);
       (void)hipEventDestroy(st.stop);
@@ -229,10 +231,8 @@ struct fhe_context {
       (void)hipFree(aux);
     if (stash_buf)
       (void)hipFree(stash_buf);
-    if (side) {
-      (void)hipStreamSynchronize(side);
+    if (side)
       (void)hipStreamDestroy(side);
-    }
     if (fork_event)
       (void)hipEventDestroy(fork_event);
     if (join_event)
@@ -900,6 +900,34 @@ int fhe_context_set_option(fhe_context *ctx, const char *name, int value) {
   });
 }
 
+int fhe_modmul_peak(fhe_context *ctx, int form, double *per_second) {
+  return guarded([&] {
+    require(ctx && per_second, "null argument");
+    require(form == 0 || form == 1, "Modmul peak: unknown form");
+    const auto &P = ctx->params;
+    // the last small prime of the chain; any twiddle-sized constant
+    const int pi = P.chain_count - 1;
+    const uint64_t q = P.q[pi], k = P.k[pi];
+    const uint64_t w = q / 3;
+    const uint64_t wq =
+        uint64_t(((unsigned __int128)w << 64) / (unsigned __int128)q);
+    int cus = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                    ctx->device),
+              "device attribute");
+    const int blocks = std::max(1, cus) * 16;
+    void *out = nullptr;
+    hip_check(hipMalloc(&out, size_t(blocks) * fhe::kBlock * 8),
+              "probe allocation");
+    const double rate = fhe::measure_modmul_rate(
+        form, blocks, 2048, q, k, w, wq, static_cast<uint64_t *>(out),
+        ctx->stream);
+    (void)hipFree(out);
+    require(rate > 0, "Modmul peak: the probe failed");
+    *per_second = rate;
+  });
+}
+
 int fhe_synchronize(fhe_context *ctx) {
   return guarded([&] {
     require(ctx, "null argument");
@@ -1049,6 +1077,18 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
         ctx->overlap_trio && ctx->lazy_key_switch &&
         fhe::ntt_parts_supported(ctx->tables) &&
         !(ctx->fuse_tensor && fhe::tensor_fused_supported(ctx->tables));
+    // once the side stream has forked, every exit (an exception included)
+    // makes the main stream wait for it, so later work on the workspace is
+    // ordered after the side-stream INTT
+    bool joined = true;
+    struct Join {
+      fhe_context *ctx;
+      bool &joined;
+      ~Join() {
+        if (!joined)
+          (void)hipStreamWaitEvent(ctx->stream, ctx->join_event, 0);
+      }
+    } join_guard{ctx, joined};
     if (ctx->fuse_tensor && fhe::tensor_fused_supported(ctx->tables)) {
       ctx->stage("ntt_pass_high(inputs)", [&] {
         fhe::ntt_forward_first(ctx->tables, a, an, duo_l, limbs, moduli_id,
@@ -1087,6 +1127,7 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
         fhe::ntt_inverse(ctx->tables, trio, trio_c, trio_l, limbs, moduli_id,
                          fhe::kInttExit, ctx->side, lazy_part);
         hip_check(hipEventRecord(ctx->join_event, ctx->side), "event");
+        joined = false;
         ctx->stage("intt(trio)", [&] {
           fhe::ntt_inverse(ctx->tables, trio, trio_c, trio_l, limbs,
                            moduli_id, fhe::kInttExit, ctx->stream,
@@ -1106,8 +1147,10 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
                     3 * limbs * N, ext, acc,
                     fused_trio ? nullptr : trio + 2 * limbs * N,
                     3 * limbs * N);
-    if (overlap)
+    if (overlap) {
+      joined = true;
       hip_check(hipStreamWaitEvent(ctx->stream, ctx->join_event, 0), "event");
+    }
     ctx->stage("moddown_relin_rescale", [&] {
       fhe::launch_moddown_relin_rescale(ctx->tables, acc, coef, out, count,
                                         limbs, moduli_id, ctx->stream);
@@ -2808,6 +2851,17 @@ void copy_out(fhe_context *ctx, TextEngine &e, const Text &t, int64_t *out,
   *out_flags = t.flags;
 }
 
+// The form of a bootstrap input: known data-flag bits only and a level on
+// the chain (checked before any device work; a level off the chain would
+// index the device tables out of range).
+void require_boot_form(const fhe::CkksParams &P, int level, int flags,
+                       int min_level, const char *op) {
+  require((flags & ~(FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)) == 0,
+          std::string(op) + ": unknown data flags");
+  require(level >= min_level && level <= P.full_level,
+          std::string(op) + ": the level is out of range.");
+}
+
 void require_bootstrap_keys(const fhe_bootstrap_keys *k) {
   require(k && k->relin && k->conjugation && k->rotation_count > 0 &&
               k->rotation && k->rotation_moduli_ids && k->rotation_deltas,
@@ -2848,6 +2902,7 @@ int fhe_coeff_to_slot(fhe_context *ctx, int count, const int64_t *in,
     require_bootstrap_keys(keys);
     require(stage_count >= 1, "Coeff to slot: stage count");
     require(count > 0, "Bootstrap: empty batch");
+    require_boot_form(ctx->params, level, flags, 1, "Coeff to slot");
     TextEngine e(ctx, keys->relin, keys->relin_moduli_id);
     e.keys = keys;
     e.count = count;
@@ -2872,6 +2927,7 @@ int fhe_slot_to_coeff(fhe_context *ctx, int count, const int64_t *real,
     require_bootstrap_keys(keys);
     require(stage_count >= 1, "Slot to coeff: stage count");
     require(count > 0, "Bootstrap: empty batch");
+    require_boot_form(ctx->params, level, flags, 1, "Slot to coeff");
     TextEngine e(ctx, keys->relin, keys->relin_moduli_id);
     e.keys = keys;
     e.count = count;
@@ -2896,6 +2952,7 @@ int fhe_bootstrap(fhe_context *ctx, int count, const int64_t *in, int level,
             "Bootstrap: an INTT-form duo and a positive stage count");
     const auto &P = ctx->params;
     require(count > 0, "Bootstrap: empty batch");
+    require_boot_form(P, level, flags, 0, "Bootstrap");
     TextEngine e(ctx, keys->relin, keys->relin_moduli_id);
     e.keys = keys;
     e.count = count;
@@ -3038,8 +3095,12 @@ int fhe_wire_write(fhe_context *ctx, int kind, const int64_t *data, int polys,
             "Wire: unknown kind");
     const auto &P = ctx->params;
     require(polys > 0 && limbs > 0 && moduli_id >= 0 &&
-                moduli_id + limbs <= P.prime_count,
+                moduli_id < P.prime_count &&
+                limbs <= P.prime_count - moduli_id,
             "Wire: shape out of range");
+    require(level >= -1 && level <= P.full_level &&
+                (flags & ~(FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)) == 0,
+            "Wire: level or flags out of range");
     const size_t words = size_t(polys) * limbs * size_t(ctx->n());
     const size_t need = kWireHeader + words * 8;
     *written = need;
@@ -3088,15 +3149,22 @@ int fhe_wire_read_header(const fhe_context *ctx, const void *buffer,
     header->flags = get<int32_t>(b, 44);
     header->scale = 0;
     std::memcpy(&header->scale, b + 48, 10);
+    // every bound is checked without overflow: limbs against the primes
+    // left above moduli_id, polys against what the record's size can hold
     require(header->kind >= FHE_WIRE_PLAINTEXT &&
                 header->kind <= FHE_WIRE_KEY && header->polys > 0 &&
                 header->limbs > 0 && header->moduli_id >= 0 &&
-                header->moduli_id + header->limbs <= P.prime_count,
+                header->moduli_id < P.prime_count &&
+                header->limbs <= P.prime_count - header->moduli_id,
             "Wire: header out of range");
-    header->payload_bytes =
-        size_t(header->polys) * header->limbs * size_t(ctx->n()) * 8;
-    require(size >= kWireHeader + header->payload_bytes,
+    require(header->level >= -1 && header->level <= P.full_level &&
+                (header->flags & ~(FHE_FLAG_NTT_FORM | FHE_FLAG_RESCALED)) ==
+                    0,
+            "Wire: header out of range");
+    const size_t poly_bytes = size_t(header->limbs) * size_t(ctx->n()) * 8;
+    require(size_t(header->polys) <= (size - kWireHeader) / poly_bytes,
             "Wire: truncated payload");
+    header->payload_bytes = size_t(header->polys) * poly_bytes;
   });
 }
 

@@ -292,6 +292,14 @@ void launch_decode(const DeviceTables &t, const DecodeDevice &d,
                    int complex_output, double *out, void *work,
                    hipStream_t stream);
 
+// Modular products per second of one form (0: Shoup, 1: Montgomery) on the
+// device, from a kernel of independent product chains over `blocks`
+// workgroups (out: blocks * kBlock words); synchronizes the stream.  A
+// negative value reports a runtime failure.
+double measure_modmul_rate(int form, int blocks, int iters, uint64_t q,
+                           uint64_t k, uint64_t w, uint64_t wq, uint64_t *out,
+                           hipStream_t stream);
+
 // Rescale of `polys` INTT-form polynomials (src/ckks/core_rescale.cpp:5-25).
 void launch_rescale(const DeviceTables &t, const int64_t *in, int64_t *out,
                     int polys, int limbs, int moduli_id, hipStream_t stream);

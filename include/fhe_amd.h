@@ -470,13 +470,36 @@ int fhe_mod_raise(fhe_context *ctx, int count, const int64_t *in,
 
 /* Pipeline choices of a context (no counterpart in the reference: they
  * select among equivalent device implementations, all bit-exact), for tuning
- * and A/B measurement.  Names and defaults: "lazy_key_switch" 1 (lazy
- * transforms where results are canonicalized later), "staged_modup" 1,
- * "fuse_modup" 0, "fuse_tensor" 0, "own_limbs" 1 (partitions' own extended
- * limbs from the NTT-form input), "split_variants" 1 (one NTT launch per
- * arithmetic variant), "ks_lazy_accumulate" 1 (small-prime key products
- * summed without intermediate reductions).  Synchronizes the context's stream. */
+ * and A/B measurement.  Names and defaults:
+ *   "lazy_key_switch" 1    lazy transforms where results are canonicalized
+ *                          later;
+ *   "staged_modup" 1       ModUp through the Garner digits;
+ *   "fuse_modup" 0, "fuse_tensor" 0   one-kernel variants (slower);
+ *   "own_limbs" 1          partitions' own extended limbs from the NTT-form
+ *                          input;
+ *   "split_variants" 1     one NTT launch per arithmetic variant;
+ *   "ks_lazy_accumulate" 1 small-prime key products summed without
+ *                          intermediate reductions;
+ *   "ks_wide_accumulate" 1 key products summed as 128-bit values with one
+ *                          Montgomery reduction per accumulator;
+ *   "overlap_trio" 1       N = 2^16 with split_variants: the INTT of d0 and
+ *                          d1 in fhe_hmult_relin_rescale runs on a second
+ *                          non-blocking stream of the context (default
+ *                          priority), joined by an event before ModDown;
+ *   "ks_occupancy" 2       waves per SIMD of the key inner product (3 trades
+ *                          registers for occupancy; slower);
+ *   "ks_prefetch" 0        the key inner product loads the next partition's
+ *                          tile ahead (slower).
+ * Synchronizes the context's stream. */
 int fhe_context_set_option(fhe_context *ctx, const char *name, int value);
+
+/* Calibration (no counterpart in the reference): the device's rate of
+ * modular products, measured by a kernel of independent product chains at
+ * full occupancy; form 0 = the Shoup product of the lazy transforms, 1 =
+ * the Montgomery product of the exact ones.  The hot path is bound by this
+ * rate rather than by HBM; benchmarks divide their algorithmic products per
+ * second by it.  Synchronizes the context's stream. */
+int fhe_modmul_peak(fhe_context *ctx, int form, double *per_second);
 
 /* Stage timing (no counterpart in the reference, which has no device
  * stages): while enabled, every pipeline stage that a call launches on the

@@ -104,3 +104,26 @@ def test_mod_raise_matches_reference(gpu_lib):
         lib.ref_free_secret_key(sk)
         ctx.close()
         ref.close()
+
+
+@pytest.mark.parametrize("level,flags", [(-1, 0), (10 ** 6, 0), (2, 8),
+                                         (2, 1 << 30)])
+def test_bootstrap_entry_points_refuse_bad_forms(level, flags, gpu_lib):
+    """A level off the chain or unknown flag bits are refused before any
+    device work (the key pointers are never dereferenced: the check comes
+    first)."""
+    with fhe_amd.Context(14, 2) as ctx:
+        n, mid = ctx.n, 2
+        d = ctx.array((1, 2, ctx.limbs_at(mid), n))
+        key = ctx.array((1, 1, n))
+        keys = fhe_amd.BootstrapKeys.make(key, 0, key, 0, [(key, 0, 1)], key,
+                                          0, key, 0)
+        out = ctx.array((1, 2, ctx.limbs_at(0), n))
+        with pytest.raises(fhe_amd.FheError, match="out of range|unknown"):
+            ctx.bootstrap(d, level, flags, keys, 3, out)
+        with pytest.raises(fhe_amd.FheError, match="out of range|unknown"):
+            ctx.coeff_to_slot(d, level, flags, keys, 3, out, out)
+        with pytest.raises(fhe_amd.FheError, match="out of range|unknown"):
+            ctx.slot_to_coeff(d, d, level, flags, keys, 3, out)
+        for arr in (d, key, out):
+            arr.free()

@@ -108,6 +108,96 @@ def test_batched_equals_single_and_oracle(name, gpu_lib):
         arr.free()
 
 
+def test_batch32_benchmark_shape_matches_digests_and_single_launches(gpu_lib):
+    """The benchmark's launch: 32 Gold pairs at moduli_id 13 (rescaled level
+    24) in one batch.  The recorded pair sits at positions 0, 13 and 31 and
+    equals the reference engine's digest there; every position equals its
+    own one-pair launch (grid sizes, XCD swizzle and workspace differ)."""
+    case = [c for c in CASES if c["name"] == "gold_mid13"][0]
+    ctx = context_for(case)
+    n, mid, limbs = ctx.n, case["moduli_id"], case["limbs"]
+    A0, B0, K = case_inputs(case)
+    rng = np.random.Generator(np.random.PCG64(32))
+    batch = 32
+    A = np.stack([np.stack([uniform_limbs(rng, case["primes"], mid, limbs, n)
+                            for _ in range(2)]) for _ in range(batch)])
+    B = np.stack([np.stack([uniform_limbs(rng, case["primes"], mid, limbs, n)
+                            for _ in range(2)]) for _ in range(batch)])
+    recorded = (0, 13, 31)
+    for c in recorded:
+        A[c], B[c] = A0, B0
+    dk = ctx.to_device(K)
+    da, db = ctx.to_device(A), ctx.to_device(B)
+    out = ctx.array((batch, 2, limbs - 1, n))
+    ctx.hmult_relin_rescale(da, db, dk, case["key_moduli_id"], out, mid)
+    batched = out.download()
+    for c in recorded:
+        assert digest(batched[c]) == case["out_sha256"], c
+    sa, sb = ctx.array((1, 2, limbs, n)), ctx.array((1, 2, limbs, n))
+    so = ctx.array((1, 2, limbs - 1, n))
+    for c in range(batch):
+        sa.upload(A[c:c + 1])
+        sb.upload(B[c:c + 1])
+        ctx.hmult_relin_rescale(sa, sb, dk, case["key_moduli_id"], so, mid)
+        assert (so.download()[0] == batched[c]).all(), c
+    for arr in (da, db, dk, out, sa, sb, so):
+        arr.free()
+
+
+@pytest.mark.reference
+@pytest.mark.skipif(not Reference.available(),
+                    reason="oracle/_ref (compiled reference) not built")
+def test_batch32_real_keys_match_reference(gpu_lib):
+    """32 pairs of real Gold encryptions (rescaled level 24, the
+    benchmark's shape) in one launch: every output equals the reference
+    engine's multiply(ct, ct, rlk) + rescale of its pair."""
+    ref = Reference(16, 4, threads=16)
+    lib, e = ref.lib, ref.e
+    sk = lib.ref_secret_key(e)
+    pk = lib.ref_public_key(e, sk)
+    rlk = lib.ref_relin_key(e, sk)
+    key_mid, key = ref.key(rlk)
+    slots = ref.slot_count
+    rng = np.random.default_rng(3232)
+    handles = []
+    ctx = fhe_amd.Context(16, 4, primes=ref.primes)
+    try:
+        # 8 rescaled encryptions give 32 distinct ordered pairs
+        cts = []
+        for _ in range(8):
+            x = rng.uniform(-1, 1, slots)
+            c0 = lib.ref_encrypt(e, ptr(x), slots, pk, 24)
+            c1 = lib.ref_rescale(e, c0)
+            handles += [c0, c1]
+            cts.append(c1)
+        mid = ref.shape(cts[0])[0]
+        assert mid == 13
+        pairs = [(i, j) for i in range(8) for j in range(8) if i != j][:32]
+        data = [ref.get(c) for c in cts]
+        A = np.stack([data[i] for i, _ in pairs])
+        B = np.stack([data[j] for _, j in pairs])
+        da, db, dk = ctx.to_device(A), ctx.to_device(B), ctx.to_device(key)
+        limbs = A.shape[2]
+        out = ctx.array((32, 2, limbs - 1, ref.n))
+        ctx.hmult_relin_rescale(da, db, dk, key_mid, out, mid)
+        got = out.download()
+        for k, (i, j) in enumerate(pairs):
+            w = lib.ref_hmult_relin_rescale(e, cts[i], cts[j], rlk)
+            want = ref.get(w)
+            lib.ref_free_ct(w)
+            assert np.array_equal(got[k], want), (k, i, j)
+        for arr in (da, db, dk, out):
+            arr.free()
+    finally:
+        ctx.close()
+        for h in handles:
+            lib.ref_free_ct(h)
+        lib.ref_free_relin_key(rlk)
+        lib.ref_free_public_key(pk)
+        lib.ref_free_secret_key(sk)
+        ref.close()
+
+
 @pytest.mark.parametrize("name", ["iron_mid1", "silver_mid1", "gold_mid30"])
 def test_relinearize_matches_oracle(name, gpu_lib):
     case = [c for c in CASES if c["name"] == name][0]

@@ -49,6 +49,17 @@ def test_wire_round_trip_and_refusals(gpu_lib):
                 ctx.wire_read(blob, small)
             for arr in (d, out, small):
                 arr.free()
+        # forged headers (the checksum covers the payload only): a limb
+        # range past the chain whose int sum would wrap, a polynomial count
+        # whose byte size would wrap, a level off the chain, unknown flags
+        import struct
+        for off, value in [(32, 2 ** 31 - 1), (36, 2 ** 30), (28, 2 ** 15),
+                           (28, 2 ** 31 - 1), (40, 1000), (40, -2),
+                           (44, 8)]:
+            forged = bytearray(blob)
+            forged[off:off + 4] = struct.pack("<i", value)
+            with pytest.raises(fhe_amd.FheError):
+                ctx.wire_header(bytes(forged))
         with fhe_amd.Context(13, 1) as other:
             with pytest.raises(fhe_amd.FheError):
                 other.wire_header(blob)
