@@ -102,11 +102,19 @@ def test_default_engine_end_to_end_matches_reference(gpu_lib):
         dmid, want_pt = ref.decrypt_limbs(rr, sk)
         assert dmid == prod_mid + 1
         assert np.array_equal(dec.download()[0], want_pt)
-        scale = ref.ct_scale(rr)
+        # the decrypted plaintext carries the scale of the level it was
+        # brought down to (level_down_ sets the level's standard scale)
+        scale = fhe_amd.params_encode_constants(
+            14, 1, prod_level - 1, primes=ref.primes)["ckks_scale"]
         got = ctx.decode(dec, dmid, scale, complex_output=True)[0]
         want = ref.decode(want_pt, prod_level - 1, scale, complex_output=True)
         assert np.array_equal(got.real, want.real)
         assert np.array_equal(got.imag, want.imag)
+        # the reference engine's own decrypt (real parts) agrees within the
+        # 2^-25 absolute tolerance of decrypted slots
+        full = np.zeros(slots)
+        assert lib.ref_decrypt(e, rr, sk, ptr(full)) == slots
+        assert np.abs(got.real - full).max() < 2.0 ** -25
         assert np.abs(got - x * y).max() < 1e-5
     finally:
         for h in handles:
