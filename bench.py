@@ -322,6 +322,136 @@ def bootstrap_bench(ev, steps):
             "timing": "wall clock, host work included"}
 
 
+UNEVEN_ALPHA = 8
+
+
+def uneven_bench(ev, steps, count=32, autotune=True):
+    """Key switching over ciphertexts at uneven levels: `count` NTT-form
+    trios at limb counts drawn from 6..24 (N = 2^16, alpha = 8, dnum <= 3)
+    are relinearized and each result rotated by its own random slot offset
+    with a RotationKey (15 power-of-two keys), one step = the whole batch.
+    Timed: the uneven batch (one launch per stage for all levels, eager and
+    replayed from a HIP graph), the same batch one level group at a time
+    (the same kernels per distinct moduli_id) and one ciphertext at a time
+    (fhe_relinearize, fhe_rotate per fixed key).  Synthetic keys and inputs
+    of the real shapes (the work does not depend on their values)."""
+    import fhe_amd
+    rng = np.random.Generator(np.random.PCG64(17))
+    with fhe_amd.Context(LOG_N, UNEVEN_ALPHA) as ctx:
+        n = ctx.n
+        full = ctx.info["chain_count"]
+        limbs = [int(v) for v in rng.integers(6, 25, count)]
+        mids = [full - l for l in limbs]
+        key_mid = ctx.key_moduli_id()
+        kp, kc, _ = ctx.key_shape(key_mid)
+
+        def key():
+            d = ctx.array((kp, kc, n))
+            for p in range(kp):
+                d.upload_at(uniform_limbs(rng, ctx.primes, key_mid, kc, n, 2),
+                            p * kc * n)
+            return d
+
+        rlk = key()
+        rot_deltas = [1 << b for b in range(LOG_N - 1)]
+        rot_keys = [key() for _ in rot_deltas]
+        rot_mids = [key_mid] * len(rot_keys)
+        deltas = [int(v) for v in rng.integers(1, n // 2, count)]
+        trios = [ctx.to_device(np.stack([uniform_limbs(rng, ctx.primes, m, l,
+                                                       n)
+                                         for _ in range(3)]))
+                 for m, l in zip(mids, limbs)]
+        relin = [ctx.array((2, l, n)) for l in limbs]
+        rot = [ctx.array((2, l, n)) for l in limbs]
+        def step(sel=None):
+            idx = range(count) if sel is None else sel
+            ctx.relinearize_uneven([trios[c] for c in idx],
+                                   [mids[c] for c in idx], rlk, key_mid,
+                                   [relin[c] for c in idx])
+            ctx.rotate_rotation_key_uneven(
+                [relin[c] for c in idx], [mids[c] for c in idx], rot_keys,
+                rot_mids, rot_deltas, [deltas[c] for c in idx],
+                [rot[c] for c in idx])
+
+        groups = {}
+        for c, m in enumerate(mids):
+            groups.setdefault(m, []).append(c)
+
+        def grouped():
+            for sel in groups.values():
+                step(sel)
+
+        def plain():
+            for c in range(count):
+                ctx.relinearize(trios[c], rlk, key_mid, relin[c], mids[c])
+                for b in range(LOG_N - 1):
+                    if deltas[c] & (1 << b):
+                        ctx.rotate(relin[c], rot_keys[b], key_mid, 1 << b,
+                                   relin[c], mids[c])
+
+        # default schedules: the batch, one level group at a time, one
+        # ciphertext at a time
+        t0 = time.perf_counter()
+        step()                      # builds the plans
+        ctx.synchronize()
+        setup_s = time.perf_counter() - t0
+        default_ms = time_steps(ctx, ev, step, steps, 1)
+        grouped()
+        grouped_ms = time_steps(ctx, ev, grouped, max(3, steps // 2), 1)
+        plain_ms = time_steps(ctx, ev, plain, max(3, steps // 4), 1)
+        # autotuned schedules (tuned on the next call of each plan), eager and
+        # replayed from a HIP graph of the whole step
+        tune_s = 0.0
+        if autotune:
+            ctx.uneven_autotune(True)
+            t0 = time.perf_counter()
+            step()
+            ctx.synchronize()
+            tune_s = time.perf_counter() - t0
+        batched_ms = time_steps(ctx, ev, step, steps, 1)
+        ctx.graph_begin()
+        try:
+            step()
+        finally:
+            graph = ctx.graph_end()
+        graph_ms = time_steps(ctx, ev, lambda: ctx.graph_launch(graph), steps,
+                              2)
+        ctx.graph_destroy(graph)
+        schedule = ctx.uneven_schedule(True, mids, key_mid)
+        rotations = sum(bin(d).count("1") for d in deltas)
+        for arr in [rlk] + rot_keys + trios + relin + rot:
+            arr.free()
+    stage_names = ["trio_intt1", "trio_intt2", "digits", "extend",
+                   "key_product", "acc_intt2", "moddown", "automorphism"]
+    return {
+        "workload": f"{count} ciphertexts at limb counts drawn from 6..24 "
+                    "(N=2^16, alpha=8, dnum<=3): relinearize, then rotate "
+                    "each by its own random slot offset with a RotationKey "
+                    f"({rotations} fixed-key rotations per step)",
+        "limbs": limbs,
+        "level_groups": len(groups),
+        "ciphertexts_per_s": round(count / (batched_ms / 1e3), 1),
+        "ms_per_step": round(batched_ms, 3),
+        "graph_ms_per_step": round(graph_ms, 3),
+        "graph_ciphertexts_per_s": round(count / (graph_ms / 1e3), 1),
+        "default_schedule_ms_per_step": round(default_ms, 3),
+        "grouped_ms_per_step": round(grouped_ms, 3),
+        "plain_ms_per_step": round(plain_ms, 3),
+        "batched_vs_grouped": round(grouped_ms / default_ms, 3),
+        "batched_vs_plain": round(plain_ms / default_ms, 3),
+        "comparison": "grouped and plain against the batch with the same "
+                      "(default) schedules; ms_per_step is the batch with "
+                      "autotuned schedules",
+        "key_switches_per_s": round((count + rotations) / (batched_ms / 1e3),
+                                    1),
+        "autotune": autotune,
+        "relin_schedule": {name: {"cap": c, "xcd": x}
+                           for name, (c, x) in zip(stage_names, schedule)},
+        "first_call_s": round(setup_s, 2),
+        "autotune_s": round(tune_s, 2),
+    }
+
+
 # The unit of work's algorithmic cost (Gold, N = 2^16, l = 25 -> 24,
 # alpha = 4): HBM bytes of one HMult+relin+rescale when B ciphertexts share
 # one key stream, 8N [4l + 2(l - 1) + 2 ceil(l / alpha)(l + alpha) / B] (read
@@ -443,6 +573,11 @@ def main():
                     help="skip the encode / decode measurement")
     ap.add_argument("--no-bootstrap", action="store_true",
                     help="skip the bootstrap measurement")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven-level key-switching measurement")
+    ap.add_argument("--only-uneven", action="store_true",
+                    help="only run the uneven-level key-switching "
+                         "measurement and print its JSON object")
     ap.add_argument("--only-bootstrap", action="store_true",
                     help="only run the bootstrap measurement (e.g. under a "
                          "profiler) and print its JSON object")
@@ -464,6 +599,9 @@ def main():
     ev = HipEvents()
     if args.only_bootstrap:
         print(json.dumps(bootstrap_bench(ev, 3)))
+        return
+    if args.only_uneven:
+        print(json.dumps(uneven_bench(ev, 10)))
         return
     with fhe_amd.Context(LOG_N, ALPHA) as ctx:
         limbs = ctx.limbs_at(MODULI_ID)
@@ -503,6 +641,7 @@ def main():
                                 not args.no_cpu)
 
     boot = None if args.no_bootstrap else bootstrap_bench(ev, 3)
+    uneven = None if args.no_uneven else uneven_bench(ev, 10)
     value = args.batch / (step_ms / 1e3)
     shape = dict(n=n, count=args.batch, limbs=limbs, ext=limbs + ALPHA,
                  parts=parts)
@@ -533,6 +672,7 @@ def main():
         "stages": breakdown,
         "encode_decode": codec,
         "bootstrap": boot,
+        "uneven_key_switch": uneven,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()

@@ -37,6 +37,10 @@ EXPORTED_SYMBOLS = (
     "fhe_round", "fhe_coeff_to_slot", "fhe_slot_to_coeff", "fhe_bootstrap",
     "fhe_bootstrap_twiddles", "fhe_wire_size", "fhe_wire_write",
     "fhe_wire_read_header", "fhe_wire_read", "fhe_modmul_peak",
+    "fhe_relinearize_uneven", "fhe_rotate_uneven",
+    "fhe_rotate_rotation_key_uneven", "fhe_context_uneven_autotune",
+    "fhe_uneven_schedule", "fhe_graph_begin", "fhe_graph_end",
+    "fhe_graph_launch", "fhe_graph_destroy",
 )
 
 # fhe_wire_* record kinds (FHE_WIRE_* in include/fhe_amd.h)
@@ -180,6 +184,18 @@ def library():
         lib.fhe_rotate_batch.argtypes = [vp, i32, i32, vp, i32, vp, vp, vp,
                                          vp]
         lib.fhe_switch_key.argtypes = [vp, i32, i32, vp, vp, i32, vp]
+        lib.fhe_relinearize_uneven.argtypes = [vp, i32, vp, vp, vp, i32, vp]
+        lib.fhe_rotate_uneven.argtypes = [vp, i32, vp, vp, vp, i32, i32, vp]
+        lib.fhe_rotate_rotation_key_uneven.argtypes = [vp, i32, vp, vp, i32,
+                                                       vp, vp, vp, vp, vp]
+        lib.fhe_context_uneven_autotune.argtypes = [vp, i32]
+        lib.fhe_uneven_schedule.argtypes = [vp, i32, i32, vp, i32, vp, vp,
+                                            i32]
+        lib.fhe_graph_begin.argtypes = [vp]
+        lib.fhe_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
+        lib.fhe_graph_launch.argtypes = [vp, vp]
+        lib.fhe_graph_destroy.argtypes = [vp]
+        lib.fhe_graph_destroy.restype = None
         lib.fhe_params_multiplier.argtypes = [
             ctypes.POINTER(Params), i32, ctypes.c_double, ctypes.c_int64, i32,
             i32, vp, i32, ctypes.POINTER(i32)]
@@ -230,6 +246,19 @@ def library():
             ctypes.POINTER(ctypes.c_int64)]
         _lib = lib
     return _lib
+
+
+def _addr(x):
+    """Device address of a DeviceArray, or an int address as it is."""
+    return x.ptr.value if isinstance(x, DeviceArray) else int(x)
+
+
+def _addrs(xs):
+    return (ctypes.c_void_p * len(xs))(*[_addr(x) for x in xs])
+
+
+def _ints(xs):
+    return (ctypes.c_int * len(xs))(*[int(x) for x in xs])
 
 
 def _longdouble(x):
@@ -572,6 +601,62 @@ class Context:
         dl = (ctypes.c_int * m)(*deltas)
         _check(library().fhe_rotate_batch(self.handle, count, moduli_id,
                                           cts.ptr, m, kp, km, dl, op))
+
+    # --- batches at uneven levels: one device buffer per ciphertext, given
+    # as DeviceArrays or raw device addresses
+    def relinearize_uneven(self, trios, moduli_ids, key, key_moduli_id,
+                           outs):
+        """fhe_relinearize_uneven: NTT-form trios[c] at moduli_ids[c] into
+        INTT-form duos outs[c]."""
+        n = len(moduli_ids)
+        _check(library().fhe_relinearize_uneven(
+            self.handle, n, _ints(moduli_ids), _addrs(trios), key.ptr,
+            key_moduli_id, _addrs(outs)))
+
+    def rotate_uneven(self, cts, moduli_ids, key, key_moduli_id, delta, outs):
+        """fhe_rotate_uneven: INTT-form duos rotated by one fixed key."""
+        n = len(moduli_ids)
+        _check(library().fhe_rotate_uneven(
+            self.handle, n, _ints(moduli_ids), _addrs(cts), key.ptr,
+            key_moduli_id, delta, _addrs(outs)))
+
+    def rotate_rotation_key_uneven(self, cts, moduli_ids, keys,
+                                   key_moduli_ids, key_deltas, deltas, outs):
+        """fhe_rotate_rotation_key_uneven: ciphertext c rotated by deltas[c]
+        with the RotationKey's fixed keys."""
+        n = len(moduli_ids)
+        _check(library().fhe_rotate_rotation_key_uneven(
+            self.handle, n, _ints(moduli_ids), _addrs(cts), len(keys),
+            _addrs(keys), _ints(key_moduli_ids), _ints(key_deltas),
+            _ints(deltas), _addrs(outs)))
+
+    def uneven_autotune(self, on):
+        _check(library().fhe_context_uneven_autotune(self.handle,
+                                                     1 if on else 0))
+
+    def uneven_schedule(self, relin, moduli_ids, key_moduli_id):
+        """Per-stage (cap, xcd) schedule of a batch shape that has run."""
+        caps = (ctypes.c_int * 8)()
+        xcds = (ctypes.c_int * 8)()
+        _check(library().fhe_uneven_schedule(
+            self.handle, 1 if relin else 0, len(moduli_ids),
+            _ints(moduli_ids), key_moduli_id, caps, xcds, 8))
+        return list(zip(caps, xcds))
+
+    def graph_begin(self):
+        _check(library().fhe_graph_begin(self.handle))
+
+    def graph_end(self):
+        g = ctypes.c_void_p()
+        _check(library().fhe_graph_end(self.handle, ctypes.byref(g)))
+        return g
+
+    def graph_launch(self, graph):
+        _check(library().fhe_graph_launch(self.handle, graph))
+
+    @staticmethod
+    def graph_destroy(graph):
+        library().fhe_graph_destroy(graph)
 
     def switch_key(self, cts, key, key_moduli_id, out, moduli_id):
         count = int(np.prod(cts.shape[:-3]))

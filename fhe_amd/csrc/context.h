@@ -103,6 +103,10 @@ inline void ok(int rc) {
 
 using namespace fhe_host;
 
+// Work lists of one uneven-level batch shape (uneven.hip).
+struct UnevenPlan;
+void uneven_plan_free(UnevenPlan *plan);
+
 struct fhe_context {
   fhe::CkksParams params;
   fhe::DeviceTables tables{};
@@ -133,6 +137,8 @@ struct fhe_context {
   hipEvent_t fork_event = nullptr, join_event = nullptr;
   // small-prime key products accumulated without guards, reduced once
   bool ks_lazy_accumulate = true;
+  // small-prime key products summed unreduced (96 bits), reduced once
+  bool ks_wide_accumulate = true;
   // key inner product loads the next partition's tile ahead
   bool ks_prefetch = false;
   // waves per SIMD targeted by the key inner product's register allocation
@@ -165,6 +171,14 @@ struct fhe_context {
   std::map<std::tuple<int, int, bool>,
            std::vector<std::vector<std::vector<std::complex<double>>>>>
       boot_twiddles;
+  // uneven-level batches: work lists per (kind, key moduli_id, moduli_ids)
+  std::map<std::vector<int>, UnevenPlan *> uneven_plans;
+  // launch schedules of the uneven-level kernels chosen by timing the
+  // candidates on the first call of each batch shape
+  bool uneven_autotune = false;
+  // the context's stream is being captured into a HIP graph: no call may
+  // synchronize, allocate or upload
+  bool capturing = false;
 
   void *pool_take(size_t bytes) {
     auto it = text_pool.find(bytes);
@@ -173,6 +187,7 @@ struct fhe_context {
       text_pool.erase(it);
       return p;
     }
+    require_not_capturing("text allocation");
     void *p = nullptr;
     hip_check(hipMalloc(&p, bytes), "text allocation");
     return p;
@@ -239,6 +254,8 @@ struct fhe_context {
       (void)hipFree(kv.second);
     for (auto &kv : boot_diagonals)
       (void)hipFree(kv.second.data);
+    for (auto &kv : uneven_plans)
+      uneven_plan_free(kv.second);
     if (workspace)
       (void)hipFree(workspace);
     if (aux)
@@ -301,8 +318,15 @@ struct fhe_context {
                                            : 1;
   }
 
+  void require_not_capturing(const char *what) const {
+    require(!capturing, std::string(what) +
+                            ": not allowed while the stream is captured (run "
+                            "the call once before capturing it)");
+  }
+
   int64_t *aux_scratch(size_t bytes) {
     if (bytes > aux_bytes) {
+      require_not_capturing("workspace growth");
       hip_check(hipStreamSynchronize(stream), "workspace sync");
       if (aux)
         hip_check(hipFree(aux), "workspace free");
@@ -316,6 +340,7 @@ struct fhe_context {
 
   int64_t *stash(size_t bytes) {
     if (bytes > stash_bytes) {
+      require_not_capturing("workspace growth");
       hip_check(hipStreamSynchronize(stream), "workspace sync");
       if (stash_buf)
         hip_check(hipFree(stash_buf), "workspace free");
@@ -350,6 +375,7 @@ struct fhe_context {
 
   int64_t *scratch(size_t bytes) {
     if (bytes > workspace_bytes) {
+      require_not_capturing("workspace growth");
       hip_check(hipStreamSynchronize(stream), "workspace sync");
       if (workspace)
         hip_check(hipFree(workspace), "workspace free");
@@ -583,6 +609,7 @@ struct fhe_context {
     auto it = modup_cache.find(key);
     if (it != modup_cache.end())
       return it->second;
+    require_not_capturing("constant upload");
     const auto host = params.modup_constants(moduli_id, key_moduli_id);
     void *p = nullptr;
     hip_check(hipMalloc(&p, host.size() * sizeof(int64_t)), "constants");
@@ -762,8 +789,8 @@ struct fhe_context {
       fhe::launch_ks_inner_product_fused(
           tables, ext, key, key_chain, key_offset, parts[0].key_index, np, acc,
           count, ext_limbs, moduli_id, lazy_key_switch, own, own_stride,
-          starts.data(), sizes.data(), ks_lazy_accumulate, ks_prefetch,
-          ks_occupancy, stream);
+          starts.data(), sizes.data(), ks_lazy_accumulate, ks_wide_accumulate,
+          ks_prefetch, ks_occupancy, stream);
     });
     stage("intt_pass_high(acc)", [&] {
       fhe::ntt_inverse_second(tables, acc, acc, int64_t(count) * 2 * ext_limbs,

@@ -140,6 +140,8 @@ int fhe_context_set_option(fhe_context *ctx, const char *name, int value) {
       ctx->ks_prefetch = value != 0;
     else if (n == "ks_lazy_accumulate")
       ctx->ks_lazy_accumulate = value != 0;
+    else if (n == "ks_wide_accumulate")
+      ctx->ks_wide_accumulate = value != 0;
     else if (n == "split_variants")
       ctx->tables.split_variants = value != 0 ? 1 : 0;
     else
@@ -964,6 +966,7 @@ int fhe_decrypt(fhe_context *ctx, int count, int moduli_id, const int64_t *ct,
 int fhe_profile_enable(fhe_context *ctx, int on) {
   return guarded([&] {
     require(ctx != nullptr, "null argument");
+    ctx->require_not_capturing("Stage timing");
     ctx->fold_timings();
     if (on) {
       ctx->stage_names.clear();

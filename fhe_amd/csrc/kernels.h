@@ -64,8 +64,9 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
                                    int ext_limbs, int moduli_id, bool lazy,
                                    const int64_t *own, int64_t own_stride,
                                    const int *starts, const int *sizes,
-                                   bool lazy_accumulate, bool prefetch,
-                                   int occupancy, hipStream_t stream);
+                                   bool lazy_accumulate, bool wide_accumulate,
+                                   bool prefetch, int occupancy,
+                                   hipStream_t stream);
 
 // out = mont_mult(a, b) elementwise (src/montgomery/unit.cpp:5-10).
 void launch_mont_mult(const DeviceTables &t, const int64_t *a,

@@ -11,37 +11,12 @@
 // single write.  Each coefficient sees the reference's exact sequence of
 // butterflies, products and guards.
 #include "kernels.h"
-#include "modup.h"
-#include "ntt_r16.h"
+#include "ks_tile.h"
 
 namespace fhe {
 
 namespace {
 
-struct KsArgs {
-  const int64_t *ext;  // [count][parts][ext_limbs][N], forward pass 1 done
-  const int64_t *key;  // [key_polys][key_chain][N]
-  int64_t *acc;        // [count][2][ext_limbs][N], inverse pass 1 done
-  int count;
-  int parts;
-  int ext_limbs;
-  int moduli_id;
-  int key_chain;
-  int key_offset;      // first key limb aligned with the ciphertext's limb 0
-  int key_partition0;  // key partition index of the ciphertext's partition 0
-  int swizzle;
-  int lazy;            // lazy transforms (the ext pass before was lazy too)
-  // NTT form of the key-switched polynomial, when the caller has it: limb i
-  // of ciphertext c at own + c * own_stride + i * N.  The extended limbs of a
-  // partition's own primes are that polynomial's limbs, so their transforms
-  // are taken from here instead of the extended polynomial (any
-  // representative serves: the key products are canonicalized later).
-  const int64_t *own;
-  int64_t own_stride;
-  int start[kMaxFusedParts];
-  int size[kMaxFusedParts];
-  int lazy_acc;        // small primes: accumulate without guards
-};
 
 // One (extended limb, tile, ciphertext) work item with arithmetic A for the
 // two transforms (the key products and the accumulation are the same in
@@ -422,48 +397,15 @@ __device__ __forceinline__ void extend_ntt_tile(const ExtendArgs &a,
                                                 int64_t *lds, int64_t cp,
                                                 int part, int tile, int i) {
   const int64_t n = int64_t(1) << t.log_n;
-  const int p = a.moduli_id + i;
-  const int tid = threadIdx.x;
-  const int c = tid & 15, r = tid >> 4;
-  const int64_t sp = int64_t(tile) * 16 + c;
-  const int size = a.size[part];
-  const int64_t *dg = a.digits + cp * a.maxz * n + sp;
-  const int64_t *cst = a.cst + int64_t(part) * a.cst_stride;
-  R16Pass s;
-  s.tw = t.root + int64_t(p) * n;
-  s.ts = t.root_shoup + int64_t(p) * n * 2;
-  s.n = n;
-  s.hi = 0;
-  s.log_n = t.log_n;
-  s.b = 8;
-  s.r = r;
-  s.q = t.q[p];
-  s.k = t.k[p];
-  s.q2 = int64_t(t.q2[p]);
-  int64_t v[16];
-#pragma unroll
-  for (int m = 0; m < 16; m++) {
-    int64_t d[MAXZ];
-#pragma unroll
-    for (int j = 0; j < MAXZ; j++)
-      d[j] = j < size ? dg[int64_t(j) * n + (int64_t(r + 16 * m) << 8)] : 0;
-    if (A == kExact)
-      v[m] = extend_limb<MAXZ, SIGNED>(d, size, i, a.ext_limbs,
-                                       cst + CkksParams::kKsExtOffset, s.q,
-                                       s.k, s.q2);
-    else
-      v[m] = extend_limb_lazy<MAXZ>(
-          d, size, i, a.ext_limbs,
-          reinterpret_cast<const uint64_t *>(cst + a.ext_shoup_offset), s.q,
-          A == kLazySmall);
-  }
-  r16_forward_high<A>(v, s);
-  r16_exchange(v, lds, c, r, true);
-  r16_forward_low<A>(v, s);
-  int64_t *out = a.ext + (cp * a.ext_limbs + i) * n + sp;
-#pragma unroll
-  for (int m = 0; m < 16; m++)
-    out[int64_t(16 * r + m) << 8] = v[m];
+  ExtItem it;
+  it.digits = a.digits + cp * a.maxz * n;
+  it.cst = a.cst + int64_t(part) * a.cst_stride;
+  it.ext = a.ext + cp * a.ext_limbs * n;
+  it.size = a.size[part];
+  it.ext_limbs = a.ext_limbs;
+  it.moduli_id = a.moduli_id;
+  it.ext_shoup_offset = a.ext_shoup_offset;
+  ext_item_tile<A, MAXZ, SIGNED>(it, t, lds, tile, i);
 }
 
 template <int MAXZ, bool LAZY>
@@ -654,8 +596,9 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
                                    int ext_limbs, int moduli_id, bool lazy,
                                    const int64_t *own, int64_t own_stride,
                                    const int *starts, const int *sizes,
-                                   bool lazy_accumulate, bool prefetch,
-                                   int occupancy, hipStream_t stream) {
+                                   bool lazy_accumulate, bool wide_accumulate,
+                                   bool prefetch, int occupancy,
+                                   hipStream_t stream) {
   const int tiles = int((int64_t(1) << t.log_n) >> 12);
   const int64_t blocks = int64_t(ext_limbs) * tiles * count;
   KsArgs a{};
@@ -677,6 +620,13 @@ void launch_ks_inner_product_fused(const DeviceTables &t, const int64_t *ext,
   for (int p = 0; p < parts && p < kMaxFusedParts; p++) {
     a.start[p] = own ? starts[p] : 0;
     a.size[p] = own ? sizes[p] : 0;
+  }
+  // small primes' key products summed unreduced (a kernel of its own
+  // translation unit, ks_wide.hip)
+  if (wide_accumulate && lazy && !prefetch && occupancy < 3 && parts > 1 &&
+      parts <= kMaxWideParts) {
+    launch_ks_inner_product_wide(a, t, blocks, stream);
+    return;
   }
   if (prefetch)
     hipLaunchKernelGGL(ks_inner_product_r16_kernel<true>,

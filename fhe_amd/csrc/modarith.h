@@ -51,15 +51,20 @@ FHE_HD uint64_t mont_reduce(U128 x, uint64_t q, uint64_t k) {
   return (hi << 2) | (lo >> kMontBits);
 }
 
-#if defined(__HIP_DEVICE_COMPILE__)
 // 32-bit add with carry in/out: v_add_co_u32 / v_addc_co_u32 on gfx950,
 // each a full-rate op, where a 64-bit add (v_lshl_add_u64) is quarter rate.
-__device__ __forceinline__ uint32_t add_carry(uint32_t a, uint32_t b,
-                                              uint32_t cin, uint32_t &cout) {
+FHE_HD uint32_t add_carry(uint32_t a, uint32_t b, uint32_t cin,
+                          uint32_t &cout) {
+#if defined(__clang__)
   unsigned int co;
   const uint32_t r = __builtin_addc(a, b, cin, &co);
   cout = co;
   return r;
+#else
+  const uint64_t s = uint64_t(a) + b + cin;
+  cout = uint32_t(s >> 32);
+  return uint32_t(s);
+#endif
 }
 
 // 128-bit product of a, b < 2^63 as four 32-bit words.  The two cross
@@ -68,7 +73,7 @@ __device__ __forceinline__ uint32_t add_carry(uint32_t a, uint32_t b,
 struct Words4 {
   uint32_t w0, w1, w2, w3;
 };
-__device__ __forceinline__ Words4 mul_words(uint64_t a, uint64_t b) {
+FHE_HD Words4 mul_words(uint64_t a, uint64_t b) {
   const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32);
   const uint32_t b0 = uint32_t(b), b1 = uint32_t(b >> 32);
   const uint64_t p00 = uint64_t(a0) * b0;
@@ -82,7 +87,6 @@ __device__ __forceinline__ Words4 mul_words(uint64_t a, uint64_t b) {
   x.w3 = uint32_t(p11 >> 32) + c;
   return x;
 }
-#endif
 
 // Montgomery product of two NON-NEGATIVE operands below 2^63 (lazy residues
 // in [0, 2q) and table constants in [0, q)).  Same value as mont_mult.

@@ -142,92 +142,16 @@ __global__ __launch_bounds__(kBlock) void ntt_pass_kernel(PassArgs a,
   }
 }
 
-// ---------------------------------------------------------------------------
-// Register-resident pass over an 8-bit range: each of the 16 sub-transforms
-// of a tile (256 points) is owned by 16 threads holding 16 points each, and
-// the eight radix-2 stages run as two radix-16 rounds in registers with one
-// LDS exchange between them (ntt_r16.h).  A second exchange is added where
-// the global side of the pass is contiguous and the register ownership is
-// not, so that every global access is coalesced.  Prime, twiddle base and
-// addresses are per-block scalars.
-// ---------------------------------------------------------------------------
+// Register-resident 8-bit pass (r16_limb_pass, ntt_r16.h) on limb l of
+// polynomial `poly` of the batch.
 template <bool INV, bool COLS, int A>
 __device__ __forceinline__ void r16_pass(const PassArgs &a,
                                          const DeviceTables &t, int64_t *lds,
                                          int64_t poly, int tile, int l) {
-  const int log_n = t.log_n;
-  const int64_t n = int64_t(1) << log_n;
-  const int b = a.low_bit;
-  const int p = a.moduli_id + l;
-
-  const int tid = threadIdx.x;
-  const int c = COLS ? (tid & 15) : (tid >> 4);
-  const int r = COLS ? (tid >> 4) : (tid & 15);
-  const int64_t sp = int64_t(tile) * 16 + c;
-  const int64_t lo = sp & ((int64_t(1) << b) - 1);
-  const int64_t hi = sp >> b;
-  const int64_t base = (poly * a.limbs + l) * n + (hi << (b + 8)) + lo;
-  const int64_t *src = a.src + base;
-  int64_t *dst = a.dst + base;
-
-  R16Pass s;
-  s.tw = (INV ? t.root_inverse : t.root) + int64_t(p) * n;
-  s.ts = (INV ? t.root_inverse_shoup : t.root_shoup) + int64_t(p) * n * 2;
-  s.n = n;
-  s.hi = hi;
-  s.log_n = log_n;
-  s.b = b;
-  s.r = r;
-  s.q = t.q[p];
-  s.k = t.k[p];
-  s.q2 = int64_t(t.q2[p]);
-
-  int64_t v[16];
-  // load in the ownership of the first round; the contiguous inverse pass
-  // loads coalesced ROWS and re-owns through LDS
-  const bool cols_load = INV && COLS;
-#pragma unroll
-  for (int m = 0; m < 16; m++)
-    v[m] = src[int64_t(cols_load ? 16 * r + m : r + 16 * m) << b];
-  if (a.pre) {
-    const int64_t cst = a.pre[p];
-#pragma unroll
-    for (int m = 0; m < 16; m++)
-      v[m] = load_product<A>(v[m], cst, s.q, s.k);
-  }
-  if (INV && !COLS)
-    r16_exchange(v, lds, c, r, true);
-
-  if (!INV) {
-    r16_forward_high<A>(v, s);
-    r16_exchange(v, lds, c, r, true);
-    r16_forward_low<A>(v, s);
-    if (a.last)
-      r16_forward_settle<A>(v, s);
-    if (!COLS)
-      r16_exchange(v, lds, c, r, false); // coalesced contiguous store
-  } else {
-    r16_inverse_low<A>(v, s);
-    r16_exchange(v, lds, c, r, false);
-    r16_inverse_high<A>(v, s);
-    if (a.post != kInttRaw) {
-      const int64_t ninv = t.ninv[p];
-#pragma unroll
-      for (int m = 0; m < 16; m++) {
-        // only the exact arithmetic carries negative values
-        const int64_t y = mont_exit(
-            A == kExact ? mont_mult_cs(ninv, v[m], s.q, s.k)
-                        : mont_mult_u(v[m], ninv, s.q, s.k),
-            s.q, s.k);
-        v[m] = a.post == kInttExitReduce ? guard(y, int64_t(s.q)) : y;
-      }
-    }
-  }
-  // store: ROWS ownership except the strided forward pass (COLS)
-  const bool cols_store = !INV && COLS;
-#pragma unroll
-  for (int m = 0; m < 16; m++)
-    dst[int64_t(cols_store ? 16 * r + m : r + 16 * m) << b] = v[m];
+  const int64_t limb = (poly * a.limbs + l) << t.log_n;
+  r16_limb_pass<INV, COLS, A>(a.src + limb, a.dst + limb, a.moduli_id + l,
+                              tile, a.low_bit, a.pre, a.post, a.last != 0, t,
+                              lds);
 }
 
 // Register-resident pass kernel with one arithmetic A per launch (a single

@@ -239,4 +239,92 @@ __device__ __forceinline__ void r16_exchange(int64_t (&v)[16], int64_t *lds,
     v[m] = lds[r16_lds(c, to_cols ? 16 * r + m : r + 16 * m)];
 }
 
+// One register-resident 8-bit pass over one 4096-point tile of one limb:
+// the 16 sub-transforms of the tile are owned by 16 threads each (16 points
+// per thread) and the eight radix-2 stages run as two radix-16 rounds with
+// one LDS exchange between them.  A second exchange is added where the
+// global side of the pass is contiguous and the register ownership is not,
+// so that every global access is coalesced.  src / dst point at the limb
+// (prime p); the pass covers index bits [b, b + 8); COLS is b > 0.  pre
+// (forward, per prime) multiplies every loaded value first; post (inverse)
+// is the InttPost epilogue; last = the pass completes a forward transform.
+template <bool INV, bool COLS, int A>
+__device__ __forceinline__ void r16_limb_pass(const int64_t *src_limb,
+                                              int64_t *dst_limb, int p,
+                                              int tile, int b,
+                                              const int64_t *pre, int post,
+                                              bool last, const DeviceTables &t,
+                                              int64_t *lds) {
+  const int log_n = t.log_n;
+  const int64_t n = int64_t(1) << log_n;
+  const int tid = threadIdx.x;
+  const int c = COLS ? (tid & 15) : (tid >> 4);
+  const int r = COLS ? (tid >> 4) : (tid & 15);
+  const int64_t sp = int64_t(tile) * 16 + c;
+  const int64_t lo = sp & ((int64_t(1) << b) - 1);
+  const int64_t hi = sp >> b;
+  const int64_t base = (hi << (b + 8)) + lo;
+  const int64_t *src = src_limb + base;
+  int64_t *dst = dst_limb + base;
+
+  R16Pass s;
+  s.tw = (INV ? t.root_inverse : t.root) + int64_t(p) * n;
+  s.ts = (INV ? t.root_inverse_shoup : t.root_shoup) + int64_t(p) * n * 2;
+  s.n = n;
+  s.hi = hi;
+  s.log_n = log_n;
+  s.b = b;
+  s.r = r;
+  s.q = t.q[p];
+  s.k = t.k[p];
+  s.q2 = int64_t(t.q2[p]);
+
+  int64_t v[16];
+  // load in the ownership of the first round; the contiguous inverse pass
+  // loads coalesced ROWS and re-owns through LDS
+  const bool cols_load = INV && COLS;
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    v[m] = src[int64_t(cols_load ? 16 * r + m : r + 16 * m) << b];
+  if (pre) {
+    const int64_t cst = pre[p];
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      v[m] = load_product<A>(v[m], cst, s.q, s.k);
+  }
+  if (INV && !COLS)
+    r16_exchange(v, lds, c, r, true);
+
+  if (!INV) {
+    r16_forward_high<A>(v, s);
+    r16_exchange(v, lds, c, r, true);
+    r16_forward_low<A>(v, s);
+    if (last)
+      r16_forward_settle<A>(v, s);
+    if (!COLS)
+      r16_exchange(v, lds, c, r, false); // coalesced contiguous store
+  } else {
+    r16_inverse_low<A>(v, s);
+    r16_exchange(v, lds, c, r, false);
+    r16_inverse_high<A>(v, s);
+    if (post != kInttRaw) {
+      const int64_t ninv = t.ninv[p];
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        // only the exact arithmetic carries negative values
+        const int64_t y = mont_exit(
+            A == kExact ? mont_mult_cs(ninv, v[m], s.q, s.k)
+                        : mont_mult_u(v[m], ninv, s.q, s.k),
+            s.q, s.k);
+        v[m] = post == kInttExitReduce ? guard(y, int64_t(s.q)) : y;
+      }
+    }
+  }
+  // store: ROWS ownership except the strided forward pass (COLS)
+  const bool cols_store = !INV && COLS;
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    dst[int64_t(cols_store ? 16 * r + m : r + 16 * m) << b] = v[m];
+}
+
 } // namespace fhe

@@ -384,6 +384,71 @@ int fhe_rotate_batch(fhe_context *ctx, int count, int moduli_id,
                      const int64_t *const *keys, const int *key_moduli_ids,
                      const int *deltas, int64_t *const *outs);
 
+/* Batches at uneven levels (N = 2^16).  The reference switches keys one
+ * ciphertext at a time; these entry points take `count` ciphertexts, each at
+ * its own moduli_id (and in its own device buffer), and run every stage of
+ * the key switch as one launch over the whole batch: persistent kernels walk
+ * per-shape work lists built on the first call of a shape and kept on the
+ * device.  Every output equals the single-ciphertext call on that input.
+ *
+ * fhe_relinearize_uneven: CKKSCore::relinearize (src/ckks/core.cpp:713-741)
+ * of NTT-form trios trios[c] [3][limbs(moduli_ids[c])][N] into INTT-form duos
+ * outs[c] [2][limbs][N] (outs[c] must not alias trios[c]). */
+int fhe_relinearize_uneven(fhe_context *ctx, int count,
+                           const int *moduli_ids, const int64_t *const *trios,
+                           const int64_t *key, int key_moduli_id,
+                           int64_t *const *outs);
+
+/* CKKSEngine::rotate(ct, FixedRotationKey*) (src/ckks/engine_rotate.cpp:
+ * 55-76, CKKSCore::rotate, src/ckks/core.cpp:938-962) of INTT-form duos
+ * ins[c] [2][limbs(moduli_ids[c])][N] at uneven levels with one fixed
+ * rotation key made for `delta`; outs[c] may alias ins[c]. */
+int fhe_rotate_uneven(fhe_context *ctx, int count, const int *moduli_ids,
+                      const int64_t *const *ins, const int64_t *key,
+                      int key_moduli_id, int delta, int64_t *const *outs);
+
+/* CKKSEngine::rotate(ct, RotationKey*, delta) (src/ckks/engine_rotate.cpp:
+ * 142-170) of INTT-form duos at uneven levels, ciphertext c by its own
+ * deltas[c]: the RotationKey's fixed keys (keys[k] at key_moduli_ids[k],
+ * made for key_deltas[k], in the key's order) rotate, key by key, the
+ * ciphertexts whose normalized delta has that key's bit, all of them in one
+ * launch sequence per key.  outs[c] may alias ins[c]; delta 0 copies. */
+int fhe_rotate_rotation_key_uneven(fhe_context *ctx, int count,
+                                   const int *moduli_ids,
+                                   const int64_t *const *ins, int key_count,
+                                   const int64_t *const *keys,
+                                   const int *key_moduli_ids,
+                                   const int *key_deltas, const int *deltas,
+                                   int64_t *const *outs);
+
+/* Launch schedules of the uneven-level kernels (no counterpart in the
+ * reference).  With autotuning on, the first call of a batch shape times
+ * each stage's schedule candidates - persistent grid caps of 1024, 2048 and
+ * 4096 blocks or one block per work unit, with and without one contiguous
+ * range of the work list per XCD - by running the whole key switch (one
+ * pass of coordinate descent) and keeps the fastest; off, the default is
+ * one block per unit with per-XCD ranges.  fhe_uneven_schedule reports the
+ * schedule of a shape that has run (relin: the shape of
+ * fhe_relinearize_uneven, else of the rotations): caps[s], xcds[s] for the
+ * 8 stages (trio inverse pass 1 and 2, digits, extension, key product,
+ * accumulator inverse pass 2, ModDown, automorphism). */
+int fhe_context_uneven_autotune(fhe_context *ctx, int on);
+int fhe_uneven_schedule(fhe_context *ctx, int relin, int count,
+                        const int *moduli_ids, int key_moduli_id, int *caps,
+                        int *xcds, int capacity);
+
+/* HIP graph capture of a context's stream (no counterpart in the
+ * reference): every call between fhe_graph_begin and fhe_graph_end is
+ * recorded instead of run, and fhe_graph_launch replays the recording on the
+ * context's stream with the same device buffers.  While capturing, calls
+ * that would allocate, upload constants, grow a workspace or synchronize
+ * fail with FHE_ERR_INVALID: run each call once before capturing it. */
+typedef struct fhe_graph fhe_graph;
+int fhe_graph_begin(fhe_context *ctx);
+int fhe_graph_end(fhe_context *ctx, fhe_graph **graph);
+int fhe_graph_launch(fhe_context *ctx, fhe_graph *graph);
+void fhe_graph_destroy(fhe_graph *graph);
+
 /* Key switching of `count` INTT-form duos to the secret of a
  * key-switching key ([key_polys][key_limbs][N] at key_moduli_id): out =
  * (c0 + KS(c1)_0, KS(c1)_1) (CKKSCore::switch_key, src/ckks/core.cpp:

@@ -904,6 +904,16 @@ void *ref_rotation_key_fixed(void *rk, int index) {
   return static_cast<RotationKey *>(rk)->fixed_rotation_keys_.at(index);
 }
 
+// CKKSEngine::rotate(ct, RotationKey, delta) (src/ckks/engine_rotate.cpp:
+// 103-130): the fixed keys whose bit the normalized delta has, in order.
+void *ref_rotate_rotation_key(void *handle, void *ct, void *rk, int delta) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->rotate(
+        static_cast<UnitCiphertext *>(ct), static_cast<RotationKey *>(rk),
+        delta);
+  });
+}
+
 // CKKSEngine::coeff_to_slot / slot_to_coeff / bootstrap with a RotationKey
 // (src/ckks/engine_bootstrap.cpp:330-454, 712-754).
 int ref_coeff_to_slot(void *handle, void *ct, void *conj, void *rk,

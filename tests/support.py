@@ -395,6 +395,7 @@ class Reference:
             "ref_free_rotation_key": (None, [VP]),
             "ref_rotation_key_count": (i32, [VP]),
             "ref_rotation_key_fixed": (VP, [VP, i32]),
+            "ref_rotate_rotation_key": (VP, [VP, VP, VP, i32]),
             "ref_coeff_to_slot": (i32, [VP, VP, VP, VP, i32, VP, VP]),
             "ref_slot_to_coeff": (VP, [VP, VP, VP, VP, VP, i32]),
             "ref_bootstrap": (VP, [VP, VP, VP, VP, VP, VP, i32]),
