@@ -452,6 +452,125 @@ def uneven_bench(ev, steps, count=32, autotune=True):
     }
 
 
+POLY_COEFFS = [0.25, -1.5, 0.75, 0.5, -0.125, 0.0625, 0.3, -0.2]
+
+
+def serve_bench(rate=200.0, seconds=5.0, max_batch=16, seed=23):
+    """The serving loop (fhe_serve) under Poisson arrivals at `rate`
+    requests/s for `seconds`: each job is an encrypted dot product of two
+    2^(logN-1)-slot vectors (multiply + relinearize + sum_1d_by_row) or a
+    degree-7 polynomial evaluation (depth 3), on N = 2^15 or 2^16 (alpha = 4),
+    starting level uniform in 8..top; keys and input ciphertexts resident on
+    the device (synthetic residues of the real shapes).  Reports completed
+    jobs/s and the p50 / p99 of arrival-to-completion latency, beside each
+    job type's latency alone."""
+    import fhe_amd
+    rng = np.random.Generator(np.random.PCG64(seed))
+    engines = []
+    keep = []
+    for log_n in (15, 16):
+        ctx = fhe_amd.Context(log_n, 4)
+        n = ctx.n
+        key_mid = ctx.key_moduli_id()
+        kp, kc, _ = ctx.key_shape(key_mid)
+
+        def key():
+            d = ctx.array((kp, kc, n))
+            for p in range(kp):
+                d.upload_at(uniform_limbs(rng, ctx.primes, key_mid, kc, n, 2),
+                            p * kc * n)
+            return d
+
+        rlk = key()
+        rot = [key() for _ in range(log_n - 1)]
+        rk = fhe_amd.RotationKey.make(rot, [key_mid] * len(rot),
+                                      [1 << b for b in range(log_n - 1)])
+        coeffs = np.array(POLY_COEFFS)
+        top = ctx.info["full_level"]
+        inputs = {}
+        for level in range(8, top + 1):
+            mid = top - level
+            inputs[level] = ctx.to_device(np.stack([uniform_limbs(
+                rng, ctx.primes, mid, ctx.limbs_at(mid), n)
+                for _ in range(2)]))
+        outs = [ctx.array((2, ctx.limbs_at(0), n)) for _ in range(64)]
+        e = fhe_amd.ServeEngine()
+        e.ctx, e.rlk, e.rlk_moduli_id = ctx.handle, rlk.ptr.value, key_mid
+        e.rotation = rk
+        e.coefficients, e.coefficient_count = coeffs.ctypes.data, len(coeffs)
+        engines.append(dict(ctx=ctx, engine=e, top=top, inputs=inputs,
+                            outs=outs))
+        keep += [rlk, rk, coeffs] + rot
+    count = int(rate * seconds)
+    arrivals = np.cumsum(rng.exponential(1e3 / rate, count))
+    jobs = (fhe_amd.Job * count)()
+    kinds = rng.integers(0, 2, count)
+    which = rng.integers(0, 2, count)
+    for j in range(count):
+        g = engines[int(which[j])]
+        level = int(rng.integers(8, g["top"] + 1))
+        jobs[j].engine, jobs[j].kind, jobs[j].level = int(which[j]), \
+            int(kinds[j]), level
+        jobs[j].a = jobs[j].b = g["inputs"][level].ptr.value
+        jobs[j].out = g["outs"][j % len(g["outs"])].ptr.value
+        jobs[j].arrival_ms = float(arrivals[j])
+
+    def alone(kind, g):
+        one = (fhe_amd.Job * 1)()
+        level = engines[g]["top"]
+        one[0].engine, one[0].kind, one[0].level = g, kind, level
+        one[0].a = one[0].b = engines[g]["inputs"][level].ptr.value
+        one[0].out = engines[g]["outs"][0].ptr.value
+        fhe_amd.serve([x["engine"] for x in engines], one)
+        fhe_amd.serve([x["engine"] for x in engines], one)
+        return round(one[0].done_ms - one[0].arrival_ms, 2)
+
+    # warm: plans, pools and workspaces of every shape
+    alone_ms = {f"{k}_n{15 + g}": alone(kk, g)
+                for g in (0, 1) for k, kk in (("dot", 0), ("poly", 1))}
+    warm = (fhe_amd.Job * min(count, 64))()
+    ctypes.memmove(warm, jobs, ctypes.sizeof(warm))
+    for j in range(len(warm)):
+        warm[j].arrival_ms = 0.0
+    fhe_amd.serve([x["engine"] for x in engines], warm, max_batch=max_batch)
+    stats = fhe_amd.serve([x["engine"] for x in engines], jobs,
+                          max_batch=max_batch)
+    for g in engines:
+        g["ctx"].synchronize()
+    lat = np.array([jobs[j].done_ms - jobs[j].arrival_ms
+                    for j in range(count)])
+    done = max(jobs[j].done_ms for j in range(count))
+    for g in engines:
+        for arr in list(g["inputs"].values()) + g["outs"]:
+            arr.free()
+    for k in keep:
+        if hasattr(k, "free"):
+            k.free()
+    for g in engines:
+        g["ctx"].close()
+    return {
+        "workload": f"Poisson arrivals at {rate:g} requests/s for "
+                    f"{seconds:g} s ({count} jobs): encrypted dot products "
+                    "(multiply + relinearize + sum_1d_by_row) and degree-7 "
+                    "polynomial evaluations, N = 2^15 or 2^16 (alpha 4), "
+                    "starting levels uniform in 8..top; keys and inputs "
+                    "resident on the device",
+        "jobs": count,
+        "throughput_jobs_per_s": round(count / (done / 1e3), 1),
+        "offered_jobs_per_s": rate,
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+        "latency_ms_max": round(float(lat.max()), 2),
+        "alone_latency_ms": alone_ms,
+        "batches": stats.batches,
+        "mean_batch_jobs": round(count / max(1, stats.batches), 2),
+        "max_batch_jobs": stats.max_batch_jobs,
+        "deferred_admissions": stats.deferred_admissions,
+        "min_free_hbm_gb": round(stats.min_free_bytes / 1e9, 1),
+        "max_batch": max_batch,
+    }
+
+
 # The unit of work's algorithmic cost (Gold, N = 2^16, l = 25 -> 24,
 # alpha = 4): HBM bytes of one HMult+relin+rescale when B ciphertexts share
 # one key stream, 8N [4l + 2(l - 1) + 2 ceil(l / alpha)(l + alpha) / B] (read
@@ -578,6 +697,10 @@ def main():
     ap.add_argument("--only-uneven", action="store_true",
                     help="only run the uneven-level key-switching "
                          "measurement and print its JSON object")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving-loop measurement")
+    ap.add_argument("--only-serve", action="store_true",
+                    help="only run the serving-loop measurement")
     ap.add_argument("--only-bootstrap", action="store_true",
                     help="only run the bootstrap measurement (e.g. under a "
                          "profiler) and print its JSON object")
@@ -602,6 +725,9 @@ def main():
         return
     if args.only_uneven:
         print(json.dumps(uneven_bench(ev, 10)))
+        return
+    if args.only_serve:
+        print(json.dumps(serve_bench()))
         return
     with fhe_amd.Context(LOG_N, ALPHA) as ctx:
         limbs = ctx.limbs_at(MODULI_ID)
@@ -642,6 +768,7 @@ def main():
 
     boot = None if args.no_bootstrap else bootstrap_bench(ev, 3)
     uneven = None if args.no_uneven else uneven_bench(ev, 10)
+    served = None if args.no_serve else serve_bench()
     value = args.batch / (step_ms / 1e3)
     shape = dict(n=n, count=args.batch, limbs=limbs, ext=limbs + ALPHA,
                  parts=parts)
@@ -673,6 +800,7 @@ def main():
         "encode_decode": codec,
         "bootstrap": boot,
         "uneven_key_switch": uneven,
+        "serving": served,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()

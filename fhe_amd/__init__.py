@@ -40,7 +40,8 @@ EXPORTED_SYMBOLS = (
     "fhe_relinearize_uneven", "fhe_rotate_uneven",
     "fhe_rotate_rotation_key_uneven", "fhe_context_uneven_autotune",
     "fhe_uneven_schedule", "fhe_graph_begin", "fhe_graph_end",
-    "fhe_graph_launch", "fhe_graph_destroy",
+    "fhe_graph_launch", "fhe_graph_destroy", "fhe_evaluate_polynomial",
+    "fhe_sum_1d_by_row", "fhe_serve",
 )
 
 # fhe_wire_* record kinds (FHE_WIRE_* in include/fhe_amd.h)
@@ -131,6 +132,74 @@ class BootstrapKeys(ctypes.Structure):
         return k
 
 
+class RotationKey(ctypes.Structure):
+    """fhe_rotation_key: a RotationKey's fixed keys on the device."""
+    _fields_ = [("count", ctypes.c_int),
+                ("keys", ctypes.POINTER(ctypes.c_void_p)),
+                ("moduli_ids", ctypes.POINTER(ctypes.c_int)),
+                ("deltas", ctypes.POINTER(ctypes.c_int))]
+
+    @classmethod
+    def make(cls, keys, moduli_ids, deltas):
+        """keys: device arrays (owned by the caller) in the key's order."""
+        m = len(keys)
+        k = cls()
+        k._keep = ((ctypes.c_void_p * m)(*[_addr(x) for x in keys]),
+                   (ctypes.c_int * m)(*moduli_ids),
+                   (ctypes.c_int * m)(*deltas))
+        k.count = m
+        k.keys = ctypes.cast(k._keep[0], ctypes.POINTER(ctypes.c_void_p))
+        k.moduli_ids = ctypes.cast(k._keep[1], ctypes.POINTER(ctypes.c_int))
+        k.deltas = ctypes.cast(k._keep[2], ctypes.POINTER(ctypes.c_int))
+        return k
+
+
+JOB_DOT, JOB_POLY = 0, 1   # FHE_JOB_* in include/fhe_amd.h
+
+
+class Job(ctypes.Structure):
+    """fhe_job: one request of the serving loop."""
+    _fields_ = [("engine", ctypes.c_int), ("kind", ctypes.c_int),
+                ("level", ctypes.c_int), ("a", ctypes.c_void_p),
+                ("b", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("arrival_ms", ctypes.c_double),
+                ("admitted_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double), ("out_level", ctypes.c_int),
+                ("out_flags", ctypes.c_int)]
+
+
+class ServeEngine(ctypes.Structure):
+    """fhe_serve_engine: a context with its resident keys."""
+    _fields_ = [("ctx", ctypes.c_void_p), ("rlk", ctypes.c_void_p),
+                ("rlk_moduli_id", ctypes.c_int), ("rotation", RotationKey),
+                ("coefficients", ctypes.c_void_p),
+                ("coefficient_count", ctypes.c_int)]
+
+
+class ServeOptions(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_int), ("max_in_flight", ctypes.c_int),
+                ("reserve_bytes", ctypes.c_size_t)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("wall_ms", ctypes.c_double), ("batches", ctypes.c_int),
+                ("max_batch_jobs", ctypes.c_int),
+                ("deferred_admissions", ctypes.c_int),
+                ("min_free_bytes", ctypes.c_size_t)]
+
+
+def serve(engines, jobs, max_batch=16, max_in_flight=2,
+          reserve_bytes=8 << 30):
+    """fhe_serve over `engines` (ServeEngine) and `jobs` (a ctypes array of
+    Job, updated in place); returns the ServeStats."""
+    e_arr = (ServeEngine * len(engines))(*engines)
+    opts = ServeOptions(max_batch, max_in_flight, reserve_bytes)
+    stats = ServeStats()
+    _check(library().fhe_serve(e_arr, len(engines), jobs, len(jobs),
+                               ctypes.byref(opts), ctypes.byref(stats)))
+    return stats
+
+
 def library():
     """Load libfhe_amd.so once; raise if it is not built."""
     global _lib
@@ -191,6 +260,13 @@ def library():
         lib.fhe_context_uneven_autotune.argtypes = [vp, i32]
         lib.fhe_uneven_schedule.argtypes = [vp, i32, i32, vp, i32, vp, vp,
                                             i32]
+        ip = ctypes.POINTER(i32)
+        lib.fhe_evaluate_polynomial.argtypes = [vp, i32, vp, i32, vp, i32, vp,
+                                                i32, vp, ip, ip]
+        lib.fhe_sum_1d_by_row.argtypes = [vp, i32, vp, i32, i32,
+                                          ctypes.POINTER(RotationKey), vp,
+                                          ip, ip]
+        lib.fhe_serve.argtypes = [vp, i32, vp, i32, vp, vp]
         lib.fhe_graph_begin.argtypes = [vp]
         lib.fhe_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
         lib.fhe_graph_launch.argtypes = [vp, vp]
@@ -642,6 +718,28 @@ class Context:
             self.handle, 1 if relin else 0, len(moduli_ids),
             _ints(moduli_ids), key_moduli_id, caps, xcds, 8))
         return list(zip(caps, xcds))
+
+    def evaluate_polynomial(self, cts, level, coefficients, rlk, rlk_mid,
+                            out):
+        """fhe_evaluate_polynomial of INTT-form duos [count][2][limbs][N] at
+        `level`; returns the result's (level, flags)."""
+        count = int(np.prod(cts.shape[:-3]))
+        c = np.ascontiguousarray(coefficients, dtype=np.float64)
+        lv, fl = ctypes.c_int(), ctypes.c_int()
+        _check(library().fhe_evaluate_polynomial(
+            self.handle, count, cts.ptr, level, c.ctypes.data, len(c),
+            rlk.ptr, rlk_mid, out.ptr, ctypes.byref(lv), ctypes.byref(fl)))
+        return lv.value, fl.value
+
+    def sum_1d_by_row(self, cts, level, flags, rotation_key, out):
+        """fhe_sum_1d_by_row with a RotationKey; returns (level, flags)."""
+        count = int(np.prod(cts.shape[:-3]))
+        lv, fl = ctypes.c_int(), ctypes.c_int()
+        _check(library().fhe_sum_1d_by_row(
+            self.handle, count, cts.ptr, level, flags,
+            ctypes.byref(rotation_key), out.ptr, ctypes.byref(lv),
+            ctypes.byref(fl)))
+        return lv.value, fl.value
 
     def graph_begin(self):
         _check(library().fhe_graph_begin(self.handle))

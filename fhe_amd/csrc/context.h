@@ -137,8 +137,10 @@ struct fhe_context {
   hipEvent_t fork_event = nullptr, join_event = nullptr;
   // small-prime key products accumulated without guards, reduced once
   bool ks_lazy_accumulate = true;
-  // small-prime key products summed unreduced (96 bits), reduced once
-  bool ks_wide_accumulate = true;
+  // small-prime key products summed unreduced (96 bits), reduced once:
+  // about 10% fewer VALU instructions in the key inner product, but it
+  // issues them at a lower rate (measured slower), so off by default
+  bool ks_wide_accumulate = false;
   // key inner product loads the next partition's tile ahead
   bool ks_prefetch = false;
   // waves per SIMD targeted by the key inner product's register allocation
@@ -649,16 +651,19 @@ struct fhe_context {
   // signed_input: the source limbs may be any int64 (the signed tile of
   // modulus raising), not residues in [0, 2q): every partition then takes
   // the signed Garner digits and the exact extension.
+  // digits (optional): room for the staged ModUp's Garner digits
+  // (ks_digits_words) instead of the context's second growable buffer.
   void key_switch(int count, int moduli_id, const int64_t *key,
                   int key_moduli_id, const int64_t *src, int64_t src_stride,
                   int64_t *ext, int64_t *acc, const int64_t *own = nullptr,
-                  int64_t own_stride = 0, bool signed_input = false) {
+                  int64_t own_stride = 0, bool signed_input = false,
+                  int64_t *digits = nullptr) {
     if (fhe::ks_fused_supported(tables)) {
       if (!own_limbs || int(params.partitions(moduli_id, key_moduli_id)
                                 .size()) > fhe::kMaxFusedParts)
         own = nullptr;
       ks_decompose(count, moduli_id, key_moduli_id, src, src_stride, ext,
-                   own != nullptr, signed_input);
+                   own != nullptr, signed_input, digits);
       ks_apply(count, moduli_id, key, key_moduli_id, ext, acc, own,
                own_stride);
       return;
@@ -700,9 +705,20 @@ struct fhe_context {
   // The extended polynomials, their products with the key and the
   // accumulators are only read by ModDown after the canonicalizing exit,
   // so with lazy_key_switch their arithmetic is lazy.
+  // Words of Garner digits the staged ModUp of `count` ciphertexts at
+  // moduli_id holds (partitions x partition size rounded to a power of 2).
+  int64_t ks_digits_words(int count, int moduli_id) const {
+    int maxz = 1;
+    while (maxz < params.alpha)
+      maxz <<= 1;
+    return int64_t(count) * params.partition_count(moduli_id) * maxz *
+           params.n;
+  }
+
   void ks_decompose(int count, int moduli_id, int key_moduli_id,
                     const int64_t *src, int64_t src_stride, int64_t *ext,
-                    bool skip_own = false, bool signed_input = false) {
+                    bool skip_own = false, bool signed_input = false,
+                    int64_t *digits_ws = nullptr) {
     const int limbs = params.limbs_at(moduli_id);
     const int ext_limbs = limbs + params.alpha;
     const int64_t N = n();
@@ -728,7 +744,8 @@ struct fhe_context {
       while (maxz < max_size)
         maxz <<= 1;
       int64_t *digits =
-          aux_scratch(size_t(count) * np * maxz * size_t(N) * 8);
+          digits_ws ? digits_ws
+                    : aux_scratch(size_t(count) * np * maxz * size_t(N) * 8);
       stage("modup_staged(ext)", [&] {
         fhe::launch_staged_modup(tables, src, src_stride, digits, ext, count,
                                  limbs, moduli_id, starts.data(), sizes.data(),
@@ -800,3 +817,12 @@ struct fhe_context {
   }
 };
 
+// The batched HMult + relinearize + rescale pipeline (context.hip) on a
+// caller-given workspace of hmult_workspace_words words and, when given, a
+// digits buffer of ks_digits_words words.
+int64_t hmult_workspace_words(const fhe_context *ctx, int count,
+                              int moduli_id);
+void hmult_relin_rescale_into(fhe_context *ctx, int count, int moduli_id,
+                              const int64_t *a, const int64_t *b,
+                              const int64_t *key, int key_moduli_id,
+                              int64_t *out, int64_t *ws, int64_t *digits);

@@ -288,15 +288,28 @@ int fhe_relinearize(fhe_context *ctx, int count, int moduli_id,
   });
 }
 
-int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
-                            const int64_t *a, const int64_t *b,
-                            const int64_t *key, int key_moduli_id,
-                            int64_t *out) {
-  return guarded([&] {
-    require(ctx && a && b && key && out, "null argument");
-    require(count > 0, "Multiply: empty batch");
-    ctx->check_moduli(moduli_id, 3, "Multiply");
-    ctx->check_key(moduli_id, key_moduli_id, "Multiply");
+} // extern "C"
+
+int64_t hmult_workspace_words(const fhe_context *ctx, int count,
+                              int moduli_id) {
+  const auto &P = ctx->params;
+  const int limbs = P.limbs_at(moduli_id);
+  const int ext_limbs = limbs + P.alpha;
+  const int64_t duo_l = int64_t(count) * 2 * limbs;
+  const int64_t trio_l = int64_t(count) * 3 * limbs;
+  const int64_t ext_l = int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
+  const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
+  return (2 * duo_l + 2 * trio_l + ext_l + acc_l) * int64_t(P.n);
+}
+
+// The batched HMult + relinearize + rescale pipeline on the workspace ws
+// (hmult_workspace_words) and, when given, the digits buffer (the staged
+// ModUp's Garner digits; else the context's second buffer).  Arguments are
+// checked by the caller.
+void hmult_relin_rescale_into(fhe_context *ctx, int count, int moduli_id,
+                              const int64_t *a, const int64_t *b,
+                              const int64_t *key, int key_moduli_id,
+                              int64_t *out, int64_t *ws, int64_t *digits) {
     const auto &P = ctx->params;
     const int limbs = P.limbs_at(moduli_id);
     const int ext_limbs = limbs + P.alpha;
@@ -305,11 +318,8 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
     const int64_t trio_l = int64_t(count) * 3 * limbs;
     const int64_t ext_l =
         int64_t(count) * ctx->ext_slots(moduli_id) * ext_limbs;
-    const int64_t acc_l = int64_t(count) * 2 * ext_limbs;
     // trio_c: the coefficient form of the trio; trio keeps the NTT form,
     // whose d2 limbs serve the partitions' own extended limbs
-    int64_t *ws = ctx->scratch(
-        size_t(2 * duo_l + 2 * trio_l + ext_l + acc_l) * N * 8);
     int64_t *an = ws, *bn = an + duo_l * N, *trio = bn + duo_l * N;
     int64_t *trio_c = trio + trio_l * N;
     int64_t *ext = trio_c + trio_l * N, *acc = ext + ext_l * N;
@@ -395,7 +405,7 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
     ctx->key_switch(count, moduli_id, key, key_moduli_id, coef + 2 * limbs * N,
                     3 * limbs * N, ext, acc,
                     fused_trio ? nullptr : trio + 2 * limbs * N,
-                    3 * limbs * N);
+                    3 * limbs * N, false, digits);
     if (overlap) {
       joined = true;
       hip_check(hipStreamWaitEvent(ctx->stream, ctx->join_event, 0), "event");
@@ -404,6 +414,23 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
       fhe::launch_moddown_relin_rescale(ctx->tables, acc, coef, out, count,
                                         limbs, moduli_id, ctx->stream);
     });
+}
+
+extern "C" {
+
+int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
+                            const int64_t *a, const int64_t *b,
+                            const int64_t *key, int key_moduli_id,
+                            int64_t *out) {
+  return guarded([&] {
+    require(ctx && a && b && key && out, "null argument");
+    require(count > 0, "Multiply: empty batch");
+    ctx->check_moduli(moduli_id, 3, "Multiply");
+    ctx->check_key(moduli_id, key_moduli_id, "Multiply");
+    int64_t *ws = ctx->scratch(
+        size_t(hmult_workspace_words(ctx, count, moduli_id)) * 8);
+    hmult_relin_rescale_into(ctx, count, moduli_id, a, b, key, key_moduli_id,
+                             out, ws, nullptr);
     ctx->check_launch("Multiply");
   });
 }

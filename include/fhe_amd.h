@@ -384,6 +384,92 @@ int fhe_rotate_batch(fhe_context *ctx, int count, int moduli_id,
                      const int64_t *const *keys, const int *key_moduli_ids,
                      const int *deltas, int64_t *const *outs);
 
+/* A RotationKey on the device (src/data/data.h:163-174): its fixed rotation
+ * keys in the key's order, each at its moduli_id, made for its delta. */
+typedef struct fhe_rotation_key {
+  int count;
+  const int64_t *const *keys;
+  const int *moduli_ids;
+  const int *deltas;
+} fhe_rotation_key;
+
+/* CKKSEngine::evaluate_polynomial(ct, coefficients, rlk) with double
+ * coefficients (src/ckks/engine_advanced.cpp:365-387): the power basis
+ * x, x^2, ... (squares, then products of a power of two and the rest,
+ * :22-57) and the weighted sum of the basis with the coefficients
+ * (:228-269), for `count` INTT-form, not rescaled duos in [count][2][limbs][N]
+ * at `level`.  out: room for the input's size, the results packed; their
+ * level and flags come back.  The reference's requirements and messages. */
+int fhe_evaluate_polynomial(fhe_context *ctx, int count, const int64_t *in,
+                            int level, const double *coefficients,
+                            int coefficient_count, const int64_t *rlk,
+                            int rlk_moduli_id, int64_t *out, int *out_level,
+                            int *out_flags);
+
+/* CKKSEngine::sum_1d_by_row(ct, RotationKey) (src/ckks/engine_advanced.cpp:
+ * 549-577) of `count` INTT-form, not rescaled duos at `level`: rotate by 1,
+ * 2, 4, ... below the slot count and add, so every slot holds the sum of
+ * all slots.  out as fhe_evaluate_polynomial. */
+int fhe_sum_1d_by_row(fhe_context *ctx, int count, const int64_t *in,
+                      int level, int flags, const fhe_rotation_key *rk,
+                      int64_t *out, int *out_level, int *out_flags);
+
+/* Serving (no counterpart in the reference, which runs one call at a time).
+ * fhe_serve runs a request loop on the calling thread until every job is
+ * done: job j arrives jobs[j].arrival_ms after the call starts and waits in
+ * a queue per (engine, kind, level); whenever an engine has fewer than
+ * max_in_flight batches on the device, its queue with the earliest waiting
+ * job gives up to max_batch jobs to one batch (continuous batching: jobs
+ * that arrive while others run join the next batch).  A batch is admitted
+ * when the free HBM minus its estimated working set stays above
+ * reserve_bytes, else it waits for a running batch to finish.  Every engine
+ * (a context with its keys, all resident on the device) has its own
+ * stream, so batches of different engines overlap on the device.  Job
+ * inputs and outputs are device buffers; each job's output limbs equal the
+ * composition run alone (fhe_multiply + fhe_sum_1d_by_row, or
+ * fhe_evaluate_polynomial).  done_ms is when the host saw the job's batch
+ * finish. */
+enum {
+  FHE_JOB_DOT = 0, /* multiply(a, b, rlk) then sum_1d_by_row: the encrypted
+                      dot product of two slot vectors */
+  FHE_JOB_POLY = 1 /* evaluate_polynomial(a) with the engine's coefficients */
+};
+typedef struct fhe_job {
+  int engine;           /* index of the engine */
+  int kind;             /* FHE_JOB_* */
+  int level;            /* of the INTT-form, not rescaled inputs */
+  const int64_t *a;     /* device duo [2][limbs][N] */
+  const int64_t *b;     /* second duo (FHE_JOB_DOT) */
+  int64_t *out;         /* device, room for [2][limbs][N] */
+  double arrival_ms;
+  double admitted_ms;   /* set by fhe_serve */
+  double done_ms;       /* set by fhe_serve */
+  int out_level, out_flags; /* set by fhe_serve */
+} fhe_job;
+typedef struct fhe_serve_engine {
+  fhe_context *ctx;
+  const int64_t *rlk;
+  int rlk_moduli_id;
+  fhe_rotation_key rotation;
+  const double *coefficients; /* of the FHE_JOB_POLY jobs */
+  int coefficient_count;
+} fhe_serve_engine;
+typedef struct fhe_serve_options {
+  int max_batch;        /* jobs per batch */
+  int max_in_flight;    /* batches per engine on the device at once */
+  size_t reserve_bytes; /* free HBM kept back from admission */
+} fhe_serve_options;
+typedef struct fhe_serve_stats {
+  double wall_ms;
+  int batches;
+  int max_batch_jobs;
+  int deferred_admissions; /* admission checks that found too little HBM */
+  size_t min_free_bytes;   /* least free HBM seen at an admission */
+} fhe_serve_stats;
+int fhe_serve(const fhe_serve_engine *engines, int engine_count,
+              fhe_job *jobs, int job_count, const fhe_serve_options *options,
+              fhe_serve_stats *stats);
+
 /* Batches at uneven levels (N = 2^16).  The reference switches keys one
  * ciphertext at a time; these entry points take `count` ciphertexts, each at
  * its own moduli_id (and in its own device buffer), and run every stage of
@@ -545,8 +631,10 @@ int fhe_mod_raise(fhe_context *ctx, int count, const int64_t *in,
  *   "split_variants" 1     one NTT launch per arithmetic variant;
  *   "ks_lazy_accumulate" 1 small-prime key products summed without
  *                          intermediate reductions;
- *   "ks_wide_accumulate" 1 key products summed as 128-bit values with one
- *                          Montgomery reduction per accumulator;
+ *   "ks_wide_accumulate" 0 small primes' key products summed as unreduced
+ *                          96-bit values with one Montgomery reduction per
+ *                          accumulator (fewer VALU instructions, measured
+ *                          slower);
  *   "overlap_trio" 1       N = 2^16 with split_variants: the INTT of d0 and
  *                          d1 in fhe_hmult_relin_rescale runs on a second
  *                          non-blocking stream of the context (default

@@ -904,6 +904,35 @@ void *ref_rotation_key_fixed(void *rk, int index) {
   return static_cast<RotationKey *>(rk)->fixed_rotation_keys_.at(index);
 }
 
+// CKKSEngine::evaluate_polynomial (src/ckks/engine_advanced.cpp:365-387)
+// with double coefficients.
+void *ref_evaluate_polynomial(void *handle, void *ct, const double *coeffs,
+                              int count, void *rlk) {
+  return guarded([&]() -> void * {
+    std::vector<double> c(coeffs, coeffs + count);
+    return static_cast<RefEngine *>(handle)->engine->evaluate_polynomial(
+        static_cast<UnitCiphertext *>(ct), c,
+        static_cast<RelinearizationKey *>(rlk));
+  });
+}
+
+// CKKSEngine::sum_1d_by_row (src/ckks/engine_advanced.cpp:549-577) of a
+// Ciphertext holding a copy of one unit ciphertext; returns a copy of the
+// result's unit ciphertext.
+void *ref_sum_1d_by_row(void *handle, void *ct, void *rk) {
+  return guarded([&]() -> void * {
+    auto e = static_cast<RefEngine *>(handle)->engine;
+    auto in = new Ciphertext(std::vector<UnitCiphertext *>{
+        e->clone(static_cast<UnitCiphertext *>(ct))});
+    Ciphertext *out = e->sum_1d_by_row(in, static_cast<RotationKey *>(rk));
+    UnitCiphertext *r = e->clone(out->get(0));
+    if (out != in)
+      delete out;
+    delete in;
+    return r;
+  });
+}
+
 // CKKSEngine::rotate(ct, RotationKey, delta) (src/ckks/engine_rotate.cpp:
 // 103-130): the fixed keys whose bit the normalized delta has, in order.
 void *ref_rotate_rotation_key(void *handle, void *ct, void *rk, int delta) {

@@ -396,6 +396,8 @@ class Reference:
             "ref_rotation_key_count": (i32, [VP]),
             "ref_rotation_key_fixed": (VP, [VP, i32]),
             "ref_rotate_rotation_key": (VP, [VP, VP, VP, i32]),
+            "ref_evaluate_polynomial": (VP, [VP, VP, VP, i32, VP]),
+            "ref_sum_1d_by_row": (VP, [VP, VP, VP]),
             "ref_coeff_to_slot": (i32, [VP, VP, VP, VP, i32, VP, VP]),
             "ref_slot_to_coeff": (VP, [VP, VP, VP, VP, VP, i32]),
             "ref_bootstrap": (VP, [VP, VP, VP, VP, VP, VP, i32]),

@@ -12,13 +12,13 @@ pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"split_variants": 1, "own_limbs": 1, "overlap_trio": 1,
             "staged_modup": 1,
-            "ks_lazy_accumulate": 1, "ks_wide_accumulate": 1,
+            "ks_lazy_accumulate": 1, "ks_wide_accumulate": 0,
             "ks_prefetch": 0, "ks_occupancy": 2,
             "lazy_key_switch": 1, "fuse_tensor": 0, "fuse_modup": 0}
 OPTIONS = [{"split_variants": 0}, {"own_limbs": 0}, {"overlap_trio": 0},
            {"staged_modup": 0},
-           {"ks_lazy_accumulate": 0}, {"ks_wide_accumulate": 0},
-           {"ks_wide_accumulate": 0, "ks_lazy_accumulate": 0},
+           {"ks_lazy_accumulate": 0}, {"ks_wide_accumulate": 1},
+           {"ks_wide_accumulate": 1, "ks_lazy_accumulate": 0},
            {"ks_prefetch": 1},
            {"ks_occupancy": 3},
            {"lazy_key_switch": 0}, {"fuse_tensor": 1},
