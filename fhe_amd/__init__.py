@@ -41,7 +41,7 @@ EXPORTED_SYMBOLS = (
     "fhe_rotate_rotation_key_uneven", "fhe_context_uneven_autotune",
     "fhe_uneven_schedule", "fhe_graph_begin", "fhe_graph_end",
     "fhe_graph_launch", "fhe_graph_destroy", "fhe_evaluate_polynomial",
-    "fhe_sum_1d_by_row", "fhe_serve",
+    "fhe_sum_1d_by_row", "fhe_serve", "fhe_hmult_relin_rescale_noalloc",
 )
 
 # fhe_wire_* record kinds (FHE_WIRE_* in include/fhe_amd.h)
@@ -267,6 +267,8 @@ def library():
                                           ctypes.POINTER(RotationKey), vp,
                                           ip, ip]
         lib.fhe_serve.argtypes = [vp, i32, vp, i32, vp, vp]
+        lib.fhe_hmult_relin_rescale_noalloc.argtypes = [vp, i32, i32, vp, vp,
+                                                        vp, i32, vp]
         lib.fhe_graph_begin.argtypes = [vp]
         lib.fhe_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
         lib.fhe_graph_launch.argtypes = [vp, vp]

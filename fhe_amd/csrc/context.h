@@ -826,3 +826,6 @@ void hmult_relin_rescale_into(fhe_context *ctx, int count, int moduli_id,
                               const int64_t *a, const int64_t *b,
                               const int64_t *key, int key_moduli_id,
                               int64_t *out, int64_t *ws, int64_t *digits);
+// Set-up of the allocation-free entry point on the current device `device`
+// (noalloc.hip).
+void noalloc_scratch_setup(int device);

@@ -61,7 +61,16 @@ int fhe_context_create(const fhe_params *p, fhe_context **out) {
     require(out != nullptr, "null argument");
     *out = nullptr;
     auto params = resolve_params(p);
-    *out = new fhe_context(std::move(params));
+    std::unique_ptr<fhe_context> ctx(new fhe_context(std::move(params)));
+    // set-up of the allocation-free entry point: this device's address of
+    // its scratch array, and the ModUp constants of every ciphertext level
+    // with keys at the maximum key-switching level
+    noalloc_scratch_setup(ctx->device);
+    const auto &P = ctx->params;
+    const int key_mid = P.moduli_id_of_level(P.max_key_switching_level());
+    for (int m = std::max(key_mid, 0); m < P.chain_count; m++)
+      (void)ctx->modup_constants(m, key_mid);
+    *out = ctx.release();
   });
 }
 

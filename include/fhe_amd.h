@@ -134,6 +134,22 @@ int fhe_hmult_relin_rescale(fhe_context *ctx, int count, int moduli_id,
                             const int64_t *key, int key_moduli_id,
                             int64_t *out);
 
+/* No counterpart in the reference: the same computation as
+ * fhe_hmult_relin_rescale, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  It calls no hipMalloc / hipFree: its workspace is the
+ * library's fixed device scratch array of 512 MiB per GPU (held until the
+ * process ends), and a batch whose workspace exceeds it is refused with
+ * FHE_ERR_INVALID before any launch (a Gold ciphertext pair at level 24 needs
+ * ~282 MB).  The key must be at the maximum key-switching level (whose ModUp
+ * constants the context prepares at creation).  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (two contexts' streams,
+ * two host threads) race. */
+int fhe_hmult_relin_rescale_noalloc(fhe_context *ctx, int count,
+                                    int moduli_id, const int64_t *a,
+                                    const int64_t *b, const int64_t *key,
+                                    int key_moduli_id, int64_t *out);
+
 /* Data flags of a text, the reference's DataFlag values (src/data/flag.h:7-14).
  * A text at level L sits at moduli_id full_level - L, or one higher when
  * rescaled. */
