@@ -303,6 +303,21 @@ def bootstrap_bench(ev, steps):
                 ctx.bootstrap(dct, level, 0, keys, 3, out)
             ctx.synchronize()
             ms[batch] = (time.perf_counter() - t0) * 1e3 / steps
+            if batch == 1:
+                # the same bootstrap replayed from a HIP graph
+                ctx.graph_begin()
+                try:
+                    ctx.bootstrap(dct, level, 0, keys, 3, out)
+                finally:
+                    graph = ctx.graph_end()
+                ctx.graph_launch(graph)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    ctx.graph_launch(graph)
+                ctx.synchronize()
+                ms["graph"] = (time.perf_counter() - t0) * 1e3 / steps
+                ctx.graph_destroy(graph)
             for arr in (dct, out):
                 arr.free()
         for arr in [relin, conj, dense, dsp] + rot:
@@ -317,6 +332,7 @@ def bootstrap_bench(ev, steps):
                          "(tests/test_gpu_bootstrap.py), whose decrypted "
                          "precision that test reports",
             "ms_per_bootstrap": round(ms[1], 1),
+            "ms_per_bootstrap_graph": round(ms["graph"], 1),
             "ms_per_batch_of_8": round(ms[8], 1),
             "ms_per_bootstrap_batched": round(ms[8] / 8, 1), "steps": steps,
             "timing": "wall clock, host work included"}
@@ -540,6 +556,24 @@ def serve_bench(rate=200.0, seconds=5.0, max_batch=16, seed=23):
     lat = np.array([jobs[j].done_ms - jobs[j].arrival_ms
                     for j in range(count)])
     done = max(jobs[j].done_ms for j in range(count))
+    # saturation: the same job mix offered ten times faster, so that jobs
+    # queue and batches fill
+    sat_rate = rate * 10
+    sat_count = min(count * 2, int(sat_rate * 1.0))
+    sat = (fhe_amd.Job * sat_count)()
+    for j in range(sat_count):
+        ctypes.memmove(ctypes.byref(sat, j * ctypes.sizeof(fhe_amd.Job)),
+                       ctypes.byref(jobs, (j % count) *
+                                    ctypes.sizeof(fhe_amd.Job)),
+                       ctypes.sizeof(fhe_amd.Job))
+        sat[j].arrival_ms = float(j * 1e3 / sat_rate)
+    sat_stats = fhe_amd.serve([x["engine"] for x in engines], sat,
+                              max_batch=max_batch)
+    for g in engines:
+        g["ctx"].synchronize()
+    sat_lat = np.array([sat[j].done_ms - sat[j].arrival_ms
+                        for j in range(sat_count)])
+    sat_done = max(sat[j].done_ms for j in range(sat_count))
     for g in engines:
         for arr in list(g["inputs"].values()) + g["outs"]:
             arr.free()
@@ -568,6 +602,16 @@ def serve_bench(rate=200.0, seconds=5.0, max_batch=16, seed=23):
         "deferred_admissions": stats.deferred_admissions,
         "min_free_hbm_gb": round(stats.min_free_bytes / 1e9, 1),
         "max_batch": max_batch,
+        "saturation": {
+            "offered_jobs_per_s": sat_rate,
+            "jobs": sat_count,
+            "throughput_jobs_per_s": round(sat_count / (sat_done / 1e3), 1),
+            "latency_ms_p50": round(float(np.percentile(sat_lat, 50)), 2),
+            "latency_ms_p99": round(float(np.percentile(sat_lat, 99)), 2),
+            "batches": sat_stats.batches,
+            "mean_batch_jobs": round(sat_count / max(1, sat_stats.batches),
+                                     2),
+        },
     }
 
 
@@ -608,23 +652,49 @@ def stage_modmuls(name, s):
 
 # Kernel name of each single-kernel stage, for the recorded HBM traffic.
 STAGE_KERNEL = {"ks_inner_product_r16": "ks_inner_product_r16_kernel"}
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles",
-                               "ks_inner_product_gold_b32_traffic.md")
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "hmult_gold_b32_traffic.md")
 
 
-def recorded_traffic(stage, batch):
-    """HBM bytes per launch of a stage's kernel from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of known
-    size), recorded for the default batch of 32; None when absent."""
-    kernel = STAGE_KERNEL.get(stage)
-    if kernel is None or batch != 32 or not os.path.exists(TRAFFIC_PROFILE):
+def recorded_figure(field, batch):
+    """A figure recorded for the default batch of 32 in the rocprofv3
+    counter summary (`<field>: <integer>` lines); None when absent."""
+    if batch != 32 or not os.path.exists(TRAFFIC_PROFILE):
         return None
-    key = f"traffic_bytes_per_launch {kernel}:"
+    key = f"{field}:"
     with open(TRAFFIC_PROFILE) as f:
         for line in f:
             if line.startswith(key):
                 return int(line[len(key):].strip())
     return None
+
+
+def recorded_traffic(stage, batch):
+    """HBM bytes per launch of a stage's kernel from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of known
+    size); None when absent."""
+    kernel = STAGE_KERNEL.get(stage)
+    if kernel is None:
+        return None
+    return recorded_figure(f"traffic_bytes_per_launch {kernel}", batch)
+
+
+def valu_issue(stage, batch, avg_ms):
+    """The kernel's VALU issue rate: its SQ_INSTS_VALU count per launch
+    (recorded) over the launch duration measured here, against the issue
+    rate of the modular-product probe (SQ_INSTS_VALU over its duration)."""
+    kernel = STAGE_KERNEL.get(stage)
+    if kernel is None:
+        return None
+    insts = recorded_figure(f"valu_insts_per_launch {kernel}", batch)
+    ceiling = recorded_figure("valu_issue_ceiling_per_s", batch)
+    if insts is None or ceiling is None:
+        return None
+    rate = insts / (avg_ms / 1e3)
+    return {"unit": "VALU wave-instructions/s",
+            "insts_per_launch": insts,
+            "achieved": round(rate, 0),
+            "ceiling": ceiling,
+            "frac": round(rate / ceiling, 4)}
 
 
 def stage_report(stages, step_ms, prof_steps, shape, peaks):
@@ -669,10 +739,13 @@ def stage_report(stages, step_ms, prof_steps, shape, peaks):
         "step_achieved": round(step_gbs, 1),
         "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "valu": valu,
+        "valu_issue": valu_issue(name, batch, avg),
         "binding": "VALU: 62-bit modular products from 32-bit multiply-adds; "
                    "the HBM fractions are of the algorithmic bytes, the "
                    "product fractions of the Shoup-product rate measured "
-                   "on this device (fhe_modmul_peak)",
+                   "on this device (fhe_modmul_peak), valu_issue the "
+                   "kernel's VALU instructions (rocprofv3 SQ_INSTS_VALU) "
+                   "against the issue rate of the product probe",
     }
     return breakdown, roofline
 

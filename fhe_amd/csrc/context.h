@@ -325,10 +325,20 @@ struct fhe_context {
                             ": not allowed while the stream is captured (run "
                             "the call once before capturing it)");
   }
+  // captured graphs alive on this context: they hold the addresses of the
+  // workspaces, which may then not move
+  int live_graphs = 0;
+  void require_workspace_fixed(const char *what) const {
+    require_not_capturing(what);
+    require(live_graphs == 0,
+            std::string(what) + ": a captured graph holds the workspace "
+                                "(destroy it, or run the larger call before "
+                                "capturing)");
+  }
 
   int64_t *aux_scratch(size_t bytes) {
     if (bytes > aux_bytes) {
-      require_not_capturing("workspace growth");
+      require_workspace_fixed("workspace growth");
       hip_check(hipStreamSynchronize(stream), "workspace sync");
       if (aux)
         hip_check(hipFree(aux), "workspace free");
@@ -342,7 +352,7 @@ struct fhe_context {
 
   int64_t *stash(size_t bytes) {
     if (bytes > stash_bytes) {
-      require_not_capturing("workspace growth");
+      require_workspace_fixed("workspace growth");
       hip_check(hipStreamSynchronize(stream), "workspace sync");
       if (stash_buf)
         hip_check(hipFree(stash_buf), "workspace free");
@@ -377,7 +387,7 @@ struct fhe_context {
 
   int64_t *scratch(size_t bytes) {
     if (bytes > workspace_bytes) {
-      require_not_capturing("workspace growth");
+      require_workspace_fixed("workspace growth");
       hip_check(hipStreamSynchronize(stream), "workspace sync");
       if (workspace)
         hip_check(hipFree(workspace), "workspace free");

@@ -1170,17 +1170,11 @@ int fhe_multiply_imaginary(fhe_context *ctx, int count, int moduli_id,
     const int64_t N = ctx->n();
     const int64_t poly = int64_t(limbs) * N;
     // the plaintext value * X^(N/2) (CKKSCore::encode_imaginary,
-    // src/ckks/core_encode.cpp:190-210), tiled, entered and transformed
-    int64_t *sample = ctx->stash(size_t(N + poly) * 8);
-    int64_t *pt = sample + N;
-    std::vector<int64_t> host(size_t(N), 0);
-    host[size_t(N / 2)] = value;
-    hip_check(hipMemcpyAsync(sample, host.data(), size_t(N) * 8,
-                             hipMemcpyHostToDevice, ctx->stream),
-              "plaintext upload");
+    // src/ckks/core_encode.cpp:190-210), tiled on the device and transformed
+    int64_t *pt = ctx->stash(size_t(poly) * 8);
     ctx->stage("multiply_imaginary", [&] {
-      fhe::launch_tile_unsigned(ctx->tables, sample, pt, 1, limbs, moduli_id,
-                                ctx->stream);
+      fhe::launch_tile_monomial(ctx->tables, value, N / 2, pt, limbs,
+                                moduli_id, ctx->stream);
       fhe::ntt_forward(ctx->tables, pt, pt, limbs, limbs, moduli_id,
                        ctx->tables.r2, ctx->stream);
       // ciphertext to NTT form, product, back to INTT form
@@ -1193,8 +1187,6 @@ int fhe_multiply_imaginary(fhe_context *ctx, int count, int moduli_id,
       fhe::ntt_inverse(ctx->tables, out, out, int64_t(count) * 2 * limbs,
                        limbs, moduli_id, fhe::kInttExit, ctx->stream);
     });
-    // the host vector must outlive the asynchronous copy
-    hip_check(hipStreamSynchronize(ctx->stream), "plaintext upload");
     ctx->check_launch("Multiply imaginary");
   });
 }

@@ -29,6 +29,20 @@ __global__ __launch_bounds__(kBlock) void tile_unsigned_kernel(
                              int64_t(t.q[moduli_id + l]));
 }
 
+// out[l][i] = make_unsigned_reduce(value, q_l) at i = pos, 0 elsewhere: the
+// tiled sample of a monomial value * X^pos, written on the device
+__global__ __launch_bounds__(kBlock) void tile_monomial_kernel(
+    int64_t *out, int64_t total, int moduli_id, int64_t value, int64_t pos,
+    DeviceTables t) {
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= total)
+    return;
+  const int l = int(idx >> t.log_n);
+  const int64_t i = idx & ((int64_t(1) << t.log_n) - 1);
+  out[idx] = i == pos ? unsigned_reduce(value, int64_t(t.q[moduli_id + l]))
+                      : 0;
+}
+
 // c0 = reduce(c0 + reduce(pt + tile(e0))), c1 = reduce(c1 + tile(e1)) with
 // reduce(x) = guard(guard(x, 2q), q) (mont_add_reduce_2q_to_q).
 __global__ __launch_bounds__(kBlock) void encrypt_finish_kernel(
@@ -61,6 +75,15 @@ void launch_tile_unsigned(const DeviceTables &t, const int64_t *samples,
   hipLaunchKernelGGL(tile_unsigned_kernel, dim3(grid_for(total)),
                      dim3(kBlock), 0, stream, samples, out, total, limbs,
                      moduli_id, t);
+}
+
+void launch_tile_monomial(const DeviceTables &t, int64_t value, int64_t pos,
+                          int64_t *out, int limbs, int moduli_id,
+                          hipStream_t stream) {
+  const int64_t total = int64_t(limbs) << t.log_n;
+  hipLaunchKernelGGL(tile_monomial_kernel, dim3(grid_for(total)),
+                     dim3(kBlock), 0, stream, out, total, moduli_id, value,
+                     pos, t);
 }
 
 void launch_encrypt_finish(const DeviceTables &t, const int64_t *pt,

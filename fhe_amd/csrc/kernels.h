@@ -244,6 +244,11 @@ void launch_scalar_rescale(const DeviceTables &t, const int64_t *in,
 void launch_tile_unsigned(const DeviceTables &t, const int64_t *samples,
                           int64_t *out, int count, int limbs, int moduli_id,
                           hipStream_t stream);
+// the monomial value * X^pos tiled over `limbs` limbs from moduli_id
+// (|value| < every q)
+void launch_tile_monomial(const DeviceTables &t, int64_t value, int64_t pos,
+                          int64_t *out, int limbs, int moduli_id,
+                          hipStream_t stream);
 void launch_encrypt_finish(const DeviceTables &t, const int64_t *pt,
                            const int64_t *e0, const int64_t *e1, int64_t *out,
                            int count, int limbs, int moduli_id,

@@ -27,16 +27,17 @@ struct Text {
 class TextEngine {
 public:
   TextEngine(fhe_context *c, const int64_t *rlk, int rlk_mid)
-      : ctx(c), key(rlk), key_mid(rlk_mid) {}
+      : ctx(c), key(rlk), key_mid(rlk_mid), sync_on_exit(!c->capturing) {}
   ~TextEngine() {
     if (sync_on_exit)
       (void)hipStreamSynchronize(ctx->stream);
   }
 
   int count = 1;
-  // the destructor waits for the stream (callers that poll events instead
-  // turn it off; pooled buffers are reused in stream order either way)
-  bool sync_on_exit = true;
+  // the destructor waits for the stream (not while the stream is captured
+  // into a graph; callers that poll events instead turn it off; pooled
+  // buffers are reused in stream order either way)
+  bool sync_on_exit;
 
   fhe_context *context() const { return ctx; }
   int limbs(int level, int flags) const {
@@ -336,6 +337,7 @@ public:
   // (src/ckks/engine_encode.cpp:10-36), one batched device encode
   Text encode_messages(const std::vector<std::complex<double>> &m,
                        int messages, int level) {
+    ctx->require_not_capturing("Encode");
     const size_t bytes = m.size() * sizeof(std::complex<double>);
     const auto dv = take(bytes);
     hip_check(hipMemcpyAsync(dv.get(), m.data(), bytes, hipMemcpyHostToDevice,

@@ -475,7 +475,7 @@ void uneven_switch(fhe_context *ctx, bool relin, int count,
 struct fhe_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipStream_t stream = nullptr;
+  fhe_context *ctx = nullptr;
 };
 
 extern "C" {
@@ -610,7 +610,8 @@ int fhe_graph_end(fhe_context *ctx, fhe_graph **out) {
               "hipStreamEndCapture");
     hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0),
               "hipGraphInstantiate");
-    g->stream = ctx->stream;
+    g->ctx = ctx;
+    ctx->live_graphs++;
     *out = g.release();
   });
 }
@@ -618,6 +619,7 @@ int fhe_graph_end(fhe_context *ctx, fhe_graph **out) {
 int fhe_graph_launch(fhe_context *ctx, fhe_graph *graph) {
   return guarded([&] {
     require(ctx && graph, "null argument");
+    require(graph->ctx == ctx, "Graph: recorded on another context");
     require(!ctx->capturing, "Graph: the stream is being captured");
     hip_check(hipGraphLaunch(graph->exec, ctx->stream), "hipGraphLaunch");
   });
@@ -626,6 +628,8 @@ int fhe_graph_launch(fhe_context *ctx, fhe_graph *graph) {
 void fhe_graph_destroy(fhe_graph *graph) {
   if (!graph)
     return;
+  if (graph->ctx && graph->ctx->live_graphs > 0)
+    graph->ctx->live_graphs--;
   if (graph->exec)
     (void)hipGraphExecDestroy(graph->exec);
   if (graph->graph)

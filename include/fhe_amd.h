@@ -544,7 +544,10 @@ int fhe_uneven_schedule(fhe_context *ctx, int relin, int count,
  * recorded instead of run, and fhe_graph_launch replays the recording on the
  * context's stream with the same device buffers.  While capturing, calls
  * that would allocate, upload constants, grow a workspace or synchronize
- * fail with FHE_ERR_INVALID: run each call once before capturing it. */
+ * fail with FHE_ERR_INVALID: run each call once before capturing it.  A
+ * graph holds the addresses of the context's workspaces: while one exists,
+ * a call that would grow a workspace fails too (destroy the graph first).
+ * Destroy a context's graphs before the context. */
 typedef struct fhe_graph fhe_graph;
 int fhe_graph_begin(fhe_context *ctx);
 int fhe_graph_end(fhe_context *ctx, fhe_graph **graph);
