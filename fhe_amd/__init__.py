@@ -42,6 +42,8 @@ EXPORTED_SYMBOLS = (
     "fhe_uneven_schedule", "fhe_graph_begin", "fhe_graph_end",
     "fhe_graph_launch", "fhe_graph_destroy", "fhe_evaluate_polynomial",
     "fhe_sum_1d_by_row", "fhe_serve", "fhe_hmult_relin_rescale_noalloc",
+    "fhe_bootstrap_key_create", "fhe_bootstrap_key_destroy",
+    "fhe_coeff_to_slot_bk", "fhe_slot_to_coeff_bk", "fhe_bootstrap_bk",
 )
 
 # fhe_wire_* record kinds (FHE_WIRE_* in include/fhe_amd.h)
@@ -128,6 +130,79 @@ class BootstrapKeys(ctypes.Structure):
         k.to_sparse = to_sparse.ptr.value if to_sparse is not None else None
         k.to_sparse_moduli_id = sparse_mid
         k.to_dense = to_dense.ptr.value if to_dense is not None else None
+        k.to_dense_moduli_id = dense_mid
+        return k
+
+
+class PlainMatrix(ctypes.Structure):
+    """fhe_plain_matrix: one butterfly stage of a BootstrapKey."""
+    _fields_ = [("level", ctypes.c_int), ("inner_count", ctypes.c_int),
+                ("outer_count", ctypes.c_int),
+                ("inner_deltas", ctypes.POINTER(ctypes.c_int)),
+                ("outer_deltas", ctypes.POINTER(ctypes.c_int)),
+                ("diagonals", ctypes.POINTER(ctypes.c_void_p))]
+
+    @classmethod
+    def make(cls, level, inner_deltas, outer_deltas, diagonals):
+        """diagonals: {(outer, inner): device array or address}, every pair
+        but (0, 0)."""
+        inner, outer = len(inner_deltas), len(outer_deltas)
+        table = (ctypes.c_void_p * (inner * outer))()
+        for (o, i), d in diagonals.items():
+            table[o * inner + i] = _addr(d)
+        m = cls()
+        m._keep = (_ints(inner_deltas), _ints(outer_deltas), table)
+        m.level, m.inner_count, m.outer_count = level, inner, outer
+        m.inner_deltas = ctypes.cast(m._keep[0], ctypes.POINTER(ctypes.c_int))
+        m.outer_deltas = ctypes.cast(m._keep[1], ctypes.POINTER(ctypes.c_int))
+        m.diagonals = ctypes.cast(table, ctypes.POINTER(ctypes.c_void_p))
+        return m
+
+
+class BootstrapKeyParts(ctypes.Structure):
+    """fhe_bootstrap_key_parts: a BootstrapKey's parts on the device."""
+    _fields_ = [("stage_count", ctypes.c_int),
+                ("coeff_to_slot", ctypes.POINTER(PlainMatrix)),
+                ("slot_to_coeff", ctypes.POINTER(PlainMatrix)),
+                ("rotation_count", ctypes.c_int),
+                ("rotation_deltas", ctypes.POINTER(ctypes.c_int)),
+                ("rotation_keys", ctypes.POINTER(ctypes.c_void_p)),
+                ("rotation_moduli_ids", ctypes.POINTER(ctypes.c_int)),
+                ("to_sparse", ctypes.c_void_p),
+                ("to_sparse_moduli_id", ctypes.c_int),
+                ("to_dense", ctypes.c_void_p),
+                ("to_dense_moduli_id", ctypes.c_int)]
+
+    @classmethod
+    def make(cls, cts, stc, rotations, to_sparse=None, sparse_mid=0,
+             to_dense=None, dense_mid=0):
+        """cts, stc: lists of PlainMatrix (either may be empty); rotations:
+        [(delta, device key, moduli_id)]."""
+        k = cls()
+        stages = max(len(cts), len(stc))
+        assert not cts or not stc or len(cts) == len(stc)
+        ca = (PlainMatrix * max(1, len(cts)))(*cts)
+        sa = (PlainMatrix * max(1, len(stc)))(*stc)
+        m = len(rotations)
+        k._keep = (ca, sa, list(cts) + list(stc),
+                   _ints([r[0] for r in rotations]),
+                   _addrs([r[1] for r in rotations]),
+                   _ints([r[2] for r in rotations]))
+        k.stage_count = stages
+        k.coeff_to_slot = (ctypes.cast(ca, ctypes.POINTER(PlainMatrix))
+                           if cts else None)
+        k.slot_to_coeff = (ctypes.cast(sa, ctypes.POINTER(PlainMatrix))
+                           if stc else None)
+        k.rotation_count = m
+        k.rotation_deltas = ctypes.cast(k._keep[3],
+                                        ctypes.POINTER(ctypes.c_int))
+        k.rotation_keys = ctypes.cast(k._keep[4],
+                                      ctypes.POINTER(ctypes.c_void_p))
+        k.rotation_moduli_ids = ctypes.cast(k._keep[5],
+                                            ctypes.POINTER(ctypes.c_int))
+        k.to_sparse = _addr(to_sparse) if to_sparse is not None else None
+        k.to_sparse_moduli_id = sparse_mid
+        k.to_dense = _addr(to_dense) if to_dense is not None else None
         k.to_dense_moduli_id = dense_mid
         return k
 
@@ -308,6 +383,19 @@ def library():
                                           ctypes.POINTER(i32)]
         lib.fhe_bootstrap.argtypes = [vp, i32, vp, i32, i32, bk, i32, vp,
                                       ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        lib.fhe_bootstrap_key_create.argtypes = [
+            vp, ctypes.POINTER(BootstrapKeyParts), ctypes.POINTER(vp)]
+        lib.fhe_bootstrap_key_destroy.argtypes = [vp]
+        lib.fhe_bootstrap_key_destroy.restype = None
+        lib.fhe_coeff_to_slot_bk.argtypes = [
+            vp, i32, vp, i32, i32, vp, i32, vp, vp, vp, ctypes.POINTER(i32),
+            ctypes.POINTER(i32)]
+        lib.fhe_slot_to_coeff_bk.argtypes = [
+            vp, i32, vp, vp, i32, i32, vp, i32, vp, vp, ctypes.POINTER(i32),
+            ctypes.POINTER(i32)]
+        lib.fhe_bootstrap_bk.argtypes = [
+            vp, i32, vp, i32, i32, vp, i32, vp, i32, vp, vp,
+            ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fhe_decode.argtypes = [vp, vp, i32, i32, ctypes.c_longdouble,
                                    i32, vp]
         lib.fhe_decoded_scale.argtypes = [vp, i32, ctypes.c_longdouble,
@@ -618,6 +706,47 @@ class Context:
         _check(library().fhe_bootstrap(
             self.handle, _count(ct), ct.ptr, level, flags, ctypes.byref(keys),
             stage_count, out.ptr, ctypes.byref(lv), ctypes.byref(fl)))
+        return lv.value, fl.value
+
+    def bootstrap_key(self, parts):
+        """fhe_bootstrap_key_create from BootstrapKeyParts; returns the
+        handle (free it with bootstrap_key_destroy before the context)."""
+        h = ctypes.c_void_p()
+        _check(library().fhe_bootstrap_key_create(
+            self.handle, ctypes.byref(parts), ctypes.byref(h)))
+        h._parts = parts
+        return h
+
+    def bootstrap_key_destroy(self, bk):
+        library().fhe_bootstrap_key_destroy(bk)
+
+    def coeff_to_slot_bk(self, ct, level, flags, conj, conj_mid, bk, real,
+                         imag):
+        """fhe_coeff_to_slot_bk; returns the parts' (level, flags)."""
+        lv, fl = ctypes.c_int(), ctypes.c_int()
+        _check(library().fhe_coeff_to_slot_bk(
+            self.handle, _count(ct), ct.ptr, level, flags, _addr(conj),
+            conj_mid, bk, real.ptr, imag.ptr, ctypes.byref(lv),
+            ctypes.byref(fl)))
+        return lv.value, fl.value
+
+    def slot_to_coeff_bk(self, real, imag, level, flags, conj, conj_mid, bk,
+                         out):
+        lv, fl = ctypes.c_int(), ctypes.c_int()
+        _check(library().fhe_slot_to_coeff_bk(
+            self.handle, _count(real), real.ptr, imag.ptr, level, flags,
+            _addr(conj), conj_mid, bk, out.ptr, ctypes.byref(lv),
+            ctypes.byref(fl)))
+        return lv.value, fl.value
+
+    def bootstrap_bk(self, ct, level, flags, relin, relin_mid, conj, conj_mid,
+                     bk, out):
+        """fhe_bootstrap_bk of INTT-form duos; returns (level, flags)."""
+        lv, fl = ctypes.c_int(), ctypes.c_int()
+        _check(library().fhe_bootstrap_bk(
+            self.handle, _count(ct), ct.ptr, level, flags, _addr(relin),
+            relin_mid, _addr(conj), conj_mid, bk, out.ptr, ctypes.byref(lv),
+            ctypes.byref(fl)))
         return lv.value, fl.value
 
     def modmul_peak(self, form=0):

@@ -25,7 +25,8 @@ HIP_SOURCES = ["ntt.hip", "ckks_kernels.hip", "ks_fused.hip", "automorphism.hip"
                "elementwise.hip", "crypt.hip", "tensor_fused.hip",
                "encode.hip", "probe.hip", "context.hip", "engine.hip",
                "wire.hip", "ks_wide.hip", "uneven_kernels.hip", "uneven.hip",
-               "advanced.hip", "serve.hip", "noalloc.hip"]
+               "advanced.hip", "serve.hip", "noalloc.hip", "bk_kernels.hip",
+               "bootstrap_key.hip"]
 # per-source flags: the encoder's double arithmetic must round every product
 # and sum separately, as the reference's x86-64 build does
 HIP_SOURCE_FLAGS = {"encode.hip": ["-ffp-contract=off"]}

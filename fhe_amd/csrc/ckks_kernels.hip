@@ -191,6 +191,30 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
   }
 }
 
+// ModDown of one extended polynomial per (poly, coefficient) into its
+// chain limbs, canonical (the relinearizing kernel above without an addend).
+template <int MAXA>
+__global__ __launch_bounds__(kBlock) void moddown_polys_kernel(
+    const int64_t *acc, int64_t *out, int64_t polys, int limbs,
+    int moduli_id, DeviceTables t) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= (polys << t.log_n))
+    return;
+  const int64_t poly = idx >> t.log_n;
+  const int64_t coef = idx & (n - 1);
+  const int64_t *a = acc + poly * (limbs + t.alpha) * n + coef;
+  ModDown<MAXA> md;
+  md.alpha = t.alpha;
+  md.limbs = limbs;
+  md.moduli_id = moduli_id;
+  md.prepare(a, n, t);
+  int64_t *o = out + poly * limbs * n + coef;
+#pragma unroll 4
+  for (int i = 0; i < limbs; i++)
+    o[int64_t(i) * n] = md.limb_lazy(a[int64_t(i) * n], i, t);
+}
+
 __global__ __launch_bounds__(kBlock) void rescale_kernel(
     const int64_t *in, int64_t *out, int polys, int limbs, int moduli_id,
     DeviceTables t) {
@@ -390,6 +414,26 @@ void launch_moddown_relin(const DeviceTables &t, const int64_t *acc,
                           int limbs, int moduli_id, hipStream_t stream) {
   launch_moddown<false>(t, acc, trio, int64_t(3) * limbs << t.log_n, 2, out,
                         count, limbs, moduli_id, stream);
+}
+
+void launch_moddown_polys(const DeviceTables &t, const int64_t *acc,
+                          int64_t *out, int64_t polys, int limbs,
+                          int moduli_id, hipStream_t stream) {
+  const dim3 grid(grid_for(polys << t.log_n));
+#define FHE_MODDOWN_POLYS(W)                                                   \
+  hipLaunchKernelGGL((moddown_polys_kernel<W>), grid, dim3(kBlock), 0,        \
+                     stream, acc, out, polys, limbs, moduli_id, t)
+  if (t.alpha <= 1)
+    FHE_MODDOWN_POLYS(1);
+  else if (t.alpha <= 2)
+    FHE_MODDOWN_POLYS(2);
+  else if (t.alpha <= 4)
+    FHE_MODDOWN_POLYS(4);
+  else if (t.alpha <= 8)
+    FHE_MODDOWN_POLYS(8);
+  else
+    FHE_MODDOWN_POLYS(16);
+#undef FHE_MODDOWN_POLYS
 }
 
 void launch_moddown_add(const DeviceTables &t, const int64_t *acc,

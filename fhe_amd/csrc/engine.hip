@@ -414,6 +414,18 @@ Text slot_to_coeff(TextEngine &e, const Text &real, const Text &imag,
   return current;
 }
 
+void require_bootstrap_keys(const fhe_bootstrap_keys *k) {
+  require(k && k->relin && k->conjugation && k->rotation_count > 0 &&
+              k->rotation && k->rotation_moduli_ids && k->rotation_deltas,
+          "Bootstrap: keys missing");
+  for (int i = 0; i < k->rotation_count; i++)
+    require(k->rotation[i] != nullptr, "Bootstrap: rotation key missing");
+}
+
+} // namespace
+
+namespace fhe_host {
+
 Text round_text(TextEngine &e, const Text &x) {
   // round = x * ROUND_BOUND - modulo(x), modulo = double angle of sine
   // (src/ckks/engine_bootstrap.cpp:661-691)
@@ -442,15 +454,44 @@ void require_boot_form(const fhe::CkksParams &P, int level, int flags,
           std::string(op) + ": the level is out of range.");
 }
 
-void require_bootstrap_keys(const fhe_bootstrap_keys *k) {
-  require(k && k->relin && k->conjugation && k->rotation_count > 0 &&
-              k->rotation && k->rotation_moduli_ids && k->rotation_deltas,
-          "Bootstrap: keys missing");
-  for (int i = 0; i < k->rotation_count; i++)
-    require(k->rotation[i] != nullptr, "Bootstrap: rotation key missing");
+// CKKSEngine::bootstrap (src/ckks/engine_bootstrap.cpp:712-754 with a
+// RotationKey, 794-834 with a BootstrapKey) on `count` ciphertexts: level
+// down to 0, mod_raise, the partial sums below full slots, coeff_to_slot,
+// round of both parts in one batched EvalMod (each ciphertext follows its
+// own sequence of operations), slot_to_coeff, the subtraction from the
+// raised text and the normalizing integer product.  The steps that depend
+// on the kind of rotation key come from `steps`.
+Text bootstrap_texts(TextEngine &e, const Text &x,
+                     const BootstrapSteps &steps) {
+  fhe_context *ctx = e.context();
+  const auto &P = ctx->params;
+  const int slots = int(P.slot_count());
+  const Text leveled = e.level_down(x, 0);
+  Text raised = e.mod_raise(leveled);
+  if (slots != ctx->n() / 2)
+    steps.partial_sum(raised);
+  const auto [real, imag] = steps.coeff_to_slot(raised);
+  Text both = e.alloc(real.polys, real.level, real.flags, 2 * real.count);
+  const int64_t part = e.words(real);
+  hip_check(hipMemcpyAsync(both.data, real.data, size_t(part) * 8,
+                           hipMemcpyDeviceToDevice, ctx->stream),
+            "copy");
+  hip_check(hipMemcpyAsync(both.data + part, imag.data, size_t(part) * 8,
+                           hipMemcpyDeviceToDevice, ctx->stream),
+            "copy");
+  const Text rounded = round_text(e, both);
+  Text rounded_real = rounded, rounded_imag = rounded;
+  rounded_real.count = rounded_imag.count = real.count;
+  rounded_imag.data = rounded.data + e.words(rounded_real);
+  const Text coeff = steps.slot_to_coeff(rounded_real, rounded_imag);
+  const Text bootstrapped = e.add(raised, coeff, true);
+  // CKKSCore::get_normalize_ratio (src/ckks/core.cpp:1121-1125)
+  const int64_t ratio = static_cast<int64_t>(
+      std::round(double(P.q[P.chain_count - 1]) / P.scale));
+  return e.multiply_int(bootstrapped, ratio);
 }
 
-} // namespace
+} // namespace fhe_host
 
 extern "C" {
 
@@ -537,40 +578,21 @@ int fhe_bootstrap(fhe_context *ctx, int count, const int64_t *in, int level,
     e.keys = keys;
     e.count = count;
     const int slots = int(P.slot_count());
-    // CKKSEngine::bootstrap with a RotationKey
-    // (src/ckks/engine_bootstrap.cpp:712-754)
-    const Text x = e.wrap(in, 2, level, flags);
-    const Text leveled = e.level_down(x, 0);
-    Text raised = e.mod_raise(leveled);
-    if (slots != ctx->n() / 2) {
-      // partial_sum_inplace (src/ckks/engine_bootstrap.cpp:315-329)
+    BootstrapSteps steps;
+    // partial_sum_inplace (src/ckks/engine_bootstrap.cpp:315-329)
+    steps.partial_sum = [&](Text &raised) {
       for (int delta = slots; delta < ctx->n() / 2; delta <<= 1) {
         const Text r = e.rotate(raised, delta, false);
         e.add_inplace(raised, r);
       }
-    }
-    const auto [real, imag] = coeff_to_slot(e, raised, slots, stage_count);
-    // the two parts through one batched EvalMod: every ciphertext of the
-    // batch follows its own sequence of operations
-    Text both = e.alloc(real.polys, real.level, real.flags, 2 * real.count);
-    const int64_t part = e.words(real);
-    hip_check(hipMemcpyAsync(both.data, real.data, size_t(part) * 8,
-                             hipMemcpyDeviceToDevice, ctx->stream),
-              "copy");
-    hip_check(hipMemcpyAsync(both.data + part, imag.data, size_t(part) * 8,
-                             hipMemcpyDeviceToDevice, ctx->stream),
-              "copy");
-    const Text rounded = round_text(e, both);
-    Text rounded_real = rounded, rounded_imag = rounded;
-    rounded_real.count = rounded_imag.count = real.count;
-    rounded_imag.data = rounded.data + e.words(rounded_real);
-    const Text coeff =
-        slot_to_coeff(e, rounded_real, rounded_imag, slots, stage_count);
-    const Text bootstrapped = e.add(raised, coeff, true);
-    // CKKSCore::get_normalize_ratio (src/ckks/core.cpp:1121-1125)
-    const int64_t ratio = static_cast<int64_t>(
-        std::round(double(P.q[P.chain_count - 1]) / P.scale));
-    const Text result = e.multiply_int(bootstrapped, ratio);
+    };
+    steps.coeff_to_slot = [&](const Text &raised) {
+      return coeff_to_slot(e, raised, slots, stage_count);
+    };
+    steps.slot_to_coeff = [&](const Text &real, const Text &imag) {
+      return slot_to_coeff(e, real, imag, slots, stage_count);
+    };
+    const Text result = bootstrap_texts(e, e.wrap(in, 2, level, flags), steps);
     copy_out(ctx, e, result, out, out_level, out_flags);
     ctx->check_launch("Bootstrap");
   });

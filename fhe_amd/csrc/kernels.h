@@ -49,6 +49,12 @@ void ntt_inverse_second(const DeviceTables &t, const int64_t *src,
                         int64_t *dst, int64_t total_limbs, int limbs,
                         int moduli_id, int post, hipStream_t stream,
                         LazySpec lazy = LazySpec());
+// The forward pass over the low index bits, after ntt_forward_first: the two
+// together are ntt_forward.
+void ntt_forward_second(const DeviceTables &t, const int64_t *src,
+                        int64_t *dst, int64_t total_limbs, int limbs,
+                        int moduli_id, hipStream_t stream,
+                        LazySpec lazy = LazySpec());
 
 // Fused key-switching core for N = 2^16: for each tile of each extended limb
 // and ciphertext, the low-bit forward NTT pass of every partition's
@@ -172,6 +178,74 @@ void launch_moddown_relin(const DeviceTables &t, const int64_t *acc,
 // for h < d_polys and zero otherwise (the c0 term of a rotation,
 // src/ckks/core_key_switch.cpp:509-548, and the copy of the second half in
 // src/ckks/core.cpp:1035-1056).
+// The butterfly stage with a BootstrapKey in the extended basis
+// (bk_kernels.hip): E = limbs + alpha limbs per polynomial at moduli_id, NTT
+// form, values kept below 2q.
+constexpr int kBkMaxTerms = 16;
+constexpr int kBkMaxGroups = 8;
+constexpr int kBkMaxOuter = 64;
+// One term of a batched ntt_rotate: key inner product of the decomposition
+// dec (ciphertext c at dec + c * dec_stride, [parts][E][N]) with the key
+// (half h of partition p at key + (2 p + h) * key_chain * N, limb 0 aligned
+// with the ciphertext's limb 0), plus the source's polynomial 0 (c0 + c *
+// src_stride) in half 0, permuted by X -> X^g; with dec == nullptr the
+// identity: polynomials c0 and c1 as they are.
+struct BkTerm {
+  const int64_t *c0;
+  const int64_t *c1;
+  const int64_t *dec;
+  const int64_t *key;
+  int64_t src_stride;
+  int64_t dec_stride;
+  int key_chain;
+  uint32_t g;
+};
+struct BkRotateArgs {
+  BkTerm term[kBkMaxTerms];
+  int terms;
+  int count;
+  int parts;
+  int ext_limbs;
+  int moduli_id;
+  int accumulate;     // sum launches: add to out instead of overwriting
+  int64_t *out;       // half 1 is E * N words after half 0
+  int64_t out_c_stride;
+  int64_t out_t_stride; // term launches: per term
+};
+// Products with the diagonals of a stage summed per group:
+// sums[h][group][c] = sum over members o of the group and inner i of
+// rotated_i[c][h] * diagonal[o][i] (o, i) != (0, 0); rotated_0 = base.
+struct BkDiagArgs {
+  const int64_t *base;       // [count][2][E][N]
+  const int64_t *rotated;    // [count][inner - 1][2][E][N]
+  const int64_t *const *diagonal; // device table [outer][inner]
+  int64_t *sums;             // [2][groups][count][E][N]
+  int inner;
+  int count;
+  int ext_limbs;
+  int moduli_id;
+  int groups;                // groups in sums
+  int group0;                // this launch's groups: group0 ..
+  int groups_here;           //   group0 + groups_here - 1
+  int member_start[kBkMaxGroups + 1];
+  int member[kBkMaxOuter];   // outer indices of the launch's groups
+};
+bool bk_supported(const DeviceTables &t);
+void launch_bk_rotate(const DeviceTables &t, const BkRotateArgs &a, bool sum,
+                      hipStream_t stream);
+void launch_bk_diagonal(const DeviceTables &t, const BkDiagArgs &a,
+                        hipStream_t stream);
+void launch_bk_attach(const DeviceTables &t, const int64_t *in, int64_t *out,
+                      int64_t polys, int limbs, int ext_limbs,
+                      hipStream_t stream);
+
+// ModDown alone (CKKSCore::mod_down after its INTT and reduction,
+// src/ckks/core_key_switch.cpp:110-160) of `polys` INTT-form extended
+// polynomials acc [polys][limbs + alpha][N] (values in [0, q)) into
+// out [polys][limbs][N], canonical.
+void launch_moddown_polys(const DeviceTables &t, const int64_t *acc,
+                          int64_t *out, int64_t polys, int limbs,
+                          int moduli_id, hipStream_t stream);
 void launch_moddown_add(const DeviceTables &t, const int64_t *acc,
                         const int64_t *d, int64_t d_stride, int d_polys,
                         int64_t *out, int count, int limbs, int moduli_id,

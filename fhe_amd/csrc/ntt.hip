@@ -381,6 +381,25 @@ void ntt_forward_first(const DeviceTables &t, const int64_t *src, int64_t *dst,
   launch_pass(false, a, t, stream);
 }
 
+void ntt_forward_second(const DeviceTables &t, const int64_t *src,
+                        int64_t *dst, int64_t total_limbs, int limbs,
+                        int moduli_id, hipStream_t stream, LazySpec lazy) {
+  int l1, l2;
+  split_bits(t.log_n, l1, l2);
+  if (l2 == 0) {
+    // single-pass sizes: the first pass was the whole transform
+    if (src != dst)
+      (void)hipMemcpyAsync(dst, src, size_t(total_limbs << t.log_n) * 8,
+                           hipMemcpyDeviceToDevice, stream);
+    return;
+  }
+  PassArgs b2{src,   dst,      total_limbs << (t.log_n - l2),
+              limbs, moduli_id, 0,
+              l2,    nullptr,   kInttRaw,
+              1,     lazy.period, lazy.count, lazy.part};
+  launch_pass(false, b2, t, stream);
+}
+
 void ntt_inverse_second(const DeviceTables &t, const int64_t *src,
                         int64_t *dst, int64_t total_limbs, int limbs,
                         int moduli_id, int post, hipStream_t stream,

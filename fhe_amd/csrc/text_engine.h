@@ -3,6 +3,8 @@
 // advanced compositions and the serving loop.
 #pragma once
 
+#include <functional>
+
 #include "context.h"
 
 namespace fhe_host {
@@ -78,6 +80,18 @@ public:
                              hipMemcpyDeviceToDevice, ctx->stream),
               "copy");
     return t;
+  }
+  // rescale_ (src/ckks/engine_rescale.cpp:5-31) of a coefficient-domain
+  // text: the text itself when it is rescaled already
+  Text rescale(const Text &a) {
+    if (a.flags & FHE_FLAG_RESCALED)
+      return a;
+    require(!(a.flags & FHE_FLAG_NTT_FORM),
+            "Rescale: a coefficient-domain text here");
+    Text out = alloc(a.polys, a.level, FHE_FLAG_RESCALED, a.count);
+    ok(fhe_rescale(ctx, a.data, out.data, a.count * a.polys,
+                   ctx->moduli_id_of(a.level, false)));
+    return out;
   }
   // intt_ (src/ckks/engine_intt.cpp): a copy in the coefficient domain
   Text intt(const Text &a) {
@@ -434,5 +448,22 @@ Text evaluate_polynomial(TextEngine &e, const Text &x,
 Text sum_1d_by_row(TextEngine &e, const Text &x);
 fhe_bootstrap_keys rotation_keys_of(const fhe_rotation_key &rk);
 void check_rotation_key(const fhe_rotation_key *rk);
+
+// The bootstrap (engine.hip): EvalMod's round, the copy of a result into a
+// caller's buffer with its form, the check of an input form, and the
+// bootstrap's sequence with the steps that depend on the kind of rotation
+// key (a RotationKey or a BootstrapKey).
+Text round_text(TextEngine &e, const Text &x);
+void copy_out(fhe_context *ctx, TextEngine &e, const Text &t, int64_t *out,
+              int *out_level, int *out_flags);
+void require_boot_form(const fhe::CkksParams &P, int level, int flags,
+                       int min_level, const char *op);
+struct BootstrapSteps {
+  std::function<void(Text &)> partial_sum; // below full slots, in place
+  std::function<std::pair<Text, Text>(const Text &)> coeff_to_slot;
+  std::function<Text(const Text &, const Text &)> slot_to_coeff;
+};
+Text bootstrap_texts(TextEngine &e, const Text &x,
+                     const BootstrapSteps &steps);
 
 } // namespace fhe_host

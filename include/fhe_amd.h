@@ -269,6 +269,82 @@ int fhe_bootstrap(fhe_context *ctx, int count, const int64_t *in, int level,
                   const fhe_bootstrap_keys *keys, int stage_count,
                   int64_t *out, int *out_level, int *out_flags);
 
+/* A BootstrapKey (src/data/bootstrap.h:24-44, built by
+ * CKKSEngine::create_bootstrap_key, src/ckks/engine.cpp:162-222) whose
+ * parts are already on the device.  Each butterfly stage is a PlainMatrix
+ * (src/data/plain_matrix.h:10-24): its level, loop counts and deltas, and the
+ * encoded diagonals (make_bootstrap_stage, src/ckks/engine_bootstrap.cpp:
+ * 8-69: encoded at the level with the special primes, rescaled, NTT form;
+ * each [limbs + alpha][N] at the level's rescaled moduli_id, limbs that of
+ * the level after rescaling), given as a host array of outer_count x
+ * inner_count device pointers [outer][inner] whose entry (0, 0) is unused.
+ * The fixed rotation keys are the key's rotation contexts by (normalized)
+ * delta, each [key polys][key limbs][N] at its moduli_id; the NTT-domain
+ * permutations are computed on the device from the deltas. */
+typedef struct fhe_plain_matrix {
+  int level;
+  int inner_count;
+  int outer_count;
+  const int *inner_deltas;          /* inner_count values, the first 0 */
+  const int *outer_deltas;          /* outer_count values */
+  const int64_t *const *diagonals;  /* outer_count * inner_count */
+} fhe_plain_matrix;
+
+typedef struct fhe_bootstrap_key_parts {
+  int stage_count;
+  const fhe_plain_matrix *coeff_to_slot; /* stage_count, or NULL */
+  const fhe_plain_matrix *slot_to_coeff; /* stage_count, or NULL */
+  int rotation_count;
+  const int *rotation_deltas;
+  const int64_t *const *rotation_keys;
+  const int *rotation_moduli_ids;
+  const int64_t *to_sparse;              /* the SmallBootstrapKey */
+  int to_sparse_moduli_id;
+  const int64_t *to_dense;
+  int to_dense_moduli_id;
+} fhe_bootstrap_key_parts;
+
+/* A device BootstrapKey of a context: it keeps the caller's device pointers
+ * (the caller keeps those buffers alive) and small tables of its own.
+ * Creation checks the stages' shapes, that every delta has a key, and
+ * uploads the ModUp constants of every stage level.  Destroy it before its
+ * context. */
+typedef struct fhe_bootstrap_key fhe_bootstrap_key;
+int fhe_bootstrap_key_create(fhe_context *ctx,
+                             const fhe_bootstrap_key_parts *parts,
+                             fhe_bootstrap_key **out);
+void fhe_bootstrap_key_destroy(fhe_bootstrap_key *bk);
+
+/* CKKSEngine::coeff_to_slot with a BootstrapKey
+ * (src/ckks/engine_bootstrap.cpp:368-398): the key's coeff-to-slot stages
+ * (apply_butterfly with a PlainMatrix, :218-289: one decomposition shared by
+ * the baby steps, rotations kept in the extended basis, one ModDown per
+ * stage), then the real and imaginary parts by conjugation.  Arguments as
+ * fhe_coeff_to_slot. */
+int fhe_coeff_to_slot_bk(fhe_context *ctx, int count, const int64_t *in,
+                         int level, int flags, const int64_t *conj,
+                         int conj_moduli_id, const fhe_bootstrap_key *bk,
+                         int64_t *real, int64_t *imag, int *out_level,
+                         int *out_flags);
+
+/* CKKSEngine::slot_to_coeff with a BootstrapKey
+ * (src/ckks/engine_bootstrap.cpp:436-454).  Arguments as
+ * fhe_slot_to_coeff. */
+int fhe_slot_to_coeff_bk(fhe_context *ctx, int count, const int64_t *real,
+                         const int64_t *imag, int level, int flags,
+                         const int64_t *conj, int conj_moduli_id,
+                         const fhe_bootstrap_key *bk, int64_t *out,
+                         int *out_level, int *out_flags);
+
+/* CKKSEngine::bootstrap with a BootstrapKey
+ * (src/ckks/engine_bootstrap.cpp:794-834): as fhe_bootstrap with the key's
+ * stages, rotations and SmallBootstrapKey. */
+int fhe_bootstrap_bk(fhe_context *ctx, int count, const int64_t *in, int level,
+                     int flags, const int64_t *relin, int relin_moduli_id,
+                     const int64_t *conj, int conj_moduli_id,
+                     const fhe_bootstrap_key *bk, int64_t *out, int *out_level,
+                     int *out_flags);
+
 /* Per-coefficient primitives over polys x limbs device limbs at moduli_id:
  * y = op(x1, x2, c[limb]) for the reference's limb operations
  * (src/montgomery/unit.cpp:5-86, src/montgomery/compound.cpp:6-109,

@@ -977,6 +977,108 @@ void *ref_bootstrap(void *handle, void *ct, void *rlk, void *conj, void *rk,
   });
 }
 
+// CKKSEngine::create_bootstrap_key (src/ckks/engine.cpp:162-222) and the
+// parts of the BootstrapKey (src/data/bootstrap.h:24-44,
+// src/data/plain_matrix.h:10-24) a device engine uploads.
+void *ref_bootstrap_key(void *handle, void *sk, int stage_count,
+                        const char *size) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->create_bootstrap_key(
+        static_cast<SecretKey *>(sk), stage_count, size);
+  });
+}
+void ref_free_bootstrap_key(void *p) { delete static_cast<BootstrapKey *>(p); }
+// out[0..3] = stage_count, inner_loop_shift, rotation contexts, CtS stages
+int ref_bootstrap_key_info(void *bk, int *out) {
+  auto k = static_cast<BootstrapKey *>(bk);
+  out[0] = k->stage_count_;
+  out[1] = k->inner_loop_shift_;
+  out[2] = int(k->rotation_contexts_.size());
+  out[3] = int(k->coeff_to_slot_stages_.size());
+  return 0;
+}
+// Rotation context `index` in the map's (delta) order: its delta and fixed
+// rotation key (kind 1 of ref_key_*); nullptr past the end.
+void *ref_bootstrap_key_rotation(void *bk, int index, int *delta) {
+  auto k = static_cast<BootstrapKey *>(bk);
+  int i = 0;
+  for (auto &[d, ctx] : k->rotation_contexts_)
+    if (i++ == index) {
+      *delta = d;
+      return ctx.first;
+    }
+  return nullptr;
+}
+void *ref_bootstrap_key_small(void *bk) {
+  return static_cast<BootstrapKey *>(bk)->small_bootstrap_key_;
+}
+// One PlainMatrix: meta[0..2] = level, inner count, outer count; the inner
+// and outer deltas (when the arrays are given).
+int ref_bootstrap_key_stage(void *bk, int to_slot, int stage, int *meta,
+                            int *inner_deltas, int *outer_deltas) {
+  return guarded([&]() -> int {
+    auto k = static_cast<BootstrapKey *>(bk);
+    auto &stages = to_slot ? k->coeff_to_slot_stages_ : k->slot_to_coeff_stages_;
+    PlainMatrix *m = stages.at(size_t(stage));
+    meta[0] = m->level_;
+    meta[1] = m->inner_loop_count_;
+    meta[2] = m->outer_loop_count_;
+    if (inner_deltas)
+      for (size_t i = 0; i < m->inner_loop_deltas_.size(); i++)
+        inner_deltas[i] = m->inner_loop_deltas_[i];
+    if (outer_deltas)
+      for (size_t i = 0; i < m->outer_loop_deltas_.size(); i++)
+        outer_deltas[i] = m->outer_loop_deltas_[i];
+    return 0;
+  });
+}
+// The encoded diagonal (outer, inner) of a stage (ref_pt_shape /
+// ref_pt_get), nullptr when the matrix has none there.
+void *ref_bootstrap_key_diagonal(void *bk, int to_slot, int stage, int outer,
+                                 int inner) {
+  auto k = static_cast<BootstrapKey *>(bk);
+  auto &stages = to_slot ? k->coeff_to_slot_stages_ : k->slot_to_coeff_stages_;
+  if (stage < 0 || size_t(stage) >= stages.size())
+    return nullptr;
+  auto &diag = stages[size_t(stage)]->encoded_diagonals_;
+  auto o = diag.find(outer);
+  if (o == diag.end())
+    return nullptr;
+  auto i = o->second.find(inner);
+  return i == o->second.end() ? nullptr : i->second;
+}
+// CKKSEngine::coeff_to_slot / slot_to_coeff / bootstrap with a BootstrapKey
+// (src/ckks/engine_bootstrap.cpp:368-398, 436-454, 794-834).
+int ref_coeff_to_slot_bk(void *handle, void *ct, void *conj, void *bk,
+                         void **real, void **imag) {
+  return guarded([&]() -> int {
+    auto [r, i] = static_cast<RefEngine *>(handle)->engine->coeff_to_slot(
+        static_cast<UnitCiphertext *>(ct), static_cast<ConjugationKey *>(conj),
+        static_cast<BootstrapKey *>(bk));
+    *real = r;
+    *imag = i;
+    return 0;
+  });
+}
+void *ref_slot_to_coeff_bk(void *handle, void *real, void *imag, void *conj,
+                           void *bk) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->slot_to_coeff(
+        static_cast<UnitCiphertext *>(real),
+        static_cast<UnitCiphertext *>(imag),
+        static_cast<ConjugationKey *>(conj), static_cast<BootstrapKey *>(bk));
+  });
+}
+void *ref_bootstrap_bk(void *handle, void *ct, void *rlk, void *conj,
+                       void *bk) {
+  return guarded([&]() -> void * {
+    return static_cast<RefEngine *>(handle)->engine->bootstrap(
+        static_cast<UnitCiphertext *>(ct),
+        static_cast<RelinearizationKey *>(rlk),
+        static_cast<ConjugationKey *>(conj), static_cast<BootstrapKey *>(bk));
+  });
+}
+
 // get_coeff_to_slot_twiddles / get_slot_to_coeff_twiddles
 // (src/operation/bootstrap.cpp:125-263), one stage as {re, im} pairs.
 int ref_bootstrap_twiddles(int log_slots, int stage_count, int to_slot,

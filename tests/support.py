@@ -402,6 +402,16 @@ class Reference:
             "ref_slot_to_coeff": (VP, [VP, VP, VP, VP, VP, i32]),
             "ref_bootstrap": (VP, [VP, VP, VP, VP, VP, VP, i32]),
             "ref_fixed_rotation_key_delta": (i32, [VP]),
+            "ref_bootstrap_key": (VP, [VP, VP, i32, ctypes.c_char_p]),
+            "ref_free_bootstrap_key": (None, [VP]),
+            "ref_bootstrap_key_info": (i32, [VP, VP]),
+            "ref_bootstrap_key_rotation": (VP, [VP, i32, VP]),
+            "ref_bootstrap_key_small": (VP, [VP]),
+            "ref_bootstrap_key_stage": (i32, [VP, i32, i32, VP, VP, VP]),
+            "ref_bootstrap_key_diagonal": (VP, [VP, i32, i32, i32, i32]),
+            "ref_coeff_to_slot_bk": (i32, [VP, VP, VP, VP, VP, VP]),
+            "ref_slot_to_coeff_bk": (VP, [VP, VP, VP, VP, VP]),
+            "ref_bootstrap_bk": (VP, [VP, VP, VP, VP, VP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
