@@ -251,7 +251,7 @@ def codec_bench(ctx, ev, batch, steps, with_cpu):
     return out
 
 
-def bootstrap_bench(ev, steps):
+def bootstrap_bench(ev, steps, graph=True):
     """Bootstraps with a RotationKey (fhe_bootstrap: level down,
     mod_raise, coeff_to_slot, EvalMod of both parts, slot_to_coeff,
     normalization) on the N = 2^16 bootstrap preset, wall-clock per
@@ -303,21 +303,21 @@ def bootstrap_bench(ev, steps):
                 ctx.bootstrap(dct, level, 0, keys, 3, out)
             ctx.synchronize()
             ms[batch] = (time.perf_counter() - t0) * 1e3 / steps
-            if batch == 1:
+            if batch == 1 and graph:
                 # the same bootstrap replayed from a HIP graph
                 ctx.graph_begin()
                 try:
                     ctx.bootstrap(dct, level, 0, keys, 3, out)
                 finally:
-                    graph = ctx.graph_end()
-                ctx.graph_launch(graph)
+                    g = ctx.graph_end()
+                ctx.graph_launch(g)
                 ctx.synchronize()
                 t0 = time.perf_counter()
                 for _ in range(steps):
-                    ctx.graph_launch(graph)
+                    ctx.graph_launch(g)
                 ctx.synchronize()
                 ms["graph"] = (time.perf_counter() - t0) * 1e3 / steps
-                ctx.graph_destroy(graph)
+                ctx.graph_destroy(g)
             for arr in (dct, out):
                 arr.free()
         for arr in [relin, conj, dense, dsp] + rot:
@@ -332,12 +332,130 @@ def bootstrap_bench(ev, steps):
                          "(tests/test_gpu_bootstrap.py), whose decrypted "
                          "precision that test reports",
             "ms_per_bootstrap": round(ms[1], 1),
-            "ms_per_bootstrap_graph": round(ms["graph"], 1),
+            "ms_per_bootstrap_graph": (round(ms["graph"], 1) if graph
+                                       else None),
             "ms_per_batch_of_8": round(ms[8], 1),
             "ms_per_bootstrap_batched": round(ms[8] / 8, 1), "steps": steps,
             "timing": "wall clock, host work included"}
 
 
+def bootstrap_bk_bench(steps, stages=3, graph=True):
+    """Bootstraps with a BootstrapKey (fhe_bootstrap_bk) on the N = 2^16
+    bootstrap preset: the key's layout (stage levels, loop counts, deltas,
+    38 rotation keys at their creation levels) from
+    fhe_bootstrap_key_stage_layout / _rotation_layout, every key and every
+    encoded diagonal in its own device buffer with synthetic values of the
+    real shapes (the work does not depend on their values)."""
+    import fhe_amd
+    rng = np.random.Generator(np.random.PCG64(12))
+    layout = fhe_amd.bootstrap_key_layout(LOG_N, ALPHA, stages)
+    with fhe_amd.Context(LOG_N, ALPHA, scale_bits=54,
+                         small_prime_bits=54) as ctx:
+        full = ctx.info["full_level"]
+        key_mid = ctx.key_moduli_id()
+        n = ctx.n
+        arrays = []
+        fills = {}
+
+        def filled(mid, polys, limbs):
+            # one random polynomial per shape, uploaded into every slot
+            key = (mid, limbs)
+            if key not in fills:
+                fills[key] = uniform_limbs(rng, ctx.primes, mid, limbs, n)
+            d = ctx.array((polys, limbs, n))
+            for p in range(polys):
+                d.upload_at(fills[key], p * limbs * n)
+            arrays.append(d)
+            return d
+
+        def device_key(mid):
+            kp, kc, _ = ctx.key_shape(mid)
+            return filled(mid, kp, kc)
+
+        relin, conj, dense = (device_key(key_mid) for _ in range(3))
+        sparse = device_key(full)
+        rotations = [(d, device_key(mid), mid)
+                     for d, mid in layout["rotations"]]
+        key_bytes = sum(a.nbytes for a in arrays)
+        mats = {}
+        for name in ("coeff_to_slot", "slot_to_coeff"):
+            mats[name] = []
+            for st in layout[name]:
+                mid = full - st["level"] + 1
+                ext = ctx.info["prime_count"] - mid
+                inner, outer = len(st["inner_deltas"]), len(st["outer_deltas"])
+                diag = {(o, i): filled(mid, 1, ext) for o in range(outer)
+                        for i in range(inner) if (o, i) != (0, 0)}
+                mats[name].append(fhe_amd.PlainMatrix.make(
+                    st["level"], st["inner_deltas"], st["outer_deltas"],
+                    diag))
+        all_bytes = sum(a.nbytes for a in arrays)
+        parts = fhe_amd.BootstrapKeyParts.make(
+            mats["coeff_to_slot"], mats["slot_to_coeff"], rotations, sparse,
+            full, dense, key_mid)
+        bk = ctx.bootstrap_key(parts)
+        level = 2
+        mid = full - level
+        ms = {}
+        try:
+            for batch in (1, 8):
+                ct = np.stack([np.stack([uniform_limbs(
+                    rng, ctx.primes, mid, ctx.limbs_at(mid), n)
+                    for _ in range(2)]) for _ in range(batch)])
+                dct = ctx.to_device(ct)
+                out = ctx.array((batch, 2, ctx.chain_count - 1, n))
+                ctx.bootstrap_bk(dct, level, 0, relin, key_mid, conj, key_mid,
+                                 bk, out)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    ctx.bootstrap_bk(dct, level, 0, relin, key_mid, conj,
+                                     key_mid, bk, out)
+                ctx.synchronize()
+                ms[batch] = (time.perf_counter() - t0) * 1e3 / steps
+                if batch == 1 and graph:
+                    ctx.graph_begin()
+                    try:
+                        ctx.bootstrap_bk(dct, level, 0, relin, key_mid, conj,
+                                         key_mid, bk, out)
+                    finally:
+                        g = ctx.graph_end()
+                    ctx.graph_launch(g)
+                    ctx.synchronize()
+                    t0 = time.perf_counter()
+                    for _ in range(steps):
+                        ctx.graph_launch(g)
+                    ctx.synchronize()
+                    ms["graph"] = (time.perf_counter() - t0) * 1e3 / steps
+                    ctx.graph_destroy(g)
+                dct.free()
+                out.free()
+        finally:
+            ctx.bootstrap_key_destroy(bk)
+            for a in arrays:
+                a.free()
+    return {"workload": f"bootstrap with a BootstrapKey (size medium, "
+                        f"{stages} CtS / StC stages of 8 x 8 diagonals, "
+                        f"{len(layout['rotations'])} fixed rotation keys), "
+                        "N=2^16 bootstrap preset, 2^15 slots, ciphertexts "
+                        "from level 2, one per call and 8 per batched call; "
+                        "every key and diagonal in its own device buffer "
+                        "(synthetic values of the real shapes)",
+            "key_gb": round(all_bytes / 1e9, 2),
+            "rotation_and_small_key_gb": round(key_bytes / 1e9, 2),
+            "precision": "bit-identical to the reference engine's bootstrap "
+                         "with a BootstrapKey "
+                         "(tests/test_gpu_bootstrap_key.py), whose decrypted "
+                         "precision that test reports",
+            "ms_per_bootstrap": round(ms[1], 1),
+            "ms_per_bootstrap_graph": (round(ms["graph"], 1) if graph
+                                       else None),
+            "ms_per_batch_of_8": round(ms[8], 1),
+            "ms_per_bootstrap_batched": round(ms[8] / 8, 1), "steps": steps,
+            "timing": "wall clock, host work included"}
+
+
+UNEVEN_ALPHA = 8
 UNEVEN_ALPHA = 8
 
 
@@ -777,6 +895,13 @@ def main():
     ap.add_argument("--only-bootstrap", action="store_true",
                     help="only run the bootstrap measurement (e.g. under a "
                          "profiler) and print its JSON object")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the HIP-graph replays of the bootstrap legs "
+                         "(for runs under rocprofv3, whose tracing of graph "
+                         "launches with copy nodes is not supported)")
+    ap.add_argument("--only-bootstrap-key", action="store_true",
+                    help="only run the BootstrapKey bootstrap measurement "
+                         "and print its JSON object")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times "
                          "(the reference kernel for scaling the FETCH_SIZE "
@@ -794,7 +919,10 @@ def main():
     import fhe_amd
     ev = HipEvents()
     if args.only_bootstrap:
-        print(json.dumps(bootstrap_bench(ev, 3)))
+        print(json.dumps(bootstrap_bench(ev, 3, not args.no_graph)))
+        return
+    if args.only_bootstrap_key:
+        print(json.dumps(bootstrap_bk_bench(3, graph=not args.no_graph)))
         return
     if args.only_uneven:
         print(json.dumps(uneven_bench(ev, 10)))
@@ -839,7 +967,10 @@ def main():
             codec = codec_bench(ctx, ev, args.batch, max(5, args.steps // 2),
                                 not args.no_cpu)
 
-    boot = None if args.no_bootstrap else bootstrap_bench(ev, 3)
+    graph = not args.no_graph
+    boot = None if args.no_bootstrap else bootstrap_bench(ev, 3, graph)
+    boot_bk = (None if args.no_bootstrap
+               else bootstrap_bk_bench(3, graph=graph))
     uneven = None if args.no_uneven else uneven_bench(ev, 10)
     served = None if args.no_serve else serve_bench()
     value = args.batch / (step_ms / 1e3)
@@ -872,6 +1003,7 @@ def main():
         "stages": breakdown,
         "encode_decode": codec,
         "bootstrap": boot,
+        "bootstrap_bootstrap_key": boot_bk,
         "uneven_key_switch": uneven,
         "serving": served,
     }

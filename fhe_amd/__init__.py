@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = (
     "fhe_sum_1d_by_row", "fhe_serve", "fhe_hmult_relin_rescale_noalloc",
     "fhe_bootstrap_key_create", "fhe_bootstrap_key_destroy",
     "fhe_coeff_to_slot_bk", "fhe_slot_to_coeff_bk", "fhe_bootstrap_bk",
+    "fhe_bootstrap_key_stage_layout", "fhe_bootstrap_key_rotation_layout",
 )
 
 # fhe_wire_* record kinds (FHE_WIRE_* in include/fhe_amd.h)
@@ -383,6 +384,12 @@ def library():
                                           ctypes.POINTER(i32)]
         lib.fhe_bootstrap.argtypes = [vp, i32, vp, i32, i32, bk, i32, vp,
                                       ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        lib.fhe_bootstrap_key_stage_layout.argtypes = [
+            ctypes.POINTER(Params), i32, i32, i32, i32, ctypes.POINTER(i32),
+            ctypes.POINTER(i32), ctypes.POINTER(i32), vp, vp, i32]
+        lib.fhe_bootstrap_key_rotation_layout.argtypes = [
+            ctypes.POINTER(Params), i32, i32, vp, vp, i32,
+            ctypes.POINTER(i32)]
         lib.fhe_bootstrap_key_create.argtypes = [
             vp, ctypes.POINTER(BootstrapKeyParts), ctypes.POINTER(vp)]
         lib.fhe_bootstrap_key_destroy.argtypes = [vp]
@@ -498,6 +505,46 @@ def params_encode_constants(log_n, special_prime_count, level, scale_bits=40,
                 log_scale_after_tile=log_after.value,
                 top_of_one_more_level=bool(top.value),
                 r_mult_scale_after_tile=out[:count.value])
+
+
+def bootstrap_key_layout(log_n, special_prime_count, stage_count=3,
+                         inner_loop_shift=0, scale_bits=54,
+                         small_prime_bits=54, one_more_level=True,
+                         primes=None):
+    """Host-only fhe_bootstrap_key_stage_layout / _rotation_layout: dict
+    with "coeff_to_slot" and "slot_to_coeff" (per stage: level, inner and
+    outer deltas) and "rotations" ([(delta, moduli_id)] in creation
+    order)."""
+    p = make_params(log_n, special_prime_count, scale_bits, small_prime_bits,
+                    one_more_level, -1, primes)
+    lib = library()
+    out = {}
+    for name, to_slot in (("coeff_to_slot", 1), ("slot_to_coeff", 0)):
+        stages = []
+        for st in range(stage_count):
+            level, ni, no = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            inner = np.zeros(4096, np.int32)
+            outer = np.zeros(4096, np.int32)
+            _check(lib.fhe_bootstrap_key_stage_layout(
+                ctypes.byref(p), stage_count, inner_loop_shift, to_slot, st,
+                ctypes.byref(level), ctypes.byref(ni), ctypes.byref(no),
+                inner.ctypes.data, outer.ctypes.data, 4096))
+            stages.append(dict(level=level.value,
+                               inner_deltas=inner[:ni.value].tolist(),
+                               outer_deltas=outer[:no.value].tolist()))
+        out[name] = stages
+    count = ctypes.c_int()
+    _check(lib.fhe_bootstrap_key_rotation_layout(
+        ctypes.byref(p), stage_count, inner_loop_shift, None, None, 0,
+        ctypes.byref(count)))
+    deltas = np.zeros(max(1, count.value), np.int32)
+    mids = np.zeros(max(1, count.value), np.int32)
+    _check(lib.fhe_bootstrap_key_rotation_layout(
+        ctypes.byref(p), stage_count, inner_loop_shift, deltas.ctypes.data,
+        mids.ctypes.data, count.value, ctypes.byref(count)))
+    out["rotations"] = list(zip(deltas[:count.value].tolist(),
+                                mids[:count.value].tolist()))
+    return out
 
 
 class DeviceArray:

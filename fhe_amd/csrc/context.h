@@ -827,6 +827,10 @@ struct fhe_context {
   }
 };
 
+// The parameters a fhe_params describes (context.hip); twiddles = false
+// skips the NTT tables (host-only queries).
+fhe::CkksParams resolve_params(const fhe_params *p, bool twiddles = true);
+
 // The batched HMult + relinearize + rescale pipeline (context.hip) on a
 // caller-given workspace of hmult_workspace_words words and, when given, a
 // digits buffer of ks_digits_words words.

@@ -6,12 +6,7 @@ namespace fhe_host {
 thread_local std::string g_last_error;
 }
 
-extern "C" {
-
-const char *fhe_last_error(void) { return g_last_error.c_str(); }
-
-static fhe::CkksParams resolve_params(const fhe_params *p,
-                                      bool twiddles = true) {
+fhe::CkksParams resolve_params(const fhe_params *p, bool twiddles) {
   require(p != nullptr, "null argument");
   require(p->scale_bits == 40 || p->scale_bits == 54,
           "CKKS Parameters: unsupported config.");
@@ -44,6 +39,10 @@ static fhe::CkksParams resolve_params(const fhe_params *p,
           "The given max level is out of range.");
   return params;
 }
+
+extern "C" {
+
+const char *fhe_last_error(void) { return g_last_error.c_str(); }
 
 int fhe_params_resolve_primes(const fhe_params *p, uint64_t *primes,
                               int capacity, int *prime_count) {

@@ -1,6 +1,8 @@
 // Engine-level compositions on the device (CKKSEngine's form bookkeeping,
 // EvalMod, the slot transforms and the bootstrap), host C++ over the device
 // operations of the C ABI.
+#include <set>
+
 #include "text_engine.h"
 
 namespace {
@@ -266,6 +268,93 @@ std::pair<int, int> stage_loop_counts(int twiddle_count) {
   return {inner, (twiddle_count + 1) / inner};
 }
 
+// get_stage_loop_counts with the BootstrapKey's inner-loop shift
+// (src/operation/bootstrap.cpp:265-278)
+std::pair<int, int> stage_loop_counts_shifted(int twiddle_count, int shift) {
+  int bits = 0;
+  for (int t = twiddle_count; t > 0; t >>= 1)
+    bits++;
+  const int inner = 1 << ((bits + 1) / 2 + shift);
+  return {inner, (twiddle_count + 1) / inner};
+}
+
+// The stage counts a BootstrapKey allows (require_bootstrap_stage_count,
+// src/ckks/engine_require.cpp:1018-1100): a table of the reference's.
+bool bootstrap_stage_count_valid(int log_slots, int shift, int stages) {
+  int lo = 3, hi = 5;
+  if (shift == 0) {
+    if (log_slots <= 3)
+      lo = hi = 1;
+    else if (log_slots <= 5)
+      lo = 1, hi = 2;
+    else if (log_slots == 6)
+      lo = hi = 2;
+    else if (log_slots <= 9)
+      lo = 2, hi = 3;
+    else if (log_slots == 10)
+      lo = 2, hi = 4;
+    else if (log_slots <= 12)
+      lo = 3, hi = 4;
+    else if (log_slots <= 14)
+      lo = 3, hi = 5;
+  } else if (shift == 1) {
+    if (log_slots <= 2)
+      return false;
+    if (log_slots <= 5)
+      lo = hi = 1;
+    else if (log_slots <= 8)
+      lo = hi = 2;
+    else if (log_slots <= 10)
+      lo = 2, hi = 3;
+    else if (log_slots == 11)
+      lo = hi = 3;
+    else if (log_slots <= 14)
+      lo = 3, hi = 4;
+  } else {
+    return false;
+  }
+  return stages >= lo && stages <= hi;
+}
+
+struct StageLayout {
+  int level = 0, inner = 0, outer = 0;
+  std::vector<int> inner_deltas, outer_deltas;
+};
+
+// One PlainMatrix's shape as make_coeff_to_slot_stages /
+// make_slot_to_coeff_stages and make_bootstrap_stage build it
+// (src/ckks/engine_bootstrap.cpp:8-111, stage levels :113-123)
+StageLayout stage_layout(const fhe::CkksParams &P, int stages, int shift,
+                         bool to_slot, int stage) {
+  const int slots = int(P.slot_count());
+  int log_slots = 0;
+  while ((1 << log_slots) < slots)
+    log_slots++;
+  const int top = P.max_key_switching_level();
+  StageLayout s;
+  int rotate_count = to_slot ? slots : 1;
+  for (int st = 0; st <= stage; st++) {
+    if (to_slot)
+      rotate_count >>= cts_log_radix(st, stages, log_slots);
+    else if (st < stage)
+      rotate_count <<= stc_log_radix(st, stages, log_slots);
+  }
+  const int log_radix = to_slot ? cts_log_radix(stage, stages, log_slots)
+                                : stc_log_radix(stage, stages, log_slots);
+  s.level = to_slot ? top - stage : top - stages - 9 - stage;
+  const auto [inner, outer] =
+      stage_loop_counts_shifted(2 * (1 << log_radix) - 1, shift);
+  s.inner = inner;
+  s.outer = outer;
+  s.inner_deltas.push_back(0);
+  for (int i = 1; i < inner; i++)
+    s.inner_deltas.push_back(positive_modulo(i * rotate_count, slots));
+  for (int o = 0; o < outer; o++)
+    s.outer_deltas.push_back(
+        positive_modulo(inner * (o - outer / 2) * rotate_count, slots));
+  return s;
+}
+
 // rotate_vector (src/ckks/engine_bootstrap.cpp:856-870)
 std::vector<Complex> rotate_vector(const std::vector<Complex> &in,
                                    int delta) {
@@ -510,6 +599,87 @@ int fhe_bootstrap_twiddles(int log_slots, int stage_count, int to_slot,
             "Bootstrap twiddles: capacity too small");
     for (size_t r = 0; r < st.size(); r++)
       std::memcpy(out + r * slots * 2, st[r].data(), size_t(slots) * 16);
+  });
+}
+
+int fhe_bootstrap_key_stage_layout(const fhe_params *params, int stage_count,
+                                   int inner_loop_shift, int to_slot,
+                                   int stage, int *level, int *inner_count,
+                                   int *outer_count, int *inner_deltas,
+                                   int *outer_deltas, int capacity) {
+  return guarded([&] {
+    require(level && inner_count && outer_count, "null argument");
+    const auto P = resolve_params(params, false);
+    int log_slots = 0;
+    while ((int64_t(1) << log_slots) < P.slot_count())
+      log_slots++;
+    require(bootstrap_stage_count_valid(log_slots, inner_loop_shift,
+                                        stage_count),
+            "Bootstrap key layout: invalid stage count.");
+    require(stage >= 0 && stage < stage_count,
+            "Bootstrap key layout: stage out of range");
+    const StageLayout s = stage_layout(P, stage_count, inner_loop_shift,
+                                       to_slot != 0, stage);
+    require(s.level >= 1, "Bootstrap key layout: too few levels");
+    *level = s.level;
+    *inner_count = s.inner;
+    *outer_count = s.outer;
+    if (inner_deltas || outer_deltas)
+      require(capacity >= std::max(s.inner, s.outer),
+              "Bootstrap key layout: capacity too small");
+    if (inner_deltas)
+      std::copy(s.inner_deltas.begin(), s.inner_deltas.end(), inner_deltas);
+    if (outer_deltas)
+      std::copy(s.outer_deltas.begin(), s.outer_deltas.end(), outer_deltas);
+  });
+}
+
+int fhe_bootstrap_key_rotation_layout(const fhe_params *params,
+                                      int stage_count, int inner_loop_shift,
+                                      int *deltas, int *moduli_ids,
+                                      int capacity, int *count) {
+  return guarded([&] {
+    require(count, "null argument");
+    const auto P = resolve_params(params, false);
+    const int slots = int(P.slot_count());
+    int log_slots = 0;
+    while ((1 << log_slots) < slots)
+      log_slots++;
+    require(bootstrap_stage_count_valid(log_slots, inner_loop_shift,
+                                        stage_count),
+            "Bootstrap key layout: invalid stage count.");
+    // MAKE_ROTATION_CONTEXT's order: partial sums at the maximum
+    // key-switching level, then every stage's inner and outer deltas at the
+    // stage's level; a delta keeps the level of its first appearance
+    std::vector<std::pair<int, int>> out;
+    std::set<int> seen;
+    auto add = [&](int delta, int level) {
+      if (delta != 0 && seen.insert(delta).second)
+        out.push_back({delta, P.moduli_id_of_level(level)});
+    };
+    const int top = P.max_key_switching_level();
+    for (int d = slots; d < P.n / 2; d <<= 1)
+      add(d, top);
+    for (int to_slot = 1; to_slot >= 0; to_slot--)
+      for (int st = 0; st < stage_count; st++) {
+        const StageLayout s =
+            stage_layout(P, stage_count, inner_loop_shift, to_slot != 0, st);
+        for (int d : s.inner_deltas)
+          add(d, s.level);
+        for (int d : s.outer_deltas)
+          add(d, s.level);
+      }
+    *count = int(out.size());
+    if (deltas || moduli_ids) {
+      require(capacity >= int(out.size()),
+              "Bootstrap key layout: capacity too small");
+      for (size_t i = 0; i < out.size(); i++) {
+        if (deltas)
+          deltas[i] = out[i].first;
+        if (moduli_ids)
+          moduli_ids[i] = out[i].second;
+      }
+    }
   });
 }
 

@@ -290,6 +290,24 @@ typedef struct fhe_plain_matrix {
   const int64_t *const *diagonals;  /* outer_count * inner_count */
 } fhe_plain_matrix;
 
+/* Host only: the shapes of a BootstrapKey as CKKSEngine::
+ * create_bootstrap_key builds them for a stage count and inner-loop shift
+ * (size "medium" 0, "large" 1; src/ckks/engine.cpp:162-222,
+ * src/ckks/engine_bootstrap.cpp:8-123): one stage's level, loop counts and
+ * deltas (to_slot 1 coeff-to-slot, 0 slot-to-coeff; the delta arrays may be
+ * NULL), and the rotation contexts in creation order, each delta with the
+ * moduli_id of the level its key is made at.  Stage counts the reference
+ * refuses are refused. */
+int fhe_bootstrap_key_stage_layout(const fhe_params *params, int stage_count,
+                                   int inner_loop_shift, int to_slot,
+                                   int stage, int *level, int *inner_count,
+                                   int *outer_count, int *inner_deltas,
+                                   int *outer_deltas, int capacity);
+int fhe_bootstrap_key_rotation_layout(const fhe_params *params,
+                                      int stage_count, int inner_loop_shift,
+                                      int *deltas, int *moduli_ids,
+                                      int capacity, int *count);
+
 typedef struct fhe_bootstrap_key_parts {
   int stage_count;
   const fhe_plain_matrix *coeff_to_slot; /* stage_count, or NULL */

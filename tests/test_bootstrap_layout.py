@@ -1,0 +1,56 @@
+"""The BootstrapKey layout (fhe_bootstrap_key_stage_layout /
+fhe_bootstrap_key_rotation_layout, host only) against the reference's own
+BootstrapKey on the N = 2^16 bootstrap preset (3 stages, size "medium";
+tests/golden/bootstrap_key_layout.json, written by
+tests/golden/make_bootstrap_key_layout.py from the reference build), and
+the stage counts the reference refuses
+(src/ckks/engine_require.cpp:1018-1100)."""
+import json
+import os
+
+import pytest
+
+import fhe_amd
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden",
+                      "bootstrap_key_layout.json")
+
+
+def test_layout_matches_reference_key():
+    with open(GOLDEN) as f:
+        want = json.load(f)
+    got = fhe_amd.bootstrap_key_layout(want["log_n"],
+                                       want["special_prime_count"],
+                                       want["stage_count"],
+                                       want["inner_loop_shift"])
+    for name in ("coeff_to_slot", "slot_to_coeff"):
+        assert got[name] == want[name], name
+    assert dict(got["rotations"]) == dict(
+        (d, m) for d, m in want["rotations_by_delta"])
+    assert len(got["rotations"]) == len(want["rotations_by_delta"])
+    # creation order: the first coeff-to-slot stage's deltas come first
+    first = want["coeff_to_slot"][0]
+    assert got["rotations"][0][0] == first["inner_deltas"][1]
+
+
+def test_layout_loop_counts_follow_shift_and_stages():
+    # 3 stages of radix 2^5: 63 diagonals per stage, inner 8 (16 with the
+    # "large" shift); 5 stages of radix 2^3: 15 diagonals, inner 4 (8)
+    for stages, shift, inner, outer in ((3, 0, 8, 8), (3, 1, 16, 4),
+                                        (5, 0, 4, 4), (5, 1, 8, 2)):
+        got = fhe_amd.bootstrap_key_layout(16, 4, stages, shift)
+        for st in got["coeff_to_slot"] + got["slot_to_coeff"]:
+            assert len(st["inner_deltas"]) == inner, (stages, shift)
+            assert len(st["outer_deltas"]) == outer, (stages, shift)
+            assert st["inner_deltas"][0] == 0
+        levels = [s["level"] for s in got["coeff_to_slot"]]
+        assert levels == list(range(levels[0], levels[0] - stages, -1))
+        stc = [s["level"] for s in got["slot_to_coeff"]]
+        # slot-to-coeff starts below coeff-to-slot and EvalMod's 9 levels
+        assert stc[0] == levels[0] - stages - 9
+
+
+@pytest.mark.parametrize("stages,shift", [(2, 0), (6, 0), (2, 1), (3, 2)])
+def test_invalid_stage_counts_are_refused(stages, shift):
+    with pytest.raises(fhe_amd.FheError, match="invalid stage count"):
+        fhe_amd.bootstrap_key_layout(16, 4, stages, shift)

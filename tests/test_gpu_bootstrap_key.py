@@ -222,6 +222,38 @@ def test_bootstrap_bk_matches_reference(boot):
     out.free()
 
 
+def test_layout_matches_reference_key(boot):
+    """fhe_bootstrap_key_stage_layout / _rotation_layout against the
+    reference's BootstrapKey: every stage's level and deltas, every
+    rotation context's delta and key level."""
+    ref, ctx, keys, k = boot
+    lib = ref.lib
+    layout = fhe_amd.bootstrap_key_layout(16, 4, 3, primes=ref.primes)
+    for name, to_slot in (("coeff_to_slot", 1), ("slot_to_coeff", 0)):
+        for st, mine in enumerate(layout[name]):
+            meta = np.zeros(3, np.int32)
+            inner = np.zeros(256, np.int32)
+            outer = np.zeros(256, np.int32)
+            assert lib.ref_bootstrap_key_stage(
+                k["bk"], to_slot, st, meta.ctypes.data_as(ctypes.c_void_p),
+                inner.ctypes.data_as(ctypes.c_void_p),
+                outer.ctypes.data_as(ctypes.c_void_p)) == 0
+            assert mine["level"] == meta[0], (name, st)
+            assert mine["inner_deltas"] == inner[:meta[1]].tolist()
+            assert mine["outer_deltas"] == outer[:meta[2]].tolist()
+    info = np.zeros(4, np.int32)
+    lib.ref_bootstrap_key_info(k["bk"], info.ctypes.data_as(ctypes.c_void_p))
+    want = {}
+    delta = ctypes.c_int()
+    for i in range(int(info[2])):
+        frk = lib.ref_bootstrap_key_rotation(k["bk"], i, ctypes.byref(delta))
+        s = np.zeros(4, np.int64)
+        lib.ref_key_shape(frk, 1, ptr(s))
+        want[delta.value] = int(s[0])
+    assert dict(layout["rotations"]) == want
+    assert len(layout["rotations"]) == len(want)
+
+
 def test_bootstrap_key_refuses_bad_parts(boot):
     ref, ctx, keys, k = boot
     # a stage whose inner delta has no key
