@@ -90,6 +90,87 @@ __global__ __launch_bounds__(kBlock) void encode_fft_stage_kernel(
   yc[j + half] = {__dsub_rn(left.re, right.re), __dsub_rn(left.im, right.im)};
 }
 
+// The same radix-2 stages fused: every butterfly is the stage kernel's (same
+// twiddle index, the same rounded products and sums), only the passes over
+// HBM change.  Stages 0 .. K-1 pair elements inside aligned blocks of 2^K,
+// so fft_low_kernel runs them on one such block in LDS; stages K .. L-1 pair
+// elements of equal index mod 2^K, so fft_high_kernel runs them on kFftCols
+// such residues x 2^(L-K) rows in LDS.  Two passes instead of L.
+constexpr int kFftLowBits = 11;   // 2^11 complex = 32 KiB of LDS
+constexpr int kFftCols = 64;      // residues per block of the high pass
+constexpr int kFftMaxHighRows = 64;
+
+__device__ __forceinline__ void fft_butterfly(Cplx *sh, int a, int b, int t,
+                                              const double *tw_cos,
+                                              const double *tw_sin) {
+  const Cplx left = sh[a], r = sh[b];
+  const Cplx right = cmul(tw_cos[t], tw_sin[t], r.re, r.im);
+  sh[a] = {__dadd_rn(left.re, right.re), __dadd_rn(left.im, right.im)};
+  sh[b] = {__dsub_rn(left.re, right.re), __dsub_rn(left.im, right.im)};
+}
+
+// stages 0 .. bits-1 of one aligned block of 2^bits elements per block
+__global__ __launch_bounds__(kBlock) void fft_low_kernel(
+    Cplx *y, int log_codec, int bits, const double *tw_cos,
+    const double *tw_sin) {
+  __shared__ Cplx sh[1 << kFftLowBits];
+  const int size = 1 << bits;
+  Cplx *blk = y + (int64_t(blockIdx.x) << bits);
+  for (int e = threadIdx.x; e < size; e += kBlock)
+    sh[e] = blk[e];
+  __syncthreads();
+  for (int s = 0; s < bits; s++) {
+    const int half = 1 << s;
+    for (int bf = threadIdx.x; bf < size / 2; bf += kBlock) {
+      // the block starts at a multiple of 2^bits: the index inside the
+      // block has the message index's low bits
+      const int k = bf & (half - 1);
+      const int j = ((bf >> s) << (s + 1)) + k;
+      fft_butterfly(sh, j, j + half, k << (log_codec - 1 - s), tw_cos,
+                    tw_sin);
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < size; e += kBlock)
+    blk[e] = sh[e];
+}
+
+// stages low_bits .. log_codec-1 of kFftCols consecutive residues mod
+// 2^low_bits (all their 2^(log_codec - low_bits) rows) per block
+__global__ __launch_bounds__(kBlock) void fft_high_kernel(
+    Cplx *y, int log_codec, int low_bits, const double *tw_cos,
+    const double *tw_sin) {
+  __shared__ Cplx sh[kFftMaxHighRows * kFftCols];
+  const int rows_log = log_codec - low_bits;
+  const int rows = 1 << rows_log;
+  const int64_t blocks_per_msg = (int64_t(1) << low_bits) / kFftCols;
+  const int64_t c = blockIdx.x / blocks_per_msg;
+  const int r0 = int(blockIdx.x % blocks_per_msg) * kFftCols;
+  Cplx *msg = y + (c << log_codec);
+  for (int e = threadIdx.x; e < rows * kFftCols; e += kBlock) {
+    const int row = e / kFftCols, col = e % kFftCols;
+    sh[e] = msg[(int64_t(row) << low_bits) + r0 + col];
+  }
+  __syncthreads();
+  for (int s = low_bits; s < log_codec; s++) {
+    const int rh = 1 << (s - low_bits); // rows between the pair
+    for (int bf = threadIdx.x; bf < (rows / 2) * kFftCols; bf += kBlock) {
+      const int col = bf % kFftCols, rb = bf / kFftCols;
+      const int kr = rb & (rh - 1);
+      const int jr = ((rb >> (s - low_bits)) << (s - low_bits + 1)) + kr;
+      // j mod 2^s = kr * 2^low_bits + (r0 + col)
+      const int k = (kr << low_bits) + r0 + col;
+      fft_butterfly(sh, jr * kFftCols + col, (jr + rh) * kFftCols + col,
+                    k << (log_codec - 1 - s), tw_cos, tw_sin);
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < rows * kFftCols; e += kBlock) {
+    const int row = e / kFftCols, col = e % kFftCols;
+    msg[(int64_t(row) << low_bits) + r0 + col] = sh[e];
+  }
+}
+
 // normalize (src/operation/fft.cpp:50-55) and apply_twister
 // (src/ckks/encode.cpp:264-275): out[i] = (re_i / codec * cos_i - im_i /
 // codec * sin_i) * scale; the per-ciphertext max |out| as ordered bits.
@@ -116,6 +197,34 @@ __global__ __launch_bounds__(kBlock) void encode_twist_kernel(
   }
   if ((threadIdx.x & 63) == 0)
     atomicMax(max_bits + (idx >> log_codec), m);
+}
+
+// All log_codec stages on `count` messages [count][2^log_codec] in place:
+// the fused passes when their shapes fit, the per-stage kernel otherwise.
+void launch_fft(Cplx *y, int count, int log_codec, const double *tw_cos,
+                const double *tw_sin, hipStream_t stream) {
+  const int low = log_codec < kFftLowBits ? log_codec : kFftLowBits;
+  const int high_rows = 1 << (log_codec - low);
+  const bool fused = log_codec >= 8 && high_rows <= kFftMaxHighRows &&
+                     (log_codec == low || (1 << low) % kFftCols == 0);
+  if (!fused) {
+    const int64_t half_codec = int64_t(1) << (log_codec - 1);
+    for (int s = 0; s < log_codec; s++)
+      hipLaunchKernelGGL(encode_fft_stage_kernel,
+                         dim3(grid_for(int64_t(count) * half_codec)),
+                         dim3(kBlock), 0, stream, y, count, log_codec, s,
+                         tw_cos, tw_sin);
+    return;
+  }
+  hipLaunchKernelGGL(fft_low_kernel,
+                     dim3(unsigned(int64_t(count) << (log_codec - low))),
+                     dim3(kBlock), 0, stream, y, log_codec, low, tw_cos,
+                     tw_sin);
+  if (log_codec > low)
+    hipLaunchKernelGGL(fft_high_kernel,
+                       dim3(unsigned((int64_t(count) << low) / kFftCols)),
+                       dim3(kBlock), 0, stream, y, log_codec, low, tw_cos,
+                       tw_sin);
 }
 
 struct ConvertArgs {
@@ -193,11 +302,7 @@ void launch_encode(const DeviceTables &t, const EncodeDevice &e,
                      dim3(grid_for(int64_t(count) * slots)), dim3(kBlock), 0,
                      stream, x, y, max_bits, count, slots, e.log_codec, e.pre,
                      complex_input);
-  for (int s = 0; s < e.log_codec; s++)
-    hipLaunchKernelGGL(encode_fft_stage_kernel,
-                       dim3(grid_for(int64_t(count) * (codec / 2))),
-                       dim3(kBlock), 0, stream, y, count, e.log_codec, s,
-                       e.fft_cos, e.fft_sin);
+  launch_fft(y, count, e.log_codec, e.fft_cos, e.fft_sin, stream);
   // the twist kernel's grid covers count * codec exactly (codec is a
   // multiple of kBlock): every thread of a wave takes part in its shuffles
   hipLaunchKernelGGL(encode_twist_kernel,
@@ -460,11 +565,7 @@ void launch_decode(const DeviceTables &t, const DecodeDevice &d,
                        0, stream, in, v, count, limbs, moduli_id, crt, t);
   hipLaunchKernelGGL(decode_skew_kernel, grid, dim3(kBlock), 0, stream, v, y,
                      count, d.log_codec, d.skew_cos, d.skew_sin);
-  for (int s = 0; s < d.log_codec; s++)
-    hipLaunchKernelGGL(encode_fft_stage_kernel,
-                       dim3(grid_for(int64_t(count) * (codec / 2))),
-                       dim3(kBlock), 0, stream, y, count, d.log_codec, s,
-                       d.ifft_cos, d.ifft_sin);
+  launch_fft(y, count, d.log_codec, d.ifft_cos, d.ifft_sin, stream);
   hipLaunchKernelGGL(decode_gather_kernel, grid, dim3(kBlock), 0, stream, y,
                      out, count, d.log_codec, d.post, scale_inverse,
                      complex_output);

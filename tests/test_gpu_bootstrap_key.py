@@ -222,6 +222,50 @@ def test_bootstrap_bk_matches_reference(boot):
     out.free()
 
 
+@pytest.mark.timeout(600)
+def test_bootstrap_bk_graph_replays_equal_eager(boot):
+    """fhe_bootstrap_bk recorded into a HIP graph after an eager call and
+    replayed three times, with eager work between the replays: every replay
+    gives the eager call's limbs."""
+    ref, ctx, keys, k = boot
+    lib, e = ref.lib, ref.e
+    x = np.random.default_rng(17).uniform(-1, 1, ref.slot_count)
+    ct = lib.ref_encrypt(e, ptr(x), ref.slot_count, k["pk"], 2)
+    level, flags = form(ref, ct)
+    d = ctx.to_device(ref.get(ct)[None])
+    out = ctx.array((1, 2, ref.chain_count - 1, ctx.n))
+    other = ctx.array(out.shape)
+    relin, relin_mid = keys["relin"]
+    conj, conj_mid = keys["conj"]
+
+    def run(dst):
+        return ctx.bootstrap_bk(d, level, flags, relin, relin_mid, conj,
+                                conj_mid, keys["bk"], dst)
+
+    run(out)
+    ctx.synchronize()
+    want = out.download()
+    ctx.graph_begin()
+    try:
+        run(out)
+    finally:
+        graph = ctx.graph_end()
+    try:
+        for rep in range(3):
+            out.upload(np.zeros(out.shape, np.int64))
+            run(other)
+            ctx.synchronize()
+            ctx.graph_launch(graph)
+            ctx.synchronize()
+            assert np.array_equal(out.download(), want), rep
+        assert np.array_equal(other.download(), want)
+    finally:
+        ctx.graph_destroy(graph)
+    lib.ref_free_ct(ct)
+    for a in (d, out, other):
+        a.free()
+
+
 def test_layout_matches_reference_key(boot):
     """fhe_bootstrap_key_stage_layout / _rotation_layout against the
     reference's BootstrapKey: every stage's level and deltas, every
