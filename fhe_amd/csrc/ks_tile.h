@@ -186,7 +186,7 @@ __device__ __forceinline__ void ks_item_tile(const Addr &a,
     }
     if (!own) {
       r16_forward_high<A>(v, f);
-      r16_exchange(v, lds, c, r, true);
+      r16_exchange<true>(v, lds, c, r, true);
       r16_forward_low<A>(v, f);
       r16_forward_settle<A>(v, f);
     }
@@ -258,13 +258,13 @@ __device__ __forceinline__ void ks_item_tile(const Addr &a,
   int64_t *d0 = a.acc(i, off);
   int64_t *d1 = d0 + int64_t(a.ext_limbs) * n;
   r16_inverse_low<A>(acc0, g);
-  r16_exchange(acc0, lds, c, r, false);
+  r16_exchange<true>(acc0, lds, c, r, false);
   r16_inverse_high<A>(acc0, g);
 #pragma unroll
   for (int m = 0; m < 16; m++)
     d0[r + 16 * m] = acc0[m];
   r16_inverse_low<A>(acc1, g);
-  r16_exchange(acc1, lds, c, r, false);
+  r16_exchange<true>(acc1, lds, c, r, false);
   r16_inverse_high<A>(acc1, g);
 #pragma unroll
   for (int m = 0; m < 16; m++)

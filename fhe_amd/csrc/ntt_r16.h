@@ -225,15 +225,32 @@ __device__ __forceinline__ int r16_lds(int c, int j) {
   return c * kR16Row + j + (j >> 4);
 }
 
+// Orders one wavefront's LDS accesses: a wave's LDS instructions execute in
+// issue order, so it is enough that the compiler moves no access across.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Re-own the thread's 16 points: ROWS -> COLS (to_cols) or COLS -> ROWS.
 // Starts with a barrier so the previous use of the buffer has finished.
+// WAVE: the 16 threads sharing c are 16 consecutive threads (c = tid >> 4),
+// hence in one wavefront, and a wavefront barrier replaces the block's.
+template <bool WAVE = false>
 __device__ __forceinline__ void r16_exchange(int64_t (&v)[16], int64_t *lds,
                                              int c, int r, bool to_cols) {
-  __syncthreads();
+  if (WAVE)
+    wave_lds_sync();
+  else
+    __syncthreads();
 #pragma unroll
   for (int m = 0; m < 16; m++)
     lds[r16_lds(c, to_cols ? r + 16 * m : 16 * r + m)] = v[m];
-  __syncthreads();
+  if (WAVE)
+    wave_lds_sync();
+  else
+    __syncthreads();
 #pragma unroll
   for (int m = 0; m < 16; m++)
     v[m] = lds[r16_lds(c, to_cols ? 16 * r + m : r + 16 * m)];
@@ -293,19 +310,19 @@ __device__ __forceinline__ void r16_limb_pass(const int64_t *src_limb,
       v[m] = load_product<A>(v[m], cst, s.q, s.k);
   }
   if (INV && !COLS)
-    r16_exchange(v, lds, c, r, true);
+    r16_exchange<!COLS>(v, lds, c, r, true);
 
   if (!INV) {
     r16_forward_high<A>(v, s);
-    r16_exchange(v, lds, c, r, true);
+    r16_exchange<!COLS>(v, lds, c, r, true);
     r16_forward_low<A>(v, s);
     if (last)
       r16_forward_settle<A>(v, s);
     if (!COLS)
-      r16_exchange(v, lds, c, r, false); // coalesced contiguous store
+      r16_exchange<!COLS>(v, lds, c, r, false); // coalesced contiguous store
   } else {
     r16_inverse_low<A>(v, s);
-    r16_exchange(v, lds, c, r, false);
+    r16_exchange<!COLS>(v, lds, c, r, false);
     r16_inverse_high<A>(v, s);
     if (post != kInttRaw) {
       const int64_t ninv = t.ninv[p];
