@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "fhe_bootstrap_key_create", "fhe_bootstrap_key_destroy",
     "fhe_coeff_to_slot_bk", "fhe_slot_to_coeff_bk", "fhe_bootstrap_bk",
     "fhe_bootstrap_key_stage_layout", "fhe_bootstrap_key_rotation_layout",
+    "fhe_bootstrap_key_encode_stage",
 )
 
 # fhe_wire_* record kinds (FHE_WIRE_* in include/fhe_amd.h)
@@ -390,6 +391,8 @@ def library():
         lib.fhe_bootstrap_key_rotation_layout.argtypes = [
             ctypes.POINTER(Params), i32, i32, vp, vp, i32,
             ctypes.POINTER(i32)]
+        lib.fhe_bootstrap_key_encode_stage.argtypes = [
+            vp, i32, i32, i32, i32, vp, ctypes.c_int64, ctypes.POINTER(i32)]
         lib.fhe_bootstrap_key_create.argtypes = [
             vp, ctypes.POINTER(BootstrapKeyParts), ctypes.POINTER(vp)]
         lib.fhe_bootstrap_key_destroy.argtypes = [vp]
@@ -763,6 +766,27 @@ class Context:
             self.handle, ctypes.byref(parts), ctypes.byref(h)))
         h._parts = parts
         return h
+
+    def bootstrap_key_stage(self, stage_count, inner_loop_shift, to_slot,
+                            stage):
+        """fhe_bootstrap_key_encode_stage: the stage's encoded diagonals as
+        one device array [count][limbs + alpha][N] (outer-major, without
+        (0, 0))."""
+        count = ctypes.c_int()
+        _check(library().fhe_bootstrap_key_encode_stage(
+            self.handle, stage_count, inner_loop_shift, to_slot, stage, None,
+            0, ctypes.byref(count)))
+        level = bootstrap_key_layout(self.log_n, self.alpha, stage_count,
+                                     inner_loop_shift,
+                                     primes=self.primes)[
+            "coeff_to_slot" if to_slot else "slot_to_coeff"][stage]["level"]
+        mid = self.info["full_level"] - level + 1
+        ext = self.info["prime_count"] - mid
+        arr = self.array((count.value, ext, self.n))
+        _check(library().fhe_bootstrap_key_encode_stage(
+            self.handle, stage_count, inner_loop_shift, to_slot, stage,
+            arr.ptr, int(np.prod(arr.shape)), ctypes.byref(count)))
+        return arr
 
     def bootstrap_key_destroy(self, bk):
         library().fhe_bootstrap_key_destroy(bk)

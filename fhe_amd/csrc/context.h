@@ -827,6 +827,11 @@ struct fhe_context {
   }
 };
 
+// Device encoding (context.hip): `count` device messages at `level` into
+// out [count][limbs (+ alpha with include_special)][N].
+void encode_texts(fhe_context *ctx, const double *values, bool complex_input,
+                  int count, int level, bool include_special, int64_t *out);
+
 // The parameters a fhe_params describes (context.hip); twiddles = false
 // skips the NTT tables (host-only queries).
 fhe::CkksParams resolve_params(const fhe_params *p, bool twiddles = true);

@@ -40,6 +40,38 @@ fhe::CkksParams resolve_params(const fhe_params *p, bool twiddles) {
   return params;
 }
 
+// CKKSCore::encode_polynomial (src/ckks/core_encode.cpp:36-150) of `count`
+// device messages at `level`, over the level's chain limbs and, with
+// include_special, the special primes after them.
+void encode_texts(fhe_context *ctx, const double *values, bool complex_input,
+                  int count, int level, bool include_special, int64_t *out) {
+  require(count > 0, "Encode: empty batch");
+  const auto &P = ctx->params;
+  require(level >= 0 && level <= P.max_level,
+          "Encode: the target level cannot be reached with the current "
+          "engine parameters.");
+  const auto e = P.encode_level(level);
+  const int mid = P.moduli_id_of_level(level);
+  const int limbs = P.limbs_at(mid) + (include_special ? P.alpha : 0);
+  // the top level of a one-more-level chain leaves limb 0 zero and encodes
+  // onto the primes of the level below (src/ckks/core_encode.cpp:64-78)
+  const int first = e.top_of_one_more_level ? 1 : 0;
+  const int enc_mid =
+      e.top_of_one_more_level ? P.moduli_id_of_level(level - 1) : mid;
+  require(limbs - first <= fhe::kMaxEwLimbs, "Encode: too many limbs");
+  require(int(e.r_mult_scale_after_tile.size()) >= limbs - first,
+          "Encode: constants missing");
+  require(P.n % fhe::kBlock == 0, "Encode: N below the kernels' block size");
+  const auto &tables = ctx->encode_tables();
+  void *work = ctx->aux_scratch(fhe::encode_work_bytes(tables, count));
+  ctx->stage("encode", [&] {
+    fhe::launch_encode(ctx->tables, tables, values, complex_input ? 1 : 0,
+                       count, e.scale_before_tile, limbs, first, enc_mid,
+                       e.r_mult_scale_after_tile.data(), out, work,
+                       ctx->stream);
+  });
+}
+
 extern "C" {
 
 const char *fhe_last_error(void) { return g_last_error.c_str(); }
@@ -716,29 +748,7 @@ int fhe_encode(fhe_context *ctx, const double *values, int complex_input,
                int count, int level, int64_t *out) {
   return guarded([&] {
     require(ctx && values && out, "null argument");
-    require(count > 0, "Encode: empty batch");
-    const auto &P = ctx->params;
-    require(level >= 0 && level <= P.max_level,
-            "Encode: the target level cannot be reached with the current "
-            "engine parameters.");
-    const auto e = P.encode_level(level);
-    const int mid = P.moduli_id_of_level(level);
-    const int limbs = P.limbs_at(mid);
-    // the top level of a one-more-level chain leaves limb 0 zero and encodes
-    // onto the primes of the level below (src/ckks/core_encode.cpp:64-78)
-    const int first = e.top_of_one_more_level ? 1 : 0;
-    const int enc_mid =
-        e.top_of_one_more_level ? P.moduli_id_of_level(level - 1) : mid;
-    require(limbs - first <= fhe::kMaxEwLimbs, "Encode: too many limbs");
-    require(P.n % fhe::kBlock == 0, "Encode: N below the kernels' block size");
-    const auto &tables = ctx->encode_tables();
-    void *work = ctx->aux_scratch(fhe::encode_work_bytes(tables, count));
-    ctx->stage("encode", [&] {
-      fhe::launch_encode(ctx->tables, tables, values, complex_input ? 1 : 0,
-                         count, e.scale_before_tile, limbs, first, enc_mid,
-                         e.r_mult_scale_after_tile.data(), out, work,
-                         ctx->stream);
-    });
+    encode_texts(ctx, values, complex_input != 0, count, level, false, out);
     ctx->check_launch("Encode");
   });
 }

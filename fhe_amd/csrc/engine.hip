@@ -683,6 +683,71 @@ int fhe_bootstrap_key_rotation_layout(const fhe_params *params,
   });
 }
 
+int fhe_bootstrap_key_encode_stage(fhe_context *ctx, int stage_count,
+                                   int inner_loop_shift, int to_slot,
+                                   int stage, int64_t *out,
+                                   int64_t capacity_words, int *count) {
+  return guarded([&] {
+    require(ctx && count, "null argument");
+    const auto &P = ctx->params;
+    const int slots = int(P.slot_count());
+    const int log_slots = log2_int(slots);
+    require(bootstrap_stage_count_valid(log_slots, inner_loop_shift,
+                                        stage_count),
+            "Bootstrap key stage: invalid stage count.");
+    require(stage >= 0 && stage < stage_count,
+            "Bootstrap key stage: stage out of range");
+    // make_bootstrap_stage (src/ckks/engine_bootstrap.cpp:8-69)
+    const StageLayout s = stage_layout(P, stage_count, inner_loop_shift,
+                                       to_slot != 0, stage);
+    require(s.level >= 1, "Bootstrap key stage: too few levels");
+    const int m = P.moduli_id_of_level(s.level) + 1; // after the rescale
+    const int ext = P.prime_count - m;
+    const int messages = s.outer * s.inner - 1;
+    *count = messages;
+    if (!out)
+      return;
+    const int64_t N = ctx->n();
+    require(capacity_words >= int64_t(messages) * ext * N,
+            "Bootstrap key stage: capacity too small");
+    ctx->require_not_capturing("Bootstrap key stage");
+    const auto &tw = cached_twiddles(ctx, slots, stage_count, to_slot != 0);
+    const Vectors &st = tw[stage];
+    const int twiddle_count = int(st.size());
+    require((twiddle_count + 1) / s.inner == s.outer,
+            "Bootstrap key stage: loop counts differ from the twiddles");
+    std::vector<Complex> factors;
+    factors.reserve(size_t(messages) * slots);
+    for (int o = 0; o < s.outer; o++) {
+      const int nidx = o - s.outer / 2;
+      for (int i = (o == 0 ? 1 : 0); i < s.inner; i++) {
+        const int ti = s.inner * nidx + i;
+        const auto f = rotate_vector(st[positive_modulo(ti, twiddle_count)],
+                                     -s.outer_deltas[o]);
+        factors.insert(factors.end(), f.begin(), f.end());
+      }
+    }
+    // encode at the stage's level with the special primes, rescale, NTT
+    TextEngine e(ctx, nullptr, 0);
+    const size_t bytes = factors.size() * sizeof(Complex);
+    const auto dv = e.take(bytes);
+    hip_check(hipMemcpyAsync(dv.get(), factors.data(), bytes,
+                             hipMemcpyHostToDevice, ctx->stream),
+              "message copy");
+    const auto enc = e.take(size_t(messages) * (ext + 1) * N * 8);
+    int64_t *encoded = static_cast<int64_t *>(enc.get());
+    encode_texts(ctx, static_cast<const double *>(dv.get()), true, messages,
+                 s.level, true, encoded);
+    ctx->stage("bootstrap_key_stage", [&] {
+      fhe::launch_rescale(ctx->tables, encoded, out, messages, ext + 1, m - 1,
+                          ctx->stream);
+      fhe::ntt_forward(ctx->tables, out, out, int64_t(messages) * ext, ext, m,
+                       ctx->tables.r2, ctx->stream);
+    });
+    ctx->check_launch("Bootstrap key stage");
+  });
+}
+
 int fhe_coeff_to_slot(fhe_context *ctx, int count, const int64_t *in,
                       int level, int flags, const fhe_bootstrap_keys *keys,
                       int stage_count, int64_t *real, int64_t *imag,

@@ -308,6 +308,18 @@ int fhe_bootstrap_key_rotation_layout(const fhe_params *params,
                                       int *deltas, int *moduli_ids,
                                       int capacity, int *count);
 
+/* make_bootstrap_stage (src/ckks/engine_bootstrap.cpp:8-69) on the device:
+ * the encoded diagonals of one stage of a BootstrapKey with the given stage
+ * count and inner-loop shift - each rotated twiddle vector encoded at the
+ * stage's level over the chain limbs and the special primes, rescaled and
+ * brought to NTT form - into out [count][limbs + alpha][N] in (outer,
+ * inner) order without (0, 0), at the level's rescaled moduli_id.  With out
+ * NULL only *count is set.  The limbs equal the reference key's. */
+int fhe_bootstrap_key_encode_stage(fhe_context *ctx, int stage_count,
+                                   int inner_loop_shift, int to_slot,
+                                   int stage, int64_t *out,
+                                   int64_t capacity_words, int *count);
+
 typedef struct fhe_bootstrap_key_parts {
   int stage_count;
   const fhe_plain_matrix *coeff_to_slot; /* stage_count, or NULL */

@@ -66,6 +66,7 @@ def boot(gpu_lib):
     dsparse, sparse_mid = dev(sbk, 5)
     ddense, dense_mid = dev(sbk, 6)
     stages = {}
+    ref_diagonals = {}
     for to_slot in (1, 0):
         mats = []
         for st in range(stage_count):
@@ -85,6 +86,7 @@ def boot(gpu_lib):
                         arr = ctx.to_device(ref.pt_get(p)[0])
                         arrays.append(arr)
                         diagonals[(o, i)] = arr
+            ref_diagonals[(to_slot, st)] = diagonals
             mats.append(fhe_amd.PlainMatrix.make(level, inner[:ni],
                                                  outer[:no], diagonals))
         stages[to_slot] = mats
@@ -94,7 +96,7 @@ def boot(gpu_lib):
     dbk = ctx.bootstrap_key(parts)
     keys = dict(relin=(drelin, relin_mid), conj=(dconj, conj_mid), bk=dbk)
     yield ref, ctx, keys, dict(sk=sk, pk=pk, rlk=rlk, conj=conj, bk=bk,
-                               sbk=sbk)
+                               sbk=sbk, diagonals=ref_diagonals)
     ctx.bootstrap_key_destroy(dbk)
     for a in arrays:
         a.free()
@@ -264,6 +266,24 @@ def test_bootstrap_bk_graph_replays_equal_eager(boot):
     lib.ref_free_ct(ct)
     for a in (d, out, other):
         a.free()
+
+
+@pytest.mark.timeout(600)
+def test_device_encoded_stages_equal_reference_key(boot):
+    """fhe_bootstrap_key_encode_stage (make_bootstrap_stage on the device:
+    encode with the special primes, rescale, NTT) gives every stage's
+    diagonals equal to the reference key's."""
+    ref, ctx, keys, k = boot
+    for (to_slot, st), diagonals in sorted(k["diagonals"].items()):
+        print(f"stage to_slot={to_slot} {st}", flush=True)
+        got = ctx.bootstrap_key_stage(3, 0, to_slot, st)
+        host = got.download()
+        order = sorted(diagonals)
+        assert host.shape[0] == len(order)
+        for idx, key in enumerate(order):
+            want = diagonals[key].download()
+            assert np.array_equal(host[idx], want), (to_slot, st, key)
+        got.free()
 
 
 def test_layout_matches_reference_key(boot):
