@@ -1,6 +1,7 @@
 """The BootstrapKey layout (fhe_bootstrap_key_stage_layout /
 fhe_bootstrap_key_rotation_layout, host only) against the reference's own
-BootstrapKey on the N = 2^16 bootstrap preset (3 stages, size "medium";
+BootstrapKey on the N = 2^16 bootstrap preset (3 stages, sizes "medium" and
+"large";
 tests/golden/bootstrap_key_layout.json, written by
 tests/golden/make_bootstrap_key_layout.py from the reference build), and
 the stage counts the reference refuses
@@ -16,9 +17,10 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden",
                       "bootstrap_key_layout.json")
 
 
-def test_layout_matches_reference_key():
+@pytest.mark.parametrize("size", [0, 1])
+def test_layout_matches_reference_key(size):
     with open(GOLDEN) as f:
-        want = json.load(f)
+        want = json.load(f)[size]
     got = fhe_amd.bootstrap_key_layout(want["log_n"],
                                        want["special_prime_count"],
                                        want["stage_count"],
