@@ -24,16 +24,18 @@ pytestmark = [pytest.mark.gpu,
                                  reason="oracle/_ref not built")]
 
 
-@pytest.fixture(scope="module")
-def boot(gpu_lib):
+def build_boot(size):
+    """The reference engine, its keys and BootstrapKey of `size`, and the
+    device copies with the device key (a generator for a fixture)."""
     ref = Reference(16, 4, bootstrap=1, threads=16)
     lib, e = ref.lib, ref.e
     sk = lib.ref_secret_key(e)
     pk = lib.ref_public_key(e, sk)
     rlk = lib.ref_relin_key(e, sk)
     conj = lib.ref_conjugation_key(e, sk)
-    print("building the reference's BootstrapKey", flush=True)
-    bk = lib.ref_bootstrap_key(e, sk, 3, b"medium")
+    print(f"building the reference's BootstrapKey ({size.decode()})",
+          flush=True)
+    bk = lib.ref_bootstrap_key(e, sk, 3, size)
     assert bk, lib.ref_last_error()
     ctx = fhe_amd.Context(16, 4, scale_bits=54, small_prime_bits=54,
                           primes=ref.primes)
@@ -107,6 +109,11 @@ def boot(gpu_lib):
     lib.ref_free_public_key(pk)
     lib.ref_free_secret_key(sk)
     ref.close()
+
+
+@pytest.fixture(scope="module")
+def boot(gpu_lib):
+    yield from build_boot(b"medium")
 
 
 def form(ref, h):
