@@ -339,7 +339,7 @@ def bootstrap_bench(ev, steps, graph=True):
             "timing": "wall clock, host work included"}
 
 
-def bootstrap_bk_bench(steps, stages=3, graph=True):
+def bootstrap_bk_bench(steps, stages=3, graph=True, shift=0):
     """Bootstraps with a BootstrapKey (fhe_bootstrap_bk) on the N = 2^16
     bootstrap preset: the key's layout (stage levels, loop counts, deltas,
     38 rotation keys at their creation levels) from
@@ -348,7 +348,7 @@ def bootstrap_bk_bench(steps, stages=3, graph=True):
     real shapes (the work does not depend on their values)."""
     import fhe_amd
     rng = np.random.Generator(np.random.PCG64(12))
-    layout = fhe_amd.bootstrap_key_layout(LOG_N, ALPHA, stages)
+    layout = fhe_amd.bootstrap_key_layout(LOG_N, ALPHA, stages, shift)
     with fhe_amd.Context(LOG_N, ALPHA, scale_bits=54,
                          small_prime_bits=54) as ctx:
         full = ctx.info["full_level"]
@@ -384,7 +384,7 @@ def bootstrap_bk_bench(steps, stages=3, graph=True):
         for name, to_slot in (("coeff_to_slot", 1), ("slot_to_coeff", 0)):
             mats[name] = []
             for si, st in enumerate(layout[name]):
-                enc = ctx.bootstrap_key_stage(stages, 0, to_slot, si)
+                enc = ctx.bootstrap_key_stage(stages, shift, to_slot, si)
                 arrays.append(enc)
                 inner, outer = len(st["inner_deltas"]), len(st["outer_deltas"])
                 step = enc.shape[1] * enc.shape[2] * 8
@@ -442,9 +442,13 @@ def bootstrap_bk_bench(steps, stages=3, graph=True):
             ctx.bootstrap_key_destroy(bk)
             for a in arrays:
                 a.free()
-    return {"workload": f"bootstrap with a BootstrapKey (size medium, "
-                        f"{stages} CtS / StC stages of 8 x 8 diagonals, "
-                        f"{len(layout['rotations'])} fixed rotation keys), "
+    inner = len(layout["coeff_to_slot"][0]["inner_deltas"])
+    outer = len(layout["coeff_to_slot"][0]["outer_deltas"])
+    size = "large" if shift else "medium"
+    return {"workload": f"bootstrap with a BootstrapKey (size {size}, "
+                        f"{stages} CtS / StC stages of {inner} x {outer} "
+                        f"diagonals, {len(layout['rotations'])} fixed "
+                        "rotation keys), "
                         "N=2^16 bootstrap preset, 2^15 slots, ciphertexts "
                         "from level 2, one per call and 8 per batched call; "
                         "the diagonals encoded on the device "
@@ -935,6 +939,8 @@ def main():
         return
     if args.only_bootstrap_key:
         print(json.dumps(bootstrap_bk_bench(3, graph=not args.no_graph)))
+        print(json.dumps(bootstrap_bk_bench(3, graph=not args.no_graph,
+                                            shift=1)))
         return
     if args.only_uneven:
         print(json.dumps(uneven_bench(ev, 10)))
@@ -983,6 +989,8 @@ def main():
     boot = None if args.no_bootstrap else bootstrap_bench(ev, 3, graph)
     boot_bk = (None if args.no_bootstrap
                else bootstrap_bk_bench(3, graph=graph))
+    boot_bk_large = (None if args.no_bootstrap
+                     else bootstrap_bk_bench(3, graph=graph, shift=1))
     uneven = None if args.no_uneven else uneven_bench(ev, 10)
     served = None if args.no_serve else serve_bench()
     value = args.batch / (step_ms / 1e3)
@@ -1016,6 +1024,7 @@ def main():
         "encode_decode": codec,
         "bootstrap": boot,
         "bootstrap_bootstrap_key": boot_bk,
+        "bootstrap_bootstrap_key_large": boot_bk_large,
         "uneven_key_switch": uneven,
         "serving": served,
     }
