@@ -288,7 +288,89 @@ __global__ __launch_bounds__(kBlock) void scalar_rescale_kernel(
   }
 }
 
+// weighted_sum (src/ckks/engine_advanced.cpp:228-269) of texts at one level
+// and form in one pass: term t is multiply(basis, weight) - the scalar
+// product with its multiplier constants and the rescale of
+// scalar_rescale_kernel - and the terms are summed in order with add's
+// guard(guard(a + b, 2q), q), so every limb equals the sequence of calls.
+struct WeightedSumArgs {
+  const int64_t *in[kWsMaxTerms]; // each [polys][limbs][N]
+  int64_t *out;                   // [polys][limbs - 1][N]
+  int terms;
+  int polys;
+  int limbs;                      // source limbs
+  int moduli_id;
+  int64_t c[kWsMaxTerms][kWsMaxLimbs];
+};
+
+__global__ __launch_bounds__(kBlock) void weighted_sum_kernel(
+    WeightedSumArgs a, DeviceTables t) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= int64_t(a.polys) * n)
+    return;
+  const int64_t poly = idx >> t.log_n;
+  const int64_t coef = idx & (n - 1);
+  const int64_t src = poly * a.limbs * n + coef;
+  int64_t *o = a.out + poly * (a.limbs - 1) * n + coef;
+  const uint64_t q0 = t.q[a.moduli_id], k0 = t.k[a.moduli_id];
+  int64_t y0[kWsMaxTerms];
+  int rounder[kWsMaxTerms];
+#pragma unroll
+  for (int x = 0; x < kWsMaxTerms; x++) {
+    y0[x] = 0;
+    rounder[x] = 0;
+    if (x < a.terms) {
+      y0[x] = guard(mont_mult(a.in[x][src], a.c[x][0], q0, k0), int64_t(q0));
+      rounder[x] = uint64_t(y0[x]) > t.qh[a.moduli_id] ? 1 : 0;
+    }
+  }
+  for (int i = 1; i < a.limbs; i++) {
+    const int p = a.moduli_id + i;
+    const uint64_t q = t.q[p], k = t.k[p];
+    const int64_t q2 = int64_t(t.q2[p]), qq = int64_t(q);
+    const int64_t rs = t.rescale[int64_t(a.moduli_id) * t.prime_count + p];
+    int64_t acc = 0;
+#pragma unroll
+    for (int x = 0; x < kWsMaxTerms; x++) {
+      if (x < a.terms) {
+        int64_t v = guard(mont_mult(a.in[x][src + int64_t(i) * n], a.c[x][i],
+                                    q, k),
+                          qq) -
+                    y0[x];
+        v = v >= 0 ? v : v + q2;
+        v = guard(v, qq);
+        v = mont_mult(v, rs, q, k);
+        v = guard(guard(v + rounder[x], q2), qq);
+        acc = x == 0 ? v : guard(guard(acc + v, q2), qq);
+      }
+    }
+    o[int64_t(i - 1) * n] = acc;
+  }
+}
+
 } // namespace
+
+void launch_weighted_sum(const DeviceTables &t,
+                         const int64_t *const *inputs,
+                         const int64_t *const *constants, int terms,
+                         int64_t *out, int polys, int limbs, int moduli_id,
+                         hipStream_t stream) {
+  WeightedSumArgs a{};
+  a.out = out;
+  a.terms = terms;
+  a.polys = polys;
+  a.limbs = limbs;
+  a.moduli_id = moduli_id;
+  for (int x = 0; x < terms; x++) {
+    a.in[x] = inputs[x];
+    for (int i = 0; i < limbs; i++)
+      a.c[x][i] = constants[x][i];
+  }
+  const dim3 grid(grid_for(int64_t(polys) << t.log_n));
+  hipLaunchKernelGGL(weighted_sum_kernel, grid, dim3(kBlock), 0, stream, a,
+                     t);
+}
 
 void launch_scalar_rescale(const DeviceTables &t, const int64_t *in,
                            int64_t *out, int polys, int limbs, int moduli_id,

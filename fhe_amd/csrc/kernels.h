@@ -307,6 +307,17 @@ void launch_elementwise(const DeviceTables &t, int op, const int64_t *x1,
 // y = rescale(guard(mont(x, c), q)) per polynomial, dropping `skip` more
 // limbs after the rescale (scalar multiply, level down).  constants: host,
 // `limbs` values; limbs <= kMaxEwLimbs.
+// weighted_sum of up to kWsMaxTerms texts at one level and form (not
+// rescaled, coefficient domain), each with its multiplier constants per
+// limb: out [polys][limbs - 1][N] equals the engine's sequence of
+// multiply(basis, weight) and add calls.
+constexpr int kWsMaxTerms = 8;
+constexpr int kWsMaxLimbs = 32;
+void launch_weighted_sum(const DeviceTables &t,
+                         const int64_t *const *inputs,
+                         const int64_t *const *constants, int terms,
+                         int64_t *out, int polys, int limbs, int moduli_id,
+                         hipStream_t stream);
 void launch_scalar_rescale(const DeviceTables &t, const int64_t *in,
                            int64_t *out, int polys, int limbs, int moduli_id,
                            const int64_t *constants, int skip,

@@ -208,6 +208,51 @@ public:
   Text weighted_sum(const std::vector<Text> &basis,
                     const std::vector<double> &weights) {
     const int degree = int(weights.size()) - 1;
+    // one pass over all terms when they share a level and the
+    // coefficient-domain, not rescaled form: the same limbs as the calls
+    {
+      std::vector<int> used;
+      for (int index = degree; index > 0; index--)
+        if (weights[index] != 0)
+          used.push_back(index - 1);
+      const Text *first = used.empty() ? nullptr : &basis[used[0]];
+      bool fused = first && int(used.size()) <= fhe::kWsMaxTerms &&
+                   first->flags == 0 && first->level >= 1 &&
+                   limbs(first->level, 0) <= fhe::kWsMaxLimbs;
+      for (int b : used)
+        fused = fused && basis[b].level == first->level &&
+                basis[b].flags == first->flags &&
+                basis[b].polys == first->polys &&
+                basis[b].count == first->count;
+      if (fused) {
+        const int level = first->level;
+        std::vector<const int64_t *> inputs;
+        std::vector<std::vector<int64_t>> constants;
+        for (int index = degree; index > 0; index--) {
+          if (weights[index] == 0)
+            continue;
+          inputs.push_back(basis[index - 1].data);
+          constants.push_back(
+              ctx->params.mult_multiplier(weights[index], level));
+        }
+        std::vector<const int64_t *> cptr;
+        for (const auto &c : constants) {
+          require(int(c.size()) == limbs(level, 0),
+                  "Weighted sum: multiplier size");
+          cptr.push_back(c.data());
+        }
+        Text result = alloc(first->polys, level - 1, 0, first->count);
+        ctx->stage("weighted_sum", [&] {
+          fhe::launch_weighted_sum(
+              ctx->tables, inputs.data(), cptr.data(), int(inputs.size()),
+              result.data, first->count * first->polys, limbs(level, 0),
+              ctx->moduli_id_of(level, false), ctx->stream);
+        });
+        if (weights[0] != 0)
+          result = add_double(result, weights[0]);
+        return result;
+      }
+    }
     Text result;
     bool have = false;
     for (int index = degree; index > 0; index--) {
