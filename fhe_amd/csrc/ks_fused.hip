@@ -114,6 +114,16 @@ __device__ __forceinline__ void ks_tile(const KsArgs &a, const DeviceTables &t,
         acc0[m] = add64(mont_mult_u(v[m], k0[m], f.q, f.k), acc0[m]);
         acc1[m] = add64(mont_mult_u(v[m], k1[m], f.q, f.k), acc1[m]);
       }
+    } else if (A == kLazyLarge && a.lazy_acc) {
+      // sums kept below 2q (one conditional subtraction of 2q): the lazy
+      // inverse transform that follows takes inputs below 2q
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        acc0[m] = guard_u(add64(mont_mult_u(v[m], k0[m], f.q, f.k), acc0[m]),
+                          f.q2);
+        acc1[m] = guard_u(add64(mont_mult_u(v[m], k1[m], f.q, f.k), acc1[m]),
+                          f.q2);
+      }
     } else {
 #pragma unroll
       for (int m = 0; m < 16; m++) {

@@ -755,7 +755,8 @@ int fhe_mod_raise(fhe_context *ctx, int count, const int64_t *in,
  *                          input;
  *   "split_variants" 1     one NTT launch per arithmetic variant;
  *   "ks_lazy_accumulate" 1 small-prime key products summed without
- *                          intermediate reductions;
+ *                          intermediate reductions, large-prime sums kept
+ *                          below 2q with one conditional subtraction;
  *   "ks_wide_accumulate" 0 small primes' key products summed as unreduced
  *                          96-bit values with one Montgomery reduction per
  *                          accumulator (fewer VALU instructions, measured
