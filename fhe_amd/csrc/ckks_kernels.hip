@@ -135,10 +135,21 @@ __global__ __launch_bounds__(kBlock) void key_inner_product_kernel(
 
 // The addend of half h of ciphertext c is d + c * d_stride + h * limbs * N
 // for h < d_polys and zero otherwise.
-template <int MAXA, bool RESCALE>
+//
+// The limbs are processed in groups of kMdGroup whose loads are issued
+// together ahead of the group's arithmetic (one thread holds one
+// coefficient, so the loads of a group are what keeps HBM busy).  The
+// pointers are restrict-qualified: the output never overlaps the
+// accumulator or the tables, and where a caller passes the addend's own
+// buffer as the output each element is read before the same thread writes
+// it.
+constexpr int kMdGroup = 8;
+
+template <int MAXA, bool FULL, bool RESCALE>
 __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
-    const int64_t *acc, const int64_t *dbase, int64_t d_stride, int d_polys,
-    int64_t *out, int count, int limbs, int moduli_id, DeviceTables t) {
+    const int64_t *__restrict__ acc, const int64_t *__restrict__ dbase,
+    int64_t d_stride, int d_polys, int64_t *__restrict__ out, int count,
+    int limbs, int moduli_id, DeviceTables t) {
   const int64_t n = int64_t(1) << t.log_n;
   const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (idx >= int64_t(count) * 2 * n)
@@ -148,10 +159,12 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
   const int64_t c = ch >> 1, h = ch & 1;
   const int ext_limbs = limbs + t.alpha;
   const int64_t *a = acc + ch * ext_limbs * n + coef;
-  const int64_t *d =
-      h < d_polys ? dbase + c * d_stride + h * limbs * n + coef : nullptr;
+  // without an addend the loads read the accumulator's limbs and are
+  // discarded, so that every load is unconditional
+  const bool has_d = h < d_polys;
+  const int64_t *d = has_d ? dbase + c * d_stride + h * limbs * n + coef : a;
 
-  ModDown<MAXA> md;
+  ModDown<MAXA, FULL> md;
   md.alpha = t.alpha;
   md.limbs = limbs;
   md.moduli_id = moduli_id;
@@ -159,44 +172,60 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
 
   // relinearize: y_i = reduce(d_i + ks_i) (src/ckks/core.cpp:736-737); a
   // canonical ks_i passes the two guards unchanged when there is no d
-  auto relin = [&](int i) {
+  auto relin = [&](int i, int64_t av, int64_t dv) {
     const int p = moduli_id + i;
-    const int64_t ks = md.limb_lazy(a[int64_t(i) * n], i, t);
-    const int64_t di = d ? d[int64_t(i) * n] : 0;
+    const int64_t ks = md.limb_lazy(av, i, t);
+    const int64_t di = has_d ? dv : 0;
     return guard(guard(di + ks, int64_t(t.q2[p])), int64_t(t.q[p]));
   };
 
-  if (!RESCALE) {
-    int64_t *o = out + ch * limbs * n + coef;
-#pragma unroll 4
-    for (int i = 0; i < limbs; i++)
-      o[int64_t(i) * n] = relin(i);
-    return;
+  // limbs [first, limbs) in groups; the slots of the last group past the
+  // end repeat the last limb (its loads, arithmetic and store, which writes
+  // the same value again), so that a group is one branch-free block
+  const int first = RESCALE ? 1 : 0;
+  int64_t y0 = 0, rounder = 0;
+  if (RESCALE) {
+    // rescale (src/ckks/core_rescale.cpp:5-25): drop limb 0 with rounding
+    y0 = relin(0, a[0], d[0]);
+    rounder = uint64_t(y0) > t.qh[moduli_id] ? 1 : 0;
   }
-  // rescale (src/ckks/core_rescale.cpp:5-25): drop limb 0 with rounding
-  const int64_t y0 = relin(0);
-  const int64_t rounder = uint64_t(y0) > t.qh[moduli_id] ? 1 : 0;
-  int64_t *o = out + ch * (limbs - 1) * n + coef;
-#pragma unroll 4
-  for (int i = 1; i < limbs; i++) {
-    const int p = moduli_id + i;
-    const uint64_t q = t.q[p], k = t.k[p];
-    const int64_t q2 = int64_t(t.q2[p]);
-    int64_t v = relin(i) - y0;
-    v = v >= 0 ? v : v + q2;
-    v = guard(v, int64_t(q));
-    v = mont_mult(v, t.rescale[int64_t(moduli_id) * t.prime_count + p], q, k);
-    v = guard(v + rounder, q2);
-    o[int64_t(i - 1) * n] = guard(v, int64_t(q));
+  int64_t *o = out + ch * (limbs - first) * n + coef;
+  for (int i0 = first; i0 < limbs; i0 += kMdGroup) {
+    int64_t av[kMdGroup], dv[kMdGroup];
+#pragma unroll
+    for (int g = 0; g < kMdGroup; g++) {
+      const int i = min(i0 + g, limbs - 1);
+      av[g] = a[int64_t(i) * n];
+      dv[g] = d[int64_t(i) * n];
+    }
+#pragma unroll
+    for (int g = 0; g < kMdGroup; g++) {
+      const int i = min(i0 + g, limbs - 1);
+      const int64_t y = relin(i, av[g], dv[g]);
+      if (!RESCALE) {
+        o[int64_t(i) * n] = y;
+        continue;
+      }
+      const int p = moduli_id + i;
+      const uint64_t q = t.q[p], k = t.k[p];
+      const int64_t q2 = int64_t(t.q2[p]);
+      int64_t v = y - y0;
+      v = v >= 0 ? v : v + q2;
+      v = guard(v, int64_t(q));
+      v = mont_mult(v, t.rescale[int64_t(moduli_id) * t.prime_count + p], q,
+                    k);
+      v = guard(v + rounder, q2);
+      o[int64_t(i - 1) * n] = guard(v, int64_t(q));
+    }
   }
 }
 
 // ModDown of one extended polynomial per (poly, coefficient) into its
 // chain limbs, canonical (the relinearizing kernel above without an addend).
-template <int MAXA>
+template <int MAXA, bool FULL>
 __global__ __launch_bounds__(kBlock) void moddown_polys_kernel(
-    const int64_t *acc, int64_t *out, int64_t polys, int limbs,
-    int moduli_id, DeviceTables t) {
+    const int64_t *__restrict__ acc, int64_t *__restrict__ out,
+    int64_t polys, int limbs, int moduli_id, DeviceTables t) {
   const int64_t n = int64_t(1) << t.log_n;
   const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (idx >= (polys << t.log_n))
@@ -204,15 +233,23 @@ __global__ __launch_bounds__(kBlock) void moddown_polys_kernel(
   const int64_t poly = idx >> t.log_n;
   const int64_t coef = idx & (n - 1);
   const int64_t *a = acc + poly * (limbs + t.alpha) * n + coef;
-  ModDown<MAXA> md;
+  ModDown<MAXA, FULL> md;
   md.alpha = t.alpha;
   md.limbs = limbs;
   md.moduli_id = moduli_id;
   md.prepare(a, n, t);
   int64_t *o = out + poly * limbs * n + coef;
-#pragma unroll 4
-  for (int i = 0; i < limbs; i++)
-    o[int64_t(i) * n] = md.limb_lazy(a[int64_t(i) * n], i, t);
+  for (int i0 = 0; i0 < limbs; i0 += kMdGroup) {
+    int64_t av[kMdGroup];
+#pragma unroll
+    for (int g = 0; g < kMdGroup; g++)
+      av[g] = a[int64_t(min(i0 + g, limbs - 1)) * n];
+#pragma unroll
+    for (int g = 0; g < kMdGroup; g++) {
+      const int i = min(i0 + g, limbs - 1);
+      o[int64_t(i) * n] = md.limb_lazy(av[g], i, t);
+    }
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void rescale_kernel(
@@ -467,9 +504,16 @@ static void launch_moddown(const DeviceTables &t, const int64_t *acc,
                            hipStream_t stream) {
   const dim3 grid(grid_for((int64_t(count) * 2) << t.log_n));
 #define FHE_MODDOWN(W)                                                         \
-  hipLaunchKernelGGL((moddown_add_kernel<W, RESCALE>), grid, dim3(kBlock), 0, \
-                     stream, acc, d, d_stride, d_polys, out, count, limbs,     \
-                     moduli_id, t)
+  do {                                                                         \
+    if (t.alpha == W)                                                          \
+      hipLaunchKernelGGL((moddown_add_kernel<W, true, RESCALE>), grid,        \
+                         dim3(kBlock), 0, stream, acc, d, d_stride, d_polys,   \
+                         out, count, limbs, moduli_id, t);                     \
+    else                                                                       \
+      hipLaunchKernelGGL((moddown_add_kernel<W, false, RESCALE>), grid,       \
+                         dim3(kBlock), 0, stream, acc, d, d_stride, d_polys,   \
+                         out, count, limbs, moduli_id, t);                     \
+  } while (0)
   if (t.alpha <= 1)
     FHE_MODDOWN(1);
   else if (t.alpha <= 2)
@@ -503,8 +547,15 @@ void launch_moddown_polys(const DeviceTables &t, const int64_t *acc,
                           int moduli_id, hipStream_t stream) {
   const dim3 grid(grid_for(polys << t.log_n));
 #define FHE_MODDOWN_POLYS(W)                                                   \
-  hipLaunchKernelGGL((moddown_polys_kernel<W>), grid, dim3(kBlock), 0,        \
-                     stream, acc, out, polys, limbs, moduli_id, t)
+  do {                                                                         \
+    if (t.alpha == W)                                                          \
+      hipLaunchKernelGGL((moddown_polys_kernel<W, true>), grid, dim3(kBlock), \
+                         0, stream, acc, out, polys, limbs, moduli_id, t);     \
+    else                                                                       \
+      hipLaunchKernelGGL((moddown_polys_kernel<W, false>), grid,              \
+                         dim3(kBlock), 0, stream, acc, out, polys, limbs,      \
+                         moduli_id, t);                                        \
+  } while (0)
   if (t.alpha <= 1)
     FHE_MODDOWN_POLYS(1);
   else if (t.alpha <= 2)

@@ -51,8 +51,9 @@ FHE_HD uint64_t mont_reduce(U128 x, uint64_t q, uint64_t k) {
   return (hi << 2) | (lo >> kMontBits);
 }
 
-// 32-bit add with carry in/out: v_add_co_u32 / v_addc_co_u32 on gfx950,
-// each a full-rate op, where a 64-bit add (v_lshl_add_u64) is quarter rate.
+// 32-bit add with carry in/out (v_add_co_u32 / v_addc_co_u32 on gfx950), for
+// the word sums of 128-bit products, where a carry out of the low half is
+// needed.
 FHE_HD uint32_t add_carry(uint32_t a, uint32_t b, uint32_t cin,
                           uint32_t &cout) {
 #if defined(__clang__)
@@ -114,19 +115,15 @@ FHE_HD int64_t mont_mult_u(int64_t a, int64_t b, uint64_t q, uint64_t k) {
 #endif
 }
 
-// 64-bit add / subtract as two 32-bit carry ops on the device.
+// 64-bit add: one v_lshl_add_u64 on gfx950 (a single full-rate
+// instruction, measured at twice the throughput of a v_add_co_u32 /
+// v_addc_co_u32 pair, whose carry read through VCC also costs two wait
+// states when the pair issues back to back).
 FHE_HD int64_t add64(int64_t a, int64_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  uint32_t c;
-  const uint32_t lo = add_carry(uint32_t(a), uint32_t(b), 0, c);
-  const uint32_t hi =
-      add_carry(uint32_t(uint64_t(a) >> 32), uint32_t(uint64_t(b) >> 32), c, c);
-  return int64_t((uint64_t(hi) << 32) | lo);
-#else
   return int64_t(uint64_t(a) + uint64_t(b));
-#endif
 }
 
+// 64-bit subtract as two 32-bit borrow ops on the device.
 FHE_HD int64_t sub64(int64_t a, int64_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
   unsigned int bo;
@@ -182,10 +179,7 @@ FHE_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
   const uint64_t ll = uint64_t(a0) * b0;
   const uint64_t lh = uint64_t(a0) * b1 + (ll >> 32);
   const uint64_t hl = uint64_t(a1) * b0 + uint32_t(lh);
-  // the last sum through 32-bit carry adds (a plain 64-bit add becomes the
-  // slower v_lshl_add_u64)
-  return uint64_t(add64(int64_t(uint64_t(a1) * b1 + (lh >> 32)),
-                        int64_t(hl >> 32)));
+  return uint64_t(a1) * b1 + (lh >> 32) + (hl >> 32);
 }
 
 // Shoup product (a representative of w y mod q in [0, 2q)) for any

@@ -9,19 +9,25 @@ namespace fhe {
 // ModDown (src/ckks/core_key_switch.cpp:110-160) of one key-switched
 // polynomial at one coefficient: the special-prime limbs are peeled off from
 // the last one down, each subtraction applied to every remaining limb.
-template <int MAXA> struct ModDown {
+// FULL: alpha == MAXA, so that every loop over the special primes has a
+// compile-time trip count and the limb loops carry no branches.
+template <int MAXA, bool FULL = false> struct ModDown {
   int64_t peeled[MAXA];
   int alpha, limbs, moduli_id;
+
+  __device__ __forceinline__ int specials() const {
+    return FULL ? MAXA : alpha;
+  }
 
   __device__ void prepare(const int64_t *a, int64_t n, const DeviceTables &t) {
     int64_t sv[MAXA];
 #pragma unroll
     for (int j = 0; j < MAXA; j++)
-      sv[j] = j < alpha ? a[int64_t(limbs + j) * n] : 0;
+      sv[j] = j < specials() ? a[int64_t(limbs + j) * n] : 0;
 #pragma unroll
     for (int sp = 0; sp < MAXA; sp++) {
-      if (sp < alpha) {
-        const int r = alpha - 1 - sp;
+      if (sp < specials()) {
+        const int r = specials() - 1 - sp;
         int64_t xr = 0;
 #pragma unroll
         for (int j = 0; j < MAXA; j++)
@@ -58,7 +64,7 @@ template <int MAXA> struct ModDown {
     uint64_t x = uint64_t(v);
 #pragma unroll
     for (int sp = 0; sp < MAXA; sp++)
-      if (sp < alpha) {
+      if (sp < specials()) {
         const uint64_t *c =
             t.p_inverse_shoup + (int64_t(sp) * t.prime_count + p) * 2;
         x = shoup_mult(x + top - uint64_t(peeled[sp]), c[0], c[1], q);
