@@ -404,7 +404,7 @@ struct ExtendArgs {
 template <int A, int MAXZ, bool SIGNED>
 __device__ __forceinline__ void extend_ntt_tile(const ExtendArgs &a,
                                                 const DeviceTables &t,
-                                                int64_t *lds, int64_t cp,
+                                                uint32_t *lds, int64_t cp,
                                                 int part, int tile, int i) {
   const int64_t n = int64_t(1) << t.log_n;
   ExtItem it;
@@ -421,7 +421,7 @@ __device__ __forceinline__ void extend_ntt_tile(const ExtendArgs &a,
 template <int MAXZ, bool LAZY>
 __global__ __launch_bounds__(kBlock) void extend_ntt_r16_kernel(
     ExtendArgs a, DeviceTables t) {
-  __shared__ int64_t lds[kR16Lds];
+  __shared__ uint32_t lds[kR16LdsWords];
   const int tiles = int((int64_t(1) << t.log_n) >> 12);
   int64_t w = blockIdx.x;
   if (a.swizzle) {

@@ -328,7 +328,7 @@ struct ExtItem {
 template <int A, int MAXZ, bool SIGNED>
 __device__ __forceinline__ void ext_item_tile(const ExtItem &a,
                                               const DeviceTables &t,
-                                              int64_t *lds, int tile, int i) {
+                                              uint32_t *lds, int tile, int i) {
   const int64_t n = int64_t(1) << t.log_n;
   const int p = a.moduli_id + i;
   const int tid = threadIdx.x;
@@ -365,7 +365,7 @@ __device__ __forceinline__ void ext_item_tile(const ExtItem &a,
           A == kLazySmall);
   }
   r16_forward_high<A>(v, s);
-  r16_exchange(v, lds, c, r, true);
+  r16_exchange_words(v, lds, c, r, true);
   r16_forward_low<A>(v, s);
   int64_t *out = a.ext + int64_t(i) * n + sp;
 #pragma unroll

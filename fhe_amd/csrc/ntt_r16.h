@@ -256,6 +256,36 @@ __device__ __forceinline__ void r16_exchange(int64_t (&v)[16], int64_t *lds,
     v[m] = lds[r16_lds(c, to_cols ? 16 * r + m : r + 16 * m)];
 }
 
+// r16_exchange through a buffer of 32-bit words, half the LDS of the 64-bit
+// exchange (so that twice as many blocks fit on a CU): the low words go
+// through first, then the high words, each between two barriers.
+constexpr int kR16LdsWords = kR16Lds;
+
+__device__ __forceinline__ void r16_exchange_words(int64_t (&v)[16],
+                                                   uint32_t *lds, int c,
+                                                   int r, bool to_cols) {
+  uint32_t lo[16];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    lds[r16_lds(c, to_cols ? r + 16 * m : 16 * r + m)] = uint32_t(v[m]);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    lo[m] = lds[r16_lds(c, to_cols ? 16 * r + m : r + 16 * m)];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    lds[r16_lds(c, to_cols ? r + 16 * m : 16 * r + m)] =
+        uint32_t(uint64_t(v[m]) >> 32);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    v[m] = int64_t(
+        (uint64_t(lds[r16_lds(c, to_cols ? 16 * r + m : r + 16 * m)]) << 32) |
+        lo[m]);
+}
+
 // One register-resident 8-bit pass over one 4096-point tile of one limb:
 // the 16 sub-transforms of the tile are owned by 16 threads each (16 points
 // per thread) and the eight radix-2 stages run as two radix-16 rounds with

@@ -112,7 +112,7 @@ struct UnevenExtArgs {
 template <int A, int MAXZ, bool SIGNED>
 __global__ __launch_bounds__(kBlock) void uneven_extend_kernel(
     UnevenExtArgs a, DeviceTables t) {
-  __shared__ int64_t lds[kR16Lds];
+  __shared__ uint32_t lds[kR16LdsWords];
   for (UnitWalk u(a.list); u.w < u.end; u.w += u.step) {
     const uint32_t unit = a.list.units[u.w];
     const int c = unit_ct(unit), part = unit_a(unit);
