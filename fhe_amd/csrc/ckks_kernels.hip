@@ -134,16 +134,9 @@ __global__ __launch_bounds__(kBlock) void key_inner_product_kernel(
 }
 
 // The addend of half h of ciphertext c is d + c * d_stride + h * limbs * N
-// for h < d_polys and zero otherwise.
-//
-// The limbs are processed in groups of kMdGroup whose loads are issued
-// together ahead of the group's arithmetic (one thread holds one
-// coefficient, so the loads of a group are what keeps HBM busy).  The
-// pointers are restrict-qualified: the output never overlaps the
-// accumulator or the tables, and where a caller passes the addend's own
-// buffer as the output each element is read before the same thread writes
-// it.
-constexpr int kMdGroup = 8;
+// for h < d_polys and zero otherwise.  The limb loops run in groups of
+// kMdGroup with the conventions of moddown_relin (moddown.h), whose
+// restrict-qualified pointers the kernel's follow.
 
 template <int MAXA, bool FULL, bool RESCALE>
 __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
@@ -163,6 +156,11 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
   // discarded, so that every load is unconditional
   const bool has_d = h < d_polys;
   const int64_t *d = has_d ? dbase + c * d_stride + h * limbs * n + coef : a;
+  if (!RESCALE) {
+    moddown_relin<MAXA, FULL>(a, d, has_d, out + ch * limbs * n + coef, limbs,
+                              moduli_id, n, t);
+    return;
+  }
 
   ModDown<MAXA, FULL> md;
   md.alpha = t.alpha;
@@ -179,18 +177,12 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
     return guard(guard(di + ks, int64_t(t.q2[p])), int64_t(t.q[p]));
   };
 
-  // limbs [first, limbs) in groups; the slots of the last group past the
-  // end repeat the last limb (its loads, arithmetic and store, which writes
-  // the same value again), so that a group is one branch-free block
-  const int first = RESCALE ? 1 : 0;
-  int64_t y0 = 0, rounder = 0;
-  if (RESCALE) {
-    // rescale (src/ckks/core_rescale.cpp:5-25): drop limb 0 with rounding
-    y0 = relin(0, a[0], d[0]);
-    rounder = uint64_t(y0) > t.qh[moduli_id] ? 1 : 0;
-  }
-  int64_t *o = out + ch * (limbs - first) * n + coef;
-  for (int i0 = first; i0 < limbs; i0 += kMdGroup) {
+  // rescale (src/ckks/core_rescale.cpp:5-25): drop limb 0 with rounding;
+  // limbs [1, limbs) in groups as in moddown_relin
+  const int64_t y0 = relin(0, a[0], d[0]);
+  const int64_t rounder = uint64_t(y0) > t.qh[moduli_id] ? 1 : 0;
+  int64_t *o = out + ch * (limbs - 1) * n + coef;
+  for (int i0 = 1; i0 < limbs; i0 += kMdGroup) {
     int64_t av[kMdGroup], dv[kMdGroup];
 #pragma unroll
     for (int g = 0; g < kMdGroup; g++) {
@@ -202,10 +194,6 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
     for (int g = 0; g < kMdGroup; g++) {
       const int i = min(i0 + g, limbs - 1);
       const int64_t y = relin(i, av[g], dv[g]);
-      if (!RESCALE) {
-        o[int64_t(i) * n] = y;
-        continue;
-      }
       const int p = moduli_id + i;
       const uint64_t q = t.q[p], k = t.k[p];
       const int64_t q2 = int64_t(t.q2[p]);

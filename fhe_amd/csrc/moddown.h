@@ -73,4 +73,54 @@ template <int MAXA, bool FULL = false> struct ModDown {
   }
 };
 
+// Limbs per group of the ModDown kernels' limb loops: one thread holds one
+// coefficient, so the loads of a group, issued together ahead of its
+// arithmetic, are what keeps HBM busy.
+constexpr int kMdGroup = 8;
+
+// The relinearized limbs y_i = reduce(d_i + ks_i) (src/ckks/core.cpp:
+// 736-737), i in [0, limbs), of one coefficient: a = the key-switched
+// accumulator's limbs (limb stride n, the special limbs after the chain
+// limbs), d = the addend's limbs when has_d (else any readable limbs, such
+// as a's: they are loaded and discarded, so that every load is
+// unconditional), o = the output limbs.  A canonical ks_i passes the two
+// guards unchanged when there is no addend.  The slots of the last group
+// past the end repeat the last limb (its loads, arithmetic and store, which
+// writes the same value again), so that a group is one branch-free block.
+// The pointers are restrict-qualified: o never overlaps a or the tables,
+// and where o is d's own buffer each element is read before the same thread
+// writes it.
+template <int MAXA, bool FULL>
+__device__ __forceinline__ void moddown_relin(const int64_t *__restrict__ a,
+                                              const int64_t *__restrict__ d,
+                                              bool has_d,
+                                              int64_t *__restrict__ o,
+                                              int limbs, int moduli_id,
+                                              int64_t n,
+                                              const DeviceTables &t) {
+  ModDown<MAXA, FULL> md;
+  md.alpha = t.alpha;
+  md.limbs = limbs;
+  md.moduli_id = moduli_id;
+  md.prepare(a, n, t);
+  for (int i0 = 0; i0 < limbs; i0 += kMdGroup) {
+    int64_t av[kMdGroup], dv[kMdGroup];
+#pragma unroll
+    for (int g = 0; g < kMdGroup; g++) {
+      const int i = min(i0 + g, limbs - 1);
+      av[g] = a[int64_t(i) * n];
+      dv[g] = d[int64_t(i) * n];
+    }
+#pragma unroll
+    for (int g = 0; g < kMdGroup; g++) {
+      const int i = min(i0 + g, limbs - 1);
+      const int p = moduli_id + i;
+      const int64_t ks = md.limb_lazy(av[g], i, t);
+      const int64_t di = has_d ? dv[g] : 0;
+      o[int64_t(i) * n] =
+          guard(guard(di + ks, int64_t(t.q2[p])), int64_t(t.q[p]));
+    }
+  }
+}
+
 } // namespace fhe

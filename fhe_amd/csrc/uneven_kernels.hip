@@ -194,7 +194,7 @@ struct UnevenModDownArgs {
   int d_polys;
 };
 
-template <int MAXA>
+template <int MAXA, bool FULL>
 __global__ __launch_bounds__(kBlock) void uneven_moddown_kernel(
     UnevenModDownArgs a, DeviceTables t) {
   const int64_t n = int64_t(1) << t.log_n;
@@ -202,29 +202,17 @@ __global__ __launch_bounds__(kBlock) void uneven_moddown_kernel(
     const uint32_t unit = a.list.units[u.w];
     const int c = unit_ct(unit), h = unit_a(unit);
     const UnevenCt &ct = a.cts[c];
-    const int limbs = ct.limbs, moduli_id = ct.moduli_id;
+    const int limbs = ct.limbs;
     const int64_t coef = int64_t(unit_tile(unit)) * kBlock + threadIdx.x;
     const int64_t *acc = a.acc.p[c] + (int64_t(h) * ct.ext << t.log_n) + coef;
-    const int64_t *d = h < a.d_polys
-                           ? a.d.p[c] + (int64_t(h) * limbs << t.log_n) + coef
-                           : nullptr;
+    // as moddown_add_kernel (no addend: the accumulator's limbs are read
+    // in its place and discarded)
+    const bool has_d = h < a.d_polys;
+    const int64_t *d =
+        has_d ? a.d.p[c] + (int64_t(h) * limbs << t.log_n) + coef : acc;
     int64_t *o = const_cast<int64_t *>(a.out.p[c]) +
                  (int64_t(h) * limbs << t.log_n) + coef;
-    ModDown<MAXA> md;
-    md.alpha = t.alpha;
-    md.limbs = limbs;
-    md.moduli_id = moduli_id;
-    md.prepare(acc, n, t);
-    // as moddown_add_kernel: y_i = reduce(d_i + ks_i)
-    // (src/ckks/core.cpp:736-737)
-#pragma unroll 4
-    for (int i = 0; i < limbs; i++) {
-      const int p = moduli_id + i;
-      const int64_t ks = md.limb_lazy(acc[int64_t(i) << t.log_n], i, t);
-      const int64_t di = d ? d[int64_t(i) << t.log_n] : 0;
-      o[int64_t(i) << t.log_n] =
-          guard(guard(di + ks, int64_t(t.q2[p])), int64_t(t.q[p]));
-    }
+    moddown_relin<MAXA, FULL>(acc, d, has_d, o, limbs, ct.moduli_id, n, t);
   }
 }
 
@@ -405,8 +393,14 @@ void launch_uneven_moddown(const DeviceTables &t, const UnevenCt *cts,
   UnevenModDownArgs a{list, cts, acc, d, out, d_polys};
   const dim3 grid(grid_of(list.count, cap));
 #define FHE_UMD(W)                                                             \
-  hipLaunchKernelGGL(uneven_moddown_kernel<W>, grid, dim3(kBlock), 0, stream, \
-                     a, t)
+  do {                                                                         \
+    if (t.alpha == W)                                                          \
+      hipLaunchKernelGGL((uneven_moddown_kernel<W, true>), grid, dim3(kBlock), \
+                         0, stream, a, t);                                     \
+    else                                                                       \
+      hipLaunchKernelGGL((uneven_moddown_kernel<W, false>), grid,             \
+                         dim3(kBlock), 0, stream, a, t);                       \
+  } while (0)
   if (t.alpha <= 1)
     FHE_UMD(1);
   else if (t.alpha <= 2)
