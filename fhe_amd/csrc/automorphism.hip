@@ -34,11 +34,34 @@ __device__ __forceinline__ int64_t xcd_block(int swizzle) {
   return int64_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
+// out never overlaps in (a permutation cannot run in place); with N a
+// multiple of kTile a block's coefficients lie in one limb, whose prime is
+// then read once and the block's gathers issue together.
 __global__ __launch_bounds__(kBlock) void automorphism_kernel(
-    const int64_t *in, int64_t *out, int64_t total, int limbs, int moduli_id,
-    uint32_t h, int swizzle, DeviceTables t) {
+    const int64_t *__restrict__ in, int64_t *__restrict__ out, int64_t total,
+    int limbs, int moduli_id, uint32_t h, int swizzle, DeviceTables t) {
   const uint32_t n = 1u << t.log_n;
   const int64_t base = xcd_block(swizzle) * kTile + threadIdx.x;
+  if (n % kTile == 0 && base < total) {
+    const int64_t f = base >> t.log_n; // flat limb, one per block
+    const int64_t q =
+        int64_t(t.q[moduli_id + int(uint32_t(f) % uint32_t(limbs))]);
+    int64_t x[kPerThread];
+    uint32_t s[kPerThread];
+#pragma unroll
+    for (int e = 0; e < kPerThread; e++) {
+      const uint32_t o = uint32_t(base + e * kBlock) & (n - 1);
+      // mod 2N of a product that wrapped mod 2^32 is still exact
+      s[e] = (h * o) & (2 * n - 1);
+      x[e] = in[f * n + (s[e] & (n - 1))];
+    }
+#pragma unroll
+    for (int e = 0; e < kPerThread; e++) {
+      const int64_t y = (s[e] & n) ? -x[e] : x[e];
+      out[base + e * kBlock] = y < 0 ? y + q : guard(y, q);
+    }
+    return;
+  }
 #pragma unroll
   for (int e = 0; e < kPerThread; e++) {
     const int64_t idx = base + e * kBlock;
@@ -47,7 +70,6 @@ __global__ __launch_bounds__(kBlock) void automorphism_kernel(
     const int64_t f = idx >> t.log_n; // flat limb
     const uint32_t o = uint32_t(idx) & (n - 1);
     const int64_t q = int64_t(t.q[moduli_id + int(f % limbs)]);
-    // mod 2N of a product that wrapped mod 2^32 is still exact
     const uint32_t s = (h * o) & (2 * n - 1);
     const int64_t x = in[f * n + (s & (n - 1))];
     const int64_t y = (s & n) ? -x : x;

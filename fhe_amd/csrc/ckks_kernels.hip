@@ -42,6 +42,31 @@ __global__ __launch_bounds__(kBlock) void mont_mult_broadcast_kernel(
   out[idx] = mont_mult(a[idx], b[r], t.q[p], t.k[p]);
 }
 
+// mont_mult_kernel and mont_mult_broadcast_kernel over kChunk coefficients
+// of one limb per block (chunked_ok); BCAST: b is one polynomial of `limbs`
+// limbs applied to every polynomial of a.
+template <bool BCAST>
+__global__ __launch_bounds__(kBlock) void mont_mult_chunk_kernel(
+    const int64_t *a, const int64_t *b, int64_t *out, int limbs,
+    int moduli_id, DeviceTables t) {
+  const int64_t e0 = int64_t(blockIdx.x) * kChunk;
+  const uint32_t flat = uint32_t(e0 >> t.log_n);
+  const uint32_t l = flat % uint32_t(limbs);
+  const int p = moduli_id + int(l);
+  const uint64_t q = t.q[p], k = t.k[p];
+  const int64_t idx = e0 + int64_t(threadIdx.x) * kPerThread;
+  const int64_t bidx =
+      BCAST ? (int64_t(l) << t.log_n) + (idx & ((int64_t(1) << t.log_n) - 1))
+            : idx;
+  int64_t x[kPerThread], y[kPerThread];
+  load4(a + idx, x);
+  load4(b + bidx, y);
+#pragma unroll
+  for (int g = 0; g < kPerThread; g++)
+    x[g] = mont_mult(x[g], y[g], q, k);
+  store4(out + idx, x);
+}
+
 __global__ __launch_bounds__(kBlock) void tensor_kernel(
     const int64_t *a, const int64_t *b, int64_t *trio, int64_t total,
     int limbs, int moduli_id, DeviceTables t) {
@@ -240,9 +265,11 @@ __global__ __launch_bounds__(kBlock) void moddown_polys_kernel(
   }
 }
 
+// The limbs go in groups of kMdGroup as in moddown_relin (moddown.h); the
+// output never overlaps the input.
 __global__ __launch_bounds__(kBlock) void rescale_kernel(
-    const int64_t *in, int64_t *out, int polys, int limbs, int moduli_id,
-    DeviceTables t) {
+    const int64_t *__restrict__ in, int64_t *__restrict__ out, int polys,
+    int limbs, int moduli_id, DeviceTables t) {
   const int64_t n = int64_t(1) << t.log_n;
   const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (idx >= int64_t(polys) * n)
@@ -253,16 +280,25 @@ __global__ __launch_bounds__(kBlock) void rescale_kernel(
   int64_t *o = out + poly * (limbs - 1) * n + coef;
   const int64_t x0 = x[0];
   const int64_t rounder = uint64_t(x0) > t.qh[moduli_id] ? 1 : 0;
-  for (int i = 1; i < limbs; i++) {
-    const int p = moduli_id + i;
-    const uint64_t q = t.q[p], k = t.k[p];
-    const int64_t q2 = int64_t(t.q2[p]);
-    int64_t v = x[int64_t(i) * n] - x0;
-    v = v >= 0 ? v : v + q2;
-    v = guard(v, int64_t(q));
-    v = mont_mult(v, t.rescale[int64_t(moduli_id) * t.prime_count + p], q, k);
-    v = guard(v + rounder, q2);
-    o[int64_t(i - 1) * n] = guard(v, int64_t(q));
+  for (int i0 = 1; i0 < limbs; i0 += kMdGroup) {
+    int64_t xv[kMdGroup];
+#pragma unroll
+    for (int g = 0; g < kMdGroup; g++)
+      xv[g] = x[int64_t(min(i0 + g, limbs - 1)) * n];
+#pragma unroll
+    for (int g = 0; g < kMdGroup; g++) {
+      const int i = min(i0 + g, limbs - 1);
+      const int p = moduli_id + i;
+      const uint64_t q = t.q[p], k = t.k[p];
+      const int64_t q2 = int64_t(t.q2[p]);
+      int64_t v = xv[g] - x0;
+      v = v >= 0 ? v : v + q2;
+      v = guard(v, int64_t(q));
+      v = mont_mult(v, t.rescale[int64_t(moduli_id) * t.prime_count + p], q,
+                    k);
+      v = guard(v + rounder, q2);
+      o[int64_t(i - 1) * n] = guard(v, int64_t(q));
+    }
   }
 }
 
@@ -419,6 +455,12 @@ void launch_mont_mult(const DeviceTables &t, const int64_t *a,
                       const int64_t *b, int64_t *out, int64_t total_limbs,
                       int limbs, int moduli_id, hipStream_t stream) {
   const int64_t total = total_limbs << t.log_n;
+  if (chunked_ok(t.log_n, {a, b, out})) {
+    hipLaunchKernelGGL(mont_mult_chunk_kernel<false>,
+                       dim3(unsigned(total / kChunk)), dim3(kBlock), 0, stream,
+                       a, b, out, limbs, moduli_id, t);
+    return;
+  }
   hipLaunchKernelGGL(mont_mult_kernel, dim3(grid_for(total)), dim3(kBlock), 0,
                      stream, a, b, out, total, limbs, moduli_id, t);
 }
@@ -428,6 +470,12 @@ void launch_mont_mult_broadcast(const DeviceTables &t, const int64_t *a,
                                 int64_t total_limbs, int limbs, int moduli_id,
                                 hipStream_t stream) {
   const int64_t total = total_limbs << t.log_n;
+  if (chunked_ok(t.log_n, {a, b, out})) {
+    hipLaunchKernelGGL(mont_mult_chunk_kernel<true>,
+                       dim3(unsigned(total / kChunk)), dim3(kBlock), 0, stream,
+                       a, b, out, limbs, moduli_id, t);
+    return;
+  }
   hipLaunchKernelGGL(mont_mult_broadcast_kernel, dim3(grid_for(total)),
                      dim3(kBlock), 0, stream, a, b, out, total, limbs,
                      moduli_id, t);

@@ -1,9 +1,10 @@
 // Device-side tables shared by every kernel, and launch helpers.
 #pragma once
 
-#include <hip/hip_runtime.h>
-
 #include <cstdint>
+#include <initializer_list>
+
+#include <hip/hip_runtime.h>
 
 #include "modarith.h"
 
@@ -51,5 +52,46 @@ constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t work, int per_block = kBlock) {
   return unsigned((work + per_block - 1) / per_block);
 }
+
+// Chunked per-coefficient kernels: a block covers kChunk consecutive
+// coefficients of one limb, kPerThread per thread (two 16-byte accesses per
+// operand), so that the limb, its prime and its constants are uniform in
+// the block.  They take limbs of N >= kChunk coefficients and operands at
+// 16-byte aligned addresses (chunked_ok); the one-coefficient-per-thread
+// kernels serve the rest.
+constexpr int kPerThread = 4;
+constexpr int64_t kChunk = int64_t(kBlock) * kPerThread;
+
+inline bool chunked_ok(int log_n, std::initializer_list<const void *> ptrs) {
+  if ((int64_t(1) << log_n) % kChunk != 0)
+    return false;
+  for (const void *p : ptrs)
+    if (reinterpret_cast<uintptr_t>(p) % 16 != 0)
+      return false;
+  return true;
+}
+
+#if defined(__HIPCC__)
+typedef int64_t i64x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void load4(const int64_t *p, int64_t (&v)[4]) {
+  const i64x2 a = reinterpret_cast<const i64x2 *>(p)[0];
+  const i64x2 b = reinterpret_cast<const i64x2 *>(p)[1];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+
+__device__ __forceinline__ void store4(int64_t *p, const int64_t (&v)[4]) {
+  i64x2 a, b;
+  a.x = v[0];
+  a.y = v[1];
+  b.x = v[2];
+  b.y = v[3];
+  reinterpret_cast<i64x2 *>(p)[0] = a;
+  reinterpret_cast<i64x2 *>(p)[1] = b;
+}
+#endif
 
 } // namespace fhe

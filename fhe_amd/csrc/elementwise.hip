@@ -102,6 +102,32 @@ __global__ __launch_bounds__(kBlock) void elementwise_kernel(EwArgs a,
   }
 }
 
+// The same over kChunk coefficients of one limb per block (chunked_ok).
+__global__ __launch_bounds__(kBlock) void elementwise_chunk_kernel(
+    EwArgs a, DeviceTables t) {
+  const int64_t e0 = int64_t(blockIdx.x) * kChunk;
+  // block-uniform: the limb, its prime and constants
+  const int l = int(uint32_t(e0 >> t.log_n) % uint32_t(a.limbs));
+  const int p = a.moduli_id + l;
+  const uint64_t q = t.q[p], k = t.k[p];
+  const int64_t q2 = int64_t(t.q2[p]), qh = int64_t(t.qh[p]);
+  const int64_t c = a.c[l];
+  const int64_t idx = e0 + int64_t(threadIdx.x) * kPerThread;
+  int64_t x1[kPerThread], x2[kPerThread] = {}, y_old[kPerThread] = {};
+  int64_t y[kPerThread];
+  load4(a.x1 + idx, x1);
+  if (a.x2)
+    load4(a.x2 + idx, x2);
+  if (a.op == kEwMultAdd)
+    load4(a.y + idx, y_old);
+  const bool first = (idx & ((int64_t(1) << t.log_n) - 1)) == 0;
+#pragma unroll
+  for (int g = 0; g < kPerThread; g++)
+    y[g] = ew_apply(a.op, x1[g], x2[g], y_old[g], c, q, k, q2, qh,
+                    first && g == 0);
+  store4(a.y + idx, y);
+}
+
 } // namespace
 
 bool ew_needs_x2(int op) {
@@ -154,6 +180,12 @@ void launch_elementwise(const DeviceTables &t, int op, const int64_t *x1,
   a.op = op;
   for (int l = 0; l < limbs && constants; l++)
     a.c[l] = constants[l];
+  if (chunked_ok(t.log_n, {x1, x2, y})) {
+    hipLaunchKernelGGL(elementwise_chunk_kernel,
+                       dim3(unsigned(a.total / kChunk)), dim3(kBlock), 0,
+                       stream, a, t);
+    return;
+  }
   // enough blocks to fill 256 CUs several times over; each thread strides
   const int64_t blocks = std::min<int64_t>(grid_for(a.total), 256 * 64);
   hipLaunchKernelGGL(elementwise_kernel, dim3(unsigned(blocks)), dim3(kBlock),
