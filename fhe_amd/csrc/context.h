@@ -502,13 +502,32 @@ struct fhe_context {
                                 int flags, bool intt_first, int dst_level,
                                 bool rescale, bool ntt, int64_t *out,
                                 int64_t *t0, int64_t *t1) {
-    const int64_t N = n();
     bool rescaled = (flags & FHE_FLAG_RESCALED) != 0;
     bool is_ntt = (flags & FHE_FLAG_NTT_FORM) != 0;
     const int64_t *cur = in;
+    // the steps that write a result, counted first so that the last one
+    // writes to out (the others alternate between t0 and t1)
+    int steps = 0;
+    {
+      bool r = rescaled, nt = is_ntt;
+      if (intt_first && nt) {
+        steps++;
+        nt = false;
+      }
+      if (level > dst_level) {
+        steps += (!r && nt) ? 3 : 1;
+        r = false;
+      }
+      if (rescale && !r)
+        steps += nt ? 3 : 1;
+      if (ntt && !nt)
+        steps++;
+    }
     int64_t *bufs[2] = {t0, t1};
-    int next = 0;
+    int next = 0, done = 0;
     auto target = [&] {
+      if (++done == steps)
+        return out;
       int64_t *b = bufs[next];
       next ^= 1;
       return b;
@@ -569,13 +588,9 @@ struct fhe_context {
     }
     if (ntt && !is_ntt)
       to_ntt();
-    if (cur == in)
-      return in;
-    const int limbs = params.limbs_at(moduli_id_of(level, rescaled));
-    hip_check(hipMemcpyAsync(out, cur, size_t(polys * limbs) * N * 8,
-                             hipMemcpyDeviceToDevice, stream),
-              "copy");
-    return out;
+    require(done == steps && (steps == 0 || cur == out),
+            "internal: form change steps miscounted");
+    return steps == 0 ? in : out;
   }
 
   // CKKSEngine::prepare_for_multiply (src/ckks/engine_multiply.cpp:13-27):

@@ -93,16 +93,18 @@ public:
                    ctx->moduli_id_of(a.level, false)));
     return out;
   }
-  // intt_ (src/ckks/engine_intt.cpp): a copy in the coefficient domain
+  // intt_ (src/ckks/engine_intt.cpp): a copy in the coefficient domain,
+  // transformed out of place
   Text intt(const Text &a) {
     if (!(a.flags & FHE_FLAG_NTT_FORM))
       return a;
-    Text t = clone(a);
-    t.flags &= ~FHE_FLAG_NTT_FORM;
+    Text t = alloc(a.polys, a.level, a.flags & ~FHE_FLAG_NTT_FORM, a.count);
     const int l = limbs(a.level, a.flags);
-    ok(fhe_intt(ctx, t.data, a.count * a.polys, l,
-                ctx->moduli_id_of(a.level, a.flags & FHE_FLAG_RESCALED),
-                1));
+    fhe::ntt_inverse(ctx->tables, a.data, t.data,
+                     int64_t(a.count) * a.polys * l, l,
+                     ctx->moduli_id_of(a.level, a.flags & FHE_FLAG_RESCALED),
+                     fhe::kInttExit, ctx->stream);
+    ctx->check_launch("INTT");
     return t;
   }
   // multiply(ct, ct[, rlk]), square (src/ckks/engine_multiply.cpp:39-199)
@@ -193,10 +195,13 @@ public:
     const int mid = ctx->moduli_id_of(trio.level, rescaled);
     Text in = trio;
     if (!(trio.flags & FHE_FLAG_NTT_FORM)) {
-      in = clone(trio);
-      in.flags |= FHE_FLAG_NTT_FORM;
-      ok(fhe_ntt(ctx, in.data, 3 * trio.count, limbs(trio.level, trio.flags),
-                 mid, 1));
+      // to the NTT form out of place
+      in = alloc(3, trio.level, trio.flags | FHE_FLAG_NTT_FORM, trio.count);
+      const int l = limbs(trio.level, trio.flags);
+      fhe::ntt_forward(ctx->tables, trio.data, in.data,
+                       int64_t(3) * trio.count * l, l, mid, ctx->tables.r2,
+                       ctx->stream);
+      ctx->check_launch("NTT");
     }
     ctx->check_key(mid, key_mid, "Relinearize");
     Text out = alloc(2, trio.level, trio.flags & ~FHE_FLAG_NTT_FORM,
