@@ -328,24 +328,32 @@ __global__ __launch_bounds__(kBlock) void scalar_rescale_kernel(
   const int64_t *x = a.in + poly * a.limbs * n + coef;
   const int out_limbs = a.limbs - 1 - a.skip;
   int64_t *o = a.out + poly * out_limbs * n + coef;
-  auto entered = [&](int i) {
+  auto entered = [&](int i, int64_t xi) {
     const int p = a.moduli_id + i;
-    return guard(mont_mult(x[int64_t(i) * n], a.c[i], t.q[p], t.k[p]),
-                 int64_t(t.q[p]));
+    return guard(mont_mult(xi, a.c[i], t.q[p], t.k[p]), int64_t(t.q[p]));
   };
-  const int64_t y0 = entered(0);
+  const int64_t y0 = entered(0, x[0]);
   const int64_t rounder = uint64_t(y0) > t.qh[a.moduli_id] ? 1 : 0;
-  for (int i = 1 + a.skip; i < a.limbs; i++) {
-    const int p = a.moduli_id + i;
-    const uint64_t q = t.q[p], k = t.k[p];
-    const int64_t q2 = int64_t(t.q2[p]);
-    int64_t v = entered(i) - y0;
-    v = v >= 0 ? v : v + q2;
-    v = guard(v, int64_t(q));
-    v = mont_mult(v, t.rescale[int64_t(a.moduli_id) * t.prime_count + p], q,
-                  k);
-    v = guard(v + rounder, q2);
-    o[int64_t(i - 1 - a.skip) * n] = guard(v, int64_t(q));
+  // limbs in load groups as in moddown_relin (moddown.h)
+  for (int i0 = 1 + a.skip; i0 < a.limbs; i0 += kMdGroup) {
+    int64_t xv[kMdGroup];
+#pragma unroll
+    for (int g = 0; g < kMdGroup; g++)
+      xv[g] = x[int64_t(min(i0 + g, a.limbs - 1)) * n];
+#pragma unroll
+    for (int g = 0; g < kMdGroup; g++) {
+      const int i = min(i0 + g, a.limbs - 1);
+      const int p = a.moduli_id + i;
+      const uint64_t q = t.q[p], k = t.k[p];
+      const int64_t q2 = int64_t(t.q2[p]);
+      int64_t v = entered(i, xv[g]) - y0;
+      v = v >= 0 ? v : v + q2;
+      v = guard(v, int64_t(q));
+      v = mont_mult(v, t.rescale[int64_t(a.moduli_id) * t.prime_count + p],
+                    q, k);
+      v = guard(v + rounder, q2);
+      o[int64_t(i - 1 - a.skip) * n] = guard(v, int64_t(q));
+    }
   }
 }
 
@@ -364,6 +372,9 @@ struct WeightedSumArgs {
   int64_t c[kWsMaxTerms][kWsMaxLimbs];
 };
 
+// TERMS = a.terms, so that the term loops are straight-line code whose
+// loads issue together.
+template <int TERMS>
 __global__ __launch_bounds__(kBlock) void weighted_sum_kernel(
     WeightedSumArgs a, DeviceTables t) {
   const int64_t n = int64_t(1) << t.log_n;
@@ -375,36 +386,31 @@ __global__ __launch_bounds__(kBlock) void weighted_sum_kernel(
   const int64_t src = poly * a.limbs * n + coef;
   int64_t *o = a.out + poly * (a.limbs - 1) * n + coef;
   const uint64_t q0 = t.q[a.moduli_id], k0 = t.k[a.moduli_id];
-  int64_t y0[kWsMaxTerms];
-  int rounder[kWsMaxTerms];
+  int64_t y0[TERMS];
+  int rounder[TERMS];
 #pragma unroll
-  for (int x = 0; x < kWsMaxTerms; x++) {
-    y0[x] = 0;
-    rounder[x] = 0;
-    if (x < a.terms) {
-      y0[x] = guard(mont_mult(a.in[x][src], a.c[x][0], q0, k0), int64_t(q0));
-      rounder[x] = uint64_t(y0[x]) > t.qh[a.moduli_id] ? 1 : 0;
-    }
+  for (int x = 0; x < TERMS; x++) {
+    y0[x] = guard(mont_mult(a.in[x][src], a.c[x][0], q0, k0), int64_t(q0));
+    rounder[x] = uint64_t(y0[x]) > t.qh[a.moduli_id] ? 1 : 0;
   }
   for (int i = 1; i < a.limbs; i++) {
     const int p = a.moduli_id + i;
     const uint64_t q = t.q[p], k = t.k[p];
     const int64_t q2 = int64_t(t.q2[p]), qq = int64_t(q);
     const int64_t rs = t.rescale[int64_t(a.moduli_id) * t.prime_count + p];
+    int64_t xv[TERMS];
+#pragma unroll
+    for (int x = 0; x < TERMS; x++)
+      xv[x] = a.in[x][src + int64_t(i) * n];
     int64_t acc = 0;
 #pragma unroll
-    for (int x = 0; x < kWsMaxTerms; x++) {
-      if (x < a.terms) {
-        int64_t v = guard(mont_mult(a.in[x][src + int64_t(i) * n], a.c[x][i],
-                                    q, k),
-                          qq) -
-                    y0[x];
-        v = v >= 0 ? v : v + q2;
-        v = guard(v, qq);
-        v = mont_mult(v, rs, q, k);
-        v = guard(guard(v + rounder[x], q2), qq);
-        acc = x == 0 ? v : guard(guard(acc + v, q2), qq);
-      }
+    for (int x = 0; x < TERMS; x++) {
+      int64_t v = guard(mont_mult(xv[x], a.c[x][i], q, k), qq) - y0[x];
+      v = v >= 0 ? v : v + q2;
+      v = guard(v, qq);
+      v = mont_mult(v, rs, q, k);
+      v = guard(guard(v + rounder[x], q2), qq);
+      acc = x == 0 ? v : guard(guard(acc + v, q2), qq);
     }
     o[int64_t(i - 1) * n] = acc;
   }
@@ -429,8 +435,24 @@ void launch_weighted_sum(const DeviceTables &t,
       a.c[x][i] = constants[x][i];
   }
   const dim3 grid(grid_for(int64_t(polys) << t.log_n));
-  hipLaunchKernelGGL(weighted_sum_kernel, grid, dim3(kBlock), 0, stream, a,
-                     t);
+  switch (terms) {
+#define FHE_WS(T)                                                              \
+  case T:                                                                      \
+    hipLaunchKernelGGL(weighted_sum_kernel<T>, grid, dim3(kBlock), 0, stream, \
+                       a, t);                                                  \
+    break;
+    FHE_WS(1)
+    FHE_WS(2)
+    FHE_WS(3)
+    FHE_WS(4)
+    FHE_WS(5)
+    FHE_WS(6)
+    FHE_WS(7)
+    FHE_WS(8)
+#undef FHE_WS
+  default:
+    throw std::invalid_argument("weighted sum: 1 to 8 terms per launch");
+  }
 }
 
 void launch_scalar_rescale(const DeviceTables &t, const int64_t *in,

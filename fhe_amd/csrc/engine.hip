@@ -55,8 +55,10 @@ Text chebyshev_ps(TextEngine &e, const std::vector<Text> &baby,
 // sine (src/ckks/engine_bootstrap.cpp:526-638): Chebyshev basis T1..T7,
 // giant steps T7, T14, T28 and the Paterson-Stockmeyer evaluation
 Text sine(TextEngine &e, const Text &x) {
+  // texts are shared, not copied: the only in-place additions below are
+  // on fresh products
   std::vector<Text> baby(kBabyDegree);
-  baby[0] = e.clone(x);
+  baby[0] = x;
   for (int bi = 2; bi <= kBabyDegree; bi++) {
     const int index = bi - 1, half = bi / 2 - 1;
     if ((bi & 1) == 0) {
@@ -75,13 +77,13 @@ Text sine(TextEngine &e, const Text &x) {
       baby[i] = e.level_down(baby[i], destination);
 
   std::vector<Text> giant(kGiantDegree);
-  giant[0] = e.clone(baby.back());
+  giant[0] = baby.back();
   for (int gi = 1; gi < kGiantDegree; gi++) {
     Text m = e.multiply(giant[gi - 1], giant[gi - 1], true);
     e.add_inplace(m, m);
     giant[gi] = e.add_double(m, -1);
   }
-  Text ps = e.clone(giant[0]);
+  Text ps = giant[0];
   for (int gi = 1; gi < kGiantDegree - 1; gi++) {
     Text m = e.multiply(ps, giant[gi], true);
     e.add_inplace(m, m);
@@ -106,7 +108,7 @@ Text sine(TextEngine &e, const Text &x) {
 
 // apply_double_angle_iterations (src/ckks/engine_bootstrap.cpp:640-659)
 Text double_angle(TextEngine &e, const Text &x) {
-  Text current = e.clone(x);
+  Text current = x; // shared: the loop replaces it, never adds into it
   const double two_pi = 2 * kPi;
   for (int it = 1; it <= kDoubleAngleIterations; it++) {
     Text m = e.multiply(current, current, true);
