@@ -72,6 +72,17 @@ inline bool chunked_ok(int log_n, std::initializer_list<const void *> ptrs) {
 }
 
 #if defined(__HIPCC__)
+// The thread's index in its block as a value that the compiler cannot take
+// for invariant across the iterations of a persistent loop: otherwise it
+// keeps every per-thread offset derived from the index (the LDS and global
+// offsets of a tile's sixteen points) live across the loop, at the cost of
+// occupancy.  An empty asm statement that claims to modify the value.
+__device__ __forceinline__ int opaque_tid() {
+  int tid = int(threadIdx.x);
+  asm volatile("" : "+v"(tid));
+  return tid;
+}
+
 typedef int64_t i64x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void load4(const int64_t *p, int64_t (&v)[4]) {

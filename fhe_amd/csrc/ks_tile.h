@@ -119,13 +119,14 @@ __device__ __forceinline__ int64_t acc96_reduce(const Acc96 &x, uint64_t q,
 // products and reduces each accumulator once (Acc96).  Every form gives the
 // same residues, and the accumulators are canonicalized after the inverse
 // transform.
-template <int A, bool PF, bool WIDE, class Addr>
+// OPAQUE: the tile runs inside a persistent loop (opaque_tid).
+template <int A, bool PF, bool WIDE, class Addr, bool OPAQUE = false>
 __device__ __forceinline__ void ks_item_tile(const Addr &a,
                                              const DeviceTables &t,
                                              int64_t *lds, int tile, int i) {
   const int64_t n = int64_t(1) << t.log_n;
   const int p = a.moduli_id + i;
-  const int tid = threadIdx.x;
+  const int tid = OPAQUE ? opaque_tid() : int(threadIdx.x);
   const int r = tid & 15;
   const int c = tid >> 4;
   const int64_t sp = int64_t(tile) * 16 + c; // 256-point block in the limb
@@ -325,13 +326,14 @@ struct ExtItem {
   int size, ext_limbs, moduli_id, ext_shoup_offset;
 };
 
-template <int A, int MAXZ, bool SIGNED>
+// OPAQUE: as for ks_item_tile.
+template <int A, int MAXZ, bool SIGNED, bool OPAQUE = false>
 __device__ __forceinline__ void ext_item_tile(const ExtItem &a,
                                               const DeviceTables &t,
                                               uint32_t *lds, int tile, int i) {
   const int64_t n = int64_t(1) << t.log_n;
   const int p = a.moduli_id + i;
-  const int tid = threadIdx.x;
+  const int tid = OPAQUE ? opaque_tid() : int(threadIdx.x);
   const int c = tid & 15, r = tid >> 4;
   const int64_t sp = int64_t(tile) * 16 + c;
   const int size = a.size;

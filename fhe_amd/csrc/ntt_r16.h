@@ -295,7 +295,8 @@ __device__ __forceinline__ void r16_exchange_words(int64_t (&v)[16],
 // (prime p); the pass covers index bits [b, b + 8); COLS is b > 0.  pre
 // (forward, per prime) multiplies every loaded value first; post (inverse)
 // is the InttPost epilogue; last = the pass completes a forward transform.
-template <bool INV, bool COLS, int A>
+// OPAQUE: the pass runs inside a persistent loop (opaque_tid).
+template <bool INV, bool COLS, int A, bool OPAQUE = false>
 __device__ __forceinline__ void r16_limb_pass(const int64_t *src_limb,
                                               int64_t *dst_limb, int p,
                                               int tile, int b,
@@ -304,7 +305,7 @@ __device__ __forceinline__ void r16_limb_pass(const int64_t *src_limb,
                                               int64_t *lds) {
   const int log_n = t.log_n;
   const int64_t n = int64_t(1) << log_n;
-  const int tid = threadIdx.x;
+  const int tid = OPAQUE ? opaque_tid() : int(threadIdx.x);
   const int c = COLS ? (tid & 15) : (tid >> 4);
   const int r = COLS ? (tid >> 4) : (tid & 15);
   const int64_t sp = int64_t(tile) * 16 + c;

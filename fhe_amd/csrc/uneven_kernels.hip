@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void uneven_ntt_kernel(UnevenNttArgs a,
     const UnevenCt &ct = a.cts[c];
     const int stride = a.ext_stride ? ct.ext : ct.limbs;
     const int64_t off = (int64_t(unit_a(unit)) * stride + l) << t.log_n;
-    r16_limb_pass<INV, COLS, A>(a.src.p[c] + off,
+    r16_limb_pass<INV, COLS, A, true>(a.src.p[c] + off,
                                 const_cast<int64_t *>(a.dst.p[c]) + off,
                                 ct.moduli_id + l, unit_tile(unit), a.b, a.pre,
                                 a.post, a.last != 0, t, lds);
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void uneven_extend_kernel(
     it.ext_limbs = ct.ext;
     it.moduli_id = ct.moduli_id;
     it.ext_shoup_offset = ct.ext_shoup_offset;
-    ext_item_tile<A, MAXZ, SIGNED>(it, t, lds, unit_tile(unit),
+    ext_item_tile<A, MAXZ, SIGNED, true>(it, t, lds, unit_tile(unit),
                                    unit_limb(unit));
   }
 }
@@ -181,8 +181,8 @@ __global__ __launch_bounds__(kBlock) void uneven_ks_kernel(UnevenKsArgs a,
   for (UnitWalk u(a.list); u.w < u.end; u.w += u.step) {
     const uint32_t unit = a.list.units[u.w];
     const UnevenKsAddr ad(a, unit_ct(unit), t.log_n);
-    ks_item_tile<A, false, WIDE>(ad, t, lds, unit_tile(unit),
-                                 unit_limb(unit));
+    ks_item_tile<A, false, WIDE, UnevenKsAddr, true>(
+        ad, t, lds, unit_tile(unit), unit_limb(unit));
   }
 }
 
