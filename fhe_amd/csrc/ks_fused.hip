@@ -389,6 +389,54 @@ __global__ __launch_bounds__(kBlock) void modup_digits_kernel(DigitArgs a,
       out[int64_t(j) * n] = d[j];
 }
 
+// modup_digits_kernel over kChunk coefficients of one (ciphertext,
+// partition) per block (chunked_ok): four independent Garner chains per
+// thread.  A partition's limbs past its size are loaded again as its last
+// limb and unused, so that every load is unconditional.
+template <int MAXZ>
+__global__ __launch_bounds__(kBlock) void modup_digits_chunk_kernel(
+    DigitArgs a, DeviceTables t) {
+  const int64_t n = int64_t(1) << t.log_n;
+  const int64_t e0 = int64_t(blockIdx.x) * kChunk;
+  const uint32_t cp = uint32_t(e0 >> t.log_n); // ciphertext * parts + part
+  const int part = int(cp % uint32_t(a.parts));
+  const int64_t ct = cp / uint32_t(a.parts);
+  const int64_t coef = (e0 & (n - 1)) + int64_t(threadIdx.x) * kPerThread;
+  const int start = a.start[part], size = a.size[part];
+  const int64_t *xs = a.x + ct * a.x_stride + coef;
+  const int64_t *cst = a.cst + int64_t(part) * a.cst_stride;
+  int64_t xv[kPerThread][MAXZ], d[kPerThread][MAXZ];
+#pragma unroll
+  for (int j = 0; j < MAXZ; j++) {
+    int64_t v[kPerThread];
+    load4(xs + int64_t(start + min(j, size - 1)) * n, v);
+#pragma unroll
+    for (int g = 0; g < kPerThread; g++)
+      xv[g][j] = j < size ? v[g] : 0;
+  }
+  if ((a.signed_mask >> part) & 1) {
+#pragma unroll
+    for (int g = 0; g < kPerThread; g++)
+      garner_digits<MAXZ, true>(xv[g], d[g], size, a.moduli_id + start, cst,
+                                t);
+  } else {
+#pragma unroll
+    for (int g = 0; g < kPerThread; g++)
+      garner_digits<MAXZ, false>(xv[g], d[g], size, a.moduli_id + start, cst,
+                                 t);
+  }
+  int64_t *out = a.digits + int64_t(cp) * MAXZ * n + coef;
+#pragma unroll
+  for (int j = 0; j < MAXZ; j++)
+    if (j < size) {
+      int64_t v[kPerThread];
+#pragma unroll
+      for (int g = 0; g < kPerThread; g++)
+        v[g] = d[g][j];
+      store4(out + int64_t(j) * n, v);
+    }
+}
+
 struct ExtendArgs {
   const int64_t *digits; // [count][parts][maxz][N]
   int64_t *ext;          // [count][parts][ext_limbs][N]
@@ -526,10 +574,17 @@ void launch_staged_modup(const DeviceTables &t, const int64_t *x,
   e.swizzle = eblocks % 8 == 0 ? 1 : 0;
   const dim3 dgrid(grid_for((int64_t(count) * parts) << t.log_n));
   const dim3 egrid{unsigned(eblocks), 1, 1};
+  const bool chunked = chunked_ok(t.log_n, {x, digits}) && x_stride % 2 == 0;
 #define FHE_STAGED(W)                                                          \
   do {                                                                         \
-    hipLaunchKernelGGL(modup_digits_kernel<W>, dgrid, dim3(kBlock), 0, stream, \
-                       d, t);                                                  \
+    if (chunked)                                                               \
+      hipLaunchKernelGGL(modup_digits_chunk_kernel<W>,                         \
+                         dim3(unsigned(((int64_t(count) * parts) << t.log_n) / \
+                                       kChunk)),                               \
+                         dim3(kBlock), 0, stream, d, t);                       \
+    else                                                                       \
+      hipLaunchKernelGGL(modup_digits_kernel<W>, dgrid, dim3(kBlock), 0,      \
+                         stream, d, t);                                        \
     if (use_lazy)                                                              \
       hipLaunchKernelGGL((extend_ntt_r16_kernel<W, true>), egrid, dim3(kBlock), \
                          0, stream, e, t);                                     \
