@@ -122,7 +122,8 @@ def test_sum_1d_by_row_matches_reference(engine, level_from_top, gpu_lib):
             assert np.array_equal(got, want), i
             dec = np.zeros(slots)
             lib.ref_decrypt(e, w, engine["sk"], ptr(dec))
-            assert np.abs(dec - xs[i].sum()).max() < 1e-5
+            # the scheme's noise (the limbs equal the reference's above)
+            assert np.abs(dec - xs[i].sum()).max() < 1e-4
             lib.ref_free_ct(w)
         din.free()
         out.free()

@@ -212,7 +212,10 @@ def test_real_keys_match_reference_and_decrypt(log_n, alpha, level, gpu_lib):
         ref.set(out_h, got)
         dec = np.zeros(slots)
         lib.ref_decrypt(e, out_h, sk, ptr(dec))
-        assert np.abs(dec - expect_msg).max() < 1e-5, delta
+        # the limbs equal the reference's above; this bounds the scheme's
+        # own noise, which varies with the reference's unseeded key and
+        # encryption randomness (observed up to 1.2e-5 at N = 2^16)
+        assert np.abs(dec - expect_msg).max() < 1e-4, delta
         out.free()
         dk.free()
         lib.ref_free_ct(out_h)
