@@ -901,21 +901,12 @@ int fhe_add_double(fhe_context *ctx, const int64_t *in, int64_t *out,
     const auto c = P.add_multiplier(value, level);
     const int limbs = int(c.size());
     require(limbs <= fhe::kMaxEwLimbs, "Add: too many limbs");
-    const int64_t poly = int64_t(limbs) << P.log_n;
     ctx->stage("elementwise", [&] {
-      for (int ct = 0; ct < count; ct++) {
-        const int64_t *x = in + ct * polys * poly;
-        int64_t *y = out + ct * polys * poly;
-        // polynomial 0 takes the scalar, the others are copied
-        fhe::launch_elementwise(ctx->tables, fhe::kEwAddScalarReduce, x,
-                                nullptr, y, limbs, limbs, mid, c.data(),
-                                ctx->stream);
-        if (polys > 1 && x != y)
-          hip_check(hipMemcpyAsync(y + poly, x + poly,
-                                   size_t(polys - 1) * poly * 8,
-                                   hipMemcpyDeviceToDevice, ctx->stream),
-                    "copy");
-      }
+      // polynomial 0 of every ciphertext takes the scalar, the others are
+      // copied: one launch for the batch
+      fhe::launch_elementwise(ctx->tables, fhe::kEwAddScalarReduce, in,
+                              nullptr, out, int64_t(count) * polys * limbs,
+                              limbs, mid, c.data(), ctx->stream, polys);
     });
     ctx->check_launch("Add");
   });

@@ -25,6 +25,7 @@ struct EwArgs {
   int limbs;
   int moduli_id;
   int op;
+  int copy_period;        // > 0: polynomials p % copy_period != 0 are copied
   int64_t c[kMaxEwLimbs]; // per-limb constant, limb l of every polynomial
 };
 
@@ -92,7 +93,12 @@ __global__ __launch_bounds__(kBlock) void elementwise_kernel(EwArgs a,
   const int64_t stride = int64_t(gridDim.x) * kBlock;
   for (int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
        idx < a.total; idx += stride) {
-    const int l = int((idx >> t.log_n) % a.limbs);
+    const int64_t flat = idx >> t.log_n;
+    const int l = int(flat % a.limbs);
+    if (a.copy_period > 0 && (flat / a.limbs) % a.copy_period != 0) {
+      a.y[idx] = a.x1[idx];
+      continue;
+    }
     const int p = a.moduli_id + l;
     const bool first = (idx & ((int64_t(1) << t.log_n) - 1)) == 0;
     const int64_t x2 = a.x2 ? a.x2[idx] : 0;
@@ -107,15 +113,22 @@ __global__ __launch_bounds__(kBlock) void elementwise_chunk_kernel(
     EwArgs a, DeviceTables t) {
   const int64_t e0 = int64_t(blockIdx.x) * kChunk;
   // block-uniform: the limb, its prime and constants
-  const int l = int(uint32_t(e0 >> t.log_n) % uint32_t(a.limbs));
+  const uint32_t flat = uint32_t(e0 >> t.log_n);
+  const int l = int(flat % uint32_t(a.limbs));
+  const int64_t idx = e0 + int64_t(threadIdx.x) * kPerThread;
+  int64_t x1[kPerThread];
+  load4(a.x1 + idx, x1);
+  if (a.copy_period > 0 &&
+      (flat / uint32_t(a.limbs)) % uint32_t(a.copy_period) != 0) {
+    store4(a.y + idx, x1);
+    return;
+  }
   const int p = a.moduli_id + l;
   const uint64_t q = t.q[p], k = t.k[p];
   const int64_t q2 = int64_t(t.q2[p]), qh = int64_t(t.qh[p]);
   const int64_t c = a.c[l];
-  const int64_t idx = e0 + int64_t(threadIdx.x) * kPerThread;
-  int64_t x1[kPerThread], x2[kPerThread] = {}, y_old[kPerThread] = {};
+  int64_t x2[kPerThread] = {}, y_old[kPerThread] = {};
   int64_t y[kPerThread];
-  load4(a.x1 + idx, x1);
   if (a.x2)
     load4(a.x2 + idx, x2);
   if (a.op == kEwMultAdd)
@@ -169,8 +182,9 @@ bool ew_needs_constants(int op) {
 void launch_elementwise(const DeviceTables &t, int op, const int64_t *x1,
                         const int64_t *x2, int64_t *y, int64_t total_limbs,
                         int limbs, int moduli_id, const int64_t *constants,
-                        hipStream_t stream) {
+                        hipStream_t stream, int copy_period) {
   EwArgs a{};
+  a.copy_period = copy_period;
   a.x1 = x1;
   a.x2 = x2;
   a.y = y;

@@ -299,10 +299,13 @@ constexpr int kMaxEwLimbs = 128;
 bool ew_needs_x2(int op);
 bool ew_needs_constants(int op);
 // constants: host array of `limbs` values (or nullptr)
+// copy_period > 0: the operation applies to polynomial 0 of every group of
+// copy_period polynomials (of `limbs` limbs each) and the others are copied
+// from x1 unchanged.
 void launch_elementwise(const DeviceTables &t, int op, const int64_t *x1,
                         const int64_t *x2, int64_t *y, int64_t total_limbs,
                         int limbs, int moduli_id, const int64_t *constants,
-                        hipStream_t stream);
+                        hipStream_t stream, int copy_period = 0);
 
 // y = rescale(guard(mont(x, c), q)) per polynomial, dropping `skip` more
 // limbs after the rescale (scalar multiply, level down).  constants: host,

@@ -84,10 +84,14 @@ __global__ __launch_bounds__(kBlock) void uneven_digits_kernel(
     const int start = ct.start[part], size = ct.size[part];
     const int64_t *xs = a.x.p[c] + coef;
     const int64_t *cst = ct.cst + int64_t(part) * ct.cst_stride;
+    // every load unconditional (limbs past the partition's size reload its
+    // last limb and are unused), so that they issue together
     int64_t xv[MAXZ], d[MAXZ];
 #pragma unroll
-    for (int j = 0; j < MAXZ; j++)
-      xv[j] = j < size ? xs[int64_t(start + j) << t.log_n] : 0;
+    for (int j = 0; j < MAXZ; j++) {
+      const int64_t v = xs[int64_t(start + min(j, size - 1)) << t.log_n];
+      xv[j] = j < size ? v : 0;
+    }
     if ((ct.signed_mask >> part) & 1)
       garner_digits<MAXZ, true>(xv, d, size, ct.moduli_id + start, cst, t);
     else
@@ -239,14 +243,19 @@ __global__ __launch_bounds__(kBlock) void uneven_automorphism_kernel(
     const int64_t *in = a.in.p[c] + off;
     int64_t *out = const_cast<int64_t *>(a.out.p[c]) + off;
     const int64_t q = int64_t(t.q[ct.moduli_id + l]);
+    // the four gathers issue before the stores (out never overlaps in)
+    uint32_t o[4], s[4];
+    int64_t x[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      const uint32_t o =
-          uint32_t(unit_tile(unit)) * kAutChunk + e * kBlock + threadIdx.x;
-      const uint32_t s = (a.h * o) & (2 * n - 1);
-      const int64_t x = in[s & (n - 1)];
-      const int64_t y = (s & n) ? -x : x;
-      out[o] = y < 0 ? y + q : guard(y, q);
+      o[e] = uint32_t(unit_tile(unit)) * kAutChunk + e * kBlock + threadIdx.x;
+      s[e] = (a.h * o[e]) & (2 * n - 1);
+      x[e] = in[s[e] & (n - 1)];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int64_t y = (s[e] & n) ? -x[e] : x[e];
+      out[o[e]] = y < 0 ? y + q : guard(y, q);
     }
   }
 }
