@@ -132,7 +132,9 @@ struct fhe_context {
   // the coefficient forms of d0 and d1 on a second stream, concurrent with
   // the key switching of d2 (a second stream and two events, default
   // priority)
-  bool overlap_trio = true;
+  // (1: forked once the trio is written; 2: once d2's INTT is done, so
+  // that the side stream runs beside the key inner product)
+  int overlap_trio = 1;
   hipStream_t side = nullptr;
   hipEvent_t fork_event = nullptr, join_event = nullptr;
   // small-prime key products accumulated without guards, reduced once

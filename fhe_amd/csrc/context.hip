@@ -173,7 +173,7 @@ int fhe_context_set_option(fhe_context *ctx, const char *name, int value) {
     else if (n == "own_limbs")
       ctx->own_limbs = value != 0;
     else if (n == "overlap_trio")
-      ctx->overlap_trio = value != 0;
+      ctx->overlap_trio = value < 0 ? 0 : value > 2 ? 2 : value;
     else if (n == "ks_occupancy")
       ctx->ks_occupancy = value;
     else if (n == "ks_prefetch")
@@ -416,22 +416,30 @@ void hmult_relin_rescale_into(fhe_context *ctx, int count, int moduli_id,
                            ctx->stream);
       });
       if (overlap) {
-        // d0, d1 (lazy) on the side stream once the trio is written; d2
-        // (exact, feeds ModUp) here
+        // d0, d1 (lazy) on the side stream, forked once the trio is written
+        // (overlap_trio 1) or once d2's INTT is done (2, alongside the key
+        // switching of d2); d2 (exact, feeds ModUp) here
         fhe::LazySpec lazy_part = d01, exact_part = d01;
         lazy_part.part = 2;
         exact_part.part = 1;
-        hip_check(hipEventRecord(ctx->fork_event, ctx->stream), "event");
-        hip_check(hipStreamWaitEvent(ctx->side, ctx->fork_event, 0), "event");
-        fhe::ntt_inverse(ctx->tables, trio, trio_c, trio_l, limbs, moduli_id,
-                         fhe::kInttExit, ctx->side, lazy_part);
-        hip_check(hipEventRecord(ctx->join_event, ctx->side), "event");
-        joined = false;
+        auto fork = [&] {
+          hip_check(hipEventRecord(ctx->fork_event, ctx->stream), "event");
+          hip_check(hipStreamWaitEvent(ctx->side, ctx->fork_event, 0),
+                    "event");
+          fhe::ntt_inverse(ctx->tables, trio, trio_c, trio_l, limbs,
+                           moduli_id, fhe::kInttExit, ctx->side, lazy_part);
+          hip_check(hipEventRecord(ctx->join_event, ctx->side), "event");
+          joined = false;
+        };
+        if (ctx->overlap_trio == 1)
+          fork();
         ctx->stage("intt(trio)", [&] {
           fhe::ntt_inverse(ctx->tables, trio, trio_c, trio_l, limbs,
                            moduli_id, fhe::kInttExit, ctx->stream,
                            exact_part);
         });
+        if (ctx->overlap_trio != 1)
+          fork();
       } else {
         ctx->stage("intt(trio)", [&] {
           fhe::ntt_inverse(ctx->tables, trio, trio_c, trio_l, limbs,

@@ -765,6 +765,10 @@ int fhe_mod_raise(fhe_context *ctx, int count, const int64_t *in,
  *                          d1 in fhe_hmult_relin_rescale runs on a second
  *                          non-blocking stream of the context (default
  *                          priority), joined by an event before ModDown;
+ *                          forked once the trio is written (1), or once
+ *                          d2's INTT is done (2: beside the key inner
+ *                          product, about 1% more throughput, that kernel's
+ *                          launches then sharing the device); 0: one stream;
  *   "ks_occupancy" 2       waves per SIMD of the key inner product (3 trades
  *                          registers for occupancy; slower);
  *   "ks_prefetch" 0        the key inner product loads the next partition's

@@ -16,6 +16,7 @@ DEFAULTS = {"split_variants": 1, "own_limbs": 1, "overlap_trio": 1,
             "ks_prefetch": 0, "ks_occupancy": 2,
             "lazy_key_switch": 1, "fuse_tensor": 0, "fuse_modup": 0}
 OPTIONS = [{"split_variants": 0}, {"own_limbs": 0}, {"overlap_trio": 0},
+           {"overlap_trio": 2},
            {"staged_modup": 0},
            {"ks_lazy_accumulate": 0}, {"ks_wide_accumulate": 1},
            {"ks_wide_accumulate": 1, "ks_lazy_accumulate": 0},

@@ -20,6 +20,7 @@ pytestmark = [pytest.mark.gpu,
                                  reason="oracle/_ref not built")]
 
 OPTIONS = [{}, {"split_variants": 0}, {"own_limbs": 0}, {"overlap_trio": 0},
+           {"overlap_trio": 2},
            {"staged_modup": 0},
            {"ks_lazy_accumulate": 0}, {"lazy_key_switch": 0},
            {"fuse_tensor": 1}, {"staged_modup": 0, "fuse_modup": 1}]
