@@ -89,18 +89,20 @@ __device__ __forceinline__ int64_t extend_limb_lazy(const int64_t (&d)[MAXZ],
                                                     int ext_limbs,
                                                     const uint64_t *es,
                                                     uint64_t q, bool small) {
-  // a negative digit (the first one is the input value itself, negative
-  // for limbs that a larger prime rescaled) enters as d + big, big a
-  // multiple of q in [2^61, 2^62) above every |d|: the same residue mod q
+  // the first digit is the input value itself, negative for limbs that a
+  // larger prime rescaled: it enters as d + big, big a multiple of q in
+  // [2^61, 2^62) above every |d| (the same residue mod q).  The other
+  // digits of a partition on this path (no signed partitions) are
+  // non-negative: each is a Montgomery product of a non-negative difference.
   const uint64_t big = q << (__builtin_clzll(q) - 2);
-  auto lift = [&](int64_t v) { return uint64_t(v) + (v < 0 ? big : 0); };
   const uint64_t *c0 = es + 2 * i;
-  uint64_t y = shoup_mult(lift(d[0]), c0[0], c0[1], q);
+  uint64_t y = shoup_mult(uint64_t(d[0]) + (d[0] < 0 ? big : 0), c0[0],
+                          c0[1], q);
 #pragma unroll
   for (int j = 1; j < MAXZ; j++)
     if (j < size) {
       const uint64_t *c = es + 2 * (j * ext_limbs + i);
-      y += shoup_mult(lift(d[j]), c[0], c[1], q);
+      y += shoup_mult(uint64_t(d[j]), c[0], c[1], q);
       if (!small && y >= 2 * q)
         y -= 2 * q;
     }
