@@ -10,8 +10,9 @@ namespace fhe {
 // Which polynomials of a batched transform may use the lazy arithmetic
 // (ntt_r16.h): those whose index modulo `period` is below `count`.  Lazy
 // outputs are other representatives of the same residues: forward outputs
-// below 2q, or below 2^48 for primes under 2^42; inverse outputs after the
-// exit as in the exact form.  The default is exact everywhere.
+// below 2q, or below 2^48 for primes under 2^42; inverse outputs with the
+// exit below 2q (canonical with kInttExitReduce).  The default is exact
+// everywhere.
 struct LazySpec {
   int period = 0;
   int count = 0;

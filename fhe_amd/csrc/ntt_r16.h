@@ -357,14 +357,25 @@ __device__ __forceinline__ void r16_limb_pass(const int64_t *src_limb,
     r16_inverse_high<A>(v, s);
     if (post != kInttRaw) {
       const int64_t ninv = t.ninv[p];
+      if (A == kExact) {
 #pragma unroll
-      for (int m = 0; m < 16; m++) {
-        // only the exact arithmetic carries negative values
-        const int64_t y = mont_exit(
-            A == kExact ? mont_mult_cs(ninv, v[m], s.q, s.k)
-                        : mont_mult_u(v[m], ninv, s.q, s.k),
-            s.q, s.k);
-        v[m] = post == kInttExitReduce ? guard(y, int64_t(s.q)) : y;
+        for (int m = 0; m < 16; m++) {
+          // only the exact arithmetic carries negative values
+          const int64_t y =
+              mont_exit(mont_mult_cs(ninv, v[m], s.q, s.k), s.q, s.k);
+          v[m] = post == kInttExitReduce ? guard(y, int64_t(s.q)) : y;
+        }
+      } else {
+        // the lazy forms need only the residue: one product with
+        // N^-1 R^-1 (the exit folded into the constant) gives it below
+        // v q / 2^62 + q < 2q (v < 2^60 for kLazySmall, v < 2q for
+        // kLazyLarge), which the reduce brings to the canonical value
+        const int64_t ninv_exit = mont_exit(ninv, s.q, s.k);
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+          const int64_t y = mont_mult_u(v[m], ninv_exit, s.q, s.k);
+          v[m] = post == kInttExitReduce ? guard_u(y, int64_t(s.q)) : y;
+        }
       }
     }
   }

@@ -151,3 +151,33 @@ def test_lazy_moddown_step_matches_reference_formula(bits):
             y = shoup(a, s_inv, (s_inv << 64) // q, q)
         got = y if y < q else y - q
         assert got == expect, (trial, v, steps)
+
+
+@pytest.mark.parametrize("bits,mode", [(40, "small"), (41, "small"),
+                                       (60, "large"), (54, "large")])
+def test_lazy_inverse_epilogue_folds_the_exit(bits, mode):
+    """The lazy inverse transforms' epilogue: one Montgomery product with
+    N^-1 R^-1 (mont_exit of the table's N^-1 R) in place of the product
+    with N^-1 R and the exit.  Over inputs up to the lazy bounds (2^59 for
+    primes below 2^42 after 16 stages, 2q otherwise) the result is below
+    2q and has the residue of the two-step form."""
+    log_n = 16
+    q = ntt_prime(bits, 1 << log_n)
+    r_bits = 62
+    mask = (1 << r_bits) - 1
+    k = (-pow(q, -1, 1 << r_bits)) % (1 << r_bits)
+
+    def mont(a, b):
+        x = a * b
+        s = ((x & mask) * k) & mask
+        return (x + s * q) >> r_bits
+
+    ninv = pow(1 << log_n, -1, q) * (1 << r_bits) % q  # N^-1 R
+    folded = mont(ninv, 1)                             # mont_exit: N^-1
+    top = (1 << 59) if mode == "small" else 2 * q
+    rng = random.Random(bits + 1)
+    for v in [0, 1, q - 1, q, 2 * q - 1, top - 1] + \
+            [rng.randrange(top) for _ in range(2000)]:
+        y = mont(v, folded)
+        assert y < 2 * q
+        assert y % q == mont(mont(v, ninv), 1) % q
