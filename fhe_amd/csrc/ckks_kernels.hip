@@ -225,8 +225,10 @@ __global__ __launch_bounds__(kBlock) void moddown_add_kernel(
       int64_t v = y - y0;
       v = v >= 0 ? v : v + q2;
       v = guard(v, int64_t(q));
-      v = mont_mult(v, t.rescale[int64_t(moduli_id) * t.prime_count + p], q,
-                    k);
+      // |v| < 2^62 (y and y0 canonical): the product with the constant in
+      // its signed-operand form, the same value as mont_mult
+      v = mont_mult_cs(t.rescale[int64_t(moduli_id) * t.prime_count + p], v,
+                       q, k);
       v = guard(v + rounder, q2);
       o[int64_t(i - 1) * n] = guard(v, int64_t(q));
     }
@@ -349,8 +351,9 @@ __global__ __launch_bounds__(kBlock) void scalar_rescale_kernel(
       int64_t v = entered(i, xv[g]) - y0;
       v = v >= 0 ? v : v + q2;
       v = guard(v, int64_t(q));
-      v = mont_mult(v, t.rescale[int64_t(a.moduli_id) * t.prime_count + p],
-                    q, k);
+      // |v| < 2^62 (entered values lie in (-q, q)): as in moddown_add_kernel
+      v = mont_mult_cs(t.rescale[int64_t(a.moduli_id) * t.prime_count + p], v,
+                       q, k);
       v = guard(v + rounder, q2);
       o[int64_t(i - 1 - a.skip) * n] = guard(v, int64_t(q));
     }
@@ -408,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void weighted_sum_kernel(
       int64_t v = guard(mont_mult(xv[x], a.c[x][i], q, k), qq) - y0[x];
       v = v >= 0 ? v : v + q2;
       v = guard(v, qq);
-      v = mont_mult(v, rs, q, k);
+      v = mont_mult_cs(rs, v, q, k); // |v| < 2^62, as in scalar_rescale
       v = guard(guard(v + rounder[x], q2), qq);
       acc = x == 0 ? v : guard(guard(acc + v, q2), qq);
     }
