@@ -83,6 +83,14 @@ __device__ __forceinline__ int opaque_tid() {
   return tid;
 }
 
+// -q for guard_neg, behind an empty asm statement that claims to modify it,
+// so that the compiler cannot fold x + (-q) back into a subtraction.
+__device__ __forceinline__ int64_t opaque_neg(int64_t q) {
+  int64_t n = -q;
+  asm("" : "+v"(n));
+  return n;
+}
+
 typedef int64_t i64x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void load4(const int64_t *p, int64_t (&v)[4]) {

@@ -199,4 +199,12 @@ FHE_HD int64_t mont_exit(int64_t x, uint64_t q, uint64_t k) {
 // Conditional subtraction (src/montgomery/unit.h:13), signed compare.
 FHE_HD int64_t guard(int64_t x, int64_t q) { return (x < q) ? x : x - q; }
 
+// guard(x, q) given nq = -q: the subtraction is one 64-bit add
+// (v_lshl_add_u64) instead of a borrow pair.  The caller keeps nq opaque to
+// the compiler (opaque_neg), which would otherwise fold x + (-q) back into
+// x - q.
+FHE_HD int64_t guard_neg(int64_t x, int64_t q, int64_t nq) {
+  return (x < q) ? x : add64(x, nq);
+}
+
 } // namespace fhe

@@ -48,23 +48,35 @@ inline bool t_host_q_small(const DeviceTables &t, int p) {
 // rescaled lie in (-q, q)).  The signed product and compares reproduce the
 // reference for both; on non-negative operands they equal the unsigned
 // forms.  Cooley-Tukey: p = mont(w, y); x' = guard(x + p, 2q);
-// y' = guard(x + 2q - p, 2q)
+// y' = guard(x + 2q - p, 2q).  nq2 = -2q, which the guards add in place of
+// subtracting 2q (guard_neg); the overloads without it serve callers that
+// do not hold it.
+__device__ __forceinline__ void ct_butterfly(int64_t &x, int64_t &y, int64_t w,
+                                             int64_t q2, int64_t nq2,
+                                             uint64_t q, uint64_t k) {
+  const int64_t p = mont_mult_cs(w, y, q, k);
+  const int64_t x0 = x;
+  x = guard_neg(add64(x0, p), q2, nq2);
+  y = guard_neg(sub64(add64(x0, q2), p), q2, nq2);
+}
 __device__ __forceinline__ void ct_butterfly(int64_t &x, int64_t &y, int64_t w,
                                              int64_t q2, uint64_t q,
                                              uint64_t k) {
-  const int64_t p = mont_mult_cs(w, y, q, k);
-  const int64_t x0 = x;
-  x = guard(add64(x0, p), q2);
-  y = guard(sub64(add64(x0, q2), p), q2);
+  ct_butterfly(x, y, w, q2, -q2, q, k);
 }
 
 // Gentleman-Sande: x' = guard(x + y, 2q); y' = mont(w, guard(x + 2q - y, 2q))
 __device__ __forceinline__ void gs_butterfly(int64_t &x, int64_t &y, int64_t w,
+                                             int64_t q2, int64_t nq2,
+                                             uint64_t q, uint64_t k) {
+  const int64_t x0 = x, y0 = y;
+  x = guard_neg(add64(x0, y0), q2, nq2);
+  y = mont_mult_cs(w, guard_neg(sub64(add64(x0, q2), y0), q2, nq2), q, k);
+}
+__device__ __forceinline__ void gs_butterfly(int64_t &x, int64_t &y, int64_t w,
                                              int64_t q2, uint64_t q,
                                              uint64_t k) {
-  const int64_t x0 = x, y0 = y;
-  x = guard(add64(x0, y0), q2);
-  y = mont_mult_cs(w, guard(sub64(add64(x0, q2), y0), q2), q, k);
+  gs_butterfly(x, y, w, q2, -q2, q, k);
 }
 
 // Load-time product with a per-prime constant (mont_enter by R^2 for the
@@ -93,6 +105,7 @@ struct R16Pass {
   int b;
   int r;
   int64_t q2;
+  int64_t nq2;  // -q2 through opaque_neg, for the exact arithmetic
   uint64_t q, k;
 };
 
@@ -101,7 +114,7 @@ template <int A>
 __device__ __forceinline__ void fwd_bfly(int64_t &x, int64_t &y,
                                          const R16Pass &s, int64_t idx) {
   if (A == kExact) {
-    ct_butterfly(x, y, s.tw[idx], s.q2, s.q, s.k);
+    ct_butterfly(x, y, s.tw[idx], s.q2, s.nq2, s.q, s.k);
     return;
   }
   const uint64_t w = s.ts[2 * idx], wq = s.ts[2 * idx + 1];
@@ -119,7 +132,7 @@ __device__ __forceinline__ void inv_bfly(int64_t &x, int64_t &y,
                                          const R16Pass &s, int64_t idx,
                                          int t) {
   if (A == kExact) {
-    gs_butterfly(x, y, s.tw[idx], s.q2, s.q, s.k);
+    gs_butterfly(x, y, s.tw[idx], s.q2, s.nq2, s.q, s.k);
     return;
   }
   const uint64_t w = s.ts[2 * idx], wq = s.ts[2 * idx + 1];
@@ -326,6 +339,7 @@ __device__ __forceinline__ void r16_limb_pass(const int64_t *src_limb,
   s.q = t.q[p];
   s.k = t.k[p];
   s.q2 = int64_t(t.q2[p]);
+  s.nq2 = A == kExact ? opaque_neg(s.q2) : 0;  // exact guards only
 
   int64_t v[16];
   // load in the ownership of the first round; the contiguous inverse pass
