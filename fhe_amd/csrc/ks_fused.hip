@@ -46,6 +46,7 @@ __device__ __forceinline__ void ks_tile(const KsArgs &a, const DeviceTables &t,
   f.k = t.k[p];
   f.q2 = int64_t(t.q2[p]);
   f.nq2 = A == kExact ? opaque_neg(f.q2) : 0;  // exact guards only
+  f.nq = A == kExact ? 0 : uint64_t(opaque_neg(int64_t(f.q)));
   const int64_t qq = int64_t(f.q);
 
   // block-uniform: every thread of the block has the same (i, pp)
@@ -262,6 +263,7 @@ __device__ __forceinline__ void modup_ntt_limb(const ModupNttArgs &a,
   s.k = t.k[p];
   s.q2 = int64_t(t.q2[p]);
   s.nq2 = A == kExact ? opaque_neg(s.q2) : 0;  // exact guards only
+  s.nq = A == kExact ? 0 : uint64_t(opaque_neg(int64_t(s.q)));
   int64_t v[16];
 #pragma clang loop unroll(full)
   for (int m = 0; m < 16; m++) {

@@ -144,6 +144,7 @@ __device__ __forceinline__ void ks_item_tile(const Addr &a,
   f.k = t.k[p];
   f.q2 = int64_t(t.q2[p]);
   f.nq2 = A == kExact ? opaque_neg(f.q2) : 0;  // exact guards only
+  f.nq = A == kExact ? 0 : uint64_t(opaque_neg(int64_t(f.q)));
   const int64_t qq = int64_t(f.q);
   // 96-bit sums reduced once: small primes, within the partition bound
   // (WIDE selects the code; the kernel chooses it only for such items)
@@ -351,6 +352,7 @@ __device__ __forceinline__ void ext_item_tile(const ExtItem &a,
   s.k = t.k[p];
   s.q2 = int64_t(t.q2[p]);
   s.nq2 = A == kExact ? opaque_neg(s.q2) : 0;  // exact guards only
+  s.nq = A == kExact ? 0 : uint64_t(opaque_neg(int64_t(s.q)));
   int64_t v[16];
 #pragma unroll
   for (int m = 0; m < 16; m++) {

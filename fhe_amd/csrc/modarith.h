@@ -188,6 +188,15 @@ FHE_HD uint64_t shoup_mult(uint64_t y, uint64_t w, uint64_t wq, uint64_t q) {
   return w * y - mulhi64(wq, y) * q;
 }
 
+// shoup_mult given nq = 2^64 - q: the same value as w y + h nq (mod 2^64),
+// a sum the multiply-adds absorb, where w y - h q ends in a borrow pair.
+// The caller keeps nq opaque to the compiler (opaque_neg), which would
+// otherwise fold the product with -q back into a subtraction.
+FHE_HD uint64_t shoup_mult_neg(uint64_t y, uint64_t w, uint64_t wq,
+                               uint64_t nq) {
+  return w * y + mulhi64(wq, y) * nq;
+}
+
 
 // Montgomery exit x * R^-1 (src/montgomery/unit.h:25-31): the single-word
 // operand is sign-extended to 128 bits.

@@ -107,6 +107,7 @@ struct R16Pass {
   int64_t q2;
   int64_t nq2;  // -q2 through opaque_neg, for the exact arithmetic
   uint64_t q, k;
+  uint64_t nq;  // 2^64 - q through opaque_neg, for the lazy forms
 };
 
 // One forward butterfly on (x, y) with twiddle entry idx.
@@ -121,7 +122,7 @@ __device__ __forceinline__ void fwd_bfly(int64_t &x, int64_t &y,
   int64_t x0 = x;
   if (A == kLazyLarge)
     x0 = guard_u(x0, s.q2);
-  const int64_t t = int64_t(shoup_mult(uint64_t(y), w, wq, s.q));
+  const int64_t t = int64_t(shoup_mult_neg(uint64_t(y), w, wq, s.nq));
   x = add64(x0, t);
   y = sub64(add64(x0, s.q2), t);
 }
@@ -144,7 +145,7 @@ __device__ __forceinline__ void inv_bfly(int64_t &x, int64_t &y,
     off = int64_t(s.q << (1 + t));
   const int64_t d = sub64(add64(x, off), y);
   x = sum;
-  y = int64_t(shoup_mult(uint64_t(d), w, wq, s.q));
+  y = int64_t(shoup_mult_neg(uint64_t(d), w, wq, s.nq));
 }
 
 // forward local bits 7..4, ROWS ownership
@@ -340,6 +341,7 @@ __device__ __forceinline__ void r16_limb_pass(const int64_t *src_limb,
   s.k = t.k[p];
   s.q2 = int64_t(t.q2[p]);
   s.nq2 = A == kExact ? opaque_neg(s.q2) : 0;  // exact guards only
+  s.nq = A == kExact ? 0 : uint64_t(opaque_neg(int64_t(s.q)));
 
   int64_t v[16];
   // load in the ownership of the first round; the contiguous inverse pass

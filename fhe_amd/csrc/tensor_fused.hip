@@ -81,6 +81,7 @@ __device__ __forceinline__ void tensor_tile(const TensorArgs &a,
   fw.k = t.k[p];
   fw.q2 = int64_t(t.q2[p]);
   fw.nq2 = opaque_neg(fw.q2);
+  fw.nq = uint64_t(opaque_neg(int64_t(fw.q)));
   R16Pass iv = fw;
   iv.tw = t.root_inverse + int64_t(p) * n;
   iv.ts = t.root_inverse_shoup + int64_t(p) * n * 2;
