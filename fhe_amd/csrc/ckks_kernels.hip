@@ -120,7 +120,9 @@ __global__ __launch_bounds__(kBlock) void modup_kernel(
     const int p = moduli_id + i;
     if (LAZY)
       out[int64_t(i) * n] = extend_limb_lazy<MAXZ>(
-          d, size, i, ext_limbs, es, t.q[p], t.q[p] < kLazySmallBound);
+          d, size, i, ext_limbs, es, t.q[p],
+          uint64_t(opaque_neg(int64_t(t.q[p]))),
+          t.q[p] < kLazySmallBound);
     else
       out[int64_t(i) * n] = extend_limb<MAXZ, SIGNED>(
           d, size, i, ext_limbs, ec, t.q[p], t.k[p], int64_t(t.q2[p]));

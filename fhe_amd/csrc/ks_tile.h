@@ -368,7 +368,7 @@ __device__ __forceinline__ void ext_item_tile(const ExtItem &a,
       v[m] = extend_limb_lazy<MAXZ>(
           d, size, i, a.ext_limbs,
           reinterpret_cast<const uint64_t *>(a.cst + a.ext_shoup_offset), s.q,
-          A == kLazySmall);
+          s.nq, A == kLazySmall);
   }
   r16_forward_high<A>(v, s);
   r16_exchange_words(v, lds, c, r, true);

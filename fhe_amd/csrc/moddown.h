@@ -12,7 +12,7 @@ namespace fhe {
 // FULL: alpha == MAXA, so that every loop over the special primes has a
 // compile-time trip count and the limb loops carry no branches.
 template <int MAXA, bool FULL = false> struct ModDown {
-  int64_t peeled[MAXA];
+  int64_t neg_peeled[MAXA];  // -p_j through opaque_neg
   int alpha, limbs, moduli_id;
 
   __device__ __forceinline__ int specials() const {
@@ -33,7 +33,7 @@ template <int MAXA, bool FULL = false> struct ModDown {
         for (int j = 0; j < MAXA; j++)
           if (j == r)
             xr = sv[j];
-        peeled[sp] = xr;
+        neg_peeled[sp] = opaque_neg(xr);
 #pragma unroll
         for (int j = 0; j < MAXA; j++) {
           if (j < r) {
@@ -57,19 +57,23 @@ template <int MAXA, bool FULL = false> struct ModDown {
   // carry; here each step is one Shoup product of v + q_top - p_j with the
   // standard-form inverse.  v + q_top - p_j lies in [0, 2^64) because every
   // p_j is a signed Montgomery product with a constant below 2^61, hence in
-  // (-2^62, 2^62 + 2^61), and q_top is in [2^63 - q, 2^63).
+  // (-2^62, 2^62 + 2^61), and q_top is in [2^63 - q, 2^63).  Every
+  // subtraction is an add of a negated value (-p_j, 2^64 - q), one 64-bit
+  // add instead of a borrow pair.
   __device__ int64_t limb_lazy(int64_t v, int i, const DeviceTables &t) const {
     const int p = moduli_id + i;
     const uint64_t q = t.q[p], top = t.q_top[p];
+    const uint64_t nq = uint64_t(opaque_neg(int64_t(q)));
     uint64_t x = uint64_t(v);
 #pragma unroll
     for (int sp = 0; sp < MAXA; sp++)
       if (sp < specials()) {
         const uint64_t *c =
             t.p_inverse_shoup + (int64_t(sp) * t.prime_count + p) * 2;
-        x = shoup_mult(x + top - uint64_t(peeled[sp]), c[0], c[1], q);
+        x = shoup_mult_neg(x + top + uint64_t(neg_peeled[sp]), c[0], c[1],
+                           nq);
       }
-    return guard_u(int64_t(x), int64_t(q));
+    return int64_t(x < q ? x : x + nq);
   }
 };
 

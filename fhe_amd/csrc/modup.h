@@ -82,13 +82,15 @@ __device__ __forceinline__ int64_t extend_limb(const int64_t (&d)[MAXZ],
 // canonicalizing INTT): Shoup products with the standard-form constants
 // (pairs {c, floor(c 2^64 / q)} at es + 2 (j ext + i)).  Primes below 2^42
 // keep the plain sum (below 2 * size * q); larger primes keep the sum below
-// 2q, as the Harvey forward transform expects.
+// 2q, as the Harvey forward transform expects.  nq = 2^64 - q, opaque to the
+// compiler (opaque_neg), for the products and the subtraction of 2q.
 template <int MAXZ>
 __device__ __forceinline__ int64_t extend_limb_lazy(const int64_t (&d)[MAXZ],
                                                     int size, int i,
                                                     int ext_limbs,
                                                     const uint64_t *es,
-                                                    uint64_t q, bool small) {
+                                                    uint64_t q, uint64_t nq,
+                                                    bool small) {
   // the first digit is the input value itself, negative for limbs that a
   // larger prime rescaled: it enters as d + big, big a multiple of q in
   // [2^61, 2^62) above every |d| (the same residue mod q).  The other
@@ -96,15 +98,15 @@ __device__ __forceinline__ int64_t extend_limb_lazy(const int64_t (&d)[MAXZ],
   // non-negative: each is a Montgomery product of a non-negative difference.
   const uint64_t big = q << (__builtin_clzll(q) - 2);
   const uint64_t *c0 = es + 2 * i;
-  uint64_t y = shoup_mult(uint64_t(d[0]) + (d[0] < 0 ? big : 0), c0[0],
-                          c0[1], q);
+  uint64_t y = shoup_mult_neg(uint64_t(d[0]) + (d[0] < 0 ? big : 0), c0[0],
+                              c0[1], nq);
 #pragma unroll
   for (int j = 1; j < MAXZ; j++)
     if (j < size) {
       const uint64_t *c = es + 2 * (j * ext_limbs + i);
-      y += shoup_mult(uint64_t(d[j]), c[0], c[1], q);
+      y += shoup_mult_neg(uint64_t(d[j]), c[0], c[1], nq);
       if (!small && y >= 2 * q)
-        y -= 2 * q;
+        y += 2 * nq;
     }
   return int64_t(y);
 }
