@@ -91,25 +91,38 @@ FHE_HD Words4 mul_words(uint64_t a, uint64_t b) {
 
 // Montgomery product of two NON-NEGATIVE operands below 2^63 (lazy residues
 // in [0, 2q) and table constants in [0, q)).  Same value as mont_mult.
+// On the device it uses no carry chain (a carry read through VCC costs wait
+// states): x = a b and y = s q come from multiply-adds as hi 2^64 + lo, and
+// since lo(x) + lo(y) = 0 mod 2^62 their low 62 bits sum to 0 (both zero)
+// or exactly 2^62 (otherwise), so
+//   (x + y) / 2^62 = 4 (hi(x) + hi(y)) + lo(x) / 2^62 + lo(y) / 2^62
+//                    + [lo(x) mod 2^62 != 0].
 FHE_HD int64_t mont_mult_u(int64_t a, int64_t b, uint64_t q, uint64_t k) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  const Words4 x = mul_words(uint64_t(a), uint64_t(b));
+  const uint32_t a0 = uint32_t(a), a1 = uint32_t(uint64_t(a) >> 32);
+  const uint32_t b0 = uint32_t(b), b1 = uint32_t(uint64_t(b) >> 32);
+  // x = a b: x0, x1 its low words; u >> 32 carries into the high half
+  // (a1 b0 + a0 b1 + 2^32 < 2^64 for a, b < 2^63)
+  const uint64_t p00 = uint64_t(a0) * b0;
+  const uint64_t u = uint64_t(a0) * b1 + (uint64_t(a1) * b0 + (p00 >> 32));
+  const uint32_t x0 = uint32_t(p00), x1 = uint32_t(u);
   // s = (x mod 2^64) * k mod 2^62
   const uint32_t k0 = uint32_t(k), k1 = uint32_t(k >> 32);
-  const uint64_t t = uint64_t(x.w0) * k0;
+  const uint64_t t = uint64_t(x0) * k0;
   const uint32_t s0 = uint32_t(t);
   const uint32_t s1 =
-      (uint32_t(t >> 32) + x.w0 * k1 + x.w1 * k0) & ((1u << 30) - 1);
-  const Words4 y = mul_words((uint64_t(s1) << 32) | s0, q);
-  // x + s*q has 62 zero low bits; keep bits 62..125
-  uint32_t c;
-  (void)add_carry(x.w0, y.w0, 0, c);
-  const uint32_t z1 = add_carry(x.w1, y.w1, c, c);
-  const uint32_t z2 = add_carry(x.w2, y.w2, c, c);
-  const uint32_t z3 = x.w3 + y.w3 + c;
-  const uint32_t lo = __builtin_amdgcn_alignbit(z2, z1, 30);
-  const uint32_t hi = __builtin_amdgcn_alignbit(z3, z2, 30);
-  return int64_t((uint64_t(hi) << 32) | lo);
+      (uint32_t(t >> 32) + x0 * k1 + x1 * k0) & ((1u << 30) - 1);
+  // y = s q: its second word is uint32_t(v)
+  const uint32_t q0 = uint32_t(q), q1 = uint32_t(q >> 32);
+  const uint64_t r00 = uint64_t(s0) * q0;
+  const uint64_t v = uint64_t(s0) * q1 + (uint64_t(s1) * q0 + (r00 >> 32));
+  // hi(x) + hi(y) (below 2^63)
+  const uint64_t hi =
+      uint64_t(a1) * b1 + uint64_t(s1) * q1 + (u >> 32) + (v >> 32);
+  const uint32_t low = x0 | (x1 << 2); // lo(x) mod 2^62, bits moved
+  const uint32_t nonzero = low < 1u ? low : 1u;
+  const uint32_t small = (x1 >> 30) + (uint32_t(v) >> 30) + nonzero;
+  return int64_t((hi << 2) + small);
 #else
   return int64_t(mont_reduce(mul_wide(uint64_t(a), uint64_t(b)), q, k));
 #endif
